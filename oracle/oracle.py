@@ -1,0 +1,137 @@
+"""ctypes wrapper of oracle/liboracle.so -- TEST INFRASTRUCTURE ONLY.
+
+Each function restates the reference arithmetic (see totri_oracle.c for the
+file:line citations) and returns float64 numpy arrays.  Modes:
+  0 FP32 op-for-op, 1 FP64 weights with FP32-pinned decisions, 2 pure FP64.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+
+FP32, PINNED64, PURE64 = 0, 1, 2
+
+_lib = None
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
+    return LIB
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB) or os.path.getmtime(LIB) < os.path.getmtime(
+                os.path.join(HERE, "totri_oracle.c")):
+            build()
+        _lib = ctypes.CDLL(LIB)
+    return _lib
+
+
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float32))
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _f3(v):
+    return (ctypes.c_float * 3)(*[float(x) for x in v])
+
+
+def set_threads(n: int) -> None:
+    lib().oracle_set_num_threads(ctypes.c_int(n))
+
+
+def num_threads() -> int:
+    return int(lib().oracle_num_threads())
+
+
+def r1_forward(volume, mp, T, origin, size, w, off, mode=FP32, i0=0, i1=None):
+    """R1 transient [i1-i0, T] (volume [X,Y,Z], mp [3,N])."""
+    vol = _f32(volume)
+    m = _f32(mp)
+    X, Y, Z = vol.shape
+    N = m.shape[1]
+    i1 = N if i1 is None else i1
+    out = np.zeros((N, T), dtype=np.float64)
+    lib().oracle_r1_forward(_p(vol), X, Y, Z, _p(m), N, T, _f3(origin), _f3(size),
+                            ctypes.c_float(w), ctypes.c_float(off), mode, _p(out),
+                            i0, i1)
+    return out[i0:i1]
+
+
+def r1_decisions(idx, mp, X, Y, Z, origin, size, w, off):
+    """Per-sample FP32 decisions for rows (i, t, xi, yi) of idx [n, 4]:
+    returns (zi, alpha, d2); zi = -1000000 where d2 <= 0."""
+    ix = np.ascontiguousarray(np.asarray(idx, dtype=np.int32))
+    m = _f32(mp)
+    n = ix.shape[0]
+    zi = np.zeros(n, dtype=np.int32)
+    al = np.zeros(n, dtype=np.float32)
+    d2 = np.zeros(n, dtype=np.float32)
+    lib().oracle_r1_decisions(_p(ix), n, _p(m), m.shape[1], X, Y, Z, _f3(origin),
+                              _f3(size), ctypes.c_float(w), ctypes.c_float(off),
+                              _p(zi), _p(al), _p(d2))
+    return zi, al, d2
+
+
+def a1_transpose(G, mp, X, Y, Z, origin, size, w, off, mode=FP32, cols=None):
+    """Exact transpose of R1 applied to G [N, T] -> volume [X, Y, Z]; only the
+    (xi, yi) columns listed in `cols` [n, 2] are computed (others are 0)."""
+    g = _f64(G)
+    m = _f32(mp)
+    N, T = g.shape
+    out = np.zeros((X, Y, Z), dtype=np.float64)
+    if cols is None:
+        cp, nc = None, 0
+    else:
+        c = np.ascontiguousarray(np.asarray(cols, dtype=np.int32))
+        cp, nc = _p(c), c.shape[0]
+    lib().oracle_a1_transpose(_p(g), _p(m), N, T, X, Y, Z, _f3(origin), _f3(size),
+                              ctypes.c_float(w), ctypes.c_float(off), mode, cp, nc,
+                              _p(out))
+    return out
+
+
+def a1_reference(G, mp, X, Y, Z, origin, size, w, off, mode=FP32, cols=None):
+    """The reference's own window-form A1 (volume_cuda.cu:95-160)."""
+    g = _f64(G)
+    m = _f32(mp)
+    N, T = g.shape
+    out = np.zeros((X, Y, Z), dtype=np.float64)
+    if cols is None:
+        cp, nc = None, 0
+    else:
+        c = np.ascontiguousarray(np.asarray(cols, dtype=np.int32))
+        cp, nc = _p(c), c.shape[0]
+    lib().oracle_a1_reference(_p(g), _p(m), N, T, X, Y, Z, _f3(origin), _f3(size),
+                              ctypes.c_float(w), ctypes.c_float(off), mode, cp, nc,
+                              _p(out))
+    return out
+
+
+def a6(volume, mp, G, origin, size, w, off, mode=FP32, i0=0, i1=None):
+    """Gradient w.r.t. measurement points [3, N] (volume_cuda.cu:188-265)."""
+    vol = _f32(volume)
+    m = _f32(mp)
+    g = _f64(G)
+    X, Y, Z = vol.shape
+    N, T = g.shape
+    i1 = N if i1 is None else i1
+    out = np.zeros((3, N), dtype=np.float64)
+    lib().oracle_a6(_p(vol), X, Y, Z, _p(m), N, _p(g), T, _f3(origin), _f3(size),
+                    ctypes.c_float(w), ctypes.c_float(off), mode, _p(out), i0, i1)
+    return out

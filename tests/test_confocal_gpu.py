@@ -1,0 +1,235 @@
+"""GPU parity of the confocal render R1 and its exact adjoint A1 against the
+CPU oracle, through the C ABI (via totrilib_amd.totri_cpp).
+
+Bar: bin/voxel decisions bit-exact; sums within 1e-5 relative L2 of the
+FP64-pinned oracle (the tolerance of the benchmark statement).
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import oracle as O
+from tests.cases import ORIGIN, SIZE, rel_l2, sparse_albedo, wall_grid
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def dev():
+    return torch.device("cuda:0")
+
+
+def gpu_r1(vol, mp, T, w, off, dtype=torch.float32, origin=ORIGIN, size=SIZE):
+    from totrilib_amd.totri_cpp.render import volume_confocal_forward
+    v = torch.from_numpy(np.ascontiguousarray(vol)).to(dev())
+    m = torch.from_numpy(np.ascontiguousarray(mp)).to(dev())
+    out = torch.full((m.shape[1], T), float("nan"), device=dev(), dtype=dtype)
+    volume_confocal_forward(v, m, out, origin, size, w, off)
+    torch.cuda.synchronize()
+    return out.float().cpu().numpy().astype(np.float64)
+
+
+def gpu_a1(G, mp, res, w, off, origin=ORIGIN, size=SIZE):
+    from totrilib_amd.totri_cpp.render import volume_confocal_grad_volume
+    g = torch.as_tensor(G).to(dev())
+    m = torch.from_numpy(np.ascontiguousarray(mp)).to(dev())
+    out = torch.full(res, float("nan"), device=dev(), dtype=torch.float32)
+    volume_confocal_grad_volume(out, m, g, origin, size, w, off)
+    torch.cuda.synchronize()
+    return out.cpu().numpy().astype(np.float64)
+
+
+def gpu_decisions(idx, mp, X, Y, Z, T, w, off):
+    from totrilib_amd import _lib
+    lib = _lib.load()
+    m = torch.from_numpy(np.ascontiguousarray(mp)).to(dev())
+    ix = torch.from_numpy(np.ascontiguousarray(idx.astype(np.int32))).to(dev())
+    n = ix.shape[0]
+    zi = torch.empty(n, dtype=torch.int32, device=dev())
+    al = torch.empty(n, dtype=torch.float32, device=dev())
+    d2 = torch.empty(n, dtype=torch.float32, device=dev())
+    rc = lib.totri_debug_confocal_samples(
+        m.data_ptr(), m.shape[1], ix.data_ptr(), n, X, Y, Z, T, _lib.f3(ORIGIN),
+        _lib.f3(SIZE), w, off, zi.data_ptr(), al.data_ptr(), d2.data_ptr(),
+        torch.cuda.current_stream().cuda_stream)
+    _lib.check(rc, "debug_confocal_samples")
+    torch.cuda.synchronize()
+    return zi.cpu().numpy(), al.cpu().numpy(), d2.cpu().numpy()
+
+
+@pytest.mark.parametrize("n_side,T,res,w", [(64, 1024, 128, 0.0025), (32, 512, 64, 0.005),
+                                            (256, 2048, 256, 0.00125)])
+def test_sample_decisions_bit_exact(n_side, T, res, w):
+    """zi = floor(z_hat) and depth^2 of 2e6 random samples, half of them drawn
+    next to voxel boundaries, equal the oracle's FP32 decisions bit for bit."""
+    rng = np.random.default_rng(11)
+    mp = wall_grid(n_side)
+    X = Y = Z = res
+    n = 1_000_000
+    idx = np.stack([rng.integers(0, mp.shape[1], n), rng.integers(0, T, n),
+                    rng.integers(0, X, n), rng.integers(0, Y, n)], axis=1)
+    # Boundary-biased half: keep the samples whose FP32 z_hat is within 1e-3
+    # of an integer, out of a larger random pool.
+    pool = np.stack([rng.integers(0, mp.shape[1], 8 * n), rng.integers(0, T, 8 * n),
+                     rng.integers(0, X, 8 * n), rng.integers(0, Y, 8 * n)], axis=1)
+    pz, pa, _ = O.r1_decisions(pool, mp, X, Y, Z, ORIGIN, SIZE, w, 0.0)
+    near = (pz > -1000000) & ((pa < 1e-3) | (pa > 1 - 1e-3))
+    idx = np.concatenate([idx, pool[near][:n]])
+    zo, ao, do = O.r1_decisions(idx, mp, X, Y, Z, ORIGIN, SIZE, w, 0.0)
+    zg, ag, dg = gpu_decisions(idx, mp, X, Y, Z, T, w, 0.0)
+    print(f"samples={len(idx)} near-boundary={near.sum()}")
+    assert np.array_equal(dg.view(np.int32), do.view(np.int32))
+    assert np.array_equal(zg, zo)
+    act = zo > -1000000
+    # alpha from the fused fast path differs from the reference by a few
+    # ulps of z_hat at most.
+    assert np.max(np.abs(ag[act] - ao[act])) < 64 * np.spacing(np.float32(res))
+
+
+def test_r1_small_matches_oracle():
+    rng = np.random.default_rng(12)
+    X, Y, Z, T = 24, 20, 28, 300
+    vol = rng.random((X, Y, Z)).astype(np.float32)
+    mp = np.stack([rng.uniform(-1, 1, 37), rng.uniform(-1, 1, 37),
+                   rng.uniform(-0.1, 0.1, 37)]).astype(np.float32)
+    w, off = 0.009, 0.003
+    got = gpu_r1(vol, mp, T, w, off)
+    ref32 = O.r1_forward(vol, mp, T, ORIGIN, SIZE, w, off, mode=O.FP32)
+    ref64 = O.r1_forward(vol, mp, T, ORIGIN, SIZE, w, off, mode=O.PINNED64)
+    e32, e64 = rel_l2(got, ref32), rel_l2(got, ref64)
+    print(f"rel-L2 vs FP32 oracle {e32:.2e}, vs FP64 pinned {e64:.2e}")
+    assert e64 < TOL and e32 < TOL
+
+
+def test_r1_cfg1_rows_match_oracle():
+    """64^3 volume, 32x32 scan, 512 bins of 0.005 m (one-way): 96 seeded rows."""
+    mp = wall_grid(32)
+    vol = sparse_albedo((64, 64, 64), 0.05, 0)
+    T, w = 512, 0.005
+    got = gpu_r1(vol, mp, T, w, 0.0)
+    rows = np.random.default_rng(13).choice(mp.shape[1], 96, replace=False)
+    ref = np.stack([O.r1_forward(vol, mp, T, ORIGIN, SIZE, w, 0.0, mode=O.PINNED64,
+                                 i0=int(r), i1=int(r) + 1)[0] for r in rows])
+    e = rel_l2(got[rows], ref)
+    print(f"cfg1 rows rel-L2 vs FP64 pinned {e:.2e}")
+    assert e < TOL
+
+
+def test_r1_bf16_fp16_storage():
+    mp = wall_grid(16)
+    vol = sparse_albedo((32, 32, 32), 0.05, 1)
+    T, w = 256, 0.01
+    ref = O.r1_forward(vol, mp, T, ORIGIN, SIZE, w, 0.0, mode=O.PINNED64)
+    for dt, tol in ((torch.bfloat16, 8e-3), (torch.float16, 1e-3)):
+        got = gpu_r1(vol, mp, T, w, 0.0, dtype=dt)
+        e = rel_l2(got, ref)
+        print(f"{dt}: rel-L2 {e:.2e}")
+        assert e < tol
+
+
+def test_a1_small_matches_oracle():
+    rng = np.random.default_rng(14)
+    X, Y, Z, T = 20, 23, 70, 260
+    mp = np.stack([rng.uniform(-1, 1, 29), rng.uniform(-1, 1, 29),
+                   rng.uniform(-0.1, 0.1, 29)]).astype(np.float32)
+    w, off = 0.01, 0.002
+    G = rng.standard_normal((29, T)).astype(np.float32)
+    got = gpu_a1(G, mp, (X, Y, Z), w, off)
+    ref32 = O.a1_transpose(G, mp, X, Y, Z, ORIGIN, SIZE, w, off, mode=O.FP32)
+    ref64 = O.a1_transpose(G, mp, X, Y, Z, ORIGIN, SIZE, w, off, mode=O.PINNED64)
+    e32, e64 = rel_l2(got, ref32), rel_l2(got, ref64)
+    print(f"rel-L2 vs FP32 oracle {e32:.2e}, vs FP64 pinned {e64:.2e}")
+    assert e64 < TOL and e32 < TOL
+
+
+def test_a1_cfg1_columns_match_oracle():
+    mp = wall_grid(32)
+    T, w = 512, 0.005
+    G = np.random.default_rng(15).random((mp.shape[1], T)).astype(np.float32)
+    got = gpu_a1(G, mp, (64, 64, 64), w, 0.0)
+    rng = np.random.default_rng(16)
+    cols = np.stack([rng.integers(0, 64, 48), rng.integers(0, 64, 48)], axis=1)
+    ref = O.a1_transpose(G, mp, 64, 64, 64, ORIGIN, SIZE, w, 0.0, mode=O.PINNED64,
+                         cols=cols)
+    e = rel_l2(got[cols[:, 0], cols[:, 1]], ref[cols[:, 0], cols[:, 1]])
+    print(f"cfg1 columns rel-L2 vs FP64 pinned {e:.2e}")
+    assert e < TOL
+
+
+def test_adjointness_on_gpu():
+    """<R v, g> = <v, A g> for the GPU pair (FP32 rounding only)."""
+    mp = wall_grid(24)
+    vol = sparse_albedo((48, 40, 56), 0.1, 2)
+    T, w = 640, 0.004
+    g = np.random.default_rng(17).standard_normal((mp.shape[1], T)).astype(np.float32)
+    rv = gpu_r1(vol, mp, T, w, 0.0)
+    ag = gpu_a1(g, mp, vol.shape, w, 0.0)
+    lhs = float(np.sum(rv * g))
+    rhs = float(np.sum(vol.astype(np.float64) * ag))
+    assert abs(lhs - rhs) < 1e-5 * np.sqrt(np.sum(rv ** 2) * np.sum(g.astype(np.float64) ** 2))
+
+
+@pytest.mark.parametrize("z", [0, 1, 62, 63])
+def test_support_of_single_voxel_render(z):
+    """A single occupied voxel at the ends of the z range: the transient is
+    nonzero exactly where the FP32 decisions put a sample into that voxel."""
+    mp = wall_grid(32)
+    X = Y = Z = 64
+    vol = np.zeros((X, Y, Z), np.float32)
+    vol[21, 40, z] = 1.0
+    T, w = 512, 0.005
+    got = gpu_r1(vol, mp, T, w, 0.0)
+    ref = O.r1_forward(vol, mp, T, ORIGIN, SIZE, w, 0.0, mode=O.FP32)
+    assert np.array_equal(got != 0, ref != 0)
+    assert rel_l2(got, ref) < TOL
+
+
+@pytest.mark.parametrize("z", [0, 1, 62, 63])
+def test_support_of_single_voxel_adjoint(z):
+    mp = wall_grid(32)
+    X = Y = Z = 64
+    T, w = 512, 0.005
+    rng = np.random.default_rng(z)
+    G = np.zeros((mp.shape[1], T), np.float32)
+    G[rng.integers(0, mp.shape[1], 40), rng.integers(0, T, 40)] = 1.0
+    got = gpu_a1(G, mp, (X, Y, Z), w, 0.0)
+    cols = np.array([[xi, yi] for xi in range(0, 64, 3) for yi in range(1, 64, 5)])
+    ref = O.a1_transpose(G, mp, X, Y, Z, ORIGIN, SIZE, w, 0.0, mode=O.FP32, cols=cols)
+    gs = got[cols[:, 0], cols[:, 1]]
+    rs = ref[cols[:, 0], cols[:, 1]]
+    assert np.array_equal(gs != 0, rs != 0)
+    assert rel_l2(gs, rs) < TOL
+
+
+def test_noncontiguous_output_and_errors():
+    from totrilib_amd.totri_cpp.render import volume_confocal_forward
+    mp = torch.from_numpy(wall_grid(4)).to(dev())
+    vol = torch.rand(8, 8, 8, device=dev())
+    out = torch.zeros(64, 16, device=dev()).T  # [16, 64] view, transposed
+    base = torch.zeros(16, 64, device=dev())
+    volume_confocal_forward(vol, mp, base, ORIGIN, SIZE, 0.05, 0.0)
+    volume_confocal_forward(vol, mp, out, ORIGIN, SIZE, 0.05, 0.0)
+    torch.cuda.synchronize()
+    assert torch.equal(out, base)
+    with pytest.raises(ValueError):
+        volume_confocal_forward(vol, mp, torch.zeros(15, 64, device=dev()), ORIGIN, SIZE,
+                                0.05, 0.0)
+    with pytest.raises(NotImplementedError):
+        volume_confocal_forward(vol.cpu(), mp, base, ORIGIN, SIZE, 0.05, 0.0)
+
+
+def test_degenerate_shapes():
+    from totrilib_amd.totri_cpp.render import (volume_confocal_forward,
+                                               volume_confocal_grad_volume)
+    mp = torch.from_numpy(wall_grid(4)).to(dev())
+    out = torch.full((16, 32), 7.0, device=dev())
+    volume_confocal_forward(torch.rand(4, 4, 1, device=dev()), mp, out, ORIGIN, SIZE,
+                            0.05, 0.0)
+    torch.cuda.synchronize()
+    assert torch.count_nonzero(out) == 0  # Z = 1: no sample has zi + 1 < Z
+    vol = torch.full((4, 4, 6), 7.0, device=dev())
+    volume_confocal_grad_volume(vol, mp[:, :0], torch.zeros(0, 32, device=dev()), ORIGIN,
+                                SIZE, 0.05, 0.0)
+    torch.cuda.synchronize()
+    assert torch.count_nonzero(vol) == 0  # no scan points: zero gradient

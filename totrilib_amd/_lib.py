@@ -1,0 +1,103 @@
+"""ctypes binding of libtotri_hip.so (the C ABI in include/totri_hip.h).
+
+The library is built in-tree by totrilib_amd.build.  Importing this module
+never falls back to another implementation: if the shared library is missing
+or does not load, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtotri_hip.so")
+
+TOTRI_OK = 0
+TOTRI_EINVAL = 1
+TOTRI_EHIP = 2
+TOTRI_EWORKSPACE = 3
+
+TOTRI_F32 = 0
+TOTRI_BF16 = 1
+TOTRI_F16 = 2
+
+_c_i64 = ctypes.c_int64
+_c_p = ctypes.c_void_p
+_c_f = ctypes.c_float
+_c_sz = ctypes.c_size_t
+_c_int = ctypes.c_int
+_F3 = ctypes.c_float * 3
+
+# name -> (restype, argtypes)
+SIGNATURES: dict[str, tuple] = {
+    "totri_last_error": (ctypes.c_char_p, []),
+    "totri_abi_version": (_c_int, []),
+    "totri_volume_confocal_forward_workspace": (_c_sz, [_c_i64] * 5),
+    "totri_volume_confocal_forward": (
+        _c_int,
+        [_c_p, _c_i64, _c_i64, _c_i64, _c_p, _c_i64, _c_p, _c_i64, _c_int,
+         _F3, _F3, _c_f, _c_f, _c_p, _c_sz, _c_p]),
+    "totri_volume_confocal_grad_volume_workspace": (_c_sz, [_c_i64] * 5),
+    "totri_volume_confocal_grad_volume": (
+        _c_int,
+        [_c_p, _c_i64, _c_i64, _c_i64, _c_p, _c_i64, _c_p, _c_i64, _c_int,
+         _F3, _F3, _c_f, _c_f, _c_p, _c_sz, _c_p]),
+    "totri_debug_confocal_samples": (
+        _c_int,
+        [_c_p, _c_i64, _c_p, _c_i64, _c_i64, _c_i64, _c_i64, _c_i64, _F3, _F3,
+         _c_f, _c_f, _c_p, _c_p, _c_p, _c_p]),
+}
+
+_lock = threading.Lock()
+_lib = None
+
+
+class TotriLibraryError(RuntimeError):
+    """The HIP library is missing or failed to load."""
+
+
+def load() -> ctypes.CDLL:
+    """Load libtotri_hip.so (once) and declare every exported signature."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise TotriLibraryError(
+                f"{LIB_PATH} is not built; run `python -m totrilib_amd.build` "
+                "(or __graft_entry__.build()) first")
+        try:
+            lib = ctypes.CDLL(LIB_PATH)
+        except OSError as e:
+            raise TotriLibraryError(f"cannot load {LIB_PATH}: {e}") from e
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+        return lib
+
+
+def last_error() -> str:
+    msg = load().totri_last_error()
+    return msg.decode() if msg else ""
+
+
+def check(rc: int, what: str) -> None:
+    """Map a status code to the exception the reference would raise."""
+    if rc == TOTRI_OK:
+        return
+    msg = f"{what}: {last_error()}"
+    if rc == TOTRI_EINVAL:
+        raise ValueError(msg)
+    raise RuntimeError(msg)
+
+
+def f3(v) -> ctypes.Array:
+    v = tuple(float(x) for x in v)
+    if len(v) != 3:
+        raise ValueError(f"expected 3 entries, got {len(v)}")
+    return _F3(*v)

@@ -1,0 +1,784 @@
+// Confocal volume render (R1) and its exact transpose (A1) for gfx950.
+//
+// Reference: src/render/volume_cuda.cu:11-186 (kernels),
+//            src/render/volume.cpp:7-41 (shape checks).
+//
+// A "sample" is one (scan point i, bin t, column (xi, yi)) triple.  Its
+// discrete decisions -- d2 > 0 and zi = floor(z_hat) with 0 <= zi <= Z-2 --
+// are taken exactly as the reference's FP32 formula takes them
+// (volume_cuda.cu:23,35,38,40-49): products and sums rounded one by one and
+// a correctly rounded square root.  The fast path uses the hardware square
+// root and a fused z_hat; it is only trusted when z_hat lies farther than a
+// proven error bound `delta` from an integer, otherwise the sample is
+// recomputed with the exact formula.  Weights (alpha, f2) are FP32 and agree
+// with the reference formula to a few ulps.
+//
+// R1 (gather into bins): a wave owns one scan point and 64*K bins, one bin
+// per lane and window, accumulating in registers.  The workgroup's waves
+// (eight scan points) share volume chunks that are streamed through LDS with
+// double buffering; per chunk each wave culls columns whose active bin range
+// misses its windows.  Column groups split the volume across workgroups when
+// there are too few scan points to fill the GPU; their partial transients are
+// summed in a fixed order by a second kernel (deterministic).
+//
+// A1 (scatter into voxels): a wave owns an 8x8 tile of columns (one column
+// per lane) and a z slab of ZS voxels whose accumulators live in LDS, one
+// row per voxel and one slot per lane, so no two lanes ever touch the same
+// address and no atomics contend.  Bins are uniform across the wave; their
+// per-bin factors come from a small precomputed table through scalar loads.
+// Scan-point ranges split the work when tiles x slabs are too few; partial
+// volumes are summed in a fixed order.
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+#include <math.h>
+
+#include "common.h"
+
+namespace totri_hip {
+namespace {
+
+constexpr int kWaves = 8;            // scan points per R1 workgroup
+constexpr int kChunkFloats = 8192;   // floats per R1 LDS chunk buffer (32 KiB)
+constexpr int kSlab = 32;            // A1 z-slab depth
+
+struct ConfGeom {
+  int X, Y, Z, N, T;
+  float ox, oy, oz;
+  float szx, szy, szz;
+  float w, off;
+  float s;       // (float)Z / size.z, volume_cuda.cu:30
+  float tc_max;  // max |tc| over all bins
+};
+
+// Guard half-width on frac(z_hat): an upper bound (with a factor-4 margin)
+// on |z_hat_fast - z_hat_exact| over samples with z_hat in [-1, Z], where
+// z_hat_fast = fma(sqrt_fast(d2), s, c) and z_hat_exact follows the
+// reference's rounding sequence.  See the derivation in the design notes:
+// each rounding contributes at most 2^-24 of its operand, the hardware
+// square root at most 2 ulp (4 * 2^-24 of depth).
+__device__ __forceinline__ float guard_delta(const ConfGeom& g, float wz, float c0,
+                                             float c_shift) {
+  const float s = fabsf(g.s);
+  float b = 4.f * s * g.tc_max + s * fabsf(g.oz) + 2.f * (float)(g.Z + 1) +
+            2.f * s * fabsf(wz - g.oz) + 3.f * (float)g.Z + fabsf(c0) +
+            fabsf(c_shift) + 3.f;
+  return fminf(b * 2.384185791015625e-07f /* 2^-22 */, 0.5f);
+}
+
+// Conservative range of bins t whose one-way radius |tc| lies in
+// [sqrt(rho2 + dlo^2), sqrt(rho2 + dhi^2)] (both branches of the sign of tc).
+// Returns lo > hi when no bin can be active.
+__device__ __forceinline__ void bin_range(const ConfGeom& g, float rho2, float dlo,
+                                          float dhi, int* lo, int* hi) {
+  if (!(dhi > 0.f)) {
+    *lo = 1;
+    *hi = 0;
+    return;
+  }
+  if (!(g.w > 0.f) || !isfinite(g.w) || !isfinite(g.off)) {
+    *lo = 0;
+    *hi = g.T - 1;
+    return;
+  }
+  const float dl = fmaxf(dlo, 0.f);
+  const float r_lo = sqrtf(rho2 + dl * dl) * (1.f - 1e-5f) - 1e-6f;
+  const float r_hi = sqrtf(rho2 + dhi * dhi) * (1.f + 1e-5f) + 1e-6f;
+  const float pa = (r_lo - g.off) / g.w, pb = (r_hi - g.off) / g.w;
+  const float na = (-r_hi - g.off) / g.w, nb = (-r_lo - g.off) / g.w;
+  float a = fminf(pa, na) - 2.f, b = fmaxf(pb, nb) + 2.f;
+  a = fminf(fmaxf(a, -1.f), (float)g.T + 1.f);
+  b = fminf(fmaxf(b, -1.f), (float)g.T + 1.f);
+  int ia = (int)floorf(a), ib = (int)ceilf(b);
+  *lo = max(ia, 0);
+  *hi = min(ib, g.T - 1);
+}
+
+// Depth interval [dlo, dhi) that keeps z_hat in [zlo, zhi) for a wall point
+// at height wz, widened by a relative margin (culling only).
+__device__ __forceinline__ void depth_range(const ConfGeom& g, float wz, float zlo,
+                                            float zhi, float* dlo, float* dhi) {
+  const float is = 1.f / g.s;
+  const float margin = 1e-4f * (fabsf(g.oz) + fabsf(wz) + ((float)g.Z + 2.f) * fabsf(is)) + 1e-6f;
+  *dlo = ((zlo + 0.5f) * is + g.oz) - wz - margin;
+  *dhi = ((zhi + 0.5f) * is + g.oz) - wz + margin;
+}
+
+// Discrete decision of one sample with depth^2 = d2 (already computed with
+// the reference rounding).  Returns floor(z_hat) - zshift and writes the
+// interpolation weight alpha = z_hat - floor(z_hat).  `cfast` is
+// (wz - oz) * s - 0.5 - zshift; the fast z_hat = fma(sqrt_fast(d2), s, cfast)
+// is trusted only when its fractional part is more than `delta` away from an
+// integer, otherwise z_hat is recomputed with the reference's exact sequence
+// (volume_cuda.cu:42-47).  For d2 <= 0 the result is unspecified; callers
+// test d2 > 0 themselves.
+__device__ __forceinline__ int confocal_decide(const ConfGeom& g, float d2, float wz,
+                                               float cfast, int zshift, float delta,
+                                               float one_m_delta, float* alpha) {
+  const float depth = sqrt_fast(d2);
+  const float zh = fmaf(depth, g.s, cfast);
+  float fl = floorf(zh);
+  float al = zh - fl;
+  int r = (int)fl;
+  if (d2 > 0.f && (al < delta || al > one_m_delta)) {
+    const float de = sqrt_rn(d2);
+    const float z = wz + de;
+    const float zs = (z - g.oz) * g.s;
+    const float zx = zs - 0.5f;
+    const float fx = floorf(zx);
+    al = zx - fx;
+    r = (int)fx - zshift;
+  }
+  *alpha = al;
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// R1
+// ---------------------------------------------------------------------------
+template <int K>
+__global__ __launch_bounds__(kWaves * 64) void r1_forward_kernel(
+    const float* __restrict__ vol, const float* __restrict__ mp,
+    float* __restrict__ dst,  // [n_groups][N][T] partial sums, or [N][T]
+    ConfGeom g, int t_base, int n_groups, int chunk_cols, int chunks_per_group) {
+  extern __shared__ float lds[];
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int grp = blockIdx.x % n_groups;
+  const int sgrp = blockIdx.x / n_groups;
+  const int i = sgrp * kWaves + wave;
+  const bool scan_ok = i < g.N;
+  const int n_cols = g.X * g.Y;
+  const int chunk_floats = chunk_cols * g.Z;
+
+  // Per-lane bin constants for each window.
+  float tc2[K], itc4[K], acc[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int t = t_base + 64 * k + lane;
+    const float tc = bin_distance(t, g.w, g.off);
+    const float q = tc * tc;
+    tc2[k] = (t < g.T) ? q : -1.f;  // -1 makes d2 negative: lane inactive
+    itc4[k] = 1.f / (q * q);
+    acc[k] = 0.f;
+  }
+
+  const int ii = scan_ok ? i : 0;
+  const float wx = uniform_f32(mp[ii]);
+  const float wy = uniform_f32(mp[g.N + ii]);
+  const float wz = uniform_f32(mp[2 * g.N + ii]);
+  float c0 = (wz - g.oz) * g.s;
+  c0 = c0 - 0.5f;
+  const float delta = guard_delta(g, wz, c0, 0.f);
+  const float one_m_delta = 1.f - delta;
+  float dlo, dhi;
+  depth_range(g, wz, 0.f, (float)(g.Z - 1), &dlo, &dhi);
+
+  const int chunk_first = grp * chunks_per_group;
+  const int total_chunks = (n_cols + chunk_cols - 1) / chunk_cols;
+  const int chunk_end = min(chunk_first + chunks_per_group, total_chunks);
+  const int nthreads = kWaves * 64;
+  const int vol_floats = n_cols * g.Z;
+
+  // Stage the first chunk.
+  if (chunk_first < chunk_end) {
+    const int base = chunk_first * chunk_floats;
+    for (int e = threadIdx.x; e < chunk_floats; e += nthreads) {
+      const int ge = base + e;
+      lds[e] = ge < vol_floats ? vol[ge] : 0.f;
+    }
+  }
+  __syncthreads();
+
+  constexpr int kPref = kChunkFloats / (kWaves * 64);  // floats per thread
+  for (int c = chunk_first; c < chunk_end; ++c) {
+    const int buf = (c - chunk_first) & 1;
+    float* cur = lds + buf * chunk_floats;
+    float* nxt = lds + (buf ^ 1) * chunk_floats;
+    const bool has_next = c + 1 < chunk_end;
+    float pref[kPref];
+    if (has_next) {
+      const int base = (c + 1) * chunk_floats;
+#pragma unroll
+      for (int r = 0; r < kPref; ++r) {
+        const int e = threadIdx.x + r * nthreads;
+        const int ge = base + e;
+        pref[r] = (e < chunk_floats && ge < vol_floats) ? vol[ge] : 0.f;
+      }
+    }
+
+    if (scan_ok) {
+      // Lane-parallel column setup: lane j describes chunk column j.
+      const int col = c * chunk_cols + lane;
+      const bool cvalid = lane < chunk_cols && col < n_cols;
+      const int colc = cvalid ? col : 0;
+      const int xi = colc / g.Y;
+      const int yi = colc - xi * g.Y;
+      const float x = cell_centre(xi, g.ox, g.szx, g.X);
+      const float y = cell_centre(yi, g.oy, g.szy, g.Y);
+      const float dx = wx - x, dy = wy - y;
+      const float dx2 = dx * dx, dy2 = dy * dy;
+      int tlo, thi;
+      bin_range(g, dx2 + dy2, dlo, dhi, &tlo, &thi);
+      const int klo = (max(tlo, t_base) - t_base) >> 6;
+      const int khi = (min(thi, t_base + 64 * K - 1) - t_base) >> 6;
+      const bool any = cvalid && thi >= t_base && tlo <= t_base + 64 * K - 1 && tlo <= thi;
+      // Bit k set <=> window k may hold an active sample of this column.
+      const unsigned packed =
+          any ? (((2u << khi) - 1u) & ~((1u << klo) - 1u)) : 0u;
+      unsigned long long mask = __ballot(any);
+      while (mask) {
+        const int j = __builtin_ctzll(mask);
+        mask &= mask - 1;
+        const unsigned wmask = (unsigned)__builtin_amdgcn_readlane((int)packed, j);
+        const float cdx2 = __builtin_bit_cast(
+            float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, dx2), j));
+        const float cdy2 = __builtin_bit_cast(
+            float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, dy2), j));
+        const float* column = cur + j * g.Z;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          if (!((wmask >> k) & 1u)) continue;
+          const float a = tc2[k] - cdx2;
+          const float d2 = a - cdy2;  // volume_cuda.cu:40
+          float al;
+          const int zi = confocal_decide(g, d2, wz, c0, 0, delta, one_m_delta, &al);
+          if (d2 > 0.f && (unsigned)zi <= (unsigned)(g.Z - 2)) {
+            const float v0 = column[zi];
+            const float v1 = column[zi + 1];
+            const float f1 = fmaf(al, v1 - v0, v0);
+            const float q = d2 * itc4[k];
+            acc[k] = fmaf(f1, q * q, acc[k]);
+          }
+        }
+      }
+    }
+
+    if (has_next) {
+#pragma unroll
+      for (int r = 0; r < kPref; ++r) {
+        const int e = threadIdx.x + r * nthreads;
+        if (e < chunk_floats) nxt[e] = pref[r];
+      }
+    }
+    __syncthreads();
+  }
+
+  if (scan_ok) {
+    float* row = dst + ((size_t)grp * g.N + i) * g.T;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int t = t_base + 64 * k + lane;
+      if (t < g.T) row[t] = acc[k];
+    }
+  }
+}
+
+// Sum of column-group partials in a fixed order, converted to the storage type.
+template <typename OutT>
+__global__ void sum_partials_kernel(const float* __restrict__ part, int n_parts,
+                                    size_t n, OutT* __restrict__ out) {
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (size_t)gridDim.x * blockDim.x) {
+    float v = part[e];
+    for (int p = 1; p < n_parts; ++p) v += part[(size_t)p * n + e];
+    store_elem(out + e, v);
+  }
+}
+
+template <typename T>
+__global__ void fill_zero_kernel(T* __restrict__ out, size_t n) {
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (size_t)gridDim.x * blockDim.x)
+    store_elem(out + e, 0.f);
+}
+
+// ---------------------------------------------------------------------------
+// A1
+// ---------------------------------------------------------------------------
+
+// Per-bin tables for A1: tc2[t] (exact reference rounding) and, per (i, t),
+// Kf[i][t] = G[i][t] / tc^8 so that a sample's weight is d2^2 * Kf.  In the
+// `safe` form (tiny radii, where 1/tc^8 could overflow) Kf = G and
+// itc4[t] = 1/tc^4 is used as (d2 * itc4)^2 * G instead.
+template <typename GT>
+__global__ void a1_tables_kernel(const GT* __restrict__ G, ConfGeom g, int safe,
+                                 int t_stride, float* __restrict__ tc2,
+                                 float* __restrict__ itc4, float* __restrict__ Kf) {
+  const size_t n = (size_t)g.N * t_stride;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const int t = (int)(e % t_stride);
+    const size_t i = e / t_stride;
+    const bool in = t < g.T;
+    const float tc = bin_distance(t, g.w, g.off);
+    const float q = tc * tc;
+    const float i4 = 1.f / (q * q);
+    if (i == 0) {
+      tc2[t] = in ? q : -1.f;
+      itc4[t] = in ? i4 : 0.f;
+    }
+    const float gv = in ? load_elem(G + i * g.T + t) : 0.f;
+    Kf[e] = safe ? gv : gv * (i4 * i4);
+  }
+}
+
+template <bool SAFE>
+__global__ __launch_bounds__(64) void a1_transpose_kernel(
+    const float* __restrict__ Kf, const float* __restrict__ tc2t,
+    const float* __restrict__ itc4t, const float* __restrict__ mp,
+    float* __restrict__ dst,  // [R][X][Y][Z] partials, or [X][Y][Z]
+    ConfGeom g, int tiles_x, int tiles_y, int n_slabs, int n_ranges, int t_stride) {
+  __shared__ float acc[(kSlab + 2) * 64];
+  const int lane = threadIdx.x;
+  int item = blockIdx.x;
+  const int tx = item % tiles_x;
+  item /= tiles_x;
+  const int ty = item % tiles_y;
+  item /= tiles_y;
+  const int slab = item % n_slabs;
+  const int range = item / n_slabs;
+
+  const int xi = tx * 8 + (lane >> 3);
+  const int yi = ty * 8 + (lane & 7);
+  const bool cvalid = xi < g.X && yi < g.Y;
+  const float x = cell_centre(xi, g.ox, g.szx, g.X);
+  const float y = cell_centre(yi, g.oy, g.szy, g.Y);
+
+  const int z0 = slab * kSlab;
+  // Row r of the accumulator holds voxel z0 - 1 + r; samples with
+  // zi in [z0-1, z0+kSlab-1] touch rows zi-z0+1 and zi-z0+2.
+  const int rmin = (z0 == 0) ? 1 : 0;                   // zi >= 0
+  const int rmax = min(kSlab, (g.Z - 2) - (z0 - 1));    // zi <= Z-2
+  const int rspan = rmax - rmin;
+  const int zshift = z0 - 1 + rmin;                     // zi = row' + zshift
+  const float zlo = (float)max(z0 - 1, 0);
+  const float zhi = (float)min(z0 + kSlab, g.Z - 1);
+
+  for (int r = 0; r < kSlab + 2; ++r) acc[r * 64 + lane] = 0.f;
+  // Only this lane touches its slots, so no barrier is needed.
+
+  const int per = (g.N + n_ranges - 1) / n_ranges;
+  const int i_begin = range * per;
+  const int i_end = min(g.N, i_begin + per);
+  float* my = acc + rmin * 64 + lane;
+
+  for (int i = i_begin; i < i_end; ++i) {
+    const float wx = mp[i], wy = mp[g.N + i], wz = mp[2 * g.N + i];
+    float c0 = (wz - g.oz) * g.s;
+    c0 = c0 - 0.5f;
+    const float cp = c0 - (float)zshift;
+    const float delta = guard_delta(g, wz, c0, cp);
+    const float one_m_delta = 1.f - delta;
+    const float dx = wx - x, dy = wy - y;
+    const float dx2 = dx * dx, dy2 = dy * dy;
+    float dlo, dhi;
+    depth_range(g, wz, zlo, zhi, &dlo, &dhi);
+    int tlo, thi;
+    bin_range(g, dx2 + dy2, dlo, dhi, &tlo, &thi);
+    if (!cvalid || rspan < 0 || tlo > thi) {
+      tlo = 0x7fffffff;
+      thi = -1;
+    }
+    const int t_begin = uniform_i32(wave_min_i32(tlo));
+    const int t_end = uniform_i32(wave_max_i32(thi));
+    // Tables are padded to a multiple of 4 bins (K = 0, tc2 = -1 beyond T),
+    // so whole groups of 4 bins can be loaded with one scalar load each.
+    const float* Ki = Kf + (size_t)i * t_stride;
+    for (int t4 = t_begin & ~3; t4 <= t_end; t4 += 4) {
+      const float4 k4 = *(const float4*)(Ki + t4);
+      const float4 q4 = *(const float4*)(tc2t + t4);
+      float4 i4 = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (SAFE) i4 = *(const float4*)(itc4t + t4);
+      const float kv[4] = {k4.x, k4.y, k4.z, k4.w};
+      const float qv[4] = {q4.x, q4.y, q4.z, q4.w};
+      const float iv[4] = {i4.x, i4.y, i4.z, i4.w};
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const float a = qv[u] - dx2;
+        const float d2 = a - dy2;
+        float al;
+        const int row = confocal_decide(g, d2, wz, cp, zshift, delta, one_m_delta, &al);
+        if (d2 > 0.f && cvalid && (unsigned)row <= (unsigned)rspan) {
+          float wt;
+          if (SAFE) {
+            const float q = d2 * iv[u];
+            wt = (q * q) * kv[u];
+          } else {
+            wt = (d2 * d2) * kv[u];
+          }
+          const float w1 = al * wt;
+          const float w0 = wt - w1;
+          atomicAdd(my + row * 64, w0);
+          atomicAdd(my + row * 64 + 64, w1);
+        }
+      }
+    }
+  }
+
+  if (cvalid) {
+    float* out = dst + (((size_t)range * g.X + xi) * g.Y + yi) * g.Z;
+    for (int r = 1; r <= kSlab; ++r) {
+      const int z = z0 + r - 1;
+      if (z < g.Z) out[z] = acc[r * 64 + lane];
+    }
+  }
+}
+
+int launch_grid(size_t n) {
+  size_t b = (n + 255) / 256;
+  if (b > 65536) b = 65536;
+  if (b < 1) b = 1;
+  return (int)b;
+}
+
+int check_common(int64_t X, int64_t Y, int64_t Z, int64_t N, int64_t T,
+                 const float* origin, const float* size, float w, float off,
+                 int dtype) {
+  TOTRI_CHECK_ARG(X >= 0 && Y >= 0 && Z >= 0 && N >= 0 && T >= 0,
+                  "negative dimension");
+  TOTRI_CHECK_ARG(X < (1ll << 31) && Y < (1ll << 31) && Z < (1ll << 31) &&
+                      N < (1ll << 31) && T < (1ll << 31),
+                  "dimension exceeds 32-bit indexing");
+  TOTRI_CHECK_ARG(X * Y * Z < (1ll << 31), "volume has more than 2^31 elements");
+  TOTRI_CHECK_ARG(N * T < (1ll << 31), "transient has more than 2^31 elements");
+  TOTRI_CHECK_ARG(origin != nullptr && size != nullptr, "null geometry");
+  TOTRI_CHECK_ARG(dtype == TOTRI_F32 || dtype == TOTRI_BF16 || dtype == TOTRI_F16,
+                  "unsupported transient dtype %d", dtype);
+  (void)w;
+  (void)off;
+  return TOTRI_OK;
+}
+
+ConfGeom make_geom(int64_t X, int64_t Y, int64_t Z, int64_t N, int64_t T,
+                   const float* origin, const float* size, float w, float off) {
+  ConfGeom g;
+  g.X = (int)X;
+  g.Y = (int)Y;
+  g.Z = (int)Z;
+  g.N = (int)N;
+  g.T = (int)T;
+  g.ox = origin[0];
+  g.oy = origin[1];
+  g.oz = origin[2];
+  g.szx = size[0];
+  g.szy = size[1];
+  g.szz = size[2];
+  g.w = w;
+  g.off = off;
+  g.s = (float)Z / size[2];
+  // Largest |tc| over bins, with the reference rounding.
+  float a = fabsf(off), b = fabsf((float)(T > 0 ? T - 1 : 0) * w + off);
+  g.tc_max = a > b ? a : b;
+  return g;
+}
+
+template <typename OutT>
+int zero_fill(OutT* out, size_t n, hipStream_t st) {
+  if (n == 0) return TOTRI_OK;
+  hipLaunchKernelGGL(fill_zero_kernel<OutT>, dim3(launch_grid(n)), dim3(256), 0, st,
+                     out, n);
+  return check_launch("fill_zero_kernel");
+}
+
+struct R1Plan {
+  int K, passes, n_groups, chunk_cols, chunks_per_group, total_chunks;
+  size_t ws;
+};
+
+R1Plan plan_r1(int64_t X, int64_t Y, int64_t Z, int64_t N, int64_t T) {
+  R1Plan p;
+  const int64_t bins_pass = T <= 64 ? 64 : T <= 128 ? 128 : T <= 256 ? 256 : T <= 512 ? 512 : 1024;
+  p.K = (int)(bins_pass / 64);
+  p.passes = (int)((T + bins_pass - 1) / bins_pass);
+  p.chunk_cols = (int)(Z > 0 ? (kChunkFloats / Z < 64 ? kChunkFloats / Z : 64) : 64);
+  if (p.chunk_cols < 1) p.chunk_cols = 1;
+  const int64_t n_cols = X * Y;
+  p.total_chunks = (int)((n_cols + p.chunk_cols - 1) / p.chunk_cols);
+  const int64_t scan_groups = (N + kWaves - 1) / kWaves;
+  const int64_t target = 2 * (int64_t)device_cu_count();
+  int64_t G = 1;
+  if (scan_groups < target) G = (target + scan_groups - 1) / scan_groups;
+  if (G > p.total_chunks) G = p.total_chunks;
+  if (G < 1) G = 1;
+  p.chunks_per_group = (int)((p.total_chunks + G - 1) / G);
+  p.n_groups = (int)((p.total_chunks + p.chunks_per_group - 1) / p.chunks_per_group);
+  // FP32 staging of the partial sums (also needed with one group when the
+  // transient is stored in a 16-bit type).
+  p.ws = (size_t)p.n_groups * N * T * sizeof(float);
+  return p;
+}
+
+size_t r1_workspace(int64_t X, int64_t Y, int64_t Z, int64_t N, int64_t T) {
+  if (X <= 0 || Y <= 0 || Z < 2 || N <= 0 || T <= 0) return 0;
+  return plan_r1(X, Y, Z, N, T).ws;
+}
+
+template <int K>
+void launch_r1(const R1Plan& p, const float* vol, const float* mp, float* dst,
+               const ConfGeom& g, int t_base, hipStream_t st) {
+  const int scan_groups = (g.N + kWaves - 1) / kWaves;
+  const int blocks = scan_groups * p.n_groups;
+  const size_t lds = 2 * (size_t)p.chunk_cols * g.Z * sizeof(float);
+  hipLaunchKernelGGL(r1_forward_kernel<K>, dim3(blocks), dim3(kWaves * 64), lds, st,
+                     vol, mp, dst, g, t_base, p.n_groups, p.chunk_cols,
+                     p.chunks_per_group);
+}
+
+template <typename OutT>
+int finish_r1(const R1Plan& p, const float* staging, size_t n, OutT* out,
+              hipStream_t st) {
+  hipLaunchKernelGGL(sum_partials_kernel<OutT>, dim3(launch_grid(n)), dim3(256), 0, st,
+                     staging, p.n_groups, n, out);
+  return check_launch("sum_partials_kernel");
+}
+
+}  // namespace
+}  // namespace totri_hip
+
+using namespace totri_hip;
+
+extern "C" size_t totri_volume_confocal_forward_workspace(int64_t X, int64_t Y,
+                                                          int64_t Z, int64_t N,
+                                                          int64_t T) {
+  return r1_workspace(X, Y, Z, N, T);
+}
+
+extern "C" int totri_volume_confocal_forward(
+    const float* volume, int64_t X, int64_t Y, int64_t Z,
+    const float* measurement_points, int64_t N, void* transient, int64_t T,
+    int transient_dtype, const float volume_origin[3], const float volume_size[3],
+    float bin_width, float bin_offset, void* workspace, size_t workspace_bytes,
+    void* stream) {
+  int rc = check_common(X, Y, Z, N, T, volume_origin, volume_size, bin_width,
+                        bin_offset, transient_dtype);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  const size_t n = (size_t)N * T;
+  if (n == 0) return TOTRI_OK;
+  TOTRI_CHECK_ARG(transient != nullptr && measurement_points != nullptr,
+                  "null transient or measurement points");
+  if (X == 0 || Y == 0 || Z < 2) {
+    // No sample can satisfy zi >= 0 && zi + 1 < Z (volume_cuda.cu:49).
+    switch (transient_dtype) {
+      case TOTRI_F32: return zero_fill((float*)transient, n, st);
+      case TOTRI_BF16: return zero_fill((__hip_bfloat16*)transient, n, st);
+      default: return zero_fill((__half*)transient, n, st);
+    }
+  }
+  TOTRI_CHECK_ARG(volume != nullptr, "null volume");
+  TOTRI_CHECK_ARG(Z <= kChunkFloats, "volume depth Z=%lld exceeds %d", (long long)Z,
+                  kChunkFloats);
+  const R1Plan p = plan_r1(X, Y, Z, N, T);
+  const size_t need = r1_workspace(X, Y, Z, N, T);
+  const bool direct = p.n_groups == 1 && transient_dtype == TOTRI_F32;
+  if (!direct && workspace_bytes < need)
+    return fail(TOTRI_EWORKSPACE, "workspace too small: need %zu bytes, got %zu",
+                need, workspace_bytes);
+  if (!direct) TOTRI_CHECK_ARG(workspace != nullptr, "null workspace");
+  const ConfGeom g = make_geom(X, Y, Z, N, T, volume_origin, volume_size,
+                               bin_width, bin_offset);
+  float* dst = direct ? (float*)transient : (float*)workspace;
+  for (int pass = 0; pass < p.passes; ++pass) {
+    const int t_base = pass * 64 * p.K;
+    switch (p.K) {
+      case 1: launch_r1<1>(p, volume, measurement_points, dst, g, t_base, st); break;
+      case 2: launch_r1<2>(p, volume, measurement_points, dst, g, t_base, st); break;
+      case 4: launch_r1<4>(p, volume, measurement_points, dst, g, t_base, st); break;
+      case 8: launch_r1<8>(p, volume, measurement_points, dst, g, t_base, st); break;
+      default: launch_r1<16>(p, volume, measurement_points, dst, g, t_base, st); break;
+    }
+    rc = check_launch("r1_forward_kernel");
+    if (rc) return rc;
+  }
+  if (direct) return TOTRI_OK;
+  switch (transient_dtype) {
+    case TOTRI_F32: return finish_r1(p, dst, n, (float*)transient, st);
+    case TOTRI_BF16: return finish_r1(p, dst, n, (__hip_bfloat16*)transient, st);
+    default: return finish_r1(p, dst, n, (__half*)transient, st);
+  }
+}
+
+namespace totri_hip {
+namespace {
+
+struct A1Plan {
+  int tiles_x, tiles_y, n_slabs, n_ranges, t_stride;
+  bool safe;
+  size_t tables_bytes, partial_bytes;
+};
+
+A1Plan plan_a1(int64_t X, int64_t Y, int64_t Z, int64_t N, int64_t T, float w,
+               float off) {
+  A1Plan p;
+  p.tiles_x = (int)((X + 7) / 8);
+  p.tiles_y = (int)((Y + 7) / 8);
+  p.n_slabs = (int)((Z + kSlab - 1) / kSlab);
+  const int64_t items = (int64_t)p.tiles_x * p.tiles_y * p.n_slabs;
+  const int64_t target = 16 * (int64_t)device_cu_count();
+  int64_t R = 1;
+  if (items < target) R = (target + items - 1) / items;
+  if (R > N) R = N;
+  if (R < 1) R = 1;
+  p.n_ranges = (int)R;
+  // Smallest nonzero |tc| decides whether 1/tc^8 may overflow.
+  float tmin = INFINITY;
+  for (int64_t t = 0; t < T && t < 4; ++t) {
+    const float tc = fabsf((float)t * w + off);
+    if (tc > 0.f && tc < tmin) tmin = tc;
+  }
+  if (w != 0.f) {
+    // Where the sign of tc changes, |tc| may be small anywhere in the range.
+    const float t0 = -off / w;
+    if (t0 > 0.f && t0 < (float)T) {
+      const int64_t tl = (int64_t)floorf(t0);
+      for (int64_t t = tl - 1; t <= tl + 2; ++t)
+        if (t >= 0 && t < T) {
+          const float tc = fabsf((float)t * w + off);
+          if (tc > 0.f && tc < tmin) tmin = tc;
+        }
+    }
+  }
+  p.safe = !(tmin >= 1e-2f);
+  p.t_stride = (int)((T + 3) & ~(int64_t)3);
+  p.tables_bytes = ((size_t)2 * p.t_stride + (size_t)N * p.t_stride) * sizeof(float);
+  p.tables_bytes = (p.tables_bytes + 255) & ~(size_t)255;
+  p.partial_bytes = p.n_ranges > 1 ? (size_t)p.n_ranges * X * Y * Z * sizeof(float) : 0;
+  return p;
+}
+
+}  // namespace
+}  // namespace totri_hip
+
+extern "C" size_t totri_volume_confocal_grad_volume_workspace(int64_t X, int64_t Y,
+                                                              int64_t Z, int64_t N,
+                                                              int64_t T) {
+  if (X <= 0 || Y <= 0 || Z < 2 || N <= 0 || T <= 0) return 0;
+  // The plan's size does not depend on the geometry, only on the safe flag's
+  // table layout, which is the same in both forms.
+  A1Plan p = plan_a1(X, Y, Z, N, T, 1.f, 1.f);
+  return p.tables_bytes + p.partial_bytes;
+}
+
+extern "C" int totri_volume_confocal_grad_volume(
+    float* volume, int64_t X, int64_t Y, int64_t Z, const float* measurement_points,
+    int64_t N, const void* transient, int64_t T, int transient_dtype,
+    const float volume_origin[3], const float volume_size[3], float bin_width,
+    float bin_offset, void* workspace, size_t workspace_bytes, void* stream) {
+  int rc = check_common(X, Y, Z, N, T, volume_origin, volume_size, bin_width,
+                        bin_offset, transient_dtype);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  const size_t nv = (size_t)X * Y * Z;
+  if (nv == 0) return TOTRI_OK;
+  TOTRI_CHECK_ARG(volume != nullptr, "null volume");
+  if (N == 0 || T == 0 || Z < 2) return zero_fill(volume, nv, st);
+  TOTRI_CHECK_ARG(transient != nullptr && measurement_points != nullptr,
+                  "null transient or measurement points");
+  const A1Plan p = plan_a1(X, Y, Z, N, T, bin_width, bin_offset);
+  const size_t need = p.tables_bytes + p.partial_bytes;
+  if (workspace_bytes < need)
+    return fail(TOTRI_EWORKSPACE, "workspace too small: need %zu bytes, got %zu",
+                need, workspace_bytes);
+  TOTRI_CHECK_ARG(workspace != nullptr, "null workspace");
+  const ConfGeom g = make_geom(X, Y, Z, N, T, volume_origin, volume_size,
+                               bin_width, bin_offset);
+  float* tc2 = (float*)workspace;
+  float* itc4 = tc2 + p.t_stride;
+  float* Kf = itc4 + p.t_stride;
+  float* partial = (float*)((char*)workspace + p.tables_bytes);
+  const size_t ntab = (size_t)N * p.t_stride;
+  switch (transient_dtype) {
+    case TOTRI_F32:
+      hipLaunchKernelGGL(a1_tables_kernel<float>, dim3(launch_grid(ntab)), dim3(256), 0,
+                         st, (const float*)transient, g, (int)p.safe, p.t_stride, tc2, itc4, Kf);
+      break;
+    case TOTRI_BF16:
+      hipLaunchKernelGGL(a1_tables_kernel<__hip_bfloat16>, dim3(launch_grid(ntab)),
+                         dim3(256), 0, st, (const __hip_bfloat16*)transient, g,
+                         (int)p.safe, p.t_stride, tc2, itc4, Kf);
+      break;
+    default:
+      hipLaunchKernelGGL(a1_tables_kernel<__half>, dim3(launch_grid(ntab)), dim3(256), 0,
+                         st, (const __half*)transient, g, (int)p.safe, p.t_stride, tc2, itc4, Kf);
+      break;
+  }
+  rc = check_launch("a1_tables_kernel");
+  if (rc) return rc;
+  float* dst = p.n_ranges > 1 ? partial : volume;
+  const int blocks = p.tiles_x * p.tiles_y * p.n_slabs * p.n_ranges;
+  if (p.safe)
+    hipLaunchKernelGGL(a1_transpose_kernel<true>, dim3(blocks), dim3(64), 0, st, Kf, tc2,
+                       itc4, measurement_points, dst, g, p.tiles_x, p.tiles_y, p.n_slabs,
+                       p.n_ranges, p.t_stride);
+  else
+    hipLaunchKernelGGL(a1_transpose_kernel<false>, dim3(blocks), dim3(64), 0, st, Kf,
+                       tc2, itc4, measurement_points, dst, g, p.tiles_x, p.tiles_y,
+                       p.n_slabs, p.n_ranges, p.t_stride);
+  rc = check_launch("a1_transpose_kernel");
+  if (rc) return rc;
+  if (p.n_ranges > 1) {
+    hipLaunchKernelGGL(sum_partials_kernel<float>, dim3(launch_grid(nv)), dim3(256), 0,
+                       st, partial, p.n_ranges, nv, volume);
+    rc = check_launch("sum_partials_kernel");
+  }
+  return rc;
+}
+
+namespace totri_hip {
+namespace {
+
+__global__ void confocal_samples_kernel(const float* __restrict__ mp,
+                                        const int* __restrict__ idx, int n, ConfGeom g,
+                                        int* __restrict__ zi_out,
+                                        float* __restrict__ alpha_out,
+                                        float* __restrict__ d2_out) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int i = idx[4 * k], t = idx[4 * k + 1], xi = idx[4 * k + 2], yi = idx[4 * k + 3];
+  if (i < 0 || i >= g.N || t < 0 || t >= g.T || xi < 0 || xi >= g.X || yi < 0 ||
+      yi >= g.Y) {
+    zi_out[k] = -2000000;
+    alpha_out[k] = 0.f;
+    d2_out[k] = 0.f;
+    return;
+  }
+  const float wx = mp[i], wy = mp[g.N + i], wz = mp[2 * g.N + i];
+  float c0 = (wz - g.oz) * g.s;
+  c0 = c0 - 0.5f;
+  const float delta = guard_delta(g, wz, c0, 0.f);
+  const float tc = bin_distance(t, g.w, g.off);
+  const float x = cell_centre(xi, g.ox, g.szx, g.X);
+  const float y = cell_centre(yi, g.oy, g.szy, g.Y);
+  const float dx = wx - x, dy = wy - y;
+  const float a = tc * tc - dx * dx;
+  const float d2 = a - dy * dy;
+  float al;
+  const int zi = confocal_decide(g, d2, wz, c0, 0, delta, 1.f - delta, &al);
+  d2_out[k] = d2;
+  zi_out[k] = d2 > 0.f ? zi : -1000000;
+  alpha_out[k] = d2 > 0.f ? al : 0.f;
+}
+
+}  // namespace
+}  // namespace totri_hip
+
+extern "C" int totri_debug_confocal_samples(
+    const float* measurement_points, int64_t N, const int32_t* idx, int64_t n,
+    int64_t X, int64_t Y, int64_t Z, int64_t T, const float volume_origin[3],
+    const float volume_size[3], float bin_width, float bin_offset, int32_t* zi_out,
+    float* alpha_out, float* d2_out, void* stream) {
+  int rc = check_common(X, Y, Z, N, T, volume_origin, volume_size, bin_width,
+                        bin_offset, TOTRI_F32);
+  if (rc) return rc;
+  TOTRI_CHECK_ARG(n >= 0 && n < (1ll << 31), "bad sample count");
+  if (n == 0) return TOTRI_OK;
+  TOTRI_CHECK_ARG(measurement_points && idx && zi_out && alpha_out && d2_out,
+                  "null pointer");
+  // Every listed bin index must be < T (T fixes the guard's bound on |tc|).
+  ConfGeom g = make_geom(X, Y, Z, N, T, volume_origin, volume_size, bin_width,
+                         bin_offset);
+  hipLaunchKernelGGL(confocal_samples_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256),
+                     0, (hipStream_t)stream, measurement_points, (const int*)idx, (int)n, g,
+                     (int*)zi_out, alpha_out, d2_out);
+  return check_launch("confocal_samples_kernel");
+}

@@ -1,0 +1,91 @@
+"""totri_cpp.render -- volume renderers and their adjoints.
+
+Same names and positional arguments as the reference's pybind11 submodule
+(src/totri.cpp:35-47); every function writes its output tensor in place and
+returns None.  The OptiX mesh functions of that submodule are not provided.
+"""
+from __future__ import annotations
+
+import typing
+
+import torch
+
+from totrilib_amd import _lib
+from totrilib_amd.totri_cpp._common import (
+    OutputSlot, require_f32, require_gpu, storage_code, stream_ptr, workspace)
+
+Vec3 = typing.Sequence[float]
+
+
+def volume_confocal_forward(volume: torch.Tensor, measurement_points: torch.Tensor,
+                            transient: torch.Tensor, volume_origin: Vec3,
+                            volume_size: Vec3, bin_width: float,
+                            bin_offset: float) -> None:
+    """R1: transient[N, T] = render(volume[X, Y, Z], measurement_points[3, N]).
+
+    Reference: src/render/volume.cpp:7-23, src/render/volume_cuda.cu:11-93.
+    """
+    dev = require_gpu(volume, measurement_points, transient,
+                      names=("volume", "measurement_points", "transient"))
+    if volume.dim() != 3 or measurement_points.dim() != 2 or transient.dim() != 2:
+        raise ValueError("expected volume [X,Y,Z], measurement_points [3,N], transient [N,T]")
+    if measurement_points.shape[1] != transient.shape[0]:
+        # volume.cpp:14-16
+        raise ValueError("Measurement and transient dimensions do not match.")
+    if measurement_points.shape[0] != 3:
+        raise ValueError("Measurement points must have shape 3xN.")
+    require_f32(volume, "volume")
+    require_f32(measurement_points, "measurement_points")
+    code = storage_code(transient, "transient")
+    lib = _lib.load()
+    X, Y, Z = (int(s) for s in volume.shape)
+    N, T = (int(s) for s in transient.shape)
+    vol = volume.contiguous()
+    mp = measurement_points.contiguous()
+    out = OutputSlot(transient)
+    with torch.cuda.device(dev):
+        ws, wsp = workspace(lib.totri_volume_confocal_forward_workspace(X, Y, Z, N, T), dev)
+        rc = lib.totri_volume_confocal_forward(
+            vol.data_ptr(), X, Y, Z, mp.data_ptr(), N, out.ptr(), T, code,
+            _lib.f3(volume_origin), _lib.f3(volume_size), float(bin_width),
+            float(bin_offset), wsp, ws.numel() if ws is not None else 0,
+            stream_ptr(dev))
+        _lib.check(rc, "volume_confocal_forward")
+        out.finish()
+
+
+def volume_confocal_grad_volume(volume: torch.Tensor, measurement_points: torch.Tensor,
+                                transient: torch.Tensor, volume_origin: Vec3,
+                                volume_size: Vec3, bin_width: float,
+                                bin_offset: float) -> None:
+    """A1: volume[X, Y, Z] = R1^T(transient[N, T]); the exact transpose of R1.
+
+    Reference: src/render/volume.cpp:25-41, src/render/volume_cuda.cu:95-186.
+    """
+    dev = require_gpu(volume, measurement_points, transient,
+                      names=("volume", "measurement_points", "transient"))
+    if volume.dim() != 3 or measurement_points.dim() != 2 or transient.dim() != 2:
+        raise ValueError("expected volume [X,Y,Z], measurement_points [3,N], transient [N,T]")
+    if measurement_points.shape[1] != transient.shape[0]:
+        # volume.cpp:32-34
+        raise ValueError("Measurement and transient dimensions do not match.")
+    if measurement_points.shape[0] != 3:
+        raise ValueError("Measurement points must have shape 3xN.")
+    require_f32(volume, "volume")
+    require_f32(measurement_points, "measurement_points")
+    code = storage_code(transient, "transient")
+    lib = _lib.load()
+    X, Y, Z = (int(s) for s in volume.shape)
+    N, T = (int(s) for s in transient.shape)
+    mp = measurement_points.contiguous()
+    tr = transient.contiguous()
+    out = OutputSlot(volume)
+    with torch.cuda.device(dev):
+        ws, wsp = workspace(lib.totri_volume_confocal_grad_volume_workspace(X, Y, Z, N, T), dev)
+        rc = lib.totri_volume_confocal_grad_volume(
+            out.ptr(), X, Y, Z, mp.data_ptr(), N, tr.data_ptr(), T, code,
+            _lib.f3(volume_origin), _lib.f3(volume_size), float(bin_width),
+            float(bin_offset), wsp, ws.numel() if ws is not None else 0,
+            stream_ptr(dev))
+        _lib.check(rc, "volume_confocal_grad_volume")
+        out.finish()
