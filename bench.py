@@ -1,0 +1,232 @@
+"""Benchmark: confocal render + backprojection (albedo gradient) throughput.
+
+Metric: voxel x scan-point pairs per second for one forward render (R1) plus
+one backward pass (A1, the albedo gradient of an L2 loss), at
+128^3 voxels / 64x64 scan points / 1024 bins of 0.005 m round trip
+(one-way bin width 0.0025 m), seeded 3 % sparse albedo, FP32.
+
+One step = transient = R1(v); G = 2 (transient - target); grad = A1(G).
+Inputs live in HBM before the timed region.  The timed region is exactly
+`--steps` steps bracketed by device synchronisation; per-step HIP events give
+p50/p99.  The JSON line also carries the roofline of the dominant kernel
+(FP32 VALU, algorithmic FLOPs per launch / measured launch time) and the CPU
+baseline (the oracle's FP32 restatement of the same formulas on host cores).
+
+Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP32 = 157.3e12   # MI355X published FP32 vector peak (packed issue)
+PEAK_HBM = 8.0e12      # MI355X HBM3E bytes/s
+
+WORKLOAD = dict(n_side=64, T=1024, res=128, w_round_trip=0.005, occupancy=0.03,
+                seed=1, target_seed=2)
+ORIGIN = (-1.0, -1.0, 0.5)
+SIZE = (2.0, 2.0, 2.0)
+
+
+def flop_model(mp: np.ndarray, res: int, T: int, w: float, off: float = 0.0):
+    """Algorithmic FLOPs per (voxel, scan point) pair, rocprof-compute
+    convention (add/mul 1, FMA 2, sqrt/div 1, floor/compare 0):
+
+      R1 per active sample 18; A1 per window bin 17 plus 23 per pair.
+
+    a_fwd = active R1 samples per pair; a_adj = bins the reference's window
+    form of A1 visits per pair.  Both are counted in closed form per
+    (scan point, column): active bins of a column form one contiguous range
+    (radius between the first and last valid voxel-centre depths), and the
+    window form telescopes to c(Z) + c(Z-1) - 2 c(0) with
+    c(k) = clamp(ceil((|p - q_k| - off) / w), 0, T) at voxel centre k.
+    """
+    X = Y = Z = res
+    s = Z / SIZE[2]
+    x = ORIGIN[0] + (np.arange(X) + 0.5) * SIZE[0] / X
+    y = ORIGIN[1] + (np.arange(Y) + 0.5) * SIZE[1] / Y
+    zc = ORIGIN[2] + (np.arange(Z) + 0.5) * SIZE[2] / Z
+    fwd = 0.0
+    adj = 0.0
+    for i in range(mp.shape[1]):
+        wx, wy, wz = (float(v) for v in mp[:, i])
+        rho2 = ((wx - x)[:, None] ** 2 + (wy - y)[None, :] ** 2).ravel()
+        dlo = max(ORIGIN[2] + 0.5 / s - wz, 0.0)
+        dhi = ORIGIN[2] + (Z - 0.5) / s - wz
+        if dhi <= 0:
+            continue
+        r_lo = np.sqrt(rho2 + dlo ** 2)
+        r_hi = np.sqrt(rho2 + dhi ** 2)
+        t0 = np.clip(np.ceil((r_lo - off) / w), 0, T)
+        t1 = np.clip(np.ceil((r_hi - off) / w), 0, T)
+        fwd += float(np.sum(t1 - t0))
+
+        def c(k):
+            d = np.sqrt(rho2 + (zc[k] - wz) ** 2) if k < Z else \
+                np.sqrt(rho2 + (zc[Z - 1] + SIZE[2] / Z - wz) ** 2)
+            return np.clip(np.ceil((d - off) / w), 0, T)
+        adj += float(np.sum(c(Z) + c(Z - 1) - 2 * c(0)))
+    pairs = float(X * Y * Z) * mp.shape[1]
+    a_fwd = fwd / pairs
+    a_adj = adj / pairs
+    return dict(a_fwd=a_fwd, a_adj=a_adj, f_r1=18 * a_fwd, f_a1=17 * a_adj + 23,
+                f_pair=18 * a_fwd + 17 * a_adj + 23)
+
+
+def make_inputs(torch, dev):
+    from tests.cases import sparse_albedo, wall_grid
+    n, res, T = WORKLOAD["n_side"], WORKLOAD["res"], WORKLOAD["T"]
+    mp = wall_grid(n)
+    vol = sparse_albedo((res, res, res), WORKLOAD["occupancy"], WORKLOAD["seed"])
+    tgt_vol = sparse_albedo((res, res, res), WORKLOAD["occupancy"], WORKLOAD["target_seed"])
+    return mp, vol, tgt_vol
+
+
+def cpu_baseline(mp, vol, T, w):
+    """Oracle FP32 restatement (same formulas, brute force as the reference
+    kernels) on host cores: R1 on a seeded subset of scan points, A1 on a
+    seeded subset of columns; work is linear in both, so the full-size time
+    is extrapolated from the subsets."""
+    from oracle import oracle as O
+    threads = min(os.cpu_count() or 1, 16)
+    O.set_threads(threads)
+    X, Y, Z = vol.shape
+    N = mp.shape[1]
+    rng = np.random.default_rng(0)
+    # R1: rows
+    rows = min(N, 256)
+    i0 = int(rng.integers(0, N - rows + 1))
+    t0 = time.perf_counter()
+    O.r1_forward(vol, mp, T, ORIGIN, SIZE, w, 0.0, mode=O.FP32, i0=i0, i1=i0 + rows)
+    t_r1 = (time.perf_counter() - t0) * N / rows
+    # A1: columns
+    ncols = min(X * Y, 256)
+    cols = np.stack([rng.integers(0, X, ncols), rng.integers(0, Y, ncols)], axis=1)
+    G = rng.standard_normal((N, T)).astype(np.float32)
+    t0 = time.perf_counter()
+    O.a1_transpose(G, mp, X, Y, Z, ORIGIN, SIZE, w, 0.0, mode=O.FP32, cols=cols)
+    t_a1 = (time.perf_counter() - t0) * (X * Y) / ncols
+    pairs = float(X * Y * Z) * N
+    return dict(value=pairs / (t_r1 + t_a1), unit="pairs/s", cores=threads, kind="port",
+                sample=(f"oracle FP32 restatement, {threads} OpenMP threads: R1 on {rows} of "
+                        f"{N} scan points, A1 on {ncols} of {X * Y} columns, extrapolated "
+                        f"linearly (R1 {t_r1:.1f} s + A1 {t_a1:.1f} s per full step)"))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this benchmark runs on one GPU; --gpus must be 1 "
+                                   "(the multi-GPU sharding is a design note, not built)"}))
+        sys.exit(2)
+
+    import torch
+    from totrilib_amd.totri_cpp.render import (volume_confocal_forward,
+                                               volume_confocal_grad_volume)
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    T, res = WORKLOAD["T"], WORKLOAD["res"]
+    w = WORKLOAD["w_round_trip"] / 2  # one-way radius per bin
+    mp_np, vol_np, tgt_np = make_inputs(torch, dev)
+    mp = torch.from_numpy(mp_np).to(dev)
+    vol = torch.from_numpy(vol_np).to(dev)
+    N = mp.shape[1]
+    target = torch.empty((N, T), device=dev)
+    volume_confocal_forward(torch.from_numpy(tgt_np).to(dev), mp, target, ORIGIN, SIZE, w, 0.0)
+    transient = torch.empty((N, T), device=dev)
+    G = torch.empty((N, T), device=dev)
+    grad = torch.empty((res, res, res), device=dev)
+
+    def step(ev=None):
+        if ev is not None:
+            ev[0].record()
+        volume_confocal_forward(vol, mp, transient, ORIGIN, SIZE, w, 0.0)
+        if ev is not None:
+            ev[1].record()
+        torch.sub(transient, target, out=G)
+        G.mul_(2.0)
+        if ev is not None:
+            ev[2].record()
+        volume_confocal_grad_volume(grad, mp, G, ORIGIN, SIZE, w, 0.0)
+        if ev is not None:
+            ev[3].record()
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        step(evs[k])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    step_ms = np.array([e[0].elapsed_time(e[3]) for e in evs])
+    r1_ms = np.array([e[0].elapsed_time(e[1]) for e in evs])
+    a1_ms = np.array([e[2].elapsed_time(e[3]) for e in evs])
+
+    pairs = float(res ** 3) * N
+    fm = flop_model(mp_np, res, T, w)
+    value = pairs * args.steps / wall
+    r1_avg, a1_avg = float(r1_ms.mean()), float(a1_ms.mean())
+    if r1_avg >= a1_avg:
+        dom, flops, dur = "r1_forward_kernel", fm["f_r1"] * pairs, r1_avg
+    else:
+        dom, flops, dur = "a1_transpose_kernel", fm["f_a1"] * pairs, a1_avg
+    achieved = flops / (dur * 1e-3)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get(dom)
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(mp_np, vol_np, T, w)
+
+    line = {
+        "metric": "render+backproject voxel·scanpoint pairs/s fwd+bwd at 128³/64²/1024; p50/p99",
+        "value": value,
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "p50_ms": float(np.percentile(step_ms, 50)),
+        "p99_ms": float(np.percentile(step_ms, 99)),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded 3% sparse U(0,1) albedo, make_wall_grid scan)",
+        "config": {"workload": "confocal R1 render + A1 albedo gradient, 128^3 voxels, "
+                               "64x64 scan on [-1,1]^2 m, 1024 bins of 0.005 m round trip, "
+                               "box x,y in [-1,1], z in [0.5,2.5] m",
+                   "voxels": res ** 3, "scan_points": N, "bins": T,
+                   "pairs_per_step": pairs},
+        "kernels_ms": {"r1_forward_avg": r1_avg, "a1_backward_avg": a1_avg},
+        "flop_model": fm,
+        "roofline": {"bound": "valu", "kernel": dom, "achieved": achieved / 1e12,
+                     "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32, "traffic": traffic},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -24,7 +24,7 @@
 // A1 (scatter into voxels): a wave owns an 8x8 tile of columns (one column
 // per lane) and a z slab of ZS voxels whose accumulators live in LDS, one
 // row per voxel and one slot per lane, so no two lanes ever touch the same
-// address and no atomics contend.  Bins are uniform across the wave; their
+// address and updates are plain read-modify-writes (no atomics).  Bins are uniform across the wave; their
 // per-bin factors come from a small precomputed table through scalar loads.
 // Scan-point ranges split the work when tiles x slabs are too few; partial
 // volumes are summed in a fixed order.
@@ -83,14 +83,27 @@ __device__ __forceinline__ void bin_range(const ConfGeom& g, float rho2, float d
   const float dl = fmaxf(dlo, 0.f);
   const float r_lo = sqrtf(rho2 + dl * dl) * (1.f - 1e-5f) - 1e-6f;
   const float r_hi = sqrtf(rho2 + dhi * dhi) * (1.f + 1e-5f) + 1e-6f;
-  const float pa = (r_lo - g.off) / g.w, pb = (r_hi - g.off) / g.w;
-  const float na = (-r_hi - g.off) / g.w, nb = (-r_lo - g.off) / g.w;
-  float a = fminf(pa, na) - 2.f, b = fmaxf(pb, nb) + 2.f;
-  a = fminf(fmaxf(a, -1.f), (float)g.T + 1.f);
-  b = fminf(fmaxf(b, -1.f), (float)g.T + 1.f);
-  int ia = (int)floorf(a), ib = (int)ceilf(b);
-  *lo = max(ia, 0);
-  *hi = min(ib, g.T - 1);
+  const float top = (float)g.T + 1.f;
+  // Positive radii tc in [r_lo, r_hi] and negative radii tc in [-r_hi, -r_lo]
+  // give two bin intervals; each is clipped to [0, T-1] on its own and the
+  // result is the hull of the non-empty ones.
+  const float pa = fminf(fmaxf((r_lo - g.off) / g.w - 2.f, -1.f), top);
+  const float pb = fminf(fmaxf((r_hi - g.off) / g.w + 2.f, -1.f), top);
+  const float na = fminf(fmaxf((-r_hi - g.off) / g.w - 2.f, -1.f), top);
+  const float nb = fminf(fmaxf((-r_lo - g.off) / g.w + 2.f, -1.f), top);
+  int plo = max((int)floorf(pa), 0), phi = min((int)ceilf(pb), g.T - 1);
+  int nlo = max((int)floorf(na), 0), nhi = min((int)ceilf(nb), g.T - 1);
+  const bool pe = plo <= phi, ne = nlo <= nhi;
+  if (pe && ne) {
+    *lo = min(plo, nlo);
+    *hi = max(phi, nhi);
+  } else if (pe) {
+    *lo = plo;
+    *hi = phi;
+  } else {
+    *lo = nlo;
+    *hi = nhi;
+  }
 }
 
 // Depth interval [dlo, dhi) that keeps z_hat in [zlo, zhi) for a wall point
@@ -322,15 +335,25 @@ __global__ void a1_tables_kernel(const GT* __restrict__ G, ConfGeom g, int safe,
   }
 }
 
+constexpr int kA1Waves = 4;  // independent waves (work items) per A1 workgroup
+
+__device__ __forceinline__ float lane_value(float v, int lane) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
+}
+
 template <bool SAFE>
-__global__ __launch_bounds__(64) void a1_transpose_kernel(
+__global__ __launch_bounds__(kA1Waves * 64) void a1_transpose_kernel(
     const float* __restrict__ Kf, const float* __restrict__ tc2t,
     const float* __restrict__ itc4t, const float* __restrict__ mp,
     float* __restrict__ dst,  // [R][X][Y][Z] partials, or [X][Y][Z]
-    ConfGeom g, int tiles_x, int tiles_y, int n_slabs, int n_ranges, int t_stride) {
-  __shared__ float acc[(kSlab + 2) * 64];
-  const int lane = threadIdx.x;
-  int item = blockIdx.x;
+    ConfGeom g, int tiles_x, int tiles_y, int n_slabs, int n_ranges, int t_stride,
+    int n_items) {
+  __shared__ float acc_all[kA1Waves][(kSlab + 2) * 64];
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  int item = blockIdx.x * kA1Waves + wave;
+  if (item >= n_items) return;  // whole wave exits; no barrier follows
+  float* acc = acc_all[wave];
   const int tx = item % tiles_x;
   item /= tiles_x;
   const int ty = item % tiles_y;
@@ -381,35 +404,40 @@ __global__ __launch_bounds__(64) void a1_transpose_kernel(
     }
     const int t_begin = uniform_i32(wave_min_i32(tlo));
     const int t_end = uniform_i32(wave_max_i32(thi));
-    // Tables are padded to a multiple of 4 bins (K = 0, tc2 = -1 beyond T),
-    // so whole groups of 4 bins can be loaded with one scalar load each.
     const float* Ki = Kf + (size_t)i * t_stride;
-    for (int t4 = t_begin & ~3; t4 <= t_end; t4 += 4) {
-      const float4 k4 = *(const float4*)(Ki + t4);
-      const float4 q4 = *(const float4*)(tc2t + t4);
-      float4 i4 = make_float4(0.f, 0.f, 0.f, 0.f);
-      if (SAFE) i4 = *(const float4*)(itc4t + t4);
-      const float kv[4] = {k4.x, k4.y, k4.z, k4.w};
-      const float qv[4] = {q4.x, q4.y, q4.z, q4.w};
-      const float iv[4] = {i4.x, i4.y, i4.z, i4.w};
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const float a = qv[u] - dx2;
+    // Bins are visited in batches of 64: one coalesced vector load per table
+    // brings a batch into registers (lane j holds bin tb + j), and each bin's
+    // value is broadcast with readlane.  Tables are padded to a multiple of
+    // 64 bins (K = 0, tc2 = -1 beyond T).
+    for (int tb = t_begin & ~63; tb <= t_end; tb += 64) {
+      const float kreg = Ki[tb + lane];
+      const float treg = tc2t[tb + lane];
+      const float ireg = SAFE ? itc4t[tb + lane] : 0.f;
+      const int u0 = max(t_begin - tb, 0);
+      const int u1 = min(t_end - tb, 63);
+      for (int u = u0; u <= u1; ++u) {
+        const float kt = lane_value(kreg, u);
+        const float a = lane_value(treg, u) - dx2;
         const float d2 = a - dy2;
         float al;
         const int row = confocal_decide(g, d2, wz, cp, zshift, delta, one_m_delta, &al);
         if (d2 > 0.f && cvalid && (unsigned)row <= (unsigned)rspan) {
           float wt;
           if (SAFE) {
-            const float q = d2 * iv[u];
-            wt = (q * q) * kv[u];
+            const float q = d2 * lane_value(ireg, u);
+            wt = (q * q) * kt;
           } else {
-            wt = (d2 * d2) * kv[u];
+            wt = (d2 * d2) * kt;
           }
           const float w1 = al * wt;
           const float w0 = wt - w1;
-          atomicAdd(my + row * 64, w0);
-          atomicAdd(my + row * 64 + 64, w1);
+          // Each lane owns its slots and a wave's LDS accesses complete in
+          // order, so a plain read-modify-write is exact.  (LDS float atomics
+          // measure ~25x slower than this on gfx950.)
+          float* p = my + row * 64;
+          const float a0 = p[0], a1 = p[64];
+          p[0] = a0 + w0;
+          p[64] = a1 + w1;
         }
       }
     }
@@ -638,8 +666,10 @@ A1Plan plan_a1(int64_t X, int64_t Y, int64_t Z, int64_t N, int64_t T, float w,
         }
     }
   }
-  p.safe = !(tmin >= 1e-2f);
-  p.t_stride = (int)((T + 3) & ~(int64_t)3);
+  // K = G / tc^8 stays finite for |G| < 3e38 * tc^8: with tc >= 1e-3 m that
+  // is |G| < 3e14, far above any transient gradient.
+  p.safe = !(tmin >= 1e-3f);
+  p.t_stride = (int)((T + 63) & ~(int64_t)63);
   p.tables_bytes = ((size_t)2 * p.t_stride + (size_t)N * p.t_stride) * sizeof(float);
   p.tables_bytes = (p.tables_bytes + 255) & ~(size_t)255;
   p.partial_bytes = p.n_ranges > 1 ? (size_t)p.n_ranges * X * Y * Z * sizeof(float) : 0;
@@ -705,15 +735,16 @@ extern "C" int totri_volume_confocal_grad_volume(
   rc = check_launch("a1_tables_kernel");
   if (rc) return rc;
   float* dst = p.n_ranges > 1 ? partial : volume;
-  const int blocks = p.tiles_x * p.tiles_y * p.n_slabs * p.n_ranges;
+  const int items = p.tiles_x * p.tiles_y * p.n_slabs * p.n_ranges;
+  const int blocks = (items + kA1Waves - 1) / kA1Waves;
   if (p.safe)
-    hipLaunchKernelGGL(a1_transpose_kernel<true>, dim3(blocks), dim3(64), 0, st, Kf, tc2,
-                       itc4, measurement_points, dst, g, p.tiles_x, p.tiles_y, p.n_slabs,
-                       p.n_ranges, p.t_stride);
+    hipLaunchKernelGGL(a1_transpose_kernel<true>, dim3(blocks), dim3(kA1Waves * 64), 0, st,
+                       Kf, tc2, itc4, measurement_points, dst, g, p.tiles_x, p.tiles_y,
+                       p.n_slabs, p.n_ranges, p.t_stride, items);
   else
-    hipLaunchKernelGGL(a1_transpose_kernel<false>, dim3(blocks), dim3(64), 0, st, Kf,
-                       tc2, itc4, measurement_points, dst, g, p.tiles_x, p.tiles_y,
-                       p.n_slabs, p.n_ranges, p.t_stride);
+    hipLaunchKernelGGL(a1_transpose_kernel<false>, dim3(blocks), dim3(kA1Waves * 64), 0,
+                       st, Kf, tc2, itc4, measurement_points, dst, g, p.tiles_x,
+                       p.tiles_y, p.n_slabs, p.n_ranges, p.t_stride, items);
   rc = check_launch("a1_transpose_kernel");
   if (rc) return rc;
   if (p.n_ranges > 1) {
