@@ -104,6 +104,13 @@ __device__ __forceinline__ float wave_sum_f32(float v) {
   return v;
 }
 
+// floor(x) converted to int in one instruction (v_cvt_flr_i32_f32).
+__device__ __forceinline__ int floor_to_int(float x) {
+  int r;
+  asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+  return r;
+}
+
 __device__ __forceinline__ float uniform_f32(float v) {
   return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
 }

@@ -38,7 +38,7 @@ namespace totri_hip {
 namespace {
 
 constexpr int kWaves = 8;            // scan points per R1 workgroup
-constexpr int kChunkFloats = 8192;   // floats per R1 LDS chunk buffer (32 KiB)
+constexpr int kChunkVoxels = 4096;   // voxels per R1 LDS chunk buffer (32 KiB of pairs)
 constexpr int kSlab = 32;            // A1 z-slab depth
 
 struct ConfGeom {
@@ -116,6 +116,31 @@ __device__ __forceinline__ void depth_range(const ConfGeom& g, float wz, float z
   *dhi = ((zhi + 0.5f) * is + g.oz) - wz + margin;
 }
 
+// Fast part of the decision: z_hat = fma(sqrt_fast(d2), s, cfast) gives the
+// candidate floor (returned) and alpha.  `near` tells whether the exact
+// sequence must confirm it (see confocal_decide).
+__device__ __forceinline__ int decide_fast(const ConfGeom& g, float d2, float cfast,
+                                           float delta, float one_m_delta, float* alpha,
+                                           bool* near) {
+  const float zh = fmaf(sqrt_fast(d2), g.s, cfast);
+  const float al = __builtin_amdgcn_fractf(zh);
+  *alpha = al;
+  *near = d2 > 0.f && (al < delta || al > one_m_delta);
+  return floor_to_int(zh);
+}
+
+// Exact reference sequence (volume_cuda.cu:42-51) for a sample flagged `near`.
+__device__ __forceinline__ int decide_exact(const ConfGeom& g, float d2, float wz,
+                                            int zshift, float* alpha) {
+  const float de = sqrt_rn(d2);
+  const float z = wz + de;
+  const float zs = (z - g.oz) * g.s;
+  const float zx = zs - 0.5f;
+  const float fx = floorf(zx);
+  *alpha = zx - fx;
+  return (int)fx - zshift;
+}
+
 // Discrete decision of one sample with depth^2 = d2 (already computed with
 // the reference rounding).  Returns floor(z_hat) - zshift and writes the
 // interpolation weight alpha = z_hat - floor(z_hat).  `cfast` is
@@ -129,9 +154,10 @@ __device__ __forceinline__ int confocal_decide(const ConfGeom& g, float d2, floa
                                                float one_m_delta, float* alpha) {
   const float depth = sqrt_fast(d2);
   const float zh = fmaf(depth, g.s, cfast);
-  float fl = floorf(zh);
-  float al = zh - fl;
-  int r = (int)fl;
+  // alpha = zh - floor(zh) (exact; v_fract only clamps results that would
+  // round to 1.0, which happens for zh in (-1, 0), a rejected range anyway).
+  float al = __builtin_amdgcn_fractf(zh);
+  int r = floor_to_int(zh);
   if (d2 > 0.f && (al < delta || al > one_m_delta)) {
     const float de = sqrt_rn(d2);
     const float z = wz + de;
@@ -153,7 +179,9 @@ __global__ __launch_bounds__(kWaves * 64) void r1_forward_kernel(
     const float* __restrict__ vol, const float* __restrict__ mp,
     float* __restrict__ dst,  // [n_groups][N][T] partial sums, or [N][T]
     ConfGeom g, int t_base, int n_groups, int chunk_cols, int chunks_per_group) {
-  extern __shared__ float lds[];
+  // LDS holds two chunk buffers of (v[z], v[z+1] - v[z]) pairs, so that one
+  // 8-byte read and one FMA give the interpolated albedo.
+  extern __shared__ float2 lds2[];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int grp = blockIdx.x % n_groups;
@@ -161,10 +189,10 @@ __global__ __launch_bounds__(kWaves * 64) void r1_forward_kernel(
   const int i = sgrp * kWaves + wave;
   const bool scan_ok = i < g.N;
   const int n_cols = g.X * g.Y;
-  const int chunk_floats = chunk_cols * g.Z;
+  const int chunk_vox = chunk_cols * g.Z;
 
   // Per-lane bin constants for each window.
-  float tc2[K], itc4[K], acc[K];
+  float tc2[K], itc4[K], a[K], acc[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int t = t_base + 64 * k + lane;
@@ -173,6 +201,7 @@ __global__ __launch_bounds__(kWaves * 64) void r1_forward_kernel(
     tc2[k] = (t < g.T) ? q : -1.f;  // -1 makes d2 negative: lane inactive
     itc4[k] = 1.f / (q * q);
     acc[k] = 0.f;
+    a[k] = 0.f;
   }
 
   const int ii = scan_ok ? i : 0;
@@ -190,32 +219,37 @@ __global__ __launch_bounds__(kWaves * 64) void r1_forward_kernel(
   const int total_chunks = (n_cols + chunk_cols - 1) / chunk_cols;
   const int chunk_end = min(chunk_first + chunks_per_group, total_chunks);
   const int nthreads = kWaves * 64;
-  const int vol_floats = n_cols * g.Z;
+  const int vol_vox = n_cols * g.Z;
 
-  // Stage the first chunk.
+  // Pair (v[z], v[z+1] - v[z]) of global voxel index ge (z = Z-1 pairs with 0).
+  auto load_pair = [&](int ge) -> float2 {
+    if (ge >= vol_vox) return make_float2(0.f, 0.f);
+    const float v0 = vol[ge];
+    const bool last = (ge % g.Z) == g.Z - 1;
+    const float v1 = last ? v0 : vol[ge + 1];
+    return make_float2(v0, v1 - v0);
+  };
+
   if (chunk_first < chunk_end) {
-    const int base = chunk_first * chunk_floats;
-    for (int e = threadIdx.x; e < chunk_floats; e += nthreads) {
-      const int ge = base + e;
-      lds[e] = ge < vol_floats ? vol[ge] : 0.f;
-    }
+    const int base = chunk_first * chunk_vox;
+    for (int e = threadIdx.x; e < chunk_vox; e += nthreads) lds2[e] = load_pair(base + e);
   }
   __syncthreads();
 
-  constexpr int kPref = kChunkFloats / (kWaves * 64);  // floats per thread
+  int cached_xi = -1;
+  constexpr int kPref = kChunkVoxels / (kWaves * 64);  // voxels per thread
   for (int c = chunk_first; c < chunk_end; ++c) {
     const int buf = (c - chunk_first) & 1;
-    float* cur = lds + buf * chunk_floats;
-    float* nxt = lds + (buf ^ 1) * chunk_floats;
+    const float2* cur = lds2 + buf * chunk_vox;
+    float2* nxt = lds2 + (buf ^ 1) * chunk_vox;
     const bool has_next = c + 1 < chunk_end;
-    float pref[kPref];
+    float2 pref[kPref];
     if (has_next) {
-      const int base = (c + 1) * chunk_floats;
+      const int base = (c + 1) * chunk_vox;
 #pragma unroll
       for (int r = 0; r < kPref; ++r) {
         const int e = threadIdx.x + r * nthreads;
-        const int ge = base + e;
-        pref[r] = (e < chunk_floats && ge < vol_floats) ? vol[ge] : 0.f;
+        pref[r] = e < chunk_vox ? load_pair(base + e) : make_float2(0.f, 0.f);
       }
     }
 
@@ -239,29 +273,67 @@ __global__ __launch_bounds__(kWaves * 64) void r1_forward_kernel(
       const unsigned packed =
           any ? (((2u << khi) - 1u) & ~((1u << klo) - 1u)) : 0u;
       unsigned long long mask = __ballot(any);
+      // Two columns per pass: their samples are independent, which gives the
+      // scheduler two dependency chains to interleave.  Windows are the union
+      // of both columns' windows; a sample outside its column's range simply
+      // fails the exact activity test.
       while (mask) {
-        const int j = __builtin_ctzll(mask);
+        const int j1 = __builtin_ctzll(mask);
         mask &= mask - 1;
-        const unsigned wmask = (unsigned)__builtin_amdgcn_readlane((int)packed, j);
-        const float cdx2 = __builtin_bit_cast(
-            float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, dx2), j));
-        const float cdy2 = __builtin_bit_cast(
-            float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, dy2), j));
-        const float* column = cur + j * g.Z;
+        const int j2 = mask ? __builtin_ctzll(mask) : j1;
+        const bool two = mask != 0;
+        if (two) mask &= mask - 1;
+        const unsigned wm1 = (unsigned)__builtin_amdgcn_readlane((int)packed, j1);
+        const unsigned wm2 = two ? (unsigned)__builtin_amdgcn_readlane((int)packed, j2) : 0u;
+        const unsigned wmask = wm1 | wm2;
+        const int xi1 = __builtin_amdgcn_readlane(xi, j1);
+        const int xi2 = __builtin_amdgcn_readlane(xi, j2);
+        if (xi1 != cached_xi) {
+          // tc^2 - dx^2 (volume_cuda.cu:40, first subtraction) depends on the
+          // column's x only; recompute it when x changes.
+          const float cdx2 = __builtin_bit_cast(
+              float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, dx2), j1));
+#pragma unroll
+          for (int k = 0; k < K; ++k) a[k] = tc2[k] - cdx2;
+          cached_xi = xi1;
+        }
+        // Second column's tc^2 - dx^2 (equal to a[] when it shares x).
+        const float cdx2b = __builtin_bit_cast(
+            float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, dx2), j2));
+        const bool same_x = xi2 == cached_xi;
+        const float cdy2a = __builtin_bit_cast(
+            float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, dy2), j1));
+        const float cdy2b = two ? __builtin_bit_cast(
+            float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, dy2), j2)) : INFINITY;
+        const float2* col1 = cur + j1 * g.Z;
+        const float2* col2 = cur + j2 * g.Z;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
           if (!((wmask >> k) & 1u)) continue;
-          const float a = tc2[k] - cdx2;
-          const float d2 = a - cdy2;  // volume_cuda.cu:40
-          float al;
-          const int zi = confocal_decide(g, d2, wz, c0, 0, delta, one_m_delta, &al);
-          if (d2 > 0.f && (unsigned)zi <= (unsigned)(g.Z - 2)) {
-            const float v0 = column[zi];
-            const float v1 = column[zi + 1];
-            const float f1 = fmaf(al, v1 - v0, v0);
-            const float q = d2 * itc4[k];
-            acc[k] = fmaf(f1, q * q, acc[k]);
+          const float d2a = a[k] - cdy2a;  // volume_cuda.cu:40
+          const float d2b = (same_x ? a[k] : tc2[k] - cdx2b) - cdy2b;
+          float ala, alb;
+          bool na, nb;
+          int za = decide_fast(g, d2a, c0, delta, one_m_delta, &ala, &na);
+          int zb = decide_fast(g, d2b, c0, delta, one_m_delta, &alb, &nb);
+          if (__ballot(na || nb)) {  // rare: confirm near-boundary samples exactly
+            if (na) za = decide_exact(g, d2a, wz, 0, &ala);
+            if (nb) zb = decide_exact(g, d2b, wz, 0, &alb);
           }
+          const bool acta = d2a > 0.f && (unsigned)za <= (unsigned)(g.Z - 2);
+          const bool actb = d2b > 0.f && (unsigned)zb <= (unsigned)(g.Z - 2);
+          // Inactive lanes read slot 0 and keep their accumulator unchanged
+          // (the discarded candidate may be NaN or inf).  Selects measure
+          // faster here than exec-masked updates.
+          const float2 va = col1[acta ? za : 0];
+          const float2 vb = col2[actb ? zb : 0];
+          const float fa = fmaf(ala, va.y, va.x);
+          const float fb = fmaf(alb, vb.y, vb.x);
+          const float qa = d2a * itc4[k];
+          const float qb = d2b * itc4[k];
+          const float ca = fmaf(fa, qa * qa, acc[k]);
+          const float cb = fmaf(fb, qb * qb, acta ? ca : acc[k]);
+          acc[k] = actb ? cb : (acta ? ca : acc[k]);
         }
       }
     }
@@ -270,7 +342,7 @@ __global__ __launch_bounds__(kWaves * 64) void r1_forward_kernel(
 #pragma unroll
       for (int r = 0; r < kPref; ++r) {
         const int e = threadIdx.x + r * nthreads;
-        if (e < chunk_floats) nxt[e] = pref[r];
+        if (e < chunk_vox) nxt[e] = pref[r];
       }
     }
     __syncthreads();
@@ -337,6 +409,8 @@ __global__ void a1_tables_kernel(const GT* __restrict__ G, ConfGeom g, int safe,
 
 constexpr int kA1Waves = 4;  // independent waves (work items) per A1 workgroup
 
+__device__ __forceinline__ float sq(float v) { return v * v; }
+
 __device__ __forceinline__ float lane_value(float v, int lane) {
   return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
 }
@@ -393,7 +467,7 @@ __global__ __launch_bounds__(kA1Waves * 64) void a1_transpose_kernel(
     const float delta = guard_delta(g, wz, c0, cp);
     const float one_m_delta = 1.f - delta;
     const float dx = wx - x, dy = wy - y;
-    const float dx2 = dx * dx, dy2 = dy * dy;
+    const float dx2 = cvalid ? dx * dx : INFINITY, dy2 = dy * dy;
     float dlo, dhi;
     depth_range(g, wz, zlo, zhi, &dlo, &dhi);
     int tlo, thi;
@@ -415,29 +489,43 @@ __global__ __launch_bounds__(kA1Waves * 64) void a1_transpose_kernel(
       const float ireg = SAFE ? itc4t[tb + lane] : 0.f;
       const int u0 = max(t_begin - tb, 0);
       const int u1 = min(t_end - tb, 63);
-      for (int u = u0; u <= u1; ++u) {
-        const float kt = lane_value(kreg, u);
-        const float a = lane_value(treg, u) - dx2;
-        const float d2 = a - dy2;
-        float al;
-        const int row = confocal_decide(g, d2, wz, cp, zshift, delta, one_m_delta, &al);
-        if (d2 > 0.f && cvalid && (unsigned)row <= (unsigned)rspan) {
-          float wt;
-          if (SAFE) {
-            const float q = d2 * lane_value(ireg, u);
-            wt = (q * q) * kt;
-          } else {
-            wt = (d2 * d2) * kt;
-          }
-          const float w1 = al * wt;
-          const float w0 = wt - w1;
-          // Each lane owns its slots and a wave's LDS accesses complete in
-          // order, so a plain read-modify-write is exact.  (LDS float atomics
-          // measure ~25x slower than this on gfx950.)
-          float* p = my + row * 64;
-          const float a0 = p[0], a1 = p[64];
-          p[0] = a0 + w0;
-          p[64] = a1 + w1;
+      // Two bins per pass: the geometry of both is computed before either
+      // accumulator update (their updates may touch the same rows, so the
+      // read-modify-writes stay in order).
+      for (int u = u0; u <= u1; u += 2) {
+        const int ub = min(u + 1, u1);
+        const float kta = lane_value(kreg, u);
+        const float ktb = (u + 1 <= u1) ? lane_value(kreg, ub) : 0.f;
+        const float d2a = (lane_value(treg, u) - dx2) - dy2;
+        const float d2b = (lane_value(treg, ub) - dx2) - dy2;
+        float ala, alb;
+        bool na, nb;
+        int ra = decide_fast(g, d2a, cp, delta, one_m_delta, &ala, &na);
+        int rb = decide_fast(g, d2b, cp, delta, one_m_delta, &alb, &nb);
+        if (__ballot(na || nb)) {  // rare: confirm near-boundary samples exactly
+          if (na) ra = decide_exact(g, d2a, wz, zshift, &ala);
+          if (nb) rb = decide_exact(g, d2b, wz, zshift, &alb);
+        }
+        // dx2 is +inf for lanes without a column, so d2 > 0 also checks that.
+        const bool acta = d2a > 0.f && (unsigned)ra <= (unsigned)rspan;
+        const bool actb = d2b > 0.f && (unsigned)rb <= (unsigned)rspan;
+        // Each lane owns its slots and a wave's LDS accesses complete in
+        // order, so plain read-modify-writes are exact.  (LDS float atomics
+        // measure ~25x slower than this on gfx950.)  Updates run under the
+        // activity mask (exec).
+        if (acta) {
+          const float wa = SAFE ? sq(d2a * lane_value(ireg, u)) * kta : (d2a * d2a) * kta;
+          float* pa = my + ra * 64;
+          const float a0 = pa[0], a1 = pa[64];
+          pa[0] = fmaf(wa, 1.f - ala, a0);
+          pa[64] = fmaf(wa, ala, a1);
+        }
+        if (actb) {
+          const float wb = SAFE ? sq(d2b * lane_value(ireg, ub)) * ktb : (d2b * d2b) * ktb;
+          float* pb = my + rb * 64;
+          const float b0 = pb[0], b1 = pb[64];
+          pb[0] = fmaf(wb, 1.f - alb, b0);
+          pb[64] = fmaf(wb, alb, b1);
         }
       }
     }
@@ -518,7 +606,7 @@ R1Plan plan_r1(int64_t X, int64_t Y, int64_t Z, int64_t N, int64_t T) {
   const int64_t bins_pass = T <= 64 ? 64 : T <= 128 ? 128 : T <= 256 ? 256 : T <= 512 ? 512 : 1024;
   p.K = (int)(bins_pass / 64);
   p.passes = (int)((T + bins_pass - 1) / bins_pass);
-  p.chunk_cols = (int)(Z > 0 ? (kChunkFloats / Z < 64 ? kChunkFloats / Z : 64) : 64);
+  p.chunk_cols = (int)(Z > 0 ? (kChunkVoxels / Z < 64 ? kChunkVoxels / Z : 64) : 64);
   if (p.chunk_cols < 1) p.chunk_cols = 1;
   const int64_t n_cols = X * Y;
   p.total_chunks = (int)((n_cols + p.chunk_cols - 1) / p.chunk_cols);
@@ -546,7 +634,7 @@ void launch_r1(const R1Plan& p, const float* vol, const float* mp, float* dst,
                const ConfGeom& g, int t_base, hipStream_t st) {
   const int scan_groups = (g.N + kWaves - 1) / kWaves;
   const int blocks = scan_groups * p.n_groups;
-  const size_t lds = 2 * (size_t)p.chunk_cols * g.Z * sizeof(float);
+  const size_t lds = 2 * (size_t)p.chunk_cols * g.Z * sizeof(float2);
   hipLaunchKernelGGL(r1_forward_kernel<K>, dim3(blocks), dim3(kWaves * 64), lds, st,
                      vol, mp, dst, g, t_base, p.n_groups, p.chunk_cols,
                      p.chunks_per_group);
@@ -594,8 +682,8 @@ extern "C" int totri_volume_confocal_forward(
     }
   }
   TOTRI_CHECK_ARG(volume != nullptr, "null volume");
-  TOTRI_CHECK_ARG(Z <= kChunkFloats, "volume depth Z=%lld exceeds %d", (long long)Z,
-                  kChunkFloats);
+  TOTRI_CHECK_ARG(Z <= kChunkVoxels, "volume depth Z=%lld exceeds %d", (long long)Z,
+                  kChunkVoxels);
   const R1Plan p = plan_r1(X, Y, Z, N, T);
   const size_t need = r1_workspace(X, Y, Z, N, T);
   const bool direct = p.n_groups == 1 && transient_dtype == TOTRI_F32;
