@@ -97,6 +97,25 @@ int totri_volume_confocal_grad_volume(float* volume, int64_t X, int64_t Y,
                                       void* stream);
 
 /*
+ * A6 -- gradient of sum(G * R1(volume; w)) with respect to the wall points w.
+ * Replaces totri_cpp.render.volume_confocal_grad_measurement_points
+ * (src/render/volume.h:26-33, src/render/volume.cpp:43-81,
+ *  kernel src/render/volume_cuda.cu:188-295).
+ *   measurement_points_grad [3][N] FP32 output, written
+ *   transient_grad          [N][T] upstream gradient G, storage transient_dtype
+ * Sums are reduced in a fixed order (the reference uses float atomics).
+ */
+size_t totri_volume_confocal_grad_measurement_points_workspace(int64_t X, int64_t Y,
+                                                               int64_t Z, int64_t N,
+                                                               int64_t T);
+int totri_volume_confocal_grad_measurement_points(
+    float* measurement_points_grad, const float* volume, int64_t X, int64_t Y,
+    int64_t Z, const float* measurement_points, int64_t N,
+    const void* transient_grad, int64_t T, int transient_dtype,
+    const float volume_origin[3], const float volume_size[3], float bin_width,
+    float bin_offset, void* workspace, size_t workspace_bytes, void* stream);
+
+/*
  * Test hook with no counterpart in the reference: evaluates the per-sample
  * decision function shared by the R1 and A1 kernels for n samples listed as
  * idx[n][4] = (scan point i, bin t < T, xi, yi) and writes

@@ -233,3 +233,52 @@ def test_degenerate_shapes():
                                 SIZE, 0.05, 0.0)
     torch.cuda.synchronize()
     assert torch.count_nonzero(vol) == 0  # no scan points: zero gradient
+
+
+def gpu_a6(vol, mp, G, w, off, origin=ORIGIN, size=SIZE):
+    from totrilib_amd.totri_cpp.render import volume_confocal_grad_measurement_points
+    v = torch.from_numpy(np.ascontiguousarray(vol)).to(dev())
+    m = torch.from_numpy(np.ascontiguousarray(mp)).to(dev())
+    g = torch.as_tensor(G).to(dev())
+    out = torch.full((3, m.shape[1]), float("nan"), device=dev())
+    volume_confocal_grad_measurement_points(out, v, m, g, origin, size, w, off)
+    torch.cuda.synchronize()
+    return out.cpu().numpy().astype(np.float64)
+
+
+def test_a6_matches_oracle():
+    rng = np.random.default_rng(21)
+    X, Y, Z, T = 20, 18, 24, 700
+    vol = rng.random((X, Y, Z)).astype(np.float32)
+    mp = np.stack([rng.uniform(-1, 1, 21), rng.uniform(-1, 1, 21),
+                   rng.uniform(-0.1, 0.1, 21)]).astype(np.float32)
+    w, off = 0.004, 0.001
+    G = rng.standard_normal((21, T)).astype(np.float32)
+    got = gpu_a6(vol, mp, G, w, off)
+    ref64 = O.a6(vol, mp, G, ORIGIN, SIZE, w, off, mode=O.PINNED64)
+    ref32 = O.a6(vol, mp, G, ORIGIN, SIZE, w, off, mode=O.FP32)
+    e64, e32 = rel_l2(got, ref64), rel_l2(got, ref32)
+    print(f"A6 rel-L2 vs FP64 pinned {e64:.2e}, vs FP32 {e32:.2e}")
+    assert e64 < TOL
+
+
+def test_autograd_volume_and_points():
+    """VolumeRenderConfocal.apply: forward R1, backward A1 (albedo) and A6
+    (wall points), as the reference's autograd Function."""
+    from totrilib_amd.render import VolumeRenderConfocal
+    rng = np.random.default_rng(22)
+    vol_np = rng.random((12, 10, 16)).astype(np.float32)
+    mp_np = np.stack([rng.uniform(-1, 1, 9), rng.uniform(-1, 1, 9),
+                      np.zeros(9)]).astype(np.float32)
+    T, w = 400, 0.006
+    v = torch.from_numpy(vol_np).to(dev()).requires_grad_(True)
+    m = torch.from_numpy(mp_np).to(dev()).requires_grad_(True)
+    tr = VolumeRenderConfocal.apply(v, m, ORIGIN, SIZE, T, w, 0.0)
+    G = torch.from_numpy(rng.standard_normal((9, T)).astype(np.float32)).to(dev())
+    (tr * G).sum().backward()
+    Gn = G.cpu().numpy()
+    ev = rel_l2(v.grad.cpu().numpy(),
+                O.a1_transpose(Gn, mp_np, 12, 10, 16, ORIGIN, SIZE, w, 0.0, mode=O.PINNED64))
+    em = rel_l2(m.grad.cpu().numpy(), O.a6(vol_np, mp_np, Gn, ORIGIN, SIZE, w, 0.0,
+                                           mode=O.PINNED64))
+    assert ev < TOL and em < TOL, (ev, em)

@@ -171,6 +171,12 @@ __device__ __forceinline__ int confocal_decide(const ConfGeom& g, float d2, floa
   return r;
 }
 
+__device__ __forceinline__ float sq(float v) { return v * v; }
+
+__device__ __forceinline__ float lane_value(float v, int lane) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
+}
+
 // ---------------------------------------------------------------------------
 // R1
 // ---------------------------------------------------------------------------
@@ -358,6 +364,142 @@ __global__ __launch_bounds__(kWaves * 64) void r1_forward_kernel(
   }
 }
 
+// ---------------------------------------------------------------------------
+// A6: gradient of <G, R1(v; w)> w.r.t. the wall points w (volume_cuda.cu:188-265)
+// ---------------------------------------------------------------------------
+// Same sample set and streaming as R1 (one scan point per wave, one bin per
+// lane and window).  Per active sample, with dx = wx - x, f1 the interpolated
+// albedo, dv = v[zi+1] - v[zi] and f2 = (depth / tc^2)^4:
+//   d/dwx = -dx (4 f1 depth^2 / tc^8 + s dv f2 / depth)   (likewise y)
+//   d/dwz = s dv f2
+// which is the reference's f1 * df2/dw + df1/dw * f2 with its
+// tmp = 4 (-depth / tc^(8/3))^3 = -4 depth^3 / tc^8 written out.  Each lane
+// sums its bins over columns, multiplies by G[i][t] once, and the wave
+// reduces to one (gx, gy, gz) per scan point, column group and bin pass.
+template <int K, typename GT>
+__global__ __launch_bounds__(kWaves * 64) void a6_kernel(
+    const float* __restrict__ vol, const float* __restrict__ mp,
+    const GT* __restrict__ G, float* __restrict__ part,  // [parts][3][N]
+    ConfGeom g, int t_base, int pass, int n_groups, int chunk_cols,
+    int chunks_per_group) {
+  extern __shared__ float2 lds2[];
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int grp = blockIdx.x % n_groups;
+  const int sgrp = blockIdx.x / n_groups;
+  const int i = sgrp * kWaves + wave;
+  const bool scan_ok = i < g.N;
+  const int n_cols = g.X * g.Y;
+  const int chunk_vox = chunk_cols * g.Z;
+
+  float tc2[K], itc4[K], a[K], gx[K], gy[K], gz[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int t = t_base + 64 * k + lane;
+    const float tc = bin_distance(t, g.w, g.off);
+    const float q = tc * tc;
+    tc2[k] = (t < g.T) ? q : -1.f;
+    itc4[k] = 1.f / (q * q);
+    a[k] = 0.f;
+    gx[k] = gy[k] = gz[k] = 0.f;
+  }
+  const int ii = scan_ok ? i : 0;
+  const float wx = uniform_f32(mp[ii]);
+  const float wy = uniform_f32(mp[g.N + ii]);
+  const float wz = uniform_f32(mp[2 * g.N + ii]);
+  float c0 = (wz - g.oz) * g.s;
+  c0 = c0 - 0.5f;
+  const float delta = guard_delta(g, wz, c0, 0.f);
+  const float one_m_delta = 1.f - delta;
+  float dlo, dhi;
+  depth_range(g, wz, 0.f, (float)(g.Z - 1), &dlo, &dhi);
+
+  const int chunk_first = grp * chunks_per_group;
+  const int total_chunks = (n_cols + chunk_cols - 1) / chunk_cols;
+  const int chunk_end = min(chunk_first + chunks_per_group, total_chunks);
+  const int nthreads = kWaves * 64;
+  const int vol_vox = n_cols * g.Z;
+  auto load_pair = [&](int ge) -> float2 {
+    if (ge >= vol_vox) return make_float2(0.f, 0.f);
+    const float v0 = vol[ge];
+    const bool last = (ge % g.Z) == g.Z - 1;
+    const float v1 = last ? v0 : vol[ge + 1];
+    return make_float2(v0, v1 - v0);
+  };
+  for (int c = chunk_first; c < chunk_end; ++c) {
+    // Single-buffered: A6 is not on the benchmark path.
+    __syncthreads();
+    const int base = c * chunk_vox;
+    for (int e = threadIdx.x; e < chunk_vox; e += nthreads) lds2[e] = load_pair(base + e);
+    __syncthreads();
+    if (!scan_ok) continue;
+    const int col = c * chunk_cols + lane;
+    const bool cvalid = lane < chunk_cols && col < n_cols;
+    const int colc = cvalid ? col : 0;
+    const int xi = colc / g.Y;
+    const int yi = colc - xi * g.Y;
+    const float x = cell_centre(xi, g.ox, g.szx, g.X);
+    const float y = cell_centre(yi, g.oy, g.szy, g.Y);
+    const float dx = wx - x, dy = wy - y;
+    const float dx2 = dx * dx, dy2 = dy * dy;
+    int tlo, thi;
+    bin_range(g, dx2 + dy2, dlo, dhi, &tlo, &thi);
+    const int klo = (max(tlo, t_base) - t_base) >> 6;
+    const int khi = (min(thi, t_base + 64 * K - 1) - t_base) >> 6;
+    const bool any = cvalid && thi >= t_base && tlo <= t_base + 64 * K - 1 && tlo <= thi;
+    const unsigned packed = any ? (((2u << khi) - 1u) & ~((1u << klo) - 1u)) : 0u;
+    unsigned long long mask = __ballot(any);
+    while (mask) {
+      const int j = __builtin_ctzll(mask);
+      mask &= mask - 1;
+      const unsigned wmask = (unsigned)__builtin_amdgcn_readlane((int)packed, j);
+      const float cdx = lane_value(dx, j), cdy = lane_value(dy, j);
+      const float cdx2 = lane_value(dx2, j), cdy2 = lane_value(dy2, j);
+#pragma unroll
+      for (int k = 0; k < K; ++k) a[k] = tc2[k] - cdx2;
+      const float2* column = lds2 + j * g.Z;
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        if (!((wmask >> k) & 1u)) continue;
+        const float d2 = a[k] - cdy2;  // volume_cuda.cu:222
+        float al;
+        const int zi = confocal_decide(g, d2, wz, c0, 0, delta, one_m_delta, &al);
+        if (d2 > 0.f && (unsigned)zi <= (unsigned)(g.Z - 2)) {
+          const float2 v = column[zi];
+          const float f1 = fmaf(al, v.y, v.x);
+          const float i8 = itc4[k] * itc4[k];
+          const float f2 = (d2 * d2) * i8;
+          const float depth = sqrt_fast(d2);
+          const float sdv = g.s * v.y;
+          const float cxy = 4.f * f1 * d2 * i8 + sdv * f2 / depth;
+          gx[k] = fmaf(-cdx, cxy, gx[k]);
+          gy[k] = fmaf(-cdy, cxy, gy[k]);
+          gz[k] = fmaf(sdv, f2, gz[k]);
+        }
+      }
+    }
+  }
+  if (!scan_ok) return;
+  float sx = 0.f, sy = 0.f, sz = 0.f;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int t = t_base + 64 * k + lane;
+    const float gt = t < g.T ? load_elem(G + (size_t)i * g.T + t) : 0.f;
+    sx = fmaf(gx[k], gt, sx);
+    sy = fmaf(gy[k], gt, sy);
+    sz = fmaf(gz[k], gt, sz);
+  }
+  sx = wave_sum_f32(sx);
+  sy = wave_sum_f32(sy);
+  sz = wave_sum_f32(sz);
+  if (lane == 0) {
+    float* o = part + ((size_t)pass * n_groups + grp) * 3 * g.N;
+    o[i] = sx;
+    o[g.N + i] = sy;
+    o[2 * g.N + i] = sz;
+  }
+}
+
 // Sum of column-group partials in a fixed order, converted to the storage type.
 template <typename OutT>
 __global__ void sum_partials_kernel(const float* __restrict__ part, int n_parts,
@@ -409,11 +551,6 @@ __global__ void a1_tables_kernel(const GT* __restrict__ G, ConfGeom g, int safe,
 
 constexpr int kA1Waves = 4;  // independent waves (work items) per A1 workgroup
 
-__device__ __forceinline__ float sq(float v) { return v * v; }
-
-__device__ __forceinline__ float lane_value(float v, int lane) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), lane));
-}
 
 template <bool SAFE>
 __global__ __launch_bounds__(kA1Waves * 64) void a1_transpose_kernel(
@@ -900,4 +1037,96 @@ extern "C" int totri_debug_confocal_samples(
                      0, (hipStream_t)stream, measurement_points, (const int*)idx, (int)n, g,
                      (int*)zi_out, alpha_out, d2_out);
   return check_launch("confocal_samples_kernel");
+}
+
+namespace totri_hip {
+namespace {
+
+template <int K, typename GT>
+void launch_a6(const R1Plan& p, const float* vol, const float* mp, const GT* G, float* part,
+               const ConfGeom& g, int t_base, int pass, hipStream_t st) {
+  const int scan_groups = (g.N + kWaves - 1) / kWaves;
+  const size_t lds = (size_t)p.chunk_cols * g.Z * sizeof(float2);
+  hipLaunchKernelGGL((a6_kernel<K, GT>), dim3(scan_groups * p.n_groups), dim3(kWaves * 64),
+                     lds, st, vol, mp, G, part, g, t_base, pass, p.n_groups, p.chunk_cols,
+                     p.chunks_per_group);
+}
+
+struct A6Plan {
+  R1Plan r;
+  int K, passes;
+  size_t ws;
+};
+
+A6Plan plan_a6(int64_t X, int64_t Y, int64_t Z, int64_t N, int64_t T) {
+  A6Plan p;
+  p.r = plan_r1(X, Y, Z, N, T);
+  p.K = p.r.K < 8 ? p.r.K : 8;
+  p.passes = (int)((T + 64 * p.K - 1) / (64 * p.K));
+  p.ws = (size_t)p.passes * p.r.n_groups * 3 * N * sizeof(float);
+  return p;
+}
+
+template <typename GT>
+int run_a6(const A6Plan& p, const float* vol, const float* mp, const void* Gv, float* part,
+           const ConfGeom& g, hipStream_t st) {
+  const GT* G = (const GT*)Gv;
+  for (int pass = 0; pass < p.passes; ++pass) {
+    const int t_base = pass * 64 * p.K;
+    switch (p.K) {
+      case 1: launch_a6<1, GT>(p.r, vol, mp, G, part, g, t_base, pass, st); break;
+      case 2: launch_a6<2, GT>(p.r, vol, mp, G, part, g, t_base, pass, st); break;
+      case 4: launch_a6<4, GT>(p.r, vol, mp, G, part, g, t_base, pass, st); break;
+      default: launch_a6<8, GT>(p.r, vol, mp, G, part, g, t_base, pass, st); break;
+    }
+    int rc = check_launch("a6_kernel");
+    if (rc) return rc;
+  }
+  return TOTRI_OK;
+}
+
+}  // namespace
+}  // namespace totri_hip
+
+extern "C" size_t totri_volume_confocal_grad_measurement_points_workspace(
+    int64_t X, int64_t Y, int64_t Z, int64_t N, int64_t T) {
+  if (X <= 0 || Y <= 0 || Z < 2 || N <= 0 || T <= 0) return 0;
+  return plan_a6(X, Y, Z, N, T).ws;
+}
+
+extern "C" int totri_volume_confocal_grad_measurement_points(
+    float* measurement_points_grad, const float* volume, int64_t X, int64_t Y, int64_t Z,
+    const float* measurement_points, int64_t N, const void* transient_grad, int64_t T,
+    int transient_dtype, const float volume_origin[3], const float volume_size[3],
+    float bin_width, float bin_offset, void* workspace, size_t workspace_bytes,
+    void* stream) {
+  int rc = check_common(X, Y, Z, N, T, volume_origin, volume_size, bin_width, bin_offset,
+                        transient_dtype);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  if (N == 0) return TOTRI_OK;
+  TOTRI_CHECK_ARG(measurement_points_grad != nullptr, "null gradient output");
+  // volume_cuda.cu:285 zero-fills the gradient before accumulating.
+  if (T == 0 || X == 0 || Y == 0 || Z < 2) return zero_fill(measurement_points_grad, (size_t)3 * N, st);
+  TOTRI_CHECK_ARG(volume && measurement_points && transient_grad, "null input");
+  TOTRI_CHECK_ARG(Z <= kChunkVoxels, "volume depth Z=%lld exceeds %d", (long long)Z,
+                  kChunkVoxels);
+  const A6Plan p = plan_a6(X, Y, Z, N, T);
+  if (workspace_bytes < p.ws)
+    return fail(TOTRI_EWORKSPACE, "workspace too small: need %zu bytes, got %zu", p.ws,
+                workspace_bytes);
+  TOTRI_CHECK_ARG(workspace != nullptr, "null workspace");
+  const ConfGeom g = make_geom(X, Y, Z, N, T, volume_origin, volume_size, bin_width,
+                               bin_offset);
+  float* part = (float*)workspace;
+  switch (transient_dtype) {
+    case TOTRI_F32: rc = run_a6<float>(p, volume, measurement_points, transient_grad, part, g, st); break;
+    case TOTRI_BF16: rc = run_a6<__hip_bfloat16>(p, volume, measurement_points, transient_grad, part, g, st); break;
+    default: rc = run_a6<__half>(p, volume, measurement_points, transient_grad, part, g, st); break;
+  }
+  if (rc) return rc;
+  const size_t n = (size_t)3 * N;
+  hipLaunchKernelGGL(sum_partials_kernel<float>, dim3(launch_grid(n)), dim3(256), 0, st, part,
+                     p.passes * p.r.n_groups, n, measurement_points_grad);
+  return check_launch("sum_partials_kernel");
 }

@@ -12,6 +12,7 @@ import torch
 
 from totrilib_amd.totri_cpp.render import (
     volume_confocal_forward,
+    volume_confocal_grad_measurement_points,
     volume_confocal_grad_volume,
 )
 
@@ -86,5 +87,8 @@ class VolumeRenderConfocal(torch.autograd.Function):
                                         ctx.volume_bbox_origin, ctx.volume_bbox_size,
                                         ctx.bin_width, ctx.bin_offset)
         if ctx.needs_input_grad[1]:
-            raise NotImplementedError('measurement point gradient not built yet')
+            measurement_points_grad = torch.empty_like(measurement_points)
+            volume_confocal_grad_measurement_points(
+                measurement_points_grad, volume, measurement_points, transient_grad,
+                ctx.volume_bbox_origin, ctx.volume_bbox_size, ctx.bin_width, ctx.bin_offset)
         return (volume_grad, measurement_points_grad) + (None,) * 6

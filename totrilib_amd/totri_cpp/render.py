@@ -89,3 +89,55 @@ def volume_confocal_grad_volume(volume: torch.Tensor, measurement_points: torch.
             stream_ptr(dev))
         _lib.check(rc, "volume_confocal_grad_volume")
         out.finish()
+
+
+def volume_confocal_grad_measurement_points(measurement_points_grad: torch.Tensor,
+                                            volume: torch.Tensor,
+                                            measurement_points: torch.Tensor,
+                                            transient_grad: torch.Tensor,
+                                            volume_origin: Vec3, volume_size: Vec3,
+                                            bin_width: float, bin_offset: float) -> None:
+    """A6: measurement_points_grad[3, N] = d<G, R1(volume; w)>/dw.
+
+    Reference: src/render/volume.cpp:43-81, src/render/volume_cuda.cu:188-295.
+    """
+    dev = require_gpu(measurement_points_grad, volume, measurement_points, transient_grad,
+                      names=("measurement_points_grad", "volume", "measurement_points",
+                             "transient_grad"))
+    # volume.cpp:51-74
+    if measurement_points_grad.dim() != 2:
+        raise ValueError("Gradient must be 2D.")
+    if volume.dim() != 3:
+        raise ValueError("volume must be 3D.")
+    if measurement_points.dim() != 2:
+        raise ValueError("Measurement points must be 2D.")
+    if transient_grad.dim() != 2:
+        raise ValueError("Transient must be 2D.")
+    if measurement_points.shape[0] != 3:
+        raise ValueError("Measurement points must have shape 3xN.")
+    if measurement_points_grad.shape[0] != 3:
+        raise ValueError("Gradient must have shape 3xN.")
+    if measurement_points.shape[1] != transient_grad.shape[0]:
+        raise ValueError("Measurement points and transient dimensions do not match.")
+    if measurement_points_grad.shape[1] != measurement_points.shape[1]:
+        raise ValueError("Measurement points and gradient dimensions do not match.")
+    require_f32(volume, "volume")
+    require_f32(measurement_points, "measurement_points")
+    require_f32(measurement_points_grad, "measurement_points_grad")
+    code = storage_code(transient_grad, "transient_grad")
+    lib = _lib.load()
+    X, Y, Z = (int(s) for s in volume.shape)
+    N, T = (int(s) for s in transient_grad.shape)
+    vol = volume.contiguous()
+    mp = measurement_points.contiguous()
+    g = transient_grad.contiguous()
+    out = OutputSlot(measurement_points_grad)
+    with torch.cuda.device(dev):
+        ws, wsp = workspace(
+            lib.totri_volume_confocal_grad_measurement_points_workspace(X, Y, Z, N, T), dev)
+        rc = lib.totri_volume_confocal_grad_measurement_points(
+            out.ptr(), vol.data_ptr(), X, Y, Z, mp.data_ptr(), N, g.data_ptr(), T, code,
+            _lib.f3(volume_origin), _lib.f3(volume_size), float(bin_width),
+            float(bin_offset), wsp, ws.numel() if ws is not None else 0, stream_ptr(dev))
+        _lib.check(rc, "volume_confocal_grad_measurement_points")
+        out.finish()
