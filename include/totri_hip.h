@@ -116,6 +116,128 @@ int totri_volume_confocal_grad_measurement_points(
     float bin_offset, void* workspace, size_t workspace_bytes, void* stream);
 
 /*
+ * A2 -- confocal Velten backprojection.
+ * Replaces totri_cpp.reconstruct.backprojection_confocal_velten_forward
+ * (src/reconstruct/backprojection.h:21-28, backprojection.cpp:22-34,
+ *  kernel src/reconstruct/backprojection_cuda.cu:152-226).
+ *   transient    [B][T][S]      FP32
+ *   scan_points  [B][S][3]      wall points (also the laser points)
+ *   scan_origin  [B][So][3]     So = 0 ignores the origin-to-wall distance
+ *   volume       [B][D][H][W]   FP32 output, written; inclusive grid from
+ *                               volume_start to volume_end
+ *   bin t collects the round trip 2|o - p| + 2|p - q| in [t w + off -/+ w/2)
+ */
+size_t totri_backprojection_confocal_velten_workspace(int64_t B, int64_t T, int64_t S);
+int totri_backprojection_confocal_velten_forward(
+    const float* transient, int64_t B, int64_t T, int64_t S, const float* scan_points,
+    const float* scan_origin, int64_t So, float* volume, int64_t D, int64_t H, int64_t W,
+    const float volume_start[3], const float volume_end[3], float bin_width,
+    float bin_offset, void* workspace, size_t workspace_bytes, void* stream);
+
+/*
+ * A3 -- exhaustive Velten backprojection.
+ * Replaces totri_cpp.reconstruct.backprojection_exhaustive_velten_forward
+ * (src/reconstruct/backprojection.h:30-39, backprojection.cpp:36-47,
+ *  kernel src/reconstruct/backprojection_cuda.cu:228-322).
+ *   transient [B][T][S][L]; scan_points [B][S][3]; laser_points [B][L][3];
+ *   scan_origin [B][So][3], laser_origin [B][Lo][3] (So in {0,S}, Lo in {0,L});
+ *   volume [B][D][H][W] written.
+ */
+size_t totri_backprojection_exhaustive_velten_workspace(int64_t B, int64_t T, int64_t S,
+                                                        int64_t L);
+int totri_backprojection_exhaustive_velten_forward(
+    const float* transient, int64_t B, int64_t T, int64_t S, int64_t L,
+    const float* scan_points, const float* laser_points, const float* scan_origin,
+    int64_t So, const float* laser_origin, int64_t Lo, float* volume, int64_t D, int64_t H,
+    int64_t W, const float volume_start[3], const float volume_end[3], float bin_width,
+    float bin_offset, void* workspace, size_t workspace_bytes, void* stream);
+
+/*
+ * A4 -- Arellano ellipsoid backprojection.
+ * Replaces totri_cpp.reconstruct.backprojection_exhaustive_arellano_forward
+ * (src/reconstruct/backprojection.h:10-19, backprojection.cpp:7-20,
+ *  kernel src/reconstruct/backprojection_cuda.cu:10-150).
+ *   transient [B][T][S][L]; scan_points [B][3][S]; laser_points [B][3][L];
+ *   scan_origin [B][3][So], laser_origin [B][3][Lo]; volume [B][D][H][W]
+ *   written (the reference accumulates into a zeroed volume).  As in the
+ *   reference, the interpolation weight between neighbouring voxels is 0.
+ */
+size_t totri_backprojection_exhaustive_arellano_workspace(int64_t B, int64_t T, int64_t S,
+                                                          int64_t L);
+int totri_backprojection_exhaustive_arellano_forward(
+    const float* transient, int64_t B, int64_t T, int64_t S, int64_t L,
+    const float* scan_points, const float* laser_points, const float* scan_origin,
+    int64_t So, const float* laser_origin, int64_t Lo, float* volume, int64_t D, int64_t H,
+    int64_t W, const float volume_start[3], const float volume_end[3], float bin_width,
+    float bin_offset, void* workspace, size_t workspace_bytes, void* stream);
+
+/*
+ * R2 -- exhaustive (non-confocal) volume render.
+ * Replaces totri_cpp.render.volume_render_exhaustive_forward
+ * (src/render/volume.h:35-44, src/render/volume.cpp:83-96,
+ *  kernel src/render/volume_cuda.cu:297-412).
+ *   volume        [B][D][H][W] FP32 occupancy
+ *   scan_points   [B][3][S], laser_points [B][3][L]
+ *   transient     [B][T][S][L] FP32 output, written (bin t holds path
+ *                 lengths |p_s - q| + |p_l - q| in [t w + off, (t+1) w + off))
+ * The reference also takes scan/laser origins and ignores them
+ * (volume_cuda.cu:312-313); they are not part of this entry point.
+ */
+size_t totri_volume_render_exhaustive_forward_workspace(int64_t B, int64_t D, int64_t H,
+                                                        int64_t W);
+int totri_volume_render_exhaustive_forward(
+    const float* volume, int64_t B, int64_t D, int64_t H, int64_t W, const float* scan_points,
+    int64_t S, const float* laser_points, int64_t L, float* transient, int64_t T,
+    const float volume_start[3], const float volume_end[3], float bin_width,
+    float bin_offset, void* workspace, size_t workspace_bytes, void* stream);
+
+/*
+ * A5 -- gradient of R2 with respect to the volume (R2's exact transpose).
+ * Replaces totri_cpp.render.volume_render_exhaustive_grad_volume
+ * (src/render/volume.h:46-55, src/render/volume.cpp:98-111), whose kernel in
+ * the reference is an empty stub (src/render/volume_cuda.cu:414-431).
+ *   transient_grad [B][T][S][L]; volume_grad [B][D][H][W] written.
+ */
+size_t totri_volume_render_exhaustive_grad_volume_workspace(int64_t B, int64_t T, int64_t S,
+                                                            int64_t L);
+int totri_volume_render_exhaustive_grad_volume(
+    const float* transient_grad, int64_t B, int64_t T, int64_t S, int64_t L,
+    const float* scan_points, const float* laser_points, float* volume_grad, int64_t D,
+    int64_t H, int64_t W, const float volume_start[3], const float volume_end[3],
+    float bin_width, float bin_offset, void* workspace, size_t workspace_bytes,
+    void* stream);
+
+/*
+ * S1 -- Gaussian RBF to volume splat.
+ * Replaces totri_cpp.util.rbf2volume_gaussian_forward
+ * (src/util/rbf2volume.h:10-14, src/util/rbf2volume.cpp:7-15,
+ *  kernel src/util/rbf2volume_cuda.cu:49-120).
+ *   rbf     [B][N][4+A] (x, y, z, sigma, A attributes), 0 <= A <= 8
+ *   volume  [B][1+A][D][H][W] output, written: channel 0 the summed
+ *           Gaussians, channels 1..A the weighted attribute means
+ *   truncation  > 0 skips RBFs farther than truncation * sigma from a voxel;
+ *               <= 0 sums all RBFs (the reference's behaviour)
+ */
+int totri_rbf2volume_gaussian_forward(const float* rbf, int64_t B, int64_t N, int64_t A,
+                                      float* volume, int64_t D, int64_t H, int64_t W,
+                                      const float volume_start[3], const float volume_end[3],
+                                      float truncation, void* stream);
+
+/*
+ * S2 -- gradient of the splat with respect to the RBF parameters.
+ * Replaces totri_cpp.util.rbf2volume_gaussian_grad
+ * (src/util/rbf2volume.h:16-22, src/util/rbf2volume.cpp:17-27,
+ *  kernel src/util/rbf2volume_cuda.cu:123-232).
+ *   volume, volume_grad [B][1+A][D][H][W] (forward output and its gradient)
+ *   rbf_grad            [B][N][4+A] output, written
+ */
+int totri_rbf2volume_gaussian_grad(const float* rbf, int64_t B, int64_t N, int64_t A,
+                                   const float* volume, const float* volume_grad,
+                                   float* rbf_grad, int64_t D, int64_t H, int64_t W,
+                                   const float volume_start[3], const float volume_end[3],
+                                   float truncation, void* stream);
+
+/*
  * Test hook with no counterpart in the reference: evaluates the per-sample
  * decision function shared by the R1 and A1 kernels for n samples listed as
  * idx[n][4] = (scan point i, bin t < T, xi, yi) and writes

@@ -135,3 +135,106 @@ def a6(volume, mp, G, origin, size, w, off, mode=FP32, i0=0, i1=None):
     lib().oracle_a6(_p(vol), X, Y, Z, _p(m), N, _p(g), T, _f3(origin), _f3(size),
                     ctypes.c_float(w), ctypes.c_float(off), mode, _p(out), i0, i1)
     return out
+
+
+def velten_confocal(tr, scan, sorigin, res, vs, ve, w, off, mode=FP32):
+    """A2 (backprojection_cuda.cu:152-194): tr [B,T,S], scan [B,S,3],
+    sorigin [B,So,3] (So in {0,S}) -> volume [B,D,H,W]."""
+    t = _f64(tr)
+    s = _f32(scan)
+    so = _f32(sorigin) if sorigin is not None and np.asarray(sorigin).size else np.zeros((t.shape[0], 0, 3), np.float32)
+    B, T, S = t.shape
+    D, H, W = res
+    out = np.zeros((B, D, H, W))
+    lib().oracle_velten_confocal(_p(t), B, T, S, _p(s), _p(so), so.shape[1], D, H, W,
+                                 _f3(vs), _f3(ve), ctypes.c_float(w), ctypes.c_float(off),
+                                 mode, _p(out))
+    return out
+
+
+def velten_exhaustive(tr, scan, laser, sorigin, lorigin, res, vs, ve, w, off, mode=FP32):
+    """A3 (backprojection_cuda.cu:228-287): tr [B,T,S,L], scan [B,S,3],
+    laser [B,L,3], origins [B,So,3] / [B,Lo,3] -> volume [B,D,H,W]."""
+    t = _f64(tr)
+    s = _f32(scan)
+    l = _f32(laser)
+    B, T, S, L = t.shape
+    so = _f32(sorigin) if sorigin is not None else np.zeros((B, 0, 3), np.float32)
+    lo = _f32(lorigin) if lorigin is not None else np.zeros((B, 0, 3), np.float32)
+    D, H, W = res
+    out = np.zeros((B, D, H, W))
+    lib().oracle_velten_exhaustive(_p(t), B, T, S, L, _p(s), _p(l), _p(so), so.shape[1],
+                                   _p(lo), lo.shape[1], D, H, W, _f3(vs), _f3(ve),
+                                   ctypes.c_float(w), ctypes.c_float(off), mode, _p(out))
+    return out
+
+
+def arellano(tr, scan, laser, sorigin, lorigin, res, vs, ve, w, off, mode=FP32):
+    """A4 (backprojection_cuda.cu:10-115): tr [B,T,S,L], scan [B,3,S],
+    laser [B,3,L], origins [B,3,So] / [B,3,Lo] -> volume [B,D,H,W]."""
+    t = _f64(tr)
+    s = _f32(scan)
+    l = _f32(laser)
+    B, T, S, L = t.shape
+    so = _f32(sorigin) if sorigin is not None else np.zeros((B, 3, 0), np.float32)
+    lo = _f32(lorigin) if lorigin is not None else np.zeros((B, 3, 0), np.float32)
+    D, H, W = res
+    out = np.zeros((B, D, H, W))
+    lib().oracle_arellano(_p(t), B, T, S, L, _p(s), _p(l), _p(so), so.shape[2], _p(lo),
+                          lo.shape[2], D, H, W, _f3(vs), _f3(ve), ctypes.c_float(w),
+                          ctypes.c_float(off), mode, _p(out))
+    return out
+
+
+def r2_forward(vol, scan, laser, T, vs, ve, w, off, mode=FP32):
+    """R2 (volume_cuda.cu:297-377): vol [B,D,H,W], scan [B,3,S], laser [B,3,L]
+    -> transient [B,T,S,L].  mode 0 FP32; 1 FP32 weights, FP64 sums;
+    2 FP32 decisions, FP64 weights and sums."""
+    v = _f32(vol)
+    s = _f32(scan)
+    l = _f32(laser)
+    B, D, H, W = v.shape
+    S, L = s.shape[2], l.shape[2]
+    out = np.zeros((B, T, S, L))
+    lib().oracle_r2_forward(_p(v), B, D, H, W, _p(s), S, _p(l), L, T, _f3(vs), _f3(ve),
+                            ctypes.c_float(w), ctypes.c_float(off), mode, _p(out))
+    return out
+
+
+def r2_adjoint(G, res, scan, laser, vs, ve, w, off, mode=FP32):
+    """Exact adjoint of R2 applied to G [B,T,S,L] -> [B,D,H,W]."""
+    g = _f64(G)
+    s = _f32(scan)
+    l = _f32(laser)
+    B, T, S, L = g.shape
+    D, H, W = res
+    out = np.zeros((B, D, H, W))
+    lib().oracle_r2_adjoint(_p(g), B, D, H, W, _p(s), S, _p(l), L, T, _f3(vs), _f3(ve),
+                            ctypes.c_float(w), ctypes.c_float(off), mode, _p(out))
+    return out
+
+
+def splat_forward(rbf, res, vs, ve, trunc=0.0, mode=FP32):
+    """S1 (rbf2volume_cuda.cu:49-93): rbf [B,N,4+A] -> [B,1+A,D,H,W]."""
+    r = _f32(rbf)
+    B, N, C = r.shape
+    A = C - 4
+    D, H, W = res
+    out = np.zeros((B, 1 + A, D, H, W))
+    lib().oracle_splat_forward(_p(r), B, N, A, D, H, W, _f3(vs), _f3(ve),
+                               ctypes.c_float(trunc), mode, _p(out))
+    return out
+
+
+def splat_grad(rbf, vol, vgrad, vs, ve, trunc=0.0, mode=FP32):
+    """S2 (rbf2volume_cuda.cu:123-201): gradient [B,N,4+A]."""
+    r = _f32(rbf)
+    v = _f64(vol)
+    g = _f64(vgrad)
+    B, N, C = r.shape
+    A = C - 4
+    D, H, W = v.shape[2:]
+    out = np.zeros((B, N, C))
+    lib().oracle_splat_grad(_p(r), B, N, A, _p(v), _p(g), D, H, W, _f3(vs), _f3(ve),
+                            ctypes.c_float(trunc), mode, _p(out))
+    return out

@@ -48,6 +48,38 @@ SIGNATURES: dict[str, tuple] = {
         _c_int,
         [_c_p, _c_p, _c_i64, _c_i64, _c_i64, _c_p, _c_i64, _c_p, _c_i64, _c_int,
          _F3, _F3, _c_f, _c_f, _c_p, _c_sz, _c_p]),
+    "totri_backprojection_confocal_velten_workspace": (_c_sz, [_c_i64] * 3),
+    "totri_backprojection_confocal_velten_forward": (
+        _c_int,
+        [_c_p, _c_i64, _c_i64, _c_i64, _c_p, _c_p, _c_i64, _c_p, _c_i64, _c_i64, _c_i64,
+         _F3, _F3, _c_f, _c_f, _c_p, _c_sz, _c_p]),
+    "totri_backprojection_exhaustive_velten_workspace": (_c_sz, [_c_i64] * 4),
+    "totri_backprojection_exhaustive_velten_forward": (
+        _c_int,
+        [_c_p, _c_i64, _c_i64, _c_i64, _c_i64, _c_p, _c_p, _c_p, _c_i64, _c_p, _c_i64,
+         _c_p, _c_i64, _c_i64, _c_i64, _F3, _F3, _c_f, _c_f, _c_p, _c_sz, _c_p]),
+    "totri_backprojection_exhaustive_arellano_workspace": (_c_sz, [_c_i64] * 4),
+    "totri_backprojection_exhaustive_arellano_forward": (
+        _c_int,
+        [_c_p, _c_i64, _c_i64, _c_i64, _c_i64, _c_p, _c_p, _c_p, _c_i64, _c_p, _c_i64,
+         _c_p, _c_i64, _c_i64, _c_i64, _F3, _F3, _c_f, _c_f, _c_p, _c_sz, _c_p]),
+    "totri_volume_render_exhaustive_forward_workspace": (_c_sz, [_c_i64] * 4),
+    "totri_volume_render_exhaustive_forward": (
+        _c_int,
+        [_c_p, _c_i64, _c_i64, _c_i64, _c_i64, _c_p, _c_i64, _c_p, _c_i64, _c_p, _c_i64,
+         _F3, _F3, _c_f, _c_f, _c_p, _c_sz, _c_p]),
+    "totri_volume_render_exhaustive_grad_volume_workspace": (_c_sz, [_c_i64] * 4),
+    "totri_volume_render_exhaustive_grad_volume": (
+        _c_int,
+        [_c_p, _c_i64, _c_i64, _c_i64, _c_i64, _c_p, _c_p, _c_p, _c_i64, _c_i64, _c_i64,
+         _F3, _F3, _c_f, _c_f, _c_p, _c_sz, _c_p]),
+    "totri_rbf2volume_gaussian_forward": (
+        _c_int,
+        [_c_p, _c_i64, _c_i64, _c_i64, _c_p, _c_i64, _c_i64, _c_i64, _F3, _F3, _c_f, _c_p]),
+    "totri_rbf2volume_gaussian_grad": (
+        _c_int,
+        [_c_p, _c_i64, _c_i64, _c_i64, _c_p, _c_p, _c_p, _c_i64, _c_i64, _c_i64, _F3, _F3,
+         _c_f, _c_p]),
     "totri_debug_confocal_samples": (
         _c_int,
         [_c_p, _c_i64, _c_p, _c_i64, _c_i64, _c_i64, _c_i64, _c_i64, _F3, _F3,

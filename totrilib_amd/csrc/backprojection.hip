@@ -1,0 +1,468 @@
+// Backprojections for gfx950: Velten confocal (A2), Velten exhaustive (A3)
+// and Arellano ellipsoid splatting (A4).
+//
+// Reference: src/reconstruct/backprojection_cuda.cu:10-322,
+//            src/reconstruct/backprojection.cpp:7-49.
+//
+// Every discrete decision (the nearest bin of a path length, the voxel an
+// ellipsoid hit falls into) follows the reference's FP32 formula with IEEE
+// correctly rounded square roots and divisions, no contraction, and
+// norm3df(d) restated as sqrt((dx*dx + dy*dy) + dz*dz).
+//
+// A2/A3 (gathers): one lane per voxel, flat over [D][H][W] so that volume
+// writes are coalesced.  The transient is first transposed so that each scan
+// point's (and laser point's) bins are contiguous; a workgroup then stages the
+// rows of a group of scan points in LDS and every lane gathers its bin from
+// there.  Sums run over scan (and laser) points in the reference's order.
+//
+// A4 (scatter): one lane per volume column (x, y); the column's D voxels are
+// accumulated in LDS slots owned by that lane (row-major [D][64]), so the
+// reference's global atomics become plain read-modify-writes.  Scan/laser
+// pairs and bins are uniform across the wave; a bin's transient value is
+// read with one scalar load.
+#include <math.h>
+
+#include "common.h"
+
+namespace totri_hip {
+namespace {
+
+constexpr int kStageFloats = 8192;  // LDS floats of staged transient rows (32 KiB)
+
+int grid1d(size_t n, int block = 256) {
+  size_t b = (n + block - 1) / block;
+  if (b > 65535 * 4) b = 65535 * 4;
+  return (int)(b < 1 ? 1 : b);
+}
+
+// out[b][c][r] = in[b][r][c] for a [B][R][C] array (tiled through LDS).
+__global__ void transpose_kernel(const float* __restrict__ in, float* __restrict__ out,
+                                 int R, int C) {
+  __shared__ float tile[32][33];
+  const int b = blockIdx.z;
+  const float* src = in + (size_t)b * R * C;
+  float* dst = out + (size_t)b * R * C;
+  const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+  for (int k = threadIdx.y; k < 32; k += 8) {
+    const int r = r0 + k, c = c0 + threadIdx.x;
+    if (r < R && c < C) tile[k][threadIdx.x] = src[(size_t)r * C + c];
+  }
+  __syncthreads();
+  for (int k = threadIdx.y; k < 32; k += 8) {
+    const int c = c0 + k, r = r0 + threadIdx.x;
+    if (r < R && c < C) dst[(size_t)c * R + r] = tile[threadIdx.x][k];
+  }
+}
+
+int launch_transpose(const float* in, float* out, int B, int R, int C, hipStream_t st) {
+  return transpose_batched(in, out, B, R, C, st);
+}
+
+struct BoxGeom {
+  int B, D, H, W, T;
+  float sx, sy, sz, ex, ey, ez;
+  float w, off;
+};
+
+// Nearest bin of a path length (backprojection_cuda.cu:187,279):
+// floorf((dist - offset) / width + 0.5f).
+__device__ __forceinline__ int nearest_bin(float dist, float w, float off) {
+  float tf = (dist - off) / w;
+  tf = tf + 0.5f;
+  return (int)floorf(tf);
+}
+
+// ---------------------------------------------------------------------------
+// A2: vol[b][z][y][x] = sum_s tr[b][t(s)][s],
+// t = nearest_bin(2 |p_s - o_s| + 2 |p_s - q|)       (backprojection_cuda.cu:152-194)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void velten_confocal_kernel(
+    const float* __restrict__ trT,    // [B][S][T]
+    const float* __restrict__ scan,   // [B][S][3]
+    const float* __restrict__ orig,   // [B][So][3], So in {0, S}
+    float* __restrict__ vol,          // [B][D][H][W]
+    BoxGeom g, int S, int So, int rows_per_stage) {
+  extern __shared__ float rows[];
+  const int b = blockIdx.y;
+  const int nvox = g.D * g.H * g.W;
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool ok = v < nvox;
+  const int vv = ok ? v : 0;
+  const int x = vv % g.W;
+  const int y = (vv / g.W) % g.H;
+  const int z = vv / (g.W * g.H);
+  const float qx = inclusive_coord(x, g.W, g.sx, g.ex);
+  const float qy = inclusive_coord(y, g.H, g.sy, g.ey);
+  const float qz = inclusive_coord(z, g.D, g.sz, g.ez);
+  const float* sb = scan + (size_t)b * S * 3;
+  const float* ob = orig + (size_t)b * So * 3;
+  const float* tb = trT + (size_t)b * S * g.T;
+  float value = 0.f;
+  for (int s0 = 0; s0 < S; s0 += rows_per_stage) {
+    const int ns = min(rows_per_stage, S - s0);
+    __syncthreads();
+    for (int e = threadIdx.x; e < ns * g.T; e += blockDim.x) rows[e] = tb[(size_t)s0 * g.T + e];
+    __syncthreads();
+    for (int k = 0; k < ns; ++k) {
+      const int s = s0 + k;
+      const float px = sb[3 * s], py = sb[3 * s + 1], pz = sb[3 * s + 2];
+      const float dsv = dist3(px, py, pz, qx, qy, qz);                     // :177
+      const float dso = So > 0 ? dist3(px, py, pz, ob[3 * s], ob[3 * s + 1], ob[3 * s + 2])
+                               : 0.f;                                        // :178-183
+      const float dist = 2.f * dso + 2.f * dsv;                              // :185
+      const int ti = nearest_bin(dist, g.w, g.off);                          // :187
+      if (ti >= 0 && ti < g.T) value += rows[k * g.T + ti];                  // :188-190
+    }
+  }
+  if (ok) vol[(size_t)b * nvox + v] = value;                                 // :192
+}
+
+// ---------------------------------------------------------------------------
+// A3: vol = sum_{s,l} tr[b][t(s,l)][s][l],
+// t = nearest_bin(|p_s - o_s| + |p_s - q| + |p_l - q| + |p_l - o_l|)
+// (backprojection_cuda.cu:228-287)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void velten_exhaustive_kernel(
+    const float* __restrict__ trT,     // [B][S][L][T]
+    const float* __restrict__ scan,    // [B][S][3]
+    const float* __restrict__ laser,   // [B][L][3]
+    const float* __restrict__ sorig,   // [B][So][3]
+    const float* __restrict__ lorig,   // [B][Lo][3]
+    float* __restrict__ vol, BoxGeom g, int S, int L, int So, int Lo,
+    int rows_per_stage) {
+  extern __shared__ float rows[];
+  const int b = blockIdx.y;
+  const int nvox = g.D * g.H * g.W;
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool ok = v < nvox;
+  const int vv = ok ? v : 0;
+  const int x = vv % g.W;
+  const int y = (vv / g.W) % g.H;
+  const int z = vv / (g.W * g.H);
+  const float qx = inclusive_coord(x, g.W, g.sx, g.ex);
+  const float qy = inclusive_coord(y, g.H, g.sy, g.ey);
+  const float qz = inclusive_coord(z, g.D, g.sz, g.ez);
+  const float* sb = scan + (size_t)b * S * 3;
+  const float* lb = laser + (size_t)b * L * 3;
+  const float* sob = sorig + (size_t)b * So * 3;
+  const float* lob = lorig + (size_t)b * Lo * 3;
+  const float* tb = trT + (size_t)b * S * L * g.T;
+  const int nrows = S * L;
+  float value = 0.f;
+  for (int r0 = 0; r0 < nrows; r0 += rows_per_stage) {
+    const int nr = min(rows_per_stage, nrows - r0);
+    __syncthreads();
+    for (int e = threadIdx.x; e < nr * g.T; e += blockDim.x) rows[e] = tb[(size_t)r0 * g.T + e];
+    __syncthreads();
+    for (int k = 0; k < nr; ++k) {
+      const int r = r0 + k;
+      const int s = r / L, l = r - s * L;
+      const float px = sb[3 * s], py = sb[3 * s + 1], pz = sb[3 * s + 2];
+      const float dsv = dist3(px, py, pz, qx, qy, qz);                     // :255
+      const float dso = So > 0 ? dist3(px, py, pz, sob[3 * s], sob[3 * s + 1], sob[3 * s + 2])
+                               : 0.f;                                        // :256-261
+      const float lx = lb[3 * l], ly = lb[3 * l + 1], lz = lb[3 * l + 2];
+      const float dlv = dist3(lx, ly, lz, qx, qy, qz);                     // :268
+      const float dlo = Lo > 0 ? dist3(lx, ly, lz, lob[3 * l], lob[3 * l + 1], lob[3 * l + 2])
+                               : 0.f;                                        // :269-274
+      float dist = dso + dsv;                                                // :276-277
+      dist = dist + dlv;
+      dist = dist + dlo;
+      const int ti = nearest_bin(dist, g.w, g.off);                          // :279
+      if (ti >= 0 && ti < g.T) value += rows[k * g.T + ti];                  // :280-282
+    }
+  }
+  if (ok) vol[(size_t)b * nvox + v] = value;                                 // :285
+}
+
+// ---------------------------------------------------------------------------
+// A4: Arellano ellipsoid backprojection (backprojection_cuda.cu:10-115).
+// ---------------------------------------------------------------------------
+template <bool IN_LDS>
+__global__ __launch_bounds__(64) void arellano_kernel(
+    const float* __restrict__ trT,     // [B][S][L][T]
+    const float* __restrict__ scan,    // [B][3][S]
+    const float* __restrict__ laser,   // [B][3][L]
+    const float* __restrict__ sorig,   // [B][3][So]
+    const float* __restrict__ lorig,   // [B][3][Lo]
+    float* __restrict__ vol,           // [B][D][H][W], written
+    BoxGeom g, int S, int L, int So, int Lo) {
+  extern __shared__ float colacc[];    // [D][64] when IN_LDS
+  const int lane = threadIdx.x;
+  const int b = blockIdx.y;
+  const int ncol = g.H * g.W;
+  const int v = blockIdx.x * 64 + lane;
+  const bool ok = v < ncol;
+  const int vv = ok ? v : 0;
+  const int xi = vv % g.W, yi = vv / g.W;                                    // :27-28
+  const float xf = inclusive_coord(xi, g.W, g.sx, g.ex);                     // :29-30
+  const float yf = inclusive_coord(yi, g.H, g.sy, g.ey);                     // :31-32
+  float* gcol = vol + (size_t)b * g.D * ncol + vv;                           // stride ncol
+  if (IN_LDS) {
+    for (int z = 0; z < g.D; ++z) colacc[z * 64 + lane] = 0.f;
+  } else if (ok) {
+    for (int z = 0; z < g.D; ++z) gcol[(size_t)z * ncol] = 0.f;
+  }
+  const float* sb = scan + (size_t)b * 3 * S;
+  const float* lb = laser + (size_t)b * 3 * L;
+  const float* sob = sorig + (size_t)b * 3 * So;
+  const float* lob = lorig + (size_t)b * 3 * Lo;
+  for (int s = 0; s < S; ++s) {
+    const float spx = sb[s], spy = sb[S + s], spz = sb[2 * S + s];
+    const float sd = So > 0 ? dist3(spx, spy, spz, sob[s], sob[So + s], sob[2 * So + s]) : 0.f;
+    for (int l = 0; l < L; ++l) {
+      const float lpx = lb[l], lpy = lb[L + l], lpz = lb[2 * L + l];
+      const float ld = Lo > 0 ? dist3(lpx, lpy, lpz, lob[l], lob[Lo + l], lob[2 * Lo + l]) : 0.f;
+      const float cx = 0.5f * (spx + lpx), cy = 0.5f * (spy + lpy), cz = 0.5f * (spz + lpz);
+      float nx = spx - lpx, ny = spy - lpy, nz = spz - lpz;                  // :62-66
+      float ls = nx * nx;
+      ls = ls + ny * ny;
+      ls = ls + nz * nz;
+      const float len = sqrt_rn(ls);                                         // :67
+      if (len > 1.e-6f) {                                                    // :68-70
+        const float inv = 1.f / len;
+        nx = inv * nx;
+        ny = inv * ny;
+        nz = inv * nz;
+      }
+      const float rsx = xf - cx, rsy = yf - cy, rsz = g.sz - cz;             // :72
+      const float rex = xf - cx, rey = yf - cy, rez = g.ez - cz;             // :74
+      float dps = rsx * nx;
+      dps = dps + rsy * ny;
+      dps = dps + rsz * nz;
+      float dpe = rex * nx;
+      dpe = dpe + rey * ny;
+      dpe = dpe + rez * nz;
+      const float* row = trT + ((size_t)(b * S + s) * L + l) * g.T;
+      for (int ti = 0; ti < g.T; ++ti) {                                     // :75
+        const float value = row[ti];
+        if (!(value > 0.f)) continue;                                        // :77
+        float tc = (float)ti * g.w;                                          // :79-80
+        tc = tc + g.off;
+        tc = tc - sd;
+        tc = tc - ld;
+        if (!(tc > len)) continue;                                           // :81
+        const float r0 = tc / 2.f;                                           // :83
+        float r1s = r0 * r0;
+        r1s = r1s - (len * len) / 4.f;                                       // :84
+        const float r1 = sqrt_rn(r1s);
+        const float sc = r1 / r0;                                            // :85
+        const float ks = (sc - 1.f) * dps;                                   // :87
+        const float stx = rsx + ks * nx, sty = rsy + ks * ny, stz = rsz + ks * nz;
+        const float ke = (sc - 1.f) * dpe;                                   // :88
+        const float etx = rex + ke * nx, ety = rey + ke * ny, etz = rez + ke * nz;
+        float dx = etx - stx, dy = ety - sty, dz = etz - stz;                // :89
+        float rl = dx * dx;
+        rl = rl + dy * dy;
+        rl = rl + dz * dz;
+        const float rlen = sqrt_rn(rl);                                      // :90
+        const float ir = 1.f / rlen;                                         // :91
+        dx = ir * dx;
+        dy = ir * dy;
+        dz = ir * dz;
+        float v0 = dx * stx;                                                 // :93
+        v0 = v0 + dy * sty;
+        v0 = v0 + dz * stz;
+        float ss = stx * stx;
+        ss = ss + sty * sty;
+        ss = ss + stz * stz;
+        float v1 = v0 * v0 - (ss - r1 * r1);                                 // :94
+        if (!(v1 > 0.f) || !ok) continue;                                    // :95
+        v1 = sqrt_rn(v1);
+        float d = -v0 - v1;
+        if (d < 0.f) d = -v0 + v1;                                           // :97-100
+        float bf = d / rlen;
+        bf = bf * (float)g.D;
+        bf = bf + 0.5f;
+        const int bini = (int)bf;                                            // :101-102
+        // :103-109: the interpolation weight is int - int = 0, so only bini
+        // receives the value.
+        if (bini >= 0 && bini < g.D) {
+          if (IN_LDS)
+            colacc[bini * 64 + lane] += value;
+          else
+            gcol[(size_t)bini * ncol] += value;
+        }
+      }
+    }
+  }
+  if (IN_LDS && ok)
+    for (int z = 0; z < g.D; ++z) gcol[(size_t)z * ncol] = colacc[z * 64 + lane];
+}
+
+BoxGeom box_geom(int64_t B, int64_t D, int64_t H, int64_t W, int64_t T,
+                 const float* vs, const float* ve, float w, float off) {
+  BoxGeom g;
+  g.B = (int)B;
+  g.D = (int)D;
+  g.H = (int)H;
+  g.W = (int)W;
+  g.T = (int)T;
+  g.sx = vs[0];
+  g.sy = vs[1];
+  g.sz = vs[2];
+  g.ex = ve[0];
+  g.ey = ve[1];
+  g.ez = ve[2];
+  g.w = w;
+  g.off = off;
+  return g;
+}
+
+int check_dims(std::initializer_list<int64_t> dims, int64_t elems_limit_a, int64_t elems_limit_b) {
+  for (int64_t d : dims)
+    TOTRI_CHECK_ARG(d >= 0 && d < (1ll << 31), "dimension out of range");
+  TOTRI_CHECK_ARG(elems_limit_a < (1ll << 31) && elems_limit_b < (1ll << 31),
+                  "tensor has more than 2^31 elements");
+  return TOTRI_OK;
+}
+
+}  // namespace
+
+int transpose_batched(const float* in, float* out, int B, int R, int C, hipStream_t st) {
+  dim3 grid((C + 31) / 32, (R + 31) / 32, B);
+  hipLaunchKernelGGL(transpose_kernel, grid, dim3(32, 8), 0, st, in, out, R, C);
+  return check_launch("transpose_kernel");
+}
+
+}  // namespace totri_hip
+
+using namespace totri_hip;
+
+extern "C" size_t totri_backprojection_confocal_velten_workspace(int64_t B, int64_t T,
+                                                                 int64_t S) {
+  return (size_t)B * T * S * sizeof(float);
+}
+
+extern "C" int totri_backprojection_confocal_velten_forward(
+    const float* transient, int64_t B, int64_t T, int64_t S, const float* scan_points,
+    const float* scan_origin, int64_t So, float* volume, int64_t D, int64_t H, int64_t W,
+    const float volume_start[3], const float volume_end[3], float bin_width,
+    float bin_offset, void* workspace, size_t workspace_bytes, void* stream) {
+  int rc = check_dims({B, T, S, So, D, H, W}, B * T * S, B * D * H * W);
+  if (rc) return rc;
+  TOTRI_CHECK_ARG(So == 0 || So == S, "scan_origin must have 0 or S points");
+  TOTRI_CHECK_ARG(volume_start && volume_end, "null volume box");
+  hipStream_t st = (hipStream_t)stream;
+  const size_t nvox = (size_t)D * H * W;
+  if (B == 0 || nvox == 0) return TOTRI_OK;
+  TOTRI_CHECK_ARG(volume != nullptr, "null volume");
+  const BoxGeom g = box_geom(B, D, H, W, T, volume_start, volume_end, bin_width, bin_offset);
+  if (T == 0 || S == 0) {
+    hipLaunchKernelGGL(velten_confocal_kernel, dim3(grid1d(nvox), (unsigned)B), dim3(256), 0,
+                       st, (const float*)nullptr, scan_points, scan_origin, volume, g, 0,
+                       (int)So, 1);
+    return check_launch("velten_confocal_kernel");
+  }
+  TOTRI_CHECK_ARG(transient && scan_points, "null input");
+  TOTRI_CHECK_ARG(So == 0 || scan_origin, "null scan_origin");
+  const size_t need = totri_backprojection_confocal_velten_workspace(B, T, S);
+  if (workspace_bytes < need)
+    return fail(TOTRI_EWORKSPACE, "workspace too small: need %zu bytes, got %zu", need,
+                workspace_bytes);
+  TOTRI_CHECK_ARG(workspace != nullptr, "null workspace");
+  TOTRI_CHECK_ARG(T <= kStageFloats, "T=%lld bins exceed the staging limit %d",
+                  (long long)T, kStageFloats);
+  float* trT = (float*)workspace;
+  rc = launch_transpose(transient, trT, (int)B, (int)T, (int)S, st);
+  if (rc) return rc;
+  const int rows = (int)(kStageFloats / T);
+  hipLaunchKernelGGL(velten_confocal_kernel, dim3(grid1d(nvox), (unsigned)B), dim3(256),
+                     (size_t)rows * T * sizeof(float), st, trT, scan_points, scan_origin,
+                     volume, g, (int)S, (int)So, rows);
+  return check_launch("velten_confocal_kernel");
+}
+
+extern "C" size_t totri_backprojection_exhaustive_velten_workspace(int64_t B, int64_t T,
+                                                                   int64_t S, int64_t L) {
+  return (size_t)B * T * S * L * sizeof(float);
+}
+
+extern "C" int totri_backprojection_exhaustive_velten_forward(
+    const float* transient, int64_t B, int64_t T, int64_t S, int64_t L,
+    const float* scan_points, const float* laser_points, const float* scan_origin,
+    int64_t So, const float* laser_origin, int64_t Lo, float* volume, int64_t D, int64_t H,
+    int64_t W, const float volume_start[3], const float volume_end[3], float bin_width,
+    float bin_offset, void* workspace, size_t workspace_bytes, void* stream) {
+  int rc = check_dims({B, T, S, L, So, Lo, D, H, W}, B * T * S * L, B * D * H * W);
+  if (rc) return rc;
+  TOTRI_CHECK_ARG(So == 0 || So == S, "scan_origin must have 0 or S points");
+  TOTRI_CHECK_ARG(Lo == 0 || Lo == L, "laser_origin must have 0 or L points");
+  TOTRI_CHECK_ARG(volume_start && volume_end, "null volume box");
+  hipStream_t st = (hipStream_t)stream;
+  const size_t nvox = (size_t)D * H * W;
+  if (B == 0 || nvox == 0) return TOTRI_OK;
+  TOTRI_CHECK_ARG(volume != nullptr, "null volume");
+  const BoxGeom g = box_geom(B, D, H, W, T, volume_start, volume_end, bin_width, bin_offset);
+  if (T == 0 || S == 0 || L == 0) {
+    hipLaunchKernelGGL(velten_exhaustive_kernel, dim3(grid1d(nvox), (unsigned)B), dim3(256),
+                       0, st, (const float*)nullptr, scan_points, laser_points, scan_origin,
+                       laser_origin, volume, g, 0, 1, (int)So, (int)Lo, 1);
+    return check_launch("velten_exhaustive_kernel");
+  }
+  TOTRI_CHECK_ARG(transient && scan_points && laser_points, "null input");
+  TOTRI_CHECK_ARG((So == 0 || scan_origin) && (Lo == 0 || laser_origin), "null origin");
+  const size_t need = totri_backprojection_exhaustive_velten_workspace(B, T, S, L);
+  if (workspace_bytes < need)
+    return fail(TOTRI_EWORKSPACE, "workspace too small: need %zu bytes, got %zu", need,
+                workspace_bytes);
+  TOTRI_CHECK_ARG(workspace != nullptr, "null workspace");
+  TOTRI_CHECK_ARG(T <= kStageFloats, "T=%lld bins exceed the staging limit %d",
+                  (long long)T, kStageFloats);
+  float* trT = (float*)workspace;
+  rc = launch_transpose(transient, trT, (int)B, (int)T, (int)(S * L), st);
+  if (rc) return rc;
+  const int rows = (int)(kStageFloats / T);
+  hipLaunchKernelGGL(velten_exhaustive_kernel, dim3(grid1d(nvox), (unsigned)B), dim3(256),
+                     (size_t)rows * T * sizeof(float), st, trT, scan_points, laser_points,
+                     scan_origin, laser_origin, volume, g, (int)S, (int)L, (int)So, (int)Lo,
+                     rows);
+  return check_launch("velten_exhaustive_kernel");
+}
+
+extern "C" size_t totri_backprojection_exhaustive_arellano_workspace(int64_t B, int64_t T,
+                                                                     int64_t S, int64_t L) {
+  return (size_t)B * T * S * L * sizeof(float);
+}
+
+extern "C" int totri_backprojection_exhaustive_arellano_forward(
+    const float* transient, int64_t B, int64_t T, int64_t S, int64_t L,
+    const float* scan_points, const float* laser_points, const float* scan_origin,
+    int64_t So, const float* laser_origin, int64_t Lo, float* volume, int64_t D, int64_t H,
+    int64_t W, const float volume_start[3], const float volume_end[3], float bin_width,
+    float bin_offset, void* workspace, size_t workspace_bytes, void* stream) {
+  int rc = check_dims({B, T, S, L, So, Lo, D, H, W}, B * T * S * L, B * D * H * W);
+  if (rc) return rc;
+  TOTRI_CHECK_ARG(So == 0 || So == S, "scan_origin must have 0 or S points");
+  TOTRI_CHECK_ARG(Lo == 0 || Lo == L, "laser_origin must have 0 or L points");
+  TOTRI_CHECK_ARG(volume_start && volume_end, "null volume box");
+  hipStream_t st = (hipStream_t)stream;
+  const size_t ncol = (size_t)H * W;
+  if (B == 0 || ncol == 0 || D == 0) return TOTRI_OK;
+  TOTRI_CHECK_ARG(volume != nullptr, "null volume");
+  const BoxGeom g = box_geom(B, D, H, W, T, volume_start, volume_end, bin_width, bin_offset);
+  const size_t need = totri_backprojection_exhaustive_arellano_workspace(B, T, S, L);
+  if (T > 0 && S > 0 && L > 0) {
+    TOTRI_CHECK_ARG(transient && scan_points && laser_points, "null input");
+    TOTRI_CHECK_ARG((So == 0 || scan_origin) && (Lo == 0 || laser_origin), "null origin");
+    if (workspace_bytes < need)
+      return fail(TOTRI_EWORKSPACE, "workspace too small: need %zu bytes, got %zu", need,
+                  workspace_bytes);
+    TOTRI_CHECK_ARG(workspace != nullptr, "null workspace");
+    rc = launch_transpose(transient, (float*)workspace, (int)B, (int)T, (int)(S * L), st);
+    if (rc) return rc;
+  } else {
+    S = 0;  // nothing to backproject: the kernel only zero-fills
+  }
+  const dim3 grid((unsigned)((ncol + 63) / 64), (unsigned)B);
+  if (D <= 256) {
+    hipLaunchKernelGGL(arellano_kernel<true>, grid, dim3(64), (size_t)D * 64 * sizeof(float),
+                       st, (const float*)workspace, scan_points, laser_points, scan_origin,
+                       laser_origin, volume, g, (int)S, (int)L, (int)So, (int)Lo);
+  } else {
+    hipLaunchKernelGGL(arellano_kernel<false>, grid, dim3(64), 0, st, (const float*)workspace,
+                       scan_points, laser_points, scan_origin, laser_origin, volume, g, (int)S,
+                       (int)L, (int)So, (int)Lo);
+  }
+  return check_launch("arellano_kernel");
+}

@@ -39,6 +39,9 @@ int check_launch(const char* what);
 // Number of compute units of the current device (cached per device).
 int device_cu_count();
 
+// out[b][c][r] = in[b][r][c] for B arrays of R x C floats, on stream st.
+int transpose_batched(const float* in, float* out, int B, int R, int C, hipStream_t st);
+
 // ---------------------------------------------------------------------------
 // Device math
 // ---------------------------------------------------------------------------

@@ -1,0 +1,396 @@
+// Exhaustive (non-confocal) volume render R2 and its adjoint A5 for gfx950.
+//
+// Reference: src/render/volume_cuda.cu:297-466 (R2 kernel and the A5 launch,
+// whose kernel body is empty), src/render/volume.cpp:83-111.
+//
+// R2 spreads each occupied voxel's occupancy over the bins of the path
+// lengths |p_s - q| + |p_l - q| between its lower and upper z faces
+// (volume_cuda.cu:330-375).  The reference scatters with global float
+// atomics from one thread per (column, scan point, laser point).  Here one
+// workgroup owns one (scan point, laser point) pair and its whole time
+// histogram: every wave accumulates into a private copy of the histogram in
+// LDS, the copies are summed in a fixed order and written once.  Occupied
+// voxels come from a compacted list built in index order, so a wave's lanes
+// are all doing useful work and the result is reproducible.
+//
+// A5 is written as R2's exact transpose with unit occupancy: one lane per
+// voxel gathers, for every (scan, laser) pair, the weighted bins of the
+// transposed upstream gradient staged in LDS.
+//
+// Decisions (bin span, clipping) follow the reference's FP32 formula with
+// IEEE correctly rounded square roots and divisions.
+#include <math.h>
+
+#include "common.h"
+
+namespace totri_hip {
+namespace {
+
+constexpr int kR2Waves = 4;
+constexpr int kStage = 8192;  // floats of staged gradient rows (32 KiB)
+
+struct ExGeom {
+  int B, D, H, W, T;
+  float sx, sy, sz, ex, ey, ez;
+  float w, off;
+};
+
+// Voxel geometry (volume_cuda.cu:316-319,335-338): column centre (xf, yf)
+// and the z of the voxel's lower and upper faces.
+__device__ __forceinline__ void voxel_faces(const ExGeom& g, int xi, int yi, int zi,
+                                            float* xf, float* yf, float* zf0, float* zf1) {
+  float a = ((float)xi + 0.5f) / (float)g.W;
+  *xf = g.sx + a * (g.ex - g.sx);
+  a = ((float)yi + 0.5f) / (float)g.H;
+  *yf = g.sy + a * (g.ey - g.sy);
+  a = ((float)zi + 0.5f) / (float)g.D;
+  const float zf = g.sz + a * (g.ez - g.sz);
+  const float h = (0.5f / (float)g.D) * (g.ez - g.sz);
+  *zf0 = zf - h;
+  *zf1 = zf + h;
+}
+
+struct Span {
+  int first, last;      // inclusive bin span; first > last when empty
+  float wf, wm, wl;     // weights of the first, inner and last bins
+};
+
+// Bin span and weights of one voxel for one (scan, laser) pair
+// (volume_cuda.cu:339-374).  With first == last the whole occupancy goes to
+// that bin (wf = occ).
+__device__ __forceinline__ Span voxel_span(const ExGeom& g, float occ, float xf, float yf,
+                                           float zf0, float zf1, float spx, float spy,
+                                           float spz, float lpx, float lpy, float lpz) {
+  Span r;
+  float d0 = dist3(spx, spy, spz, xf, yf, zf0) + dist3(lpx, lpy, lpz, xf, yf, zf0);
+  float d1 = dist3(spx, spy, spz, xf, yf, zf1) + dist3(lpx, lpy, lpz, xf, yf, zf1);
+  if (d1 < d0) {
+    const float t = d0;
+    d0 = d1;
+    d1 = t;
+  }
+  float bf0 = (d0 - g.off) / g.w;
+  float bf1 = (d1 - g.off) / g.w;
+  const float range = bf1 - bf0;
+  int fb = (int)floorf(bf0), lb = (int)floorf(bf1);
+  r.first = 1;
+  r.last = 0;
+  r.wf = r.wm = r.wl = 0.f;
+  if (fb < 0) {
+    if (lb < 0) return r;
+    bf0 = 0.f;
+    fb = 0;
+  }
+  if (lb >= g.T) {
+    if (fb >= g.T) return r;
+    bf1 = (float)(g.T - 1);
+    lb = g.T - 1;
+  }
+  r.first = fb;
+  r.last = lb;
+  if (fb == lb) {
+    r.wf = occ;
+    return r;
+  }
+  const float frac = occ / range;
+  r.wf = (1.f - (bf0 - (float)fb)) * frac;
+  r.wm = frac;
+  r.wl = (bf1 - (float)lb) * frac;
+  return r;
+}
+
+// ---- compaction of occupied voxels (index order) -------------------------
+constexpr int kCompactBlock = 1024;
+
+__global__ void count_nonzero_kernel(const float* __restrict__ vol, int nvox,
+                                     int* __restrict__ counts, int nblocks) {
+  const int b = blockIdx.y;
+  const int base = blockIdx.x * kCompactBlock;
+  int c = 0;
+  for (int e = threadIdx.x; e < kCompactBlock; e += blockDim.x) {
+    const int v = base + e;
+    if (v < nvox && vol[(size_t)b * nvox + v] != 0.f) ++c;
+  }
+  __shared__ int red[256];
+  red[threadIdx.x] = c;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) counts[(size_t)b * nblocks + blockIdx.x] = red[0];
+}
+
+// Exclusive scan of the per-block counts of one batch (one workgroup per b).
+__global__ void scan_counts_kernel(int* __restrict__ counts, int nblocks,
+                                   int* __restrict__ totals) {
+  const int b = blockIdx.x;
+  int* c = counts + (size_t)b * nblocks;
+  if (threadIdx.x == 0) {
+    int run = 0;
+    for (int k = 0; k < nblocks; ++k) {
+      const int v = c[k];
+      c[k] = run;
+      run += v;
+    }
+    totals[b] = run;
+  }
+}
+
+__global__ void scatter_nonzero_kernel(const float* __restrict__ vol, int nvox,
+                                       const int* __restrict__ offsets, int nblocks,
+                                       int* __restrict__ list) {
+  const int b = blockIdx.y;
+  const int base = blockIdx.x * kCompactBlock;
+  __shared__ int wave_tot[4];
+  int out = offsets[(size_t)b * nblocks + blockIdx.x];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int e0 = 0; e0 < kCompactBlock; e0 += blockDim.x) {
+    const int v = base + e0 + threadIdx.x;
+    const bool nz = v < nvox && vol[(size_t)b * nvox + v] != 0.f;
+    const unsigned long long m = __ballot(nz);
+    const int before = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) wave_tot[wave] = __popcll(m);
+    __syncthreads();
+    int wbase = 0;
+    for (int k = 0; k < wave; ++k) wbase += wave_tot[k];
+    const int step = wave_tot[0] + wave_tot[1] + wave_tot[2] + wave_tot[3];
+    if (nz) list[(size_t)b * nvox + out + wbase + before] = v;
+    out += step;
+    __syncthreads();
+  }
+}
+
+// ---- R2 -------------------------------------------------------------------
+__global__ __launch_bounds__(kR2Waves * 64) void r2_forward_kernel(
+    const float* __restrict__ vol, const int* __restrict__ list,
+    const int* __restrict__ totals, const float* __restrict__ scan,  // [B][3][S]
+    const float* __restrict__ laser,                                  // [B][3][L]
+    float* __restrict__ transient,                                    // [B][T][S][L]
+    ExGeom g, int S, int L) {
+  extern __shared__ float hist[];  // [kR2Waves][T]
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int b = blockIdx.y;
+  const int pair = blockIdx.x;
+  const int s = pair / L, l = pair - s * L;
+  const int nvox = g.D * g.H * g.W;
+  float* h = hist + wave * g.T;
+  for (int t = lane; t < g.T; t += 64) h[t] = 0.f;
+  const float spx = scan[(size_t)b * 3 * S + s], spy = scan[(size_t)b * 3 * S + S + s],
+              spz = scan[(size_t)b * 3 * S + 2 * S + s];
+  const float lpx = laser[(size_t)b * 3 * L + l], lpy = laser[(size_t)b * 3 * L + L + l],
+              lpz = laser[(size_t)b * 3 * L + 2 * L + l];
+  const int n = totals[b];
+  const int* lst = list + (size_t)b * nvox;
+  const float* vb = vol + (size_t)b * nvox;
+  for (int k0 = wave * 64; k0 < n; k0 += kR2Waves * 64) {
+    const int k = k0 + lane;
+    if (k < n) {
+      const int v = lst[k];
+      const int xi = v % g.W, yi = (v / g.W) % g.H, zi = v / (g.W * g.H);
+      float xf, yf, zf0, zf1;
+      voxel_faces(g, xi, yi, zi, &xf, &yf, &zf0, &zf1);
+      const Span sp = voxel_span(g, vb[v], xf, yf, zf0, zf1, spx, spy, spz, lpx, lpy, lpz);
+      if (sp.first <= sp.last) {
+        // Lanes of a wave may hit the same bins: LDS atomics on the wave's
+        // private histogram.
+        if (sp.first == sp.last) {
+          atomicAdd(h + sp.first, sp.wf);
+        } else {
+          atomicAdd(h + sp.first, sp.wf);
+          for (int bn = sp.first + 1; bn < sp.last; ++bn) atomicAdd(h + bn, sp.wm);
+          atomicAdd(h + sp.last, sp.wl);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  // Fixed-order reduction of the wave histograms; one write per bin.
+  float* out = transient + (size_t)b * g.T * S * L;
+  for (int t = threadIdx.x; t < g.T; t += blockDim.x) {
+    float v = hist[t];
+    for (int w = 1; w < kR2Waves; ++w) v += hist[w * g.T + t];
+    out[((size_t)t * S + s) * L + l] = v;
+  }
+}
+
+// ---- A5 -------------------------------------------------------------------
+__global__ __launch_bounds__(256) void r2_adjoint_kernel(
+    const float* __restrict__ gT,      // [B][S][L][T] transposed upstream gradient
+    const float* __restrict__ scan, const float* __restrict__ laser,
+    float* __restrict__ grad,          // [B][D][H][W]
+    ExGeom g, int S, int L, int rows_per_stage) {
+  extern __shared__ float rows[];
+  const int b = blockIdx.y;
+  const int nvox = g.D * g.H * g.W;
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool ok = v < nvox;
+  const int vv = ok ? v : 0;
+  const int xi = vv % g.W, yi = (vv / g.W) % g.H, zi = vv / (g.W * g.H);
+  float xf, yf, zf0, zf1;
+  voxel_faces(g, xi, yi, zi, &xf, &yf, &zf0, &zf1);
+  const float* sb = scan + (size_t)b * 3 * S;
+  const float* lb = laser + (size_t)b * 3 * L;
+  const float* gb = gT + (size_t)b * S * L * g.T;
+  const int nrows = S * L;
+  float acc = 0.f;
+  for (int r0 = 0; r0 < nrows; r0 += rows_per_stage) {
+    const int nr = min(rows_per_stage, nrows - r0);
+    __syncthreads();
+    for (int e = threadIdx.x; e < nr * g.T; e += blockDim.x) rows[e] = gb[(size_t)r0 * g.T + e];
+    __syncthreads();
+    for (int k = 0; k < nr; ++k) {
+      const int r = r0 + k;
+      const int s = r / L, l = r - s * L;
+      const Span sp = voxel_span(g, 1.f, xf, yf, zf0, zf1, sb[s], sb[S + s], sb[2 * S + s],
+                                 lb[l], lb[L + l], lb[2 * L + l]);
+      if (sp.first > sp.last) continue;
+      const float* row = rows + k * g.T;
+      if (sp.first == sp.last) {
+        acc += sp.wf * row[sp.first];
+      } else {
+        acc += sp.wf * row[sp.first];
+        for (int bn = sp.first + 1; bn < sp.last; ++bn) acc += sp.wm * row[bn];
+        acc += sp.wl * row[sp.last];
+      }
+    }
+  }
+  if (ok) grad[(size_t)b * nvox + v] = acc;
+}
+
+ExGeom ex_geom(int64_t B, int64_t D, int64_t H, int64_t W, int64_t T, const float* vs,
+               const float* ve, float w, float off) {
+  ExGeom g;
+  g.B = (int)B;
+  g.D = (int)D;
+  g.H = (int)H;
+  g.W = (int)W;
+  g.T = (int)T;
+  g.sx = vs[0];
+  g.sy = vs[1];
+  g.sz = vs[2];
+  g.ex = ve[0];
+  g.ey = ve[1];
+  g.ez = ve[2];
+  g.w = w;
+  g.off = off;
+  return g;
+}
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+int check_sizes(int64_t B, int64_t D, int64_t H, int64_t W, int64_t T, int64_t S, int64_t L) {
+  for (int64_t d : {B, D, H, W, T, S, L})
+    TOTRI_CHECK_ARG(d >= 0 && d < (1ll << 31), "dimension out of range");
+  TOTRI_CHECK_ARG(B * D * H * W < (1ll << 31) && B * T * S * L < (1ll << 31),
+                  "tensor has more than 2^31 elements");
+  return TOTRI_OK;
+}
+
+}  // namespace
+}  // namespace totri_hip
+
+using namespace totri_hip;
+
+extern "C" size_t totri_volume_render_exhaustive_forward_workspace(int64_t B, int64_t D,
+                                                                   int64_t H, int64_t W) {
+  const size_t nvox = (size_t)D * H * W;
+  const size_t nblocks = (nvox + kCompactBlock - 1) / kCompactBlock;
+  return align256((size_t)B * nblocks * sizeof(int)) + align256((size_t)B * sizeof(int)) +
+         align256((size_t)B * nvox * sizeof(int));
+}
+
+extern "C" int totri_volume_render_exhaustive_forward(
+    const float* volume, int64_t B, int64_t D, int64_t H, int64_t W, const float* scan_points,
+    int64_t S, const float* laser_points, int64_t L, float* transient, int64_t T,
+    const float volume_start[3], const float volume_end[3], float bin_width,
+    float bin_offset, void* workspace, size_t workspace_bytes, void* stream) {
+  int rc = check_sizes(B, D, H, W, T, S, L);
+  if (rc) return rc;
+  TOTRI_CHECK_ARG(volume_start && volume_end, "null volume box");
+  hipStream_t st = (hipStream_t)stream;
+  if (B == 0 || T == 0 || S == 0 || L == 0) return TOTRI_OK;
+  TOTRI_CHECK_ARG(transient && scan_points && laser_points, "null input");
+  TOTRI_CHECK_ARG((size_t)T * kR2Waves * sizeof(float) <= 64 * 1024,
+                  "T=%lld bins exceed the LDS histogram limit %d", (long long)T,
+                  (int)(64 * 1024 / (kR2Waves * sizeof(float))));
+  const size_t nvox = (size_t)D * H * W;
+  const ExGeom g = ex_geom(B, D, H, W, T, volume_start, volume_end, bin_width, bin_offset);
+  const size_t need = totri_volume_render_exhaustive_forward_workspace(B, D, H, W);
+  if (workspace_bytes < need)
+    return fail(TOTRI_EWORKSPACE, "workspace too small: need %zu bytes, got %zu", need,
+                workspace_bytes);
+  TOTRI_CHECK_ARG(workspace != nullptr, "null workspace");
+  TOTRI_CHECK_ARG(nvox == 0 || volume != nullptr, "null volume");
+  const int nblocks = (int)((nvox + kCompactBlock - 1) / kCompactBlock);
+  char* ws = (char*)workspace;
+  int* counts = (int*)ws;
+  int* totals = (int*)(ws + align256((size_t)B * nblocks * sizeof(int)));
+  int* list = (int*)(ws + align256((size_t)B * nblocks * sizeof(int)) +
+                     align256((size_t)B * sizeof(int)));
+  if (nvox > 0) {
+    hipLaunchKernelGGL(count_nonzero_kernel, dim3(nblocks, (unsigned)B), dim3(256), 0, st,
+                       volume, (int)nvox, counts, nblocks);
+    rc = check_launch("count_nonzero_kernel");
+    if (rc) return rc;
+    hipLaunchKernelGGL(scan_counts_kernel, dim3((unsigned)B), dim3(64), 0, st, counts,
+                       nblocks, totals);
+    rc = check_launch("scan_counts_kernel");
+    if (rc) return rc;
+    hipLaunchKernelGGL(scatter_nonzero_kernel, dim3(nblocks, (unsigned)B), dim3(256), 0, st,
+                       volume, (int)nvox, counts, nblocks, list);
+    rc = check_launch("scatter_nonzero_kernel");
+    if (rc) return rc;
+  } else {
+    hipLaunchKernelGGL(scan_counts_kernel, dim3((unsigned)B), dim3(64), 0, st, counts, 0,
+                       totals);
+    rc = check_launch("scan_counts_kernel");
+    if (rc) return rc;
+  }
+  hipLaunchKernelGGL(r2_forward_kernel, dim3((unsigned)(S * L), (unsigned)B),
+                     dim3(kR2Waves * 64), (size_t)kR2Waves * T * sizeof(float), st, volume,
+                     list, totals, scan_points, laser_points, transient, g, (int)S, (int)L);
+  return check_launch("r2_forward_kernel");
+}
+
+extern "C" size_t totri_volume_render_exhaustive_grad_volume_workspace(int64_t B, int64_t T,
+                                                                       int64_t S, int64_t L) {
+  return (size_t)B * T * S * L * sizeof(float);
+}
+
+extern "C" int totri_volume_render_exhaustive_grad_volume(
+    const float* transient_grad, int64_t B, int64_t T, int64_t S, int64_t L,
+    const float* scan_points, const float* laser_points, float* volume_grad, int64_t D,
+    int64_t H, int64_t W, const float volume_start[3], const float volume_end[3],
+    float bin_width, float bin_offset, void* workspace, size_t workspace_bytes,
+    void* stream) {
+  int rc = check_sizes(B, D, H, W, T, S, L);
+  if (rc) return rc;
+  TOTRI_CHECK_ARG(volume_start && volume_end, "null volume box");
+  hipStream_t st = (hipStream_t)stream;
+  const size_t nvox = (size_t)D * H * W;
+  if (B == 0 || nvox == 0) return TOTRI_OK;
+  TOTRI_CHECK_ARG(volume_grad != nullptr, "null volume_grad");
+  const ExGeom g = ex_geom(B, D, H, W, T, volume_start, volume_end, bin_width, bin_offset);
+  const dim3 grid((unsigned)((nvox + 255) / 256), (unsigned)B);
+  if (T == 0 || S == 0 || L == 0) {
+    hipLaunchKernelGGL(r2_adjoint_kernel, grid, dim3(256), 0, st, (const float*)nullptr,
+                       scan_points, laser_points, volume_grad, g, 0, 1, 1);
+    return check_launch("r2_adjoint_kernel");
+  }
+  TOTRI_CHECK_ARG(transient_grad && scan_points && laser_points, "null input");
+  TOTRI_CHECK_ARG(T <= kStage, "T=%lld bins exceed the staging limit %d", (long long)T, kStage);
+  const size_t need = totri_volume_render_exhaustive_grad_volume_workspace(B, T, S, L);
+  if (workspace_bytes < need)
+    return fail(TOTRI_EWORKSPACE, "workspace too small: need %zu bytes, got %zu", need,
+                workspace_bytes);
+  TOTRI_CHECK_ARG(workspace != nullptr, "null workspace");
+  float* gT = (float*)workspace;
+  // [B][T][S*L] -> [B][S*L][T]
+  rc = transpose_batched(transient_grad, gT, (int)B, (int)T, (int)(S * L), st);
+  if (rc) return rc;
+  const int rows = (int)(kStage / T);
+  hipLaunchKernelGGL(r2_adjoint_kernel, grid, dim3(256), (size_t)rows * T * sizeof(float), st,
+                     gT, scan_points, laser_points, volume_grad, g, (int)S, (int)L, rows);
+  return check_launch("r2_adjoint_kernel");
+}

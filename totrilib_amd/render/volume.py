@@ -14,6 +14,8 @@ from totrilib_amd.totri_cpp.render import (
     volume_confocal_forward,
     volume_confocal_grad_measurement_points,
     volume_confocal_grad_volume,
+    volume_render_exhaustive_forward,
+    volume_render_exhaustive_grad_volume,
 )
 
 
@@ -92,3 +94,72 @@ class VolumeRenderConfocal(torch.autograd.Function):
                 measurement_points_grad, volume, measurement_points, transient_grad,
                 ctx.volume_bbox_origin, ctx.volume_bbox_size, ctx.bin_width, ctx.bin_offset)
         return (volume_grad, measurement_points_grad) + (None,) * 6
+
+
+def _origin_3xn(origin, points, name):
+    """Origin broadcasting for [B, 3, N] wall points
+    (reference totri/render/volume.py:113-130)."""
+    if origin is None:
+        return torch.empty_like(points[:, :, :0])
+    if origin.ndim == 2:
+        return origin[:, :, None].expand_as(points)
+    if origin.ndim == 3:
+        return origin.expand_as(points) if origin.shape[2] > 0 else origin
+    raise ValueError(f'Shape {origin.shape} of {name} is invalid.')
+
+
+class VolumeRenderExhaustive(torch.autograd.Function):
+    """Render an occupancy volume to a non-confocal transient
+    (reference totri/render/volume.py:78-170).
+
+    volume [B, D, H, W], scan_points [B, 3, S], laser_points [B, 3, L],
+    volume_def (start/end (x, y, z), resolution (D, H, W)), bin_def
+    (width/offset of the scan+laser path length) -> transient [B, N, S, L].
+    The origins are accepted and, as in the reference, ignored.  The backward
+    pass returns R2's exact transpose for the volume (the reference's
+    backward kernel is empty).
+    """
+
+    @staticmethod
+    def forward(ctx, volume: torch.Tensor, scan_points: torch.Tensor,
+                laser_points: torch.Tensor, volume_def, bin_def,
+                scan_origin: typing.Optional[torch.Tensor] = None,
+                laser_origin: typing.Optional[torch.Tensor] = None) -> torch.Tensor:
+        if not torch.cuda.is_available():
+            raise RuntimeError('VolumeRenderExhaustive needs a GPU, but no devices are available.')
+        dev = volume.device if volume.is_cuda else torch.device('cuda', torch.cuda.current_device())
+        transient = torch.zeros((volume.shape[0], bin_def.num, scan_points.shape[2],
+                                 laser_points.shape[2]), device=dev)
+        scan_origin = _origin_3xn(scan_origin, scan_points, 'scan_origin')
+        laser_origin = _origin_3xn(laser_origin, laser_points, 'laser_origin')
+        ctx.set_materialize_grads(False)
+        ctx.save_for_backward(volume, scan_points, laser_points, scan_origin, laser_origin)
+        ctx.volume_def = volume_def
+        ctx.bin_def = bin_def
+        volume_render_exhaustive_forward(
+            volume.to(dev), scan_points.to(dev), laser_points.to(dev), scan_origin.to(dev),
+            laser_origin.to(dev), transient, volume_def.start, volume_def.end,
+            bin_def.width, bin_def.offset)
+        return transient.to(volume.device)
+
+    @staticmethod
+    def backward(ctx, transient_grad: torch.Tensor):
+        n_in = len(ctx.needs_input_grad)
+        grad = [None] * n_in
+        if transient_grad is None:
+            return tuple(grad)
+        volume, scan_points, laser_points, scan_origin, laser_origin = ctx.saved_tensors
+        if ctx.needs_input_grad[0]:
+            dev = transient_grad.device if transient_grad.is_cuda else \
+                torch.device('cuda', torch.cuda.current_device())
+            volume_grad = torch.empty(volume.shape, device=dev)
+            volume_render_exhaustive_grad_volume(
+                transient_grad.to(dev).float(), scan_points.to(dev), laser_points.to(dev),
+                scan_origin.to(dev), laser_origin.to(dev), volume_grad,
+                ctx.volume_def.start, ctx.volume_def.end, ctx.bin_def.width,
+                ctx.bin_def.offset)
+            grad[0] = volume_grad.to(volume.device)
+        for k in range(1, n_in):
+            if ctx.needs_input_grad[k]:
+                raise NotImplementedError()
+        return tuple(grad)

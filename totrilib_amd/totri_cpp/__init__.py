@@ -6,6 +6,6 @@ output tensor passed by the caller.  They run the HIP kernels of
 libtotri_hip.so through its C ABI and never fall back to another
 implementation.
 """
-from totrilib_amd.totri_cpp import render  # noqa: F401
+from totrilib_amd.totri_cpp import reconstruct, render, util  # noqa: F401
 
 __version__ = "0.1.0"

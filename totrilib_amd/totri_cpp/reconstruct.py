@@ -1,0 +1,139 @@
+"""totri_cpp.reconstruct -- backprojections.
+
+Same names and positional arguments as the reference's pybind11 submodule
+(src/totri.cpp:28-33); every function writes the `volume` tensor passed by
+the caller and returns None.
+"""
+from __future__ import annotations
+
+import typing
+
+import torch
+
+from totrilib_amd import _lib
+from totrilib_amd.totri_cpp._common import (
+    OutputSlot, require_f32, require_gpu, stream_ptr, workspace)
+
+Vec3 = typing.Sequence[float]
+
+
+def _prep(t: torch.Tensor, name: str) -> torch.Tensor:
+    require_f32(t, name)
+    return t.contiguous()
+
+
+def backprojection_confocal_velten_forward(transient: torch.Tensor,
+                                           scan_points: torch.Tensor,
+                                           scan_origin: torch.Tensor,
+                                           volume: torch.Tensor, volume_start: Vec3,
+                                           volume_end: Vec3, bin_width: float,
+                                           bin_offset: float) -> None:
+    """A2: volume[B, D, H, W] = sum_s transient[b, t(s), s] with the nearest
+    bin of 2|o - p_s| + 2|p_s - q| (backprojection_cuda.cu:152-226).
+    transient [B, T, S], scan_points [B, S, 3], scan_origin [B, 0 or S, 3]."""
+    dev = require_gpu(transient, scan_points, scan_origin, volume,
+                      names=("transient", "scan_points", "scan_origin", "volume"))
+    if transient.dim() != 3 or scan_points.dim() != 3 or scan_origin.dim() != 3 \
+            or volume.dim() != 4:
+        raise ValueError("expected transient [B,T,S], scan_points [B,S,3], "
+                         "scan_origin [B,S',3], volume [B,D,H,W]")
+    B, T, S = (int(s) for s in transient.shape)
+    if tuple(scan_points.shape) != (B, S, 3):
+        raise ValueError(f"scan_points must be [{B},{S},3], got {tuple(scan_points.shape)}")
+    So = int(scan_origin.shape[1])
+    if scan_origin.shape[0] != B or scan_origin.shape[2] != 3 or So not in (0, S):
+        raise ValueError(f"scan_origin must be [{B},0 or {S},3], got {tuple(scan_origin.shape)}")
+    if volume.shape[0] != B:
+        raise ValueError("volume batch size does not match the transient")
+    require_f32(volume, "volume")
+    tr, sp, so = _prep(transient, "transient"), _prep(scan_points, "scan_points"), \
+        _prep(scan_origin, "scan_origin")
+    D, H, W = (int(s) for s in volume.shape[1:])
+    lib = _lib.load()
+    out = OutputSlot(volume)
+    with torch.cuda.device(dev):
+        ws, wsp = workspace(lib.totri_backprojection_confocal_velten_workspace(B, T, S), dev)
+        rc = lib.totri_backprojection_confocal_velten_forward(
+            tr.data_ptr(), B, T, S, sp.data_ptr(), so.data_ptr(), So, out.ptr(), D, H, W,
+            _lib.f3(volume_start), _lib.f3(volume_end), float(bin_width), float(bin_offset),
+            wsp, ws.numel() if ws is not None else 0, stream_ptr(dev))
+        _lib.check(rc, "backprojection_confocal_velten_forward")
+        out.finish()
+
+
+def _exhaustive_args(transient, scan_points, laser_points, scan_origin, laser_origin,
+                     volume, points_last: bool):
+    dev = require_gpu(transient, scan_points, laser_points, scan_origin, laser_origin, volume,
+                      names=("transient", "scan_points", "laser_points", "scan_origin",
+                             "laser_origin", "volume"))
+    if transient.dim() != 4 or volume.dim() != 4:
+        raise ValueError("expected transient [B,T,S,L] and volume [B,D,H,W]")
+    B, T, S, L = (int(s) for s in transient.shape)
+    if points_last:   # [B, S, 3] layout (Velten)
+        sp_shape, lp_shape = (B, S, 3), (B, L, 3)
+        So, Lo = int(scan_origin.shape[1]), int(laser_origin.shape[1])
+        so_ok = scan_origin.dim() == 3 and scan_origin.shape[0] == B and scan_origin.shape[2] == 3
+        lo_ok = laser_origin.dim() == 3 and laser_origin.shape[0] == B and laser_origin.shape[2] == 3
+    else:             # [B, 3, S] layout (Arellano)
+        sp_shape, lp_shape = (B, 3, S), (B, 3, L)
+        So, Lo = int(scan_origin.shape[2]), int(laser_origin.shape[2])
+        so_ok = scan_origin.dim() == 3 and scan_origin.shape[0] == B and scan_origin.shape[1] == 3
+        lo_ok = laser_origin.dim() == 3 and laser_origin.shape[0] == B and laser_origin.shape[1] == 3
+    if tuple(scan_points.shape) != sp_shape or tuple(laser_points.shape) != lp_shape:
+        raise ValueError(f"scan/laser points must be {sp_shape} / {lp_shape}, got "
+                         f"{tuple(scan_points.shape)} / {tuple(laser_points.shape)}")
+    if not so_ok or So not in (0, S) or not lo_ok or Lo not in (0, L):
+        raise ValueError("scan_origin / laser_origin must hold 0 or one point per wall point")
+    if volume.shape[0] != B:
+        raise ValueError("volume batch size does not match the transient")
+    require_f32(volume, "volume")
+    tensors = [_prep(t, n) for t, n in ((transient, "transient"), (scan_points, "scan_points"),
+                                         (laser_points, "laser_points"),
+                                         (scan_origin, "scan_origin"),
+                                         (laser_origin, "laser_origin"))]
+    return dev, (B, T, S, L, So, Lo), tensors
+
+
+def backprojection_exhaustive_velten_forward(transient, scan_points, laser_points,
+                                             scan_origin, laser_origin, volume,
+                                             volume_start, volume_end, bin_width,
+                                             bin_offset) -> None:
+    """A3: exhaustive Velten backprojection (backprojection_cuda.cu:228-322).
+    transient [B,T,S,L], scan_points [B,S,3], laser_points [B,L,3]."""
+    dev, (B, T, S, L, So, Lo), (tr, sp, lp, so, lo) = _exhaustive_args(
+        transient, scan_points, laser_points, scan_origin, laser_origin, volume, True)
+    D, H, W = (int(s) for s in volume.shape[1:])
+    lib = _lib.load()
+    out = OutputSlot(volume)
+    with torch.cuda.device(dev):
+        ws, wsp = workspace(lib.totri_backprojection_exhaustive_velten_workspace(B, T, S, L), dev)
+        rc = lib.totri_backprojection_exhaustive_velten_forward(
+            tr.data_ptr(), B, T, S, L, sp.data_ptr(), lp.data_ptr(), so.data_ptr(), So,
+            lo.data_ptr(), Lo, out.ptr(), D, H, W, _lib.f3(volume_start), _lib.f3(volume_end),
+            float(bin_width), float(bin_offset), wsp, ws.numel() if ws is not None else 0,
+            stream_ptr(dev))
+        _lib.check(rc, "backprojection_exhaustive_velten_forward")
+        out.finish()
+
+
+def backprojection_exhaustive_arellano_forward(transient, scan_points, laser_points,
+                                               scan_origin, laser_origin, volume,
+                                               volume_start, volume_end, bin_width,
+                                               bin_offset) -> None:
+    """A4: Arellano ellipsoid backprojection (backprojection_cuda.cu:10-150).
+    transient [B,T,S,L], scan_points [B,3,S], laser_points [B,3,L]."""
+    dev, (B, T, S, L, So, Lo), (tr, sp, lp, so, lo) = _exhaustive_args(
+        transient, scan_points, laser_points, scan_origin, laser_origin, volume, False)
+    D, H, W = (int(s) for s in volume.shape[1:])
+    lib = _lib.load()
+    out = OutputSlot(volume)
+    with torch.cuda.device(dev):
+        ws, wsp = workspace(lib.totri_backprojection_exhaustive_arellano_workspace(B, T, S, L),
+                            dev)
+        rc = lib.totri_backprojection_exhaustive_arellano_forward(
+            tr.data_ptr(), B, T, S, L, sp.data_ptr(), lp.data_ptr(), so.data_ptr(), So,
+            lo.data_ptr(), Lo, out.ptr(), D, H, W, _lib.f3(volume_start), _lib.f3(volume_end),
+            float(bin_width), float(bin_offset), wsp, ws.numel() if ws is not None else 0,
+            stream_ptr(dev))
+        _lib.check(rc, "backprojection_exhaustive_arellano_forward")
+        out.finish()

@@ -141,3 +141,85 @@ def volume_confocal_grad_measurement_points(measurement_points_grad: torch.Tenso
             float(bin_offset), wsp, ws.numel() if ws is not None else 0, stream_ptr(dev))
         _lib.check(rc, "volume_confocal_grad_measurement_points")
         out.finish()
+
+
+def _exhaustive_render_shapes(volume_like_B, scan_points, laser_points):
+    if scan_points.dim() != 3 or laser_points.dim() != 3 or scan_points.shape[1] != 3 \
+            or laser_points.shape[1] != 3:
+        raise ValueError("scan_points / laser_points must be [B,3,S] / [B,3,L]")
+    B = int(scan_points.shape[0])
+    if laser_points.shape[0] != B or volume_like_B != B:
+        raise ValueError("batch sizes do not match")
+    return B, int(scan_points.shape[2]), int(laser_points.shape[2])
+
+
+def volume_render_exhaustive_forward(volume: torch.Tensor, scan_points: torch.Tensor,
+                                     laser_points: torch.Tensor, scan_origin: torch.Tensor,
+                                     laser_origin: torch.Tensor, transient: torch.Tensor,
+                                     volume_start: Vec3, volume_end: Vec3, bin_width: float,
+                                     bin_offset: float) -> None:
+    """R2: transient[B, T, S, L] = exhaustive render of volume[B, D, H, W]
+    (volume_cuda.cu:297-412).  The origins are accepted and ignored, as in
+    the reference (volume_cuda.cu:312-313)."""
+    dev = require_gpu(volume, scan_points, laser_points, transient,
+                      names=("volume", "scan_points", "laser_points", "transient"))
+    if volume.dim() != 4 or transient.dim() != 4:
+        raise ValueError("expected volume [B,D,H,W] and transient [B,T,S,L]")
+    B, S, L = _exhaustive_render_shapes(int(volume.shape[0]), scan_points, laser_points)
+    if tuple(transient.shape[::2]) != (B, S) or transient.shape[3] != L:
+        raise ValueError(f"transient must be [{B},T,{S},{L}], got {tuple(transient.shape)}")
+    for t, n in ((volume, "volume"), (scan_points, "scan_points"),
+                 (laser_points, "laser_points"), (transient, "transient")):
+        require_f32(t, n)
+    D, H, W = (int(s) for s in volume.shape[1:])
+    T = int(transient.shape[1])
+    vol = volume.contiguous()
+    sp, lp = scan_points.contiguous(), laser_points.contiguous()
+    lib = _lib.load()
+    out = OutputSlot(transient)
+    with torch.cuda.device(dev):
+        ws, wsp = workspace(lib.totri_volume_render_exhaustive_forward_workspace(B, D, H, W), dev)
+        rc = lib.totri_volume_render_exhaustive_forward(
+            vol.data_ptr(), B, D, H, W, sp.data_ptr(), S, lp.data_ptr(), L, out.ptr(), T,
+            _lib.f3(volume_start), _lib.f3(volume_end), float(bin_width), float(bin_offset),
+            wsp, ws.numel() if ws is not None else 0, stream_ptr(dev))
+        _lib.check(rc, "volume_render_exhaustive_forward")
+        out.finish()
+
+
+def volume_render_exhaustive_grad_volume(transient_grad: torch.Tensor,
+                                         scan_points: torch.Tensor,
+                                         laser_points: torch.Tensor,
+                                         scan_origin: torch.Tensor,
+                                         laser_origin: torch.Tensor,
+                                         volume_grad: torch.Tensor, volume_start: Vec3,
+                                         volume_end: Vec3, bin_width: float,
+                                         bin_offset: float) -> None:
+    """A5: volume_grad[B, D, H, W] = R2^T(transient_grad[B, T, S, L]).  The
+    reference's kernel for this entry is empty (volume_cuda.cu:414-431)."""
+    dev = require_gpu(transient_grad, scan_points, laser_points, volume_grad,
+                      names=("transient_grad", "scan_points", "laser_points", "volume_grad"))
+    if volume_grad.dim() != 4 or transient_grad.dim() != 4:
+        raise ValueError("expected transient_grad [B,T,S,L] and volume_grad [B,D,H,W]")
+    B, S, L = _exhaustive_render_shapes(int(volume_grad.shape[0]), scan_points, laser_points)
+    if transient_grad.shape[0] != B or transient_grad.shape[2] != S \
+            or transient_grad.shape[3] != L:
+        raise ValueError("transient_grad does not match the scan/laser points")
+    for t, n in ((transient_grad, "transient_grad"), (scan_points, "scan_points"),
+                 (laser_points, "laser_points"), (volume_grad, "volume_grad")):
+        require_f32(t, n)
+    D, H, W = (int(s) for s in volume_grad.shape[1:])
+    T = int(transient_grad.shape[1])
+    g = transient_grad.contiguous()
+    sp, lp = scan_points.contiguous(), laser_points.contiguous()
+    lib = _lib.load()
+    out = OutputSlot(volume_grad)
+    with torch.cuda.device(dev):
+        ws, wsp = workspace(lib.totri_volume_render_exhaustive_grad_volume_workspace(B, T, S, L),
+                            dev)
+        rc = lib.totri_volume_render_exhaustive_grad_volume(
+            g.data_ptr(), B, T, S, L, sp.data_ptr(), lp.data_ptr(), out.ptr(), D, H, W,
+            _lib.f3(volume_start), _lib.f3(volume_end), float(bin_width), float(bin_offset),
+            wsp, ws.numel() if ws is not None else 0, stream_ptr(dev))
+        _lib.check(rc, "volume_render_exhaustive_grad_volume")
+        out.finish()
