@@ -121,3 +121,16 @@ def test_bindings_reject_cpu_tensors_and_bad_shapes():
     with pytest.raises(NotImplementedError):
         util.rbf2volume_gaussian_forward(torch.zeros(1, 2, 5), torch.zeros(1, 2, 2, 2, 2),
                                          (0, 0, 0), (1, 1, 1))
+
+
+def test_install_as_totri_aliases_reference_modules():
+    import sys
+    import totrilib_amd
+    totrilib_amd.install_as_totri()
+    from totri.render import VolumeRenderConfocal  # noqa: F401
+    from totri.reconstruct import BackprojectionConfocalVelten  # noqa: F401
+    from totri.types import BinDef, VolumeDef  # noqa: F401
+    from totri.util import Rbf2VolumeGaussian  # noqa: F401
+    from totri_cpp.render import volume_confocal_forward  # noqa: F401
+    from totri_cpp.util import rbf2volume_gaussian_grad  # noqa: F401
+    assert sys.modules["totri_cpp.reconstruct"].backprojection_confocal_velten_forward
