@@ -53,9 +53,14 @@ struct ConfGeom {
 // Guard half-width on frac(z_hat): an upper bound (with a factor-4 margin)
 // on |z_hat_fast - z_hat_exact| over samples with z_hat in [-1, Z], where
 // z_hat_fast = fma(sqrt_fast(d2), s, c) and z_hat_exact follows the
-// reference's rounding sequence.  See the derivation in the design notes:
-// each rounding contributes at most 2^-24 of its operand, the hardware
-// square root at most 2 ulp (4 * 2^-24 of depth).
+// reference's rounding sequence.  Terms of the bound, in units of 2^-24:
+//   4 s tc_max       v_sqrt_f32 within 2 ulp of sqrt_rn (depth <= tc_max), times s
+//   s|oz| + 2(Z+1)   roundings of z = wz + depth and of (z - oz) * s
+//   2 s|wz - oz|     rounding of the sum inside z, scaled by s
+//   3Z + |c0| + |c_shift| + 3
+//                    the final -0.5 and the roundings of the precomputed
+//                    constant c0 = (wz - oz) * s - 0.5 - shift
+// The sum B is multiplied by 4 (2^-22 instead of 2^-24) for margin.
 __device__ __forceinline__ float guard_delta(const ConfGeom& g, float wz, float c0,
                                              float c_shift) {
   const float s = fabsf(g.s);
@@ -692,7 +697,8 @@ int check_common(int64_t X, int64_t Y, int64_t Z, int64_t N, int64_t T,
   TOTRI_CHECK_ARG(X < (1ll << 31) && Y < (1ll << 31) && Z < (1ll << 31) &&
                       N < (1ll << 31) && T < (1ll << 31),
                   "dimension exceeds 32-bit indexing");
-  TOTRI_CHECK_ARG(X * Y * Z < (1ll << 31), "volume has more than 2^31 elements");
+  TOTRI_CHECK_ARG(X * Y < (1ll << 31) && X * Y * Z < (1ll << 31),
+                  "volume has more than 2^31 elements");
   TOTRI_CHECK_ARG(N * T < (1ll << 31), "transient has more than 2^31 elements");
   TOTRI_CHECK_ARG(origin != nullptr && size != nullptr, "null geometry");
   TOTRI_CHECK_ARG(dtype == TOTRI_F32 || dtype == TOTRI_BF16 || dtype == TOTRI_F16,
