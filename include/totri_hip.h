@@ -238,6 +238,57 @@ int totri_rbf2volume_gaussian_grad(const float* rbf, int64_t B, int64_t N, int64
                                    float truncation, void* stream);
 
 /*
+ * Batched confocal step (serving).  No counterpart in the reference, whose
+ * volume_confocal_forward / volume_confocal_grad_volume each run one request
+ * (src/render/volume.h:10-24); this runs up to TOTRI_BATCH_MAX_JOBS of them,
+ * renders (R1) and backprojections (A1) of different sizes, in one step of
+ * persistent kernels that take work items from a queue.
+ *
+ * Each job is computed with the same plan and the same per-item code as the
+ * single-request entry points, so its result is bit-identical to
+ * totri_volume_confocal_forward / _grad_volume (FP32 transient) on the same
+ * inputs, whatever else shares the step.
+ *
+ *   totri_confocal_job_workspace(job)   device scratch bytes the job needs
+ *   totri_confocal_batch_table_bytes()  size of one step table
+ *   totri_confocal_batch_prepare()      host only: validates the jobs and
+ *                                       writes a step table into host memory
+ *   totri_confocal_batch_run()          copies the table (host, should be
+ *                                       pinned) to device_table and enqueues
+ *                                       the step on `stream`
+ *
+ * batch_run's launches depend only on its arguments, not on the table
+ * contents, so one call can be captured into a HIP graph and replayed after
+ * each batch_prepare into the same host table.  The host table must not be
+ * rewritten until the previous step on it has finished its copy (wait for the
+ * stream).  Job buffers must not overlap each other or the table.
+ */
+#define TOTRI_JOB_RENDER 0      /* R1: volume (input) -> transient (output)      */
+#define TOTRI_JOB_BACKPROJECT 1 /* A1: transient (input) -> volume (output)      */
+#define TOTRI_BATCH_MAX_JOBS 64
+
+typedef struct totri_confocal_job {
+  int kind;                         /* TOTRI_JOB_RENDER or TOTRI_JOB_BACKPROJECT */
+  int reserved;                     /* 0 */
+  float* volume;                    /* [X][Y][Z] FP32 */
+  const float* measurement_points;  /* [3][N] FP32 */
+  float* transient;                 /* [N][T] FP32 */
+  int64_t X, Y, Z, N, T;
+  float volume_origin[3];
+  float volume_size[3];
+  float bin_width, bin_offset;
+  void* workspace;                  /* >= totri_confocal_job_workspace(job) */
+  size_t workspace_bytes;
+} totri_confocal_job;
+
+size_t totri_confocal_job_workspace(const totri_confocal_job* job);
+size_t totri_confocal_batch_table_bytes(void);
+int totri_confocal_batch_prepare(const totri_confocal_job* jobs, int n_jobs,
+                                 void* host_table, size_t table_bytes);
+int totri_confocal_batch_run(const void* host_table, void* device_table,
+                             size_t table_bytes, void* stream);
+
+/*
  * Test hook with no counterpart in the reference: evaluates the per-sample
  * decision function shared by the R1 and A1 kernels for n samples listed as
  * idx[n][4] = (scan point i, bin t < T, xi, yi) and writes

@@ -1,0 +1,142 @@
+"""GPU tests of the serving engine: every result of a batched, graph-replayed
+step is bit-identical to the single-request entry points on the same inputs,
+each request's result is the same alone and in a mix, and admission holds a
+fixed device budget."""
+import numpy as np
+import pytest
+import torch
+
+from tests.cases import ORIGIN, SIZE, sparse_albedo, wall_grid
+
+pytestmark = pytest.mark.gpu
+
+
+def dev():
+    return torch.device("cuda:0")
+
+
+def make_requests(seed=0):
+    """Renders and backprojections of mixed sizes that exercise every plan:
+    column groups (few scan points), 1/2/4/8/16 bin windows and two bin
+    passes, A1 scan-point ranges and the tiny-radius (safe) A1 form, inputs
+    on the host (pinned and pageable) and resident on the device."""
+    from totrilib_amd.serve import BackprojectRequest, RenderRequest
+    rng = np.random.default_rng(seed)
+    specs = [  # kind, scan side, (X, Y, Z), T, one-way bin width, offset, where
+        ("r", 8, (24, 20, 40), 300, 0.009, 0.0, "pinned"),
+        ("r", 16, (32, 32, 32), 1100, 0.0025, 0.0, "device"),
+        ("b", 8, (24, 20, 40), 300, 0.009, 0.0, "pageable"),
+        ("b", 32, (64, 64, 64), 512, 0.005, 0.0, "pinned"),
+        ("r", 32, (64, 64, 64), 512, 0.005, 0.0, "pageable"),
+        ("b", 12, (16, 24, 20), 300, 0.009, 0.0002, "device"),
+        ("r", 6, (20, 16, 24), 64, 0.04, 0.3, "pinned"),
+        ("r", 6, (20, 16, 24), 100, 0.03, 0.1, "device"),
+        ("r", 10, (18, 18, 30), 200, 0.02, 0.0, "pinned"),
+        ("b", 20, (40, 36, 33), 1500, 0.002, 0.0, "device"),
+    ]
+    reqs = []
+    for k, (kind, n, res, T, w, off, where) in enumerate(specs):
+        mp = torch.from_numpy(wall_grid(n))
+        if kind == "r":
+            data = torch.from_numpy(sparse_albedo(res, 0.05, 100 + k))
+        else:
+            data = torch.from_numpy(rng.standard_normal((n * n, T)).astype(np.float32))
+        if where == "pinned":
+            data = data.pin_memory()
+        elif where == "device":
+            data, mp = data.to(dev()), mp.to(dev())
+        if kind == "r":
+            reqs.append(RenderRequest(data, mp, ORIGIN, SIZE, T, w, off))
+        else:
+            reqs.append(BackprojectRequest(data, mp, ORIGIN, SIZE, res, w, off))
+    return reqs
+
+
+def single_request(req):
+    """The same request through the single-request entry points."""
+    from totrilib_amd.serve import RenderRequest
+    from totrilib_amd.totri_cpp.render import (volume_confocal_forward,
+                                               volume_confocal_grad_volume)
+    mp = req.measurement_points.to(dev())
+    if isinstance(req, RenderRequest):
+        out = torch.empty(req.output_shape(), device=dev())
+        volume_confocal_forward(req.volume.to(dev()), mp, out, req.volume_bbox_origin,
+                                req.volume_bbox_size, req.bin_width, req.bin_offset)
+    else:
+        out = torch.empty(req.output_shape(), device=dev())
+        volume_confocal_grad_volume(out, mp, req.transient.to(dev()), req.volume_bbox_origin,
+                                    req.volume_bbox_size, req.bin_width, req.bin_offset)
+    torch.cuda.synchronize()
+    return out.cpu()
+
+
+def collect(tickets):
+    out = []
+    for tk in tickets:
+        assert tk.error is None, tk.error
+        out.append(tk.result().clone())
+        tk.release()
+    return out
+
+
+@pytest.mark.parametrize("graphs", [True, False])
+def test_batched_step_matches_single_request_entry_points(graphs):
+    from totrilib_amd.serve import ServingEngine
+    reqs = make_requests()
+    eng = ServingEngine(device_budget=1 << 30, host_budget=256 << 20, lanes=2,
+                        use_graphs=graphs)
+    tickets = [eng.submit(r) for r in reqs]
+    eng.drain()
+    got = collect(tickets)
+    assert eng.steps >= 1
+    for k, (r, g) in enumerate(zip(reqs, got)):
+        want = single_request(r)
+        assert torch.equal(g, want), (k, float((g - want).abs().max()))
+        assert torch.count_nonzero(want) > 0, k
+
+
+def test_alone_equals_in_mix_and_graph_replays():
+    """Each request alone (its own step) and in a mix of overlapping steps
+    with Poisson arrivals: identical bits.  Three jobs per step over two
+    lanes replays each lane's captured graph several times, with eager
+    single-request work and synchronisations between the steps."""
+    from totrilib_amd.serve import ServingEngine
+    reqs = make_requests(seed=1)
+    eng = ServingEngine(device_budget=1 << 30, host_budget=256 << 20, lanes=2, max_jobs=3)
+    alone = []
+    for r in reqs:
+        tk = eng.submit(r)
+        eng.drain()
+        alone.append(collect([tk])[0])
+        single_request(r)  # eager work between replays
+    rng = np.random.default_rng(5)
+    # a burst of five, then Poisson arrivals
+    arrivals = np.concatenate([np.zeros(5), np.cumsum(rng.exponential(0.001, len(reqs) - 5))])
+    mixed = collect(eng.run(reqs, list(arrivals)))
+    assert max(eng.jobs_per_step) > 1
+    assert eng.lanes[0].steps >= 3 and eng.steps > len(reqs)
+    for k, (a, m) in enumerate(zip(alone, mixed)):
+        assert torch.equal(a, m), k
+        assert torch.equal(a, single_request(reqs[k])), k
+
+
+def test_admission_respects_device_budget():
+    from totrilib_amd.serve import BackprojectRequest, ServingEngine, request_pairs
+    reqs = make_requests(seed=2)
+    budget = 48 << 20
+    eng = ServingEngine(device_budget=budget, host_budget=64 << 20, lanes=2)
+    peak = 0
+    tickets = [eng.submit(r) for r in reqs]
+    # one request that can never fit
+    big = BackprojectRequest(torch.zeros(4096, 2048), torch.from_numpy(wall_grid(64)),
+                             ORIGIN, SIZE, (128, 128, 128), 0.0025)
+    t_big = eng.submit(big)
+    while eng.poll():
+        peak = max(peak, eng.dev.arena.used)
+    assert peak <= budget
+    assert t_big.error is not None and "budget" in t_big.error
+    got = collect(tickets)
+    for r, g in zip(reqs, got):
+        assert torch.equal(g, single_request(r))
+    assert eng.dev.arena.used == 0 and eng.host.arena.used == 0
+    assert sum(request_pairs(r) for r in reqs) > 0

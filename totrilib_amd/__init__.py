@@ -36,10 +36,14 @@ _ALIASES = {
 }
 
 
-def install_as_totri() -> None:
+def install_as_totri(extension_only: bool = False) -> None:
     """Register this package under the reference's module names (`totri`,
     `totri.render`, ..., `totri_cpp`, `totri_cpp.render`, ...) so that code
-    written against the reference imports it unchanged."""
+    written against the reference imports it unchanged.  With
+    `extension_only=True` only the compiled-module names (`totri_cpp.*`) are
+    registered, so the reference's own Python layer runs on these kernels."""
     for alias, target in _ALIASES.items():
+        if extension_only and not alias.startswith("totri_cpp"):
+            continue
         if alias not in sys.modules:
             sys.modules[alias] = importlib.import_module(target)

@@ -80,11 +80,38 @@ SIGNATURES: dict[str, tuple] = {
         _c_int,
         [_c_p, _c_i64, _c_i64, _c_i64, _c_p, _c_p, _c_p, _c_i64, _c_i64, _c_i64, _F3, _F3,
          _c_f, _c_p]),
+    "totri_confocal_job_workspace": (_c_sz, [_c_p]),
+    "totri_confocal_batch_table_bytes": (_c_sz, []),
+    "totri_confocal_batch_prepare": (_c_int, [_c_p, _c_int, _c_p, _c_sz]),
+    "totri_confocal_batch_run": (_c_int, [_c_p, _c_p, _c_sz, _c_p]),
     "totri_debug_confocal_samples": (
         _c_int,
         [_c_p, _c_i64, _c_p, _c_i64, _c_i64, _c_i64, _c_i64, _c_i64, _F3, _F3,
          _c_f, _c_f, _c_p, _c_p, _c_p, _c_p]),
 }
+
+TOTRI_JOB_RENDER = 0
+TOTRI_JOB_BACKPROJECT = 1
+TOTRI_BATCH_MAX_JOBS = 64
+
+
+class ConfocalJob(ctypes.Structure):
+    """totri_confocal_job of include/totri_hip.h (one request of a batched step)."""
+    _fields_ = [
+        ("kind", ctypes.c_int),
+        ("reserved", ctypes.c_int),
+        ("volume", _c_p),
+        ("measurement_points", _c_p),
+        ("transient", _c_p),
+        ("X", _c_i64), ("Y", _c_i64), ("Z", _c_i64), ("N", _c_i64), ("T", _c_i64),
+        ("volume_origin", _F3),
+        ("volume_size", _F3),
+        ("bin_width", _c_f),
+        ("bin_offset", _c_f),
+        ("workspace", _c_p),
+        ("workspace_bytes", _c_sz),
+    ]
+
 
 _lock = threading.Lock()
 _lib = None

@@ -31,6 +31,7 @@
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
 #include <math.h>
+#include <string.h>
 
 #include "common.h"
 
@@ -185,18 +186,19 @@ __device__ __forceinline__ float lane_value(float v, int lane) {
 // ---------------------------------------------------------------------------
 // R1
 // ---------------------------------------------------------------------------
+// One R1 work item: scan points sgrp*8 .. sgrp*8+7 (one per wave), bins
+// t_base .. t_base + 64K - 1, column group grp.  Every thread of the
+// workgroup calls it (it synchronises the workgroup).  LDS holds two chunk
+// buffers of (v[z], v[z+1] - v[z]) pairs, so that one 8-byte read and one
+// FMA give the interpolated albedo.
 template <int K>
-__global__ __launch_bounds__(kWaves * 64) void r1_forward_kernel(
+__device__ __forceinline__ void r1_block(
     const float* __restrict__ vol, const float* __restrict__ mp,
     float* __restrict__ dst,  // [n_groups][N][T] partial sums, or [N][T]
-    ConfGeom g, int t_base, int n_groups, int chunk_cols, int chunks_per_group) {
-  // LDS holds two chunk buffers of (v[z], v[z+1] - v[z]) pairs, so that one
-  // 8-byte read and one FMA give the interpolated albedo.
-  extern __shared__ float2 lds2[];
+    const ConfGeom& g, int t_base, int grp, int sgrp, int n_groups, int chunk_cols,
+    int chunks_per_group, float2* __restrict__ lds2) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int grp = blockIdx.x % n_groups;
-  const int sgrp = blockIdx.x / n_groups;
   const int i = sgrp * kWaves + wave;
   const bool scan_ok = i < g.N;
   const int n_cols = g.X * g.Y;
@@ -369,6 +371,16 @@ __global__ __launch_bounds__(kWaves * 64) void r1_forward_kernel(
   }
 }
 
+template <int K>
+__global__ __launch_bounds__(kWaves * 64) void r1_forward_kernel(
+    const float* __restrict__ vol, const float* __restrict__ mp,
+    float* __restrict__ dst, ConfGeom g, int t_base, int n_groups, int chunk_cols,
+    int chunks_per_group) {
+  extern __shared__ float2 lds2[];
+  r1_block<K>(vol, mp, dst, g, t_base, (int)(blockIdx.x % n_groups),
+              (int)(blockIdx.x / n_groups), n_groups, chunk_cols, chunks_per_group, lds2);
+}
+
 // ---------------------------------------------------------------------------
 // A6: gradient of <G, R1(v; w)> w.r.t. the wall points w (volume_cuda.cu:188-265)
 // ---------------------------------------------------------------------------
@@ -533,43 +545,50 @@ __global__ void fill_zero_kernel(T* __restrict__ out, size_t n) {
 // `safe` form (tiny radii, where 1/tc^8 could overflow) Kf = G and
 // itc4[t] = 1/tc^4 is used as (d2 * itc4)^2 * G instead.
 template <typename GT>
+__device__ __forceinline__ void a1_table_elem(const GT* __restrict__ G, const ConfGeom& g,
+                                              int safe, int t_stride, size_t e,
+                                              float* __restrict__ tc2,
+                                              float* __restrict__ itc4,
+                                              float* __restrict__ Kf) {
+  const int t = (int)(e % t_stride);
+  const size_t i = e / t_stride;
+  const bool in = t < g.T;
+  const float tc = bin_distance(t, g.w, g.off);
+  const float q = tc * tc;
+  const float i4 = 1.f / (q * q);
+  if (i == 0) {
+    tc2[t] = in ? q : -1.f;
+    itc4[t] = in ? i4 : 0.f;
+  }
+  const float gv = in ? load_elem(G + i * g.T + t) : 0.f;
+  Kf[e] = safe ? gv : gv * (i4 * i4);
+}
+
+template <typename GT>
 __global__ void a1_tables_kernel(const GT* __restrict__ G, ConfGeom g, int safe,
                                  int t_stride, float* __restrict__ tc2,
                                  float* __restrict__ itc4, float* __restrict__ Kf) {
   const size_t n = (size_t)g.N * t_stride;
   for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
-       e += (size_t)gridDim.x * blockDim.x) {
-    const int t = (int)(e % t_stride);
-    const size_t i = e / t_stride;
-    const bool in = t < g.T;
-    const float tc = bin_distance(t, g.w, g.off);
-    const float q = tc * tc;
-    const float i4 = 1.f / (q * q);
-    if (i == 0) {
-      tc2[t] = in ? q : -1.f;
-      itc4[t] = in ? i4 : 0.f;
-    }
-    const float gv = in ? load_elem(G + i * g.T + t) : 0.f;
-    Kf[e] = safe ? gv : gv * (i4 * i4);
-  }
+       e += (size_t)gridDim.x * blockDim.x)
+    a1_table_elem(G, g, safe, t_stride, e, tc2, itc4, Kf);
 }
 
 constexpr int kA1Waves = 4;  // independent waves (work items) per A1 workgroup
 
 
+// One A1 work item, run by one wave on its own LDS accumulator block `acc`
+// ((kSlab + 2) rows of 64 floats): the 8x8 column tile (tx, ty), z slab
+// `slab` and scan-point range `range` of n_ranges.  `item` enumerates
+// (tx, ty, slab, range) with tx fastest.  No workgroup barrier is used.
 template <bool SAFE>
-__global__ __launch_bounds__(kA1Waves * 64) void a1_transpose_kernel(
+__device__ __forceinline__ void a1_item(
     const float* __restrict__ Kf, const float* __restrict__ tc2t,
     const float* __restrict__ itc4t, const float* __restrict__ mp,
     float* __restrict__ dst,  // [R][X][Y][Z] partials, or [X][Y][Z]
-    ConfGeom g, int tiles_x, int tiles_y, int n_slabs, int n_ranges, int t_stride,
-    int n_items) {
-  __shared__ float acc_all[kA1Waves][(kSlab + 2) * 64];
+    const ConfGeom& g, int item, int tiles_x, int tiles_y, int n_slabs, int n_ranges,
+    int t_stride, float* __restrict__ acc) {
   const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  int item = blockIdx.x * kA1Waves + wave;
-  if (item >= n_items) return;  // whole wave exits; no barrier follows
-  float* acc = acc_all[wave];
   const int tx = item % tiles_x;
   item /= tiles_x;
   const int ty = item % tiles_y;
@@ -680,6 +699,20 @@ __global__ __launch_bounds__(kA1Waves * 64) void a1_transpose_kernel(
       if (z < g.Z) out[z] = acc[r * 64 + lane];
     }
   }
+}
+
+template <bool SAFE>
+__global__ __launch_bounds__(kA1Waves * 64) void a1_transpose_kernel(
+    const float* __restrict__ Kf, const float* __restrict__ tc2t,
+    const float* __restrict__ itc4t, const float* __restrict__ mp,
+    float* __restrict__ dst, ConfGeom g, int tiles_x, int tiles_y, int n_slabs,
+    int n_ranges, int t_stride, int n_items) {
+  __shared__ float acc_all[kA1Waves][(kSlab + 2) * 64];
+  const int wave = threadIdx.x >> 6;
+  const int item = blockIdx.x * kA1Waves + wave;
+  if (item >= n_items) return;  // whole wave exits; no barrier follows
+  a1_item<SAFE>(Kf, tc2t, itc4t, mp, dst, g, item, tiles_x, tiles_y, n_slabs, n_ranges,
+                t_stride, acc_all[wave]);
 }
 
 int launch_grid(size_t n) {
@@ -984,6 +1017,331 @@ extern "C" int totri_volume_confocal_grad_volume(
     rc = check_launch("sum_partials_kernel");
   }
   return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Batched step: many R1 / A1 jobs in one set of persistent launches
+// ---------------------------------------------------------------------------
+namespace totri_hip {
+namespace {
+
+constexpr int kBatchMax = TOTRI_BATCH_MAX_JOBS;
+constexpr int kClassA1 = 5;  // classes 0..4: R1 with K = 1, 2, 4, 8, 16
+constexpr int kClasses = 7;  // 5: A1, 6: A1 in the safe (tiny radius) form
+constexpr unsigned kBatchMagic = 0x74427431u;
+
+int r1_class(int K) { return K == 1 ? 0 : K == 2 ? 1 : K == 4 ? 2 : K == 8 ? 3 : 4; }
+
+// One job as the kernels see it: geometry, plan and buffer addresses.
+struct BatchJob {
+  ConfGeom g;
+  int kind;
+  int K, passes, scan_groups, n_groups, chunk_cols, chunks_per_group;
+  int tiles_x, tiles_y, n_slabs, n_ranges, t_stride, safe;
+  const float* vol_in;
+  const float* mp;
+  const float* tr_in;
+  float* r1_dst;
+  float* tc2;
+  float* itc4;
+  float* kf;
+  float* a1_dst;
+  const float* fin_src;
+  float* fin_dst;
+  long long fin_n;
+  int fin_parts;
+  int pad_;
+};
+
+// One step.  The work-queue heads come first, in a 256-byte block of their
+// own, and are zero in every table that batch_prepare writes, so uploading
+// the table resets them.  Per class: the jobs with work and the prefix sums
+// of their item counts; likewise the A1 table elements and the partial-sum
+// elements of the finishing pass.
+struct BatchTable {
+  int counters[kClasses];
+  unsigned magic;
+  int n_jobs;
+  int pad_[64 - kClasses - 2];
+  int cls_n[kClasses];
+  int cls_jobs[kClasses][kBatchMax];
+  int cls_prefix[kClasses][kBatchMax + 1];
+  int tab_n;
+  int tab_jobs[kBatchMax];
+  long long tab_prefix[kBatchMax + 1];
+  int fin_n;
+  int fin_jobs[kBatchMax];
+  long long fin_prefix[kBatchMax + 1];
+  BatchJob jobs[kBatchMax];
+};
+
+// Largest slot j in [0, n) with prefix[j] <= e (prefix is non-decreasing and
+// every listed slot is non-empty, so this is the slot that holds e).
+template <typename IT>
+__device__ __forceinline__ int find_slot(const IT* prefix, int n, IT e) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (prefix[mid] <= e) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Persistent R1 over the class-`cls` jobs (all with K windows).  Each
+// workgroup takes the next item from the class queue until it is empty; the
+// item index goes through LDS slot 0, which every thread has read before
+// r1_block refills the chunk buffers.
+// The table is read through a read-only view (job fields become scalar
+// loads) and the queue heads are updated through `counters`, which points at
+// the same table's counter block (the view never reads it).
+template <int K>
+__global__ __launch_bounds__(kWaves * 64) void r1_batch_kernel(
+    const BatchTable* __restrict__ tab, int* __restrict__ counters, int cls) {
+  extern __shared__ float2 lds2[];
+  const int n = tab->cls_n[cls];
+  const int total = tab->cls_prefix[cls][n];
+  int* slot0 = reinterpret_cast<int*>(lds2);
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0) slot0[0] = atomicAdd(&counters[cls], 1);
+    __syncthreads();
+    const int item = uniform_i32(slot0[0]);
+    __syncthreads();
+    if (item >= total) break;
+    const int j = find_slot(tab->cls_prefix[cls], n, item);
+    const BatchJob* jb = &tab->jobs[tab->cls_jobs[cls][j]];
+    const ConfGeom g = jb->g;
+    const int n_groups = jb->n_groups, scan_groups = jb->scan_groups;
+    int local = item - tab->cls_prefix[cls][j];
+    const int grp = local % n_groups;
+    local /= n_groups;
+    const int sgrp = local % scan_groups;
+    const int pass = local / scan_groups;
+    r1_block<K>(jb->vol_in, jb->mp, jb->r1_dst, g, pass * 64 * K, grp, sgrp, n_groups,
+                jb->chunk_cols, jb->chunks_per_group, lds2);
+  }
+}
+
+// Persistent A1 over the class-`cls` jobs: every wave takes its own items.
+template <bool SAFE>
+__global__ __launch_bounds__(kA1Waves * 64) void a1_batch_kernel(
+    const BatchTable* __restrict__ tab, int* __restrict__ counters, int cls) {
+  __shared__ float acc_all[kA1Waves][(kSlab + 2) * 64];
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int n = tab->cls_n[cls];
+  const int total = tab->cls_prefix[cls][n];
+  for (;;) {
+    int item = 0;
+    if (lane == 0) item = atomicAdd(&counters[cls], 1);
+    item = uniform_i32(__shfl(item, 0, 64));
+    if (item >= total) break;
+    const int j = find_slot(tab->cls_prefix[cls], n, item);
+    const BatchJob* jb = &tab->jobs[tab->cls_jobs[cls][j]];
+    const ConfGeom g = jb->g;
+    a1_item<SAFE>(jb->kf, jb->tc2, jb->itc4, jb->mp, jb->a1_dst, g,
+                  item - tab->cls_prefix[cls][j], jb->tiles_x, jb->tiles_y, jb->n_slabs,
+                  jb->n_ranges, jb->t_stride, acc_all[wave]);
+  }
+}
+
+__global__ __launch_bounds__(256) void a1_tables_batch_kernel(const BatchTable* tab) {
+  const int n = tab->tab_n;
+  const long long total = tab->tab_prefix[n];
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int j = find_slot(tab->tab_prefix, n, e);
+    const BatchJob* jb = &tab->jobs[tab->tab_jobs[j]];
+    a1_table_elem(jb->tr_in, jb->g, jb->safe, jb->t_stride,
+                  (size_t)(e - tab->tab_prefix[j]), jb->tc2, jb->itc4, jb->kf);
+  }
+}
+
+// Partial sums in the same fixed order as sum_partials_kernel.
+__global__ __launch_bounds__(256) void finish_batch_kernel(const BatchTable* tab) {
+  const int n = tab->fin_n;
+  const long long total = tab->fin_prefix[n];
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int j = find_slot(tab->fin_prefix, n, e);
+    const BatchJob* jb = &tab->jobs[tab->fin_jobs[j]];
+    const size_t le = (size_t)(e - tab->fin_prefix[j]);
+    const size_t m = (size_t)jb->fin_n;
+    float v = jb->fin_src[le];
+    for (int p = 1; p < jb->fin_parts; ++p) v += jb->fin_src[(size_t)p * m + le];
+    jb->fin_dst[le] = v;
+  }
+}
+
+// Validates one job and fills its kernel-side description; *need receives
+// the workspace it needs.  With check_buffers false only the sizes are
+// validated (the workspace query, made before buffers exist).
+int plan_job(const totri_confocal_job& j, BatchJob* out, size_t* need, bool check_buffers) {
+  *need = 0;
+  TOTRI_CHECK_ARG(j.kind == TOTRI_JOB_RENDER || j.kind == TOTRI_JOB_BACKPROJECT,
+                  "unknown job kind %d", j.kind);
+  int rc = check_common(j.X, j.Y, j.Z, j.N, j.T, j.volume_origin, j.volume_size,
+                        j.bin_width, j.bin_offset, TOTRI_F32);
+  if (rc) return rc;
+  TOTRI_CHECK_ARG(j.X >= 1 && j.Y >= 1 && j.Z >= 2 && j.N >= 1 && j.T >= 1,
+                  "batched jobs need X, Y, N, T >= 1 and Z >= 2");
+  TOTRI_CHECK_ARG(j.Z <= kChunkVoxels, "volume depth Z=%lld exceeds %d", (long long)j.Z,
+                  kChunkVoxels);
+  if (check_buffers)
+    TOTRI_CHECK_ARG(j.volume && j.measurement_points && j.transient, "null job buffer");
+  BatchJob b;
+  memset(&b, 0, sizeof(b));
+  b.g = make_geom(j.X, j.Y, j.Z, j.N, j.T, j.volume_origin, j.volume_size, j.bin_width,
+                  j.bin_offset);
+  b.kind = j.kind;
+  b.mp = j.measurement_points;
+  float* ws = (float*)j.workspace;
+  if (j.kind == TOTRI_JOB_RENDER) {
+    const R1Plan p = plan_r1(j.X, j.Y, j.Z, j.N, j.T);
+    b.K = p.K;
+    b.passes = p.passes;
+    b.scan_groups = (int)((j.N + kWaves - 1) / kWaves);
+    b.n_groups = p.n_groups;
+    b.chunk_cols = p.chunk_cols;
+    b.chunks_per_group = p.chunks_per_group;
+    b.vol_in = j.volume;
+    *need = p.n_groups > 1 ? p.ws : 0;
+    b.r1_dst = p.n_groups > 1 ? ws : j.transient;
+    if (p.n_groups > 1) {
+      b.fin_src = ws;
+      b.fin_dst = j.transient;
+      b.fin_n = (long long)j.N * j.T;
+      b.fin_parts = p.n_groups;
+    }
+  } else {
+    const A1Plan p = plan_a1(j.X, j.Y, j.Z, j.N, j.T, j.bin_width, j.bin_offset);
+    b.tiles_x = p.tiles_x;
+    b.tiles_y = p.tiles_y;
+    b.n_slabs = p.n_slabs;
+    b.n_ranges = p.n_ranges;
+    b.t_stride = p.t_stride;
+    b.safe = p.safe ? 1 : 0;
+    b.tr_in = j.transient;
+    *need = p.tables_bytes + p.partial_bytes;
+    b.tc2 = ws;
+    b.itc4 = ws ? ws + p.t_stride : nullptr;
+    b.kf = ws ? ws + 2 * (size_t)p.t_stride : nullptr;
+    float* partial = ws ? (float*)((char*)ws + p.tables_bytes) : nullptr;
+    b.a1_dst = p.n_ranges > 1 ? partial : j.volume;
+    if (p.n_ranges > 1) {
+      b.fin_src = partial;
+      b.fin_dst = j.volume;
+      b.fin_n = (long long)j.X * j.Y * j.Z;
+      b.fin_parts = p.n_ranges;
+    }
+  }
+  if (*need > 0 && check_buffers) {
+    TOTRI_CHECK_ARG(j.workspace != nullptr, "null job workspace");
+    if (j.workspace_bytes < *need)
+      return fail(TOTRI_EWORKSPACE, "job workspace too small: need %zu bytes, got %zu",
+                  *need, j.workspace_bytes);
+  }
+  *out = b;
+  return TOTRI_OK;
+}
+
+long long job_items(const BatchJob& b) {
+  if (b.kind == TOTRI_JOB_RENDER) return (long long)b.passes * b.scan_groups * b.n_groups;
+  return (long long)b.tiles_x * b.tiles_y * b.n_slabs * b.n_ranges;
+}
+
+}  // namespace
+}  // namespace totri_hip
+
+extern "C" size_t totri_confocal_batch_table_bytes(void) { return sizeof(BatchTable); }
+
+extern "C" size_t totri_confocal_job_workspace(const totri_confocal_job* job) {
+  if (!job) return 0;
+  BatchJob b;
+  size_t need = 0;
+  if (plan_job(*job, &b, &need, false) != TOTRI_OK) return 0;
+  return need;
+}
+
+extern "C" int totri_confocal_batch_prepare(const totri_confocal_job* jobs, int n_jobs,
+                                            void* host_table, size_t table_bytes) {
+  TOTRI_CHECK_ARG(host_table != nullptr, "null host table");
+  TOTRI_CHECK_ARG(table_bytes >= sizeof(BatchTable), "table needs %zu bytes, got %zu",
+                  sizeof(BatchTable), table_bytes);
+  TOTRI_CHECK_ARG(n_jobs >= 0 && n_jobs <= kBatchMax, "n_jobs=%d outside [0, %d]", n_jobs,
+                  kBatchMax);
+  TOTRI_CHECK_ARG(n_jobs == 0 || jobs != nullptr, "null jobs");
+  BatchTable* t = (BatchTable*)host_table;
+  memset(t, 0, sizeof(BatchTable));
+  for (int k = 0; k < n_jobs; ++k) {
+    size_t need = 0;
+    int rc = plan_job(jobs[k], &t->jobs[k], &need, true);
+    if (rc) {
+      memset(t, 0, sizeof(BatchTable));
+      return rc;
+    }
+  }
+  long long cls_total[kClasses] = {0};
+  for (int k = 0; k < n_jobs; ++k) {
+    const BatchJob& b = t->jobs[k];
+    const long long items = job_items(b);
+    const int c = b.kind == TOTRI_JOB_RENDER ? r1_class(b.K) : kClassA1 + b.safe;
+    if (items > 0) {
+      TOTRI_CHECK_ARG(cls_total[c] + items < (1ll << 30), "too many work items in one step");
+      const int m = t->cls_n[c]++;
+      t->cls_jobs[c][m] = k;
+      t->cls_prefix[c][m] = (int)cls_total[c];
+      cls_total[c] += items;
+      t->cls_prefix[c][m + 1] = (int)cls_total[c];
+    }
+    if (b.kind == TOTRI_JOB_BACKPROJECT) {
+      const int m = t->tab_n++;
+      t->tab_jobs[m] = k;
+      t->tab_prefix[m + 1] = t->tab_prefix[m] + (long long)b.g.N * b.t_stride;
+    }
+    if (b.fin_parts > 1) {
+      const int m = t->fin_n++;
+      t->fin_jobs[m] = k;
+      t->fin_prefix[m + 1] = t->fin_prefix[m] + b.fin_n;
+    }
+  }
+  t->n_jobs = n_jobs;
+  t->magic = kBatchMagic;
+  return TOTRI_OK;
+}
+
+extern "C" int totri_confocal_batch_run(const void* host_table, void* device_table,
+                                        size_t table_bytes, void* stream) {
+  TOTRI_CHECK_ARG(host_table != nullptr && device_table != nullptr, "null table");
+  TOTRI_CHECK_ARG(table_bytes >= sizeof(BatchTable), "table needs %zu bytes, got %zu",
+                  sizeof(BatchTable), table_bytes);
+  TOTRI_CHECK_ARG(((const BatchTable*)host_table)->magic == kBatchMagic,
+                  "host table was not written by totri_confocal_batch_prepare");
+  hipStream_t st = (hipStream_t)stream;
+  TOTRI_CHECK_HIP(hipMemcpyAsync(device_table, host_table, sizeof(BatchTable),
+                                 hipMemcpyHostToDevice, st));
+  const BatchTable* tab = (const BatchTable*)device_table;
+  int* ctr = ((BatchTable*)device_table)->counters;
+  const int cu = device_cu_count();
+  const size_t r1_lds = 2 * (size_t)kChunkVoxels * sizeof(float2);
+  hipLaunchKernelGGL(a1_tables_batch_kernel, dim3(4 * cu), dim3(256), 0, st, tab);
+  int rc = check_launch("a1_tables_batch_kernel");
+  if (rc) return rc;
+  hipLaunchKernelGGL(r1_batch_kernel<16>, dim3(2 * cu), dim3(kWaves * 64), r1_lds, st, tab, ctr, 4);
+  hipLaunchKernelGGL(r1_batch_kernel<8>, dim3(2 * cu), dim3(kWaves * 64), r1_lds, st, tab, ctr, 3);
+  hipLaunchKernelGGL(r1_batch_kernel<4>, dim3(2 * cu), dim3(kWaves * 64), r1_lds, st, tab, ctr, 2);
+  hipLaunchKernelGGL(r1_batch_kernel<2>, dim3(2 * cu), dim3(kWaves * 64), r1_lds, st, tab, ctr, 1);
+  hipLaunchKernelGGL(r1_batch_kernel<1>, dim3(2 * cu), dim3(kWaves * 64), r1_lds, st, tab, ctr, 0);
+  rc = check_launch("r1_batch_kernel");
+  if (rc) return rc;
+  hipLaunchKernelGGL(a1_batch_kernel<false>, dim3(4 * cu), dim3(kA1Waves * 64), 0, st, tab,
+                     ctr, kClassA1);
+  hipLaunchKernelGGL(a1_batch_kernel<true>, dim3(4 * cu), dim3(kA1Waves * 64), 0, st, tab,
+                     ctr, kClassA1 + 1);
+  rc = check_launch("a1_batch_kernel");
+  if (rc) return rc;
+  hipLaunchKernelGGL(finish_batch_kernel, dim3(4 * cu), dim3(256), 0, st, tab);
+  return check_launch("finish_batch_kernel");
 }
 
 namespace totri_hip {

@@ -50,4 +50,4 @@ int device_cu_count() {
 
 extern "C" const char* totri_last_error(void) { return totri_hip::g_msg; }
 
-extern "C" int totri_abi_version(void) { return 100; }
+extern "C" int totri_abi_version(void) { return 101; }

@@ -1,0 +1,404 @@
+"""Continuous-batching request loop for confocal renders and backprojections.
+
+Requests are the two operations of VolumeRenderConfocal (reference
+totri/render/volume.py:12-76): a render R1 (`forward`: albedo volume ->
+transient) and a backprojection A1 (`apply_transposed`: transient -> volume),
+each with its own scan, bin count and volume size.
+
+How a request moves through the engine:
+
+* submit() puts it in a FIFO queue.
+* Admission (every poll()) takes queued requests in order while their
+  device memory -- input copies, output, scratch -- fits in the fixed-size
+  HBM arena, and their host result buffer fits in the pinned arena.  Inputs
+  that already live on the device are used in place (resident data).
+* A step starts whenever a lane is idle and admitted requests wait: up to
+  TOTRI_BATCH_MAX_JOBS of them are copied in (on the lane's stream), run by
+  one batched step of persistent kernels (totri_confocal_batch_run), and
+  copied out into pinned host memory.  A lane's step is a HIP graph captured
+  once and replayed for every step, since its launches do not depend on the
+  step's contents (the kernels read a device-side job table that the replay
+  uploads first).
+* With two or more lanes, a step is queued on the device while the previous
+  one runs, so requests that arrive meanwhile start as soon as the previous
+  step's work drains, and copies overlap compute.
+* poll() retires steps whose completion event has fired: their requests get
+  their result and completion time, and their memory returns to the arenas
+  (the result's pinned block when the caller releases the ticket).
+
+Every job's result is bit-identical to the single-request entry points on the
+same inputs, whatever shares its step (same plan, same per-item code, fixed
+summation order).
+"""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+import math
+import time
+from collections import deque
+from typing import Deque, List, Optional, Sequence, Tuple
+
+import torch
+
+from totrilib_amd import _lib
+from totrilib_amd.serve.arena import TensorArena
+
+_ALIGN = 256
+
+
+@dataclasses.dataclass
+class RenderRequest:
+    """R1: volume [X, Y, Z] albedo -> transient [N, num_bins] (FP32)."""
+    volume: torch.Tensor
+    measurement_points: torch.Tensor
+    volume_bbox_origin: Tuple[float, float, float]
+    volume_bbox_size: Tuple[float, float, float]
+    bin_num: int
+    bin_width: float
+    bin_offset: float = 0.0
+
+    kind = _lib.TOTRI_JOB_RENDER
+
+    def dims(self) -> Tuple[int, int, int, int, int]:
+        X, Y, Z = self.volume.shape
+        return X, Y, Z, self.measurement_points.shape[1], int(self.bin_num)
+
+    def output_shape(self) -> Tuple[int, ...]:
+        X, Y, Z, N, T = self.dims()
+        return (N, T)
+
+
+@dataclasses.dataclass
+class BackprojectRequest:
+    """A1: transient [N, T] -> volume [X, Y, Z] = R1^T transient (FP32)."""
+    transient: torch.Tensor
+    measurement_points: torch.Tensor
+    volume_bbox_origin: Tuple[float, float, float]
+    volume_bbox_size: Tuple[float, float, float]
+    volume_resolution: Tuple[int, int, int]
+    bin_width: float
+    bin_offset: float = 0.0
+
+    kind = _lib.TOTRI_JOB_BACKPROJECT
+
+    def dims(self) -> Tuple[int, int, int, int, int]:
+        X, Y, Z = (int(v) for v in self.volume_resolution)
+        N, T = self.transient.shape
+        return X, Y, Z, N, T
+
+    def output_shape(self) -> Tuple[int, ...]:
+        return tuple(int(v) for v in self.volume_resolution)
+
+
+def request_pairs(req) -> int:
+    """Voxel x scan-point pairs of a request (the throughput unit)."""
+    X, Y, Z, N, _ = req.dims()
+    return X * Y * Z * N
+
+
+@dataclasses.dataclass
+class Ticket:
+    """Handle of a submitted request."""
+    request: object
+    id: int
+    t_submit: float
+    t_admit: Optional[float] = None
+    t_done: Optional[float] = None
+    error: Optional[str] = None
+    _host_out: Optional[torch.Tensor] = None
+    _release: Optional[object] = None
+
+    @property
+    def done(self) -> bool:
+        return self.t_done is not None
+
+    @property
+    def latency(self) -> Optional[float]:
+        return None if self.t_done is None else self.t_done - self.t_submit
+
+    def result(self) -> torch.Tensor:
+        """The output in pinned host memory (valid until release())."""
+        if self.error is not None:
+            raise RuntimeError(self.error)
+        if self._host_out is None:
+            raise RuntimeError("request has not completed")
+        return self._host_out
+
+    def release(self) -> None:
+        """Return the result's pinned block to the engine."""
+        if self._release is not None:
+            self._release()
+            self._release = None
+            self._host_out = None
+
+
+class _Job:
+    """An admitted request: its arena blocks and the C job description."""
+
+    def __init__(self, ticket: Ticket, dev_blocks: List[int], host_block: int):
+        self.ticket = ticket
+        self.dev_blocks = dev_blocks
+        self.host_block = host_block
+        self.copies_in: List[Tuple[torch.Tensor, torch.Tensor]] = []
+        self.inputs: List[torch.Tensor] = []
+        self.out_dev: Optional[torch.Tensor] = None
+        self.out_host: Optional[torch.Tensor] = None
+        self.cjob = _lib.ConfocalJob()
+
+
+class _Lane:
+    """A stream with its own step table (pinned host + device) and graph."""
+
+    def __init__(self, device: torch.device, table_bytes: int):
+        self.stream = torch.cuda.Stream(device=device)
+        self.host_table = torch.zeros(table_bytes, dtype=torch.uint8, pin_memory=True)
+        self.dev_table = torch.zeros(table_bytes, dtype=torch.uint8, device=device)
+        self.graph: Optional[torch.cuda.CUDAGraph] = None
+        self.jobs: List[_Job] = []
+        self.event: Optional[torch.cuda.Event] = None
+        self.steps = 0
+
+
+class ServingEngine:
+    """Continuous batching of confocal render / backprojection requests.
+
+    device_budget  bytes of HBM for request data and scratch (one allocation)
+    host_budget    bytes of pinned host memory for results (one allocation)
+    lanes          steps that may be queued on the device at once
+    use_graphs     replay each lane's step as a captured HIP graph
+    """
+
+    def __init__(self, device_budget: int = 16 << 30, host_budget: int = 4 << 30,
+                 lanes: int = 2, max_jobs: int = _lib.TOTRI_BATCH_MAX_JOBS,
+                 use_graphs: bool = True, device: Optional[torch.device] = None):
+        if not torch.cuda.is_available():
+            raise RuntimeError("ServingEngine needs a GPU")
+        self.lib = _lib.load()
+        self.device = device or torch.device("cuda", torch.cuda.current_device())
+        if not 1 <= max_jobs <= _lib.TOTRI_BATCH_MAX_JOBS:
+            raise ValueError(f"max_jobs must be in [1, {_lib.TOTRI_BATCH_MAX_JOBS}]")
+        self.max_jobs = max_jobs
+        self.use_graphs = use_graphs
+        self.table_bytes = int(self.lib.totri_confocal_batch_table_bytes())
+        self.dev = TensorArena(device_budget, self.device, align=_ALIGN)
+        self.host = TensorArena(host_budget, self.device, pinned=True, align=_ALIGN)
+        self.lanes = [_Lane(self.device, self.table_bytes) for _ in range(max(1, lanes))]
+        self.queue: Deque[Ticket] = deque()
+        self.admitted: Deque[_Job] = deque()
+        self._next_id = 0
+        self.steps = 0
+        self.jobs_per_step: List[int] = []
+
+    # -- submission -------------------------------------------------------
+    def submit(self, request, t_submit: Optional[float] = None) -> Ticket:
+        self._validate(request)
+        tk = Ticket(request, self._next_id, time.perf_counter() if t_submit is None else t_submit)
+        self._next_id += 1
+        self.queue.append(tk)
+        return tk
+
+    def _validate(self, req) -> None:
+        if not isinstance(req, (RenderRequest, BackprojectRequest)):
+            raise TypeError(f"unsupported request {type(req)}")
+        X, Y, Z, N, T = req.dims()
+        mp = req.measurement_points
+        if mp.ndim != 2 or mp.shape[0] != 3 or mp.dtype != torch.float32:
+            raise ValueError("measurement_points must be a float32 [3, N] tensor")
+        data = req.volume if isinstance(req, RenderRequest) else req.transient
+        if data.dtype != torch.float32:
+            raise TypeError("request data must be float32")
+        if isinstance(req, RenderRequest) and data.ndim != 3:
+            raise ValueError("volume must be [X, Y, Z]")
+        if isinstance(req, BackprojectRequest) and (data.ndim != 2 or data.shape[0] != N):
+            raise ValueError("transient must be [N, T] with N = measurement points")
+        if min(X, Y, N, T) < 1 or Z < 2:
+            raise ValueError("requests need X, Y, N, T >= 1 and Z >= 2")
+        for t in (mp, data):
+            if t.is_cuda and t.device != self.device:
+                raise ValueError(f"tensor on {t.device}, engine on {self.device}")
+
+    # -- admission --------------------------------------------------------
+    def _footprint(self, req) -> Tuple[List[int], int, int]:
+        """Device block sizes [inputs copied in..., output, scratch], the
+        host result size and the scratch size."""
+        X, Y, Z, N, T = req.dims()
+        cj = _lib.ConfocalJob(kind=req.kind, X=X, Y=Y, Z=Z, N=N, T=T,
+                              volume_origin=_lib.f3(req.volume_bbox_origin),
+                              volume_size=_lib.f3(req.volume_bbox_size),
+                              bin_width=float(req.bin_width), bin_offset=float(req.bin_offset))
+        ws = int(self.lib.totri_confocal_job_workspace(ctypes.byref(cj)))
+        data = req.volume if isinstance(req, RenderRequest) else req.transient
+        sizes = []
+        for t in (req.measurement_points, data):
+            if not t.is_cuda:
+                sizes.append(t.numel() * 4)
+        out = math.prod(req.output_shape()) * 4
+        sizes.append(out)
+        sizes.append(ws)
+        return sizes, out, ws
+
+    def _admit(self, now: float) -> None:
+        while self.queue:
+            tk = self.queue[0]
+            req = tk.request
+            sizes, out_bytes, ws = self._footprint(req)
+            if not self.dev.arena.fits(sizes) or out_bytes > self.host.arena.capacity:
+                self.queue.popleft()
+                tk.error = "request exceeds the engine's memory budget"
+                tk.t_done = now
+                continue
+            blocks = self.dev.arena.alloc_many(sizes)
+            if blocks is None:
+                return
+            hb = self.host.arena.alloc(out_bytes)
+            if hb is None:
+                for b in blocks:
+                    self.dev.arena.free(b)
+                return
+            self.queue.popleft()
+            tk.t_admit = now
+            self.admitted.append(self._make_job(tk, blocks, hb, ws))
+
+    def _make_job(self, tk: Ticket, blocks: List[int], hb: int, ws: int) -> _Job:
+        req = tk.request
+        job = _Job(tk, blocks, hb)
+        X, Y, Z, N, T = req.dims()
+        data = req.volume if isinstance(req, RenderRequest) else req.transient
+        k = 0
+        placed = []
+        for t in (req.measurement_points, data):
+            if t.is_cuda:
+                placed.append(t.contiguous())
+            else:
+                dst = self.dev.view(blocks[k], t.shape)
+                job.copies_in.append((dst, t))
+                placed.append(dst)
+                k += 1
+        mp_dev, data_dev = placed
+        out_shape = req.output_shape()
+        job.out_dev = self.dev.view(blocks[k], out_shape)
+        job.out_host = self.host.view(hb, out_shape)
+        ws_ptr = self.dev.ptr(blocks[k + 1]) if ws > 0 else None
+        cj = job.cjob
+        cj.kind = req.kind
+        cj.reserved = 0
+        cj.measurement_points = mp_dev.data_ptr()
+        if isinstance(req, RenderRequest):
+            cj.volume = data_dev.data_ptr()
+            cj.transient = job.out_dev.data_ptr()
+        else:
+            cj.volume = job.out_dev.data_ptr()
+            cj.transient = data_dev.data_ptr()
+        cj.X, cj.Y, cj.Z, cj.N, cj.T = X, Y, Z, N, T
+        cj.volume_origin = _lib.f3(req.volume_bbox_origin)
+        cj.volume_size = _lib.f3(req.volume_bbox_size)
+        cj.bin_width = float(req.bin_width)
+        cj.bin_offset = float(req.bin_offset)
+        cj.workspace = ws_ptr
+        cj.workspace_bytes = ws
+        # keep device-resident inputs alive while the job runs
+        job.inputs = placed
+        return job
+
+    # -- steps --------------------------------------------------------------
+    def _launch(self, lane: _Lane) -> None:
+        jobs = []
+        while self.admitted and len(jobs) < self.max_jobs:
+            jobs.append(self.admitted.popleft())
+        arr = (_lib.ConfocalJob * len(jobs))(*[j.cjob for j in jobs])
+        rc = self.lib.totri_confocal_batch_prepare(arr, len(jobs), lane.host_table.data_ptr(),
+                                                   self.table_bytes)
+        _lib.check(rc, "totri_confocal_batch_prepare")
+        st = lane.stream
+        # the lane's host table may be rewritten only after its last step's
+        # upload; the lane is idle here, so that upload has completed.
+        with torch.cuda.stream(st):
+            for j in jobs:
+                for dst, src in j.copies_in:
+                    dst.copy_(src, non_blocking=True)
+            if self.use_graphs:
+                if lane.graph is None:
+                    lane.graph = torch.cuda.CUDAGraph()
+                    with torch.cuda.graph(lane.graph, stream=st):
+                        self._run_step(lane)
+                lane.graph.replay()
+            else:
+                self._run_step(lane)
+            for j in jobs:
+                j.out_host.copy_(j.out_dev, non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record(st)
+        lane.jobs = jobs
+        lane.event = ev
+        lane.steps += 1
+        self.steps += 1
+        self.jobs_per_step.append(len(jobs))
+
+    def _run_step(self, lane: _Lane) -> None:
+        rc = self.lib.totri_confocal_batch_run(lane.host_table.data_ptr(),
+                                               lane.dev_table.data_ptr(), self.table_bytes,
+                                               lane.stream.cuda_stream)
+        _lib.check(rc, "totri_confocal_batch_run")
+
+    def _retire(self, lane: _Lane, now: float) -> None:
+        for j in lane.jobs:
+            for b in j.dev_blocks:
+                self.dev.arena.free(b)
+            tk = j.ticket
+            tk._host_out = j.out_host
+            hb = j.host_block
+            tk._release = (lambda hb=hb: self.host.arena.free(hb))
+            tk.t_done = now
+        lane.jobs = []
+        lane.event = None
+
+    def poll(self) -> int:
+        """Retire finished steps, admit queued requests, start steps on idle
+        lanes.  Returns the number of requests still queued or running."""
+        now = time.perf_counter()
+        for lane in self.lanes:
+            if lane.event is not None and lane.event.query():
+                self._retire(lane, now)
+        self._admit(now)
+        for lane in self.lanes:
+            if not self.admitted:
+                break
+            if lane.event is None:
+                self._launch(lane)
+        return self.pending()
+
+    def pending(self) -> int:
+        running = sum(len(l.jobs) for l in self.lanes)
+        return len(self.queue) + len(self.admitted) + running
+
+    def drain(self, sleep: float = 1e-4) -> None:
+        while self.poll():
+            time.sleep(sleep)
+
+    def run(self, requests: Sequence, arrivals: Sequence[float], sleep: float = 1e-4,
+            on_done=None) -> List[Ticket]:
+        """Submit each request at its arrival time (seconds from now), serve
+        until all are done, and return the tickets in request order.
+
+        `on_done(ticket)` is called once per finished request, after which
+        its result is released; without it, the caller releases results
+        (the pinned arena must hold every unreleased result)."""
+        order = sorted(range(len(requests)), key=lambda k: arrivals[k])
+        tickets: List[Optional[Ticket]] = [None] * len(requests)
+        t0 = time.perf_counter()
+        nxt = 0
+        while nxt < len(order) or self.pending():
+            now = time.perf_counter()
+            while nxt < len(order) and t0 + arrivals[order[nxt]] <= now:
+                k = order[nxt]
+                tickets[k] = self.submit(requests[k], t_submit=t0 + arrivals[k])
+                nxt += 1
+            self.poll()
+            if on_done is not None:
+                for tk in tickets:
+                    if tk is not None and tk.done and tk._release is not None:
+                        on_done(tk)
+                        tk.release()
+            time.sleep(sleep)
+        return tickets  # type: ignore[return-value]
