@@ -130,12 +130,22 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 256 MiB device-to-device copy (a known-traffic "
                          "kernel that calibrates FETCH_SIZE/WRITE_SIZE counters)")
+    ap.add_argument("--workload", choices=["confocal", "serve"], default="confocal",
+                    help="confocal: the render + gradient step (default); serve: the "
+                         "request mix through the serving engine")
+    ap.add_argument("--requests", type=int, default=48, help="serve: requests per run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU; --gpus must be 1 "
                                    "(the multi-GPU sharding is a design note, not built)"}))
         sys.exit(2)
+    if args.workload == "serve":
+        serve_main(args)
+    else:
+        confocal_main(args)
 
+
+def confocal_main(args):
     import torch
     from totrilib_amd.totri_cpp.render import (volume_confocal_forward,
                                                volume_confocal_grad_volume)
@@ -233,6 +243,160 @@ def main():
                      "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32, "traffic": traffic},
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+# ---------------------------------------------------------------------------
+# Serving mix
+# ---------------------------------------------------------------------------
+SERVE = dict(seed=6, sides=(32, 64, 128), bins=(512, 1024, 2048), res=(64, 128, 192),
+             occupancy=(0.01, 0.05), path_length=5.12, device_budget=16 << 30,
+             host_budget=6 << 30, loads=(0.5, 1.0, 2.0))
+
+
+def serve_requests(torch, dev, count):
+    """Seeded mix: render or backproject (50/50); scan side, bins and volume
+    side uniform over SERVE's sets; occupancy U(1 %, 5 %); one-way bin width
+    2.56 m / T (5.12 m of round-trip path over the bins).  Inputs are put in
+    pinned host memory, as a service receives them.  A backprojection's input
+    is the noisy transient of its own sparse albedo."""
+    from tests.cases import sparse_albedo, wall_grid
+    from totrilib_amd.serve import BackprojectRequest, RenderRequest
+    from totrilib_amd.totri_cpp.render import volume_confocal_forward
+    rng = np.random.default_rng(SERVE["seed"])
+    reqs = []
+    grids = {}
+    for k in range(count):
+        kind = "render" if rng.random() < 0.5 else "backproject"
+        n = int(rng.choice(SERVE["sides"]))
+        T = int(rng.choice(SERVE["bins"]))
+        r = int(rng.choice(SERVE["res"]))
+        p = float(rng.uniform(*SERVE["occupancy"]))
+        w = SERVE["path_length"] / 2 / T
+        if n not in grids:
+            grids[n] = torch.from_numpy(wall_grid(n)).pin_memory()
+        mp = grids[n]
+        vol = torch.from_numpy(sparse_albedo((r, r, r), p, 1000 + k))
+        if kind == "render":
+            reqs.append(RenderRequest(vol.pin_memory(), mp, ORIGIN, SIZE, T, w, 0.0))
+        else:
+            tr = torch.empty((n * n, T), device=dev)
+            volume_confocal_forward(vol.to(dev), mp.to(dev), tr, ORIGIN, SIZE, w, 0.0)
+            g = torch.Generator(device=dev).manual_seed(2000 + k)
+            tr += 1e-3 * tr.max() * torch.randn(tr.shape, device=dev, generator=g)
+            reqs.append(BackprojectRequest(tr.cpu().pin_memory(), mp, ORIGIN, SIZE,
+                                           (r, r, r), w, 0.0))
+    torch.cuda.synchronize()
+    return reqs
+
+
+def per_request_baseline(torch, dev, reqs):
+    """Plain per-request launches of the same kernels: allocate, copy in,
+    launch, synchronise, copy out, one request at a time."""
+    from totrilib_amd.serve import RenderRequest
+    from totrilib_amd.totri_cpp.render import (volume_confocal_forward,
+                                               volume_confocal_grad_volume)
+    lat = []
+    t0 = time.perf_counter()
+    for r in reqs:
+        ts = time.perf_counter()
+        mp = r.measurement_points.to(dev)
+        if isinstance(r, RenderRequest):
+            out = torch.empty(r.output_shape(), device=dev)
+            volume_confocal_forward(r.volume.to(dev), mp, out, r.volume_bbox_origin,
+                                    r.volume_bbox_size, r.bin_width, r.bin_offset)
+        else:
+            out = torch.empty(r.output_shape(), device=dev)
+            volume_confocal_grad_volume(out, mp, r.transient.to(dev), r.volume_bbox_origin,
+                                        r.volume_bbox_size, r.bin_width, r.bin_offset)
+        torch.cuda.synchronize()
+        out.cpu()
+        lat.append(time.perf_counter() - ts)
+    return time.perf_counter() - t0, np.array(lat)
+
+
+def serve_main(args):
+    import torch
+    from totrilib_amd.serve import ServingEngine, request_pairs
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    reqs = serve_requests(torch, dev, args.requests)
+    pairs = float(sum(request_pairs(r) for r in reqs))
+    print(f"[serve] {len(reqs)} requests, {pairs:.3e} pairs", file=sys.stderr, flush=True)
+
+    # warm both paths (allocator caches, graph capture) on small requests
+    small = [r for r in reqs if request_pairs(r) <= 64 ** 3 * 32 * 32][:4] or reqs[:1]
+    per_request_baseline(torch, dev, small)
+    eng = ServingEngine(device_budget=SERVE["device_budget"], host_budget=SERVE["host_budget"],
+                        lanes=2)
+    eng.run(small, [0.0] * len(small), on_done=lambda tk: None)
+
+    base_t, base_lat = per_request_baseline(torch, dev, reqs)
+    print(f"[serve] per-request baseline {base_t:.2f} s", file=sys.stderr, flush=True)
+
+    def serve(arrivals):
+        steps0 = eng.steps
+        jobs0 = len(eng.jobs_per_step)
+        t0 = time.perf_counter()
+        tks = eng.run(reqs, list(arrivals), on_done=lambda tk: None)
+        t_end = max(tk.t_done for tk in tks)
+        lat = np.array([tk.latency for tk in tks]) * 1e3
+        assert all(tk.error is None for tk in tks), [tk.error for tk in tks if tk.error]
+        jps = eng.jobs_per_step[jobs0:]
+        return dict(span=t_end - t0, p50_ms=float(np.percentile(lat, 50)),
+                    p99_ms=float(np.percentile(lat, 99)), steps=eng.steps - steps0,
+                    mean_jobs_per_step=float(np.mean(jps)) if jps else 0.0)
+
+    sat = serve(np.zeros(len(reqs)))
+    print(f"[serve] saturated {sat['span']:.2f} s", file=sys.stderr, flush=True)
+    rate_sat = len(reqs) / sat["span"]
+    rng = np.random.default_rng(SERVE["seed"] + 1)
+    rates = []
+    for load in SERVE["loads"]:
+        lam = load * rate_sat
+        arr = np.cumsum(rng.exponential(1.0 / lam, len(reqs)))
+        arr -= arr[0]
+        r = serve(arr)
+        rates.append(dict(load=load, rate_rps=lam, p50_ms=r["p50_ms"], p99_ms=r["p99_ms"],
+                          pairs_per_s=pairs / r["span"], steps=r["steps"],
+                          mean_jobs_per_step=r["mean_jobs_per_step"]))
+        print(f"[serve] load {load}: p50 {r['p50_ms']:.1f} ms p99 {r['p99_ms']:.1f} ms",
+              file=sys.stderr, flush=True)
+
+    value = pairs / sat["span"]
+    base_value = pairs / base_t
+    line = {
+        "metric": "serving: voxel·scanpoint pairs/s of a seeded render/backproject request "
+                  "mix through the continuous-batching engine; p50/p99 latency",
+        "value": value,
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": sat["steps"],
+        "warmup": len(small),
+        "ms_per_step": sat["span"] * 1e3 / max(sat["steps"], 1),
+        "p50_ms": sat["p50_ms"],
+        "p99_ms": sat["p99_ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded sparse albedo, noisy rendered transients, make_wall_grid scans)",
+        "config": {"workload": f"{len(reqs)} requests, 50/50 render (R1) / backproject (A1); "
+                               "scan 32^2|64^2|128^2, bins 512|1024|2048 over 5.12 m round "
+                               "trip, volume 64^3|128^3|192^3, occupancy U(1%,5%); inputs "
+                               "in pinned host memory, results copied back",
+                   "requests": len(reqs), "pairs": pairs,
+                   "device_budget_bytes": SERVE["device_budget"], "lanes": 2},
+        "saturated": sat,
+        "per_request_baseline": {"value": base_value, "unit": "pairs/s",
+                                 "p50_ms": float(np.percentile(base_lat, 50) * 1e3),
+                                 "p99_ms": float(np.percentile(base_lat, 99) * 1e3),
+                                 "how": "one request at a time: torch allocations, H2D, "
+                                        "single-request entry point, synchronize, D2H"},
+        "speedup_vs_per_request": value / base_value,
+        "rates": rates,
     }
     print(json.dumps(line))
 
