@@ -59,18 +59,26 @@ def num_threads() -> int:
     return int(lib().oracle_num_threads())
 
 
-def r1_forward(volume, mp, T, origin, size, w, off, mode=FP32, i0=0, i1=None):
-    """R1 transient [i1-i0, T] (volume [X,Y,Z], mp [3,N])."""
+def r1_forward(volume, mp, T, origin, size, w, off, mode=FP32, i0=0, i1=None, cols=None):
+    """R1 transient [i1-i0, T] (volume [X,Y,Z], mp [3,N]).  With `cols`
+    [n, 2] only those (xi, yi) columns are summed, in the listed order -- in
+    (xi, yi) index order that is the exact render of a volume that is zero
+    outside them."""
     vol = _f32(volume)
     m = _f32(mp)
     X, Y, Z = vol.shape
     N = m.shape[1]
     i1 = N if i1 is None else i1
-    out = np.zeros((N, T), dtype=np.float64)
+    out = np.zeros((max(i1 - i0, 0), T), dtype=np.float64)
+    if cols is None:
+        cp, nc = None, 0
+    else:
+        c = np.ascontiguousarray(np.asarray(cols, dtype=np.int32))
+        cp, nc = _p(c), c.shape[0]
     lib().oracle_r1_forward(_p(vol), X, Y, Z, _p(m), N, T, _f3(origin), _f3(size),
                             ctypes.c_float(w), ctypes.c_float(off), mode, _p(out),
-                            i0, i1)
-    return out[i0:i1]
+                            i0, i1, cp, nc)
+    return out
 
 
 def r1_decisions(idx, mp, X, Y, Z, origin, size, w, off):
@@ -214,15 +222,23 @@ def r2_adjoint(G, res, scan, laser, vs, ve, w, off, mode=FP32):
     return out
 
 
-def splat_forward(rbf, res, vs, ve, trunc=0.0, mode=FP32):
-    """S1 (rbf2volume_cuda.cu:49-93): rbf [B,N,4+A] -> [B,1+A,D,H,W]."""
+def splat_forward(rbf, res, vs, ve, trunc=0.0, mode=FP32, vox=None):
+    """S1 (rbf2volume_cuda.cu:49-93): rbf [B,N,4+A] -> [B,1+A,D,H,W].  With
+    `vox` (flat indices b*D*H*W + (z*H + y)*W + x) only those voxels are
+    computed; the others are left 0."""
     r = _f32(rbf)
     B, N, C = r.shape
     A = C - 4
     D, H, W = res
     out = np.zeros((B, 1 + A, D, H, W))
+    if vox is None:
+        vp, nv = None, 0
+    else:
+        v = np.ascontiguousarray(np.asarray(vox, dtype=np.int64))
+        vp, nv = _p(v), v.shape[0]
     lib().oracle_splat_forward(_p(r), B, N, A, D, H, W, _f3(vs), _f3(ve),
-                               ctypes.c_float(trunc), mode, _p(out))
+                               ctypes.c_float(trunc), mode, _p(out), vp,
+                               ctypes.c_longlong(nv))
     return out
 
 

@@ -134,30 +134,35 @@ static inline void r1_weights64(int ti, float w, float off, float wx, float wy,
 }
 
 /* transient[i][t] for i in [i0, i1).  volume [X][Y][Z], mp [3][N],
- * out [N][T] (double). */
+ * out [i1 - i0][T] (double).  With cols != NULL only the ncols listed (xi, yi)
+ * columns are summed: the exact render of a volume that is zero elsewhere
+ * (a zero column adds exact zeros in every mode). */
 void oracle_r1_forward(const float* vol, int X, int Y, int Z,
                        const float* mp, int N, int T,
                        const float* origin, const float* size,
                        float w, float off, int mode,
-                       double* out, int i0, int i1) {
+                       double* out, int i0, int i1, const int* cols, int ncols) {
   const float ox = origin[0], oy = origin[1], oz = origin[2];
   const float szx = size[0], szy = size[1], szz = size[2];
   const float s = (float)Z / szz; /* :30 */
   const double s64 = (double)Z / (double)szz;
   if (i0 < 0) i0 = 0;
   if (i1 > N) i1 = N;
-#pragma omp parallel for schedule(dynamic, 1)
+#pragma omp parallel for collapse(2) schedule(dynamic, 16)
   for (int i = i0; i < i1; ++i) {
-    const float wx = mp[0 * N + i], wy = mp[1 * N + i], wz = mp[2 * N + i];
     for (int ti = 0; ti < T; ++ti) {
+      const float wx = mp[0 * N + i], wy = mp[1 * N + i], wz = mp[2 * N + i];
       const float tc = r1_tc32(ti, w, off);
       float v32 = 0.f;
       double v64 = 0.0;
       const double tc64 = (double)ti * (double)w + (double)off;
-      for (int xi = 0; xi < X; ++xi) {
+      const int nc = cols ? ncols : X * Y;
+      for (int c = 0; c < nc; ++c) {
+        const int xi = cols ? cols[2 * c] : c / Y;
+        const int yi = cols ? cols[2 * c + 1] : c % Y;
         const float x = r1_x32(xi, ox, szx, X);
         const double x64 = (double)ox + ((double)xi + 0.5) * (double)szx / (double)X;
-        for (int yi = 0; yi < Y; ++yi) {
+        {
           const float y = r1_x32(yi, oy, szy, Y);
           const float* col = vol + ((size_t)xi * Y + yi) * Z;
           if (mode == 2) {
@@ -195,7 +200,7 @@ void oracle_r1_forward(const float* vol, int X, int Y, int Z,
           }
         }
       }
-      out[(size_t)i * T + ti] = (mode == 0) ? (double)v32 : v64;
+      out[(size_t)(i - i0) * T + ti] = (mode == 0) ? (double)v32 : v64;
     }
   }
 }
@@ -948,18 +953,29 @@ static inline float gauss32(float x, float y, float z, float xr, float yr,
   return expf(e);
 }
 
+/* S1 over the whole grid, or (vox != NULL) only at the nvox listed flat
+ * indices b * D*H*W + (z * H + y) * W + x (other outputs stay untouched). */
 void oracle_splat_forward(const float* rbf, int B, int N, int A, int D,
                           int H, int W, const float* vs, const float* ve,
-                          float trunc, int mode, double* out) {
+                          float trunc, int mode, double* out,
+                          const long long* vox, long long nvox) {
   const int C = 4 + A;
   const size_t V = (size_t)D * H * W;
-#pragma omp parallel for collapse(2) schedule(dynamic, 1)
-  for (int b = 0; b < B; ++b) {
-    for (int zi = 0; zi < D; ++zi) {
+  const long long total = vox ? nvox : (long long)B * D * H;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (long long job = 0; job < total; ++job) {
+    const long long flat = vox ? vox[job] : -1;
+    const int b = vox ? (int)(flat / (long long)V) : (int)(job / ((long long)D * H));
+    const int zi = vox ? (int)((flat % (long long)V) / ((long long)H * W))
+                       : (int)((job / H) % D);
+    const int y_lo = vox ? (int)((flat % ((long long)H * W)) / W) : (int)(job % H);
+    const int x_lo = vox ? (int)(flat % W) : 0;
+    const int x_hi = vox ? x_lo + 1 : W;
+    {
       double* a64 = (double*)malloc(sizeof(double) * (A > 0 ? A : 1));
       float* a32 = (float*)malloc(sizeof(float) * (A > 0 ? A : 1));
-      for (int yi = 0; yi < H; ++yi) {
-        for (int xi = 0; xi < W; ++xi) {
+      for (int yi = y_lo; yi <= y_lo; ++yi) {
+        for (int xi = x_lo; xi < x_hi; ++xi) {
           const float x = splat_coord32(xi, W, vs[0], ve[0]);
           const float y = splat_coord32(yi, H, vs[1], ve[1]);
           const float z = splat_coord32(zi, D, vs[2], ve[2]);
@@ -1031,9 +1047,29 @@ void oracle_splat_grad(const float* rbf, int B, int N, int A,
       for (int c = 0; c < C; ++c) { acc64[c] = 0; acc32[c] = 0; }
       const double* vb = vol + (size_t)b * (1 + A) * V;
       const double* gb = vgrad + (size_t)b * (1 + A) * V;
-      for (int zi = 0; zi < D; ++zi)
-        for (int yi = 0; yi < H; ++yi)
-          for (int xi = 0; xi < W; ++xi) {
+      /* With truncation only voxels inside the support box can pass the
+       * radius test; the box is widened by two grid steps so it contains
+       * every voxel that passes, and the loop order is unchanged. */
+      int lo[3] = {0, 0, 0}, hi[3] = {W - 1, H - 1, D - 1};
+      if (trunc > 0.f) {
+        const int n3[3] = {W, H, D};
+        const float c3[3] = {xr, yr, zr};
+        for (int d = 0; d < 3; ++d) {
+          if (n3[d] < 2) continue;
+          const double step = ((double)ve[d] - vs[d]) / (n3[d] - 1);
+          if (step == 0.0) continue;
+          const double rad = (double)trunc * sg;
+          double a = (c3[d] - rad - vs[d]) / step, e = (c3[d] + rad - vs[d]) / step;
+          if (a > e) { const double t = a; a = e; e = t; }
+          a = floor(a) - 2;
+          e = ceil(e) + 2;
+          lo[d] = a < 0 ? 0 : (a > n3[d] - 1 ? n3[d] : (int)a);
+          hi[d] = e > n3[d] - 1 ? n3[d] - 1 : (e < 0 ? -1 : (int)e);
+        }
+      }
+      for (int zi = lo[2]; zi <= hi[2]; ++zi)
+        for (int yi = lo[1]; yi <= hi[1]; ++yi)
+          for (int xi = lo[0]; xi <= hi[0]; ++xi) {
             const size_t vi = ((size_t)zi * H + yi) * W + xi;
             const float x = splat_coord32(xi, W, vs[0], ve[0]);
             const float y = splat_coord32(yi, H, vs[1], ve[1]);

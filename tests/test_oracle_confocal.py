@@ -271,3 +271,22 @@ def test_measurement_point_gradient_matches_finite_differences():
         lm = np.sum(r1_numpy64(vol, mpm, T, ORIGIN, SIZE, w, off) * G)
         fd = (lp - lm) / (2 * h)
         assert abs(fd - ana[c]) < 2e-2 * np.abs(ana).max(), (c, fd, ana[c])
+
+
+def test_column_restricted_render_equals_full_render():
+    """r1_forward(cols=...) -- used for full-size finite differences -- is the
+    render of the volume with every other column zero, in every mode."""
+    rng = np.random.default_rng(29)
+    X, Y, Z, T = 9, 7, 12, 90
+    mp = np.stack([rng.uniform(-1, 1, 11), rng.uniform(-1, 1, 11),
+                   np.zeros(11)]).astype(np.float32)
+    cols = np.array([[2, 3], [4, 6], [8, 0]])  # index order
+    vol = np.zeros((X, Y, Z), np.float32)
+    for xi, yi in cols:
+        vol[xi, yi] = rng.random(Z)
+    for mode in (O.FP32, O.PINNED64, O.PURE64):
+        full = O.r1_forward(vol, mp, T, ORIGIN, SIZE, 0.03, 0.01, mode=mode)
+        sub = O.r1_forward(vol, mp, T, ORIGIN, SIZE, 0.03, 0.01, mode=mode, cols=cols)
+        rows = O.r1_forward(vol, mp, T, ORIGIN, SIZE, 0.03, 0.01, mode=mode, i0=3, i1=7)
+        assert np.count_nonzero(full) > 0
+        assert np.array_equal(full, sub) and np.array_equal(full[3:7], rows)

@@ -114,3 +114,52 @@ def test_splat_gradient_matches_finite_differences():
             lm = np.sum(O.splat_forward(rm32, res, VS, VE, mode=O.PURE64) * G)
             fd = (lp - lm) / step
             assert abs(fd - ana[0, i, c]) < 2e-4 * max(1.0, abs(ana[0, i, c])), (i, c, fd, ana[0, i, c])
+
+
+def test_splat_voxel_subset_and_support_box_match_full_grid():
+    """The subset form of the splat and the support-box walk of its gradient
+    (used at full size) give the full-grid results exactly."""
+    rng = np.random.default_rng(44)
+    N = 40
+    rbf = np.concatenate([rng.uniform(-1, 1, (1, N, 2)), rng.uniform(0.5, 2.5, (1, N, 1)),
+                          rng.uniform(0.05, 0.3, (1, N, 1)), rng.random((1, N, 1))], 2).astype(np.float32)
+    res = (14, 12, 11)
+    for trunc in (0.0, 3.0):
+        full = O.splat_forward(rbf, res, VS, VE, trunc=trunc, mode=O.PINNED64)
+        vox = rng.choice(np.prod(res), 200, replace=False)
+        sub = O.splat_forward(rbf, res, VS, VE, trunc=trunc, mode=O.PINNED64, vox=vox)
+        flat_full = full.reshape(1, 2, -1)[:, :, vox]
+        assert np.array_equal(sub.reshape(1, 2, -1)[:, :, vox], flat_full)
+        assert np.count_nonzero(np.delete(sub.reshape(1, 2, -1), vox, axis=2)) == 0
+    # One RBF at a time: its truncated gradient (support-box walk) equals the
+    # untruncated gradient (whole-grid walk) of an upstream gradient that is
+    # zero outside the truncation sphere, where the sphere test is restated
+    # in float32 (zero terms add exactly).
+    D, H, W = res
+    def coord(i, n, a, b):
+        t = np.float32(b - a) * np.float32(i)
+        return np.float32(a) + t / np.float32(n - 1)
+    z = coord(np.arange(D), D, VS[2], VE[2])[:, None, None]
+    y = coord(np.arange(H), H, VS[1], VE[1])[None, :, None]
+    x = coord(np.arange(W), W, VS[0], VE[0])[None, None, :]
+    for i in range(0, N, 7):
+        one = rbf[:, i:i + 1]
+        vol = O.splat_forward(one, res, VS, VE, mode=O.PINNED64)
+        G = rng.standard_normal(vol.shape)
+        xr, yr, zr, sg = (np.float32(v) for v in one[0, 0, :4])
+        r2 = (xr - x) * (xr - x)
+        r2 = r2 + (yr - y) * (yr - y)
+        r2 = r2 + (zr - z) * (zr - z)
+        lim = np.float32(3.0) * sg
+        inside = ~(r2 > lim * lim)
+        assert 0 < inside.sum() < inside.size
+        Gm = G * inside[None, None]
+        a = O.splat_grad(one, vol, G, VS, VE, trunc=3.0, mode=O.FP32)
+        b = O.splat_grad(one, vol, Gm, VS, VE, trunc=0.0, mode=O.FP32)
+        assert np.array_equal(a, b), i
+    # a wide truncation walks the whole grid: identical to no truncation
+    full = O.splat_forward(rbf, res, VS, VE, mode=O.PINNED64)
+    G = rng.standard_normal(full.shape)
+    a = O.splat_grad(rbf, full, G, VS, VE, trunc=0.0, mode=O.PINNED64)
+    b = O.splat_grad(rbf, full, G, VS, VE, trunc=1e4, mode=O.PINNED64)
+    assert np.array_equal(a, b)

@@ -41,6 +41,7 @@ namespace {
 constexpr int kWaves = 8;            // scan points per R1 workgroup
 constexpr int kChunkVoxels = 4096;   // voxels per R1 LDS chunk buffer (32 KiB of pairs)
 constexpr int kSlab = 32;            // A1 z-slab depth
+constexpr int kFlush = 32;           // A1 scan points per first-level sum
 
 struct ConfGeom {
   int X, Y, Z, N, T;
@@ -614,6 +615,13 @@ __device__ __forceinline__ void a1_item(
 
   for (int r = 0; r < kSlab + 2; ++r) acc[r * 64 + lane] = 0.f;
   // Only this lane touches its slots, so no barrier is needed.
+  // Two-level sums: the LDS rows hold the sum over a block of kFlush scan
+  // points, which is then added to a per-lane register total and cleared,
+  // so no voxel's sum is one long FP32 chain over every scan point.  (Rows
+  // 0 and kSlab + 1 belong to the neighbouring slabs and are never read.)
+  float tot[kSlab];
+#pragma unroll
+  for (int r = 0; r < kSlab; ++r) tot[r] = 0.f;
 
   const int per = (g.N + n_ranges - 1) / n_ranges;
   const int i_begin = range * per;
@@ -621,6 +629,13 @@ __device__ __forceinline__ void a1_item(
   float* my = acc + rmin * 64 + lane;
 
   for (int i = i_begin; i < i_end; ++i) {
+    if (i > i_begin && (i - i_begin) % kFlush == 0) {
+#pragma unroll
+      for (int r = 0; r < kSlab; ++r) {
+        tot[r] += acc[(r + 1) * 64 + lane];
+        acc[(r + 1) * 64 + lane] = 0.f;
+      }
+    }
     const float wx = mp[i], wy = mp[g.N + i], wz = mp[2 * g.N + i];
     float c0 = (wz - g.oz) * g.s;
     c0 = c0 - 0.5f;
@@ -694,9 +709,10 @@ __device__ __forceinline__ void a1_item(
 
   if (cvalid) {
     float* out = dst + (((size_t)range * g.X + xi) * g.Y + yi) * g.Z;
-    for (int r = 1; r <= kSlab; ++r) {
-      const int z = z0 + r - 1;
-      if (z < g.Z) out[z] = acc[r * 64 + lane];
+#pragma unroll
+    for (int r = 0; r < kSlab; ++r) {
+      const int z = z0 + r;
+      if (z < g.Z) out[z] = tot[r] + acc[(r + 1) * 64 + lane];
     }
   }
 }
