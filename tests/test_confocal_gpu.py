@@ -282,3 +282,49 @@ def test_autograd_volume_and_points():
     em = rel_l2(m.grad.cpu().numpy(), O.a6(vol_np, mp_np, Gn, ORIGIN, SIZE, w, 0.0,
                                            mode=O.PINNED64))
     assert ev < TOL and em < TOL, (ev, em)
+
+
+@pytest.mark.parametrize("scan", ["mirror", "perturbed", "random"])
+def test_r1_mirror_quads_and_general_path(scan):
+    """A scan of mirror quads on a box centred at x = y = 0 takes the
+    symmetric kernel (one decision per four samples); one moved point or a
+    random scan takes the general kernel.  All match the oracle."""
+    rng = np.random.default_rng(31)
+    X, Y, Z, T = 32, 30, 40, 700
+    vol = sparse_albedo((X, Y, Z), 0.2, 32)
+    mp = wall_grid(8)
+    if scan == "perturbed":
+        mp[0, 5] = np.nextafter(mp[0, 5], np.float32(2))
+    elif scan == "random":
+        mp = np.stack([rng.uniform(-1, 1, 64), rng.uniform(-1, 1, 64),
+                       np.zeros(64)]).astype(np.float32)
+    w = 0.0045
+    got = gpu_r1(vol, mp, T, w, 0.0)
+    ref = O.r1_forward(vol, mp, T, ORIGIN, SIZE, w, 0.0, mode=O.PINNED64)
+    e = rel_l2(got, ref)
+    print(f"{scan}: rel-L2 vs FP64 pinned {e:.2e}")
+    assert e < TOL
+    assert np.count_nonzero(got) > 0
+
+
+@pytest.mark.parametrize("scan", ["mirror", "perturbed", "random"])
+def test_a1_mirror_pairs_and_general_path(scan):
+    """x-mirror pairs of scan points on a box centred at x = 0 take the
+    mirror kernel (one decision per two samples); otherwise the general
+    kernel runs.  Both match the oracle's exact adjoint."""
+    rng = np.random.default_rng(33)
+    X, Y, Z, T = 30, 26, 44, 600
+    mp = wall_grid(8)
+    if scan == "perturbed":
+        mp[1, 9] = np.nextafter(mp[1, 9], np.float32(2))   # breaks one x-pair
+        mp[0, 9] = np.nextafter(mp[0, 9], np.float32(2))
+    elif scan == "random":
+        mp = np.stack([rng.uniform(-1, 1, 64), rng.uniform(-1, 1, 64),
+                       np.zeros(64)]).astype(np.float32)
+    G = rng.standard_normal((mp.shape[1], T)).astype(np.float32)
+    w = 0.0045
+    got = gpu_a1(G, mp, (X, Y, Z), w, 0.0)
+    ref = O.a1_transpose(G, mp, X, Y, Z, ORIGIN, SIZE, w, 0.0, mode=O.PINNED64)
+    e = rel_l2(got, ref)
+    print(f"{scan}: rel-L2 vs FP64 pinned {e:.2e}")
+    assert e < TOL

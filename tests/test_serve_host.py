@@ -85,9 +85,14 @@ def test_job_workspace_matches_single_request_entry_points():
     # the query needs no buffers (it is made before they are allocated)
     b.volume = b.measurement_points = b.transient = None
     assert lib.totri_confocal_job_workspace(ctypes.byref(b)) == want
-    # enough scan points for one column group: an FP32 render needs none
+    # enough scan points for one column group and a box off the x = y = 0
+    # mirror (no mirror path): an FP32 render needs none
     r = job(_lib.TOTRI_JOB_RENDER, 16, 16, 16, 8 * 4096, 64)
+    r.volume_origin = _lib.f3((-0.9, -1, 0.5))
     assert lib.totri_confocal_job_workspace(ctypes.byref(r)) == 0
+    r.volume_origin = _lib.f3((-1, -1, 0.5))   # centred: scratch for mirror detection
+    assert lib.totri_confocal_job_workspace(ctypes.byref(r)) == \
+        lib.totri_volume_confocal_forward_workspace(16, 16, 16, 8 * 4096, 64) > 0
 
 
 def test_batch_prepare_validates_jobs():
@@ -95,6 +100,7 @@ def test_batch_prepare_validates_jobs():
     nb = lib.totri_confocal_batch_table_bytes()
     table = (ctypes.c_uint8 * nb)()
     ok = job(_lib.TOTRI_JOB_RENDER, 8, 8, 8, 4096, 64)
+    ok.volume_origin = _lib.f3((-0.9, -1, 0.5))   # no mirror path: needs no scratch
     arr = (_lib.ConfocalJob * 1)(ok)
     assert lib.totri_confocal_batch_prepare(arr, 1, table, nb) == _lib.TOTRI_OK
     assert lib.totri_confocal_batch_prepare(arr, 1, table, nb - 1) == _lib.TOTRI_EINVAL

@@ -372,14 +372,379 @@ __device__ __forceinline__ void r1_block(
   }
 }
 
+// `skip` (optional): a device flag; when it is set the launch does nothing
+// (the mirror-symmetric kernel below has done the work).
 template <int K>
 __global__ __launch_bounds__(kWaves * 64) void r1_forward_kernel(
     const float* __restrict__ vol, const float* __restrict__ mp,
     float* __restrict__ dst, ConfGeom g, int t_base, int n_groups, int chunk_cols,
-    int chunks_per_group) {
+    int chunks_per_group, const int* __restrict__ skip) {
+  if (skip && *skip) return;
   extern __shared__ float2 lds2[];
   r1_block<K>(vol, mp, dst, g, t_base, (int)(blockIdx.x % n_groups),
               (int)(blockIdx.x / n_groups), n_groups, chunk_cols, chunks_per_group, lds2);
+}
+
+// ---------------------------------------------------------------------------
+// Mirror-symmetric R1
+// ---------------------------------------------------------------------------
+// When the voxel grid is antisymmetric in x and y (cell centre X-1-xi is
+// exactly -centre(xi), likewise in y) and the scan points come in quads
+//   s0 = (wx, wy, wz), s1 = (-wx, wy, wz), s2 = (wx, -wy, wz), s3 = (-wx, -wy, wz)
+// (exact float values), the samples (s_m, column c_(m^f)) for m = 0..3 share
+// dx^2 and dy^2 bit for bit, where c_0 = (x, y), c_1 = (-x, y), c_2 = (x, -y),
+// c_3 = (-x, -y) are the mirror images of a column in the quadrant
+// xi < X/2, yi < Y/2.  Their decisions (zi, alpha) and weights are therefore
+// identical, so one decision serves four samples: family f of a column
+// quad has dx^2 = (wx - x_(f&1))^2 and dy^2 = (wy - y_(f&2))^2.
+//
+// The quads are found on the device (scan_mirror_kernel, scan_quads_kernel),
+// which writes a flag; the symmetric kernel runs when it is set, the general
+// kernel otherwise.  Results equal the general kernel's up to the FP32
+// summation order over columns.
+
+// For each scan point, the index of its x-, y- and xy-mirror image (the
+// first match; -1 if none), by comparing against every point.  One 256-thread
+// workgroup handles points block*256 .. block*256+255.
+__device__ __forceinline__ void scan_mirror_block(const float* __restrict__ mp, int N,
+                                                  int* __restrict__ partner, int block) {
+  __shared__ float sx[256], sy[256], sz[256];
+  const int i = block * 256 + threadIdx.x;
+  const int ii = i < N ? i : 0;
+  const float wx = mp[ii], wy = mp[N + ii], wz = mp[2 * N + ii];
+  int px = -1, py = -1, pxy = -1;
+  for (int j0 = 0; j0 < N; j0 += 256) {
+    __syncthreads();
+    const int jl = j0 + threadIdx.x;
+    if (jl < N) {
+      sx[threadIdx.x] = mp[jl];
+      sy[threadIdx.x] = mp[N + jl];
+      sz[threadIdx.x] = mp[2 * N + jl];
+    }
+    __syncthreads();
+    const int nj = min(256, N - j0);
+    for (int jj = 0; jj < nj; ++jj) {
+      const float x = sx[jj], y = sy[jj];
+      const bool zs = sz[jj] == wz;
+      const bool xm = x == -wx, xs = x == wx, ym = y == -wy, ys = y == wy;
+      if (px < 0 && zs && xm && ys) px = j0 + jj;
+      if (py < 0 && zs && xs && ym) py = j0 + jj;
+      if (pxy < 0 && zs && xm && ym) pxy = j0 + jj;
+    }
+  }
+  if (i < N) {
+    partner[i] = px;
+    partner[N + i] = py;
+    partner[2 * N + i] = pxy;
+  }
+}
+
+__global__ __launch_bounds__(256) void scan_mirror_kernel(const float* __restrict__ mp, int N,
+                                                          int* __restrict__ partner) {
+  scan_mirror_block(mp, N, partner, (int)blockIdx.x);
+}
+
+// Inclusive scan of one int per thread over a 1024-thread workgroup.
+__device__ __forceinline__ int block_scan_1024(int v, int* s_cnt) {
+  const int tid = threadIdx.x;
+  __syncthreads();
+  s_cnt[tid] = v;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const int u = tid >= off ? s_cnt[tid - off] : 0;
+    __syncthreads();
+    s_cnt[tid] += u;
+    __syncthreads();
+  }
+  return s_cnt[tid];
+}
+
+// One workgroup: checks whether the mirror relation splits all N scan
+// points into quads of four distinct points, and whether the x-mirror alone
+// splits them into pairs, and lists them in order of their smallest index:
+// quads (s0, s1, s2, s3), pairs (s0, s1).
+//   info[0] = 1 when every point is in a quad, info[1] = number of quads,
+//   info[2] = 1 when every point is in an x-pair, info[3] = number of pairs.
+__device__ __forceinline__ void scan_quads_block(const int* __restrict__ partner, int N,
+                                                 int* __restrict__ quads,
+                                                 int* __restrict__ pairs,
+                                                 int* __restrict__ info) {
+  __shared__ int s_cnt[1024];
+  __shared__ int s_bad[2];
+  const int tid = threadIdx.x;
+  if (tid < 2) s_bad[tid] = 0;
+  __syncthreads();
+  const int per = (N + 1023) / 1024;
+  const int b0 = min(N, tid * per), b1 = min(N, b0 + per);
+  const int* PX = partner;
+  const int* PY = partner + N;
+  const int* PXY = partner + 2 * N;
+  auto quad_ok = [&](int i) {
+    const int px = PX[i], py = PY[i], pxy = PXY[i];
+    return px >= 0 && py >= 0 && pxy >= 0 && px != i && py != i && pxy != i && px != py &&
+           px != pxy && py != pxy && PX[px] == i && PY[py] == i && PX[pxy] == py &&
+           PY[pxy] == px;
+  };
+  auto pair_ok = [&](int i) {
+    const int px = PX[i];
+    return px >= 0 && px != i && PX[px] == i;
+  };
+  int qc = 0, pc = 0, qbad = 0, pbad = 0;
+  for (int i = b0; i < b1; ++i) {
+    if (!quad_ok(i)) qbad = 1;
+    else if (i < PX[i] && i < PY[i] && i < PXY[i]) ++qc;
+    if (!pair_ok(i)) pbad = 1;
+    else if (i < PX[i]) ++pc;
+  }
+  if (qbad) atomicOr(&s_bad[0], 1);
+  if (pbad) atomicOr(&s_bad[1], 1);
+  const int q_incl = block_scan_1024(qc, s_cnt);
+  const int n_quads = s_cnt[1023];
+  const int p_incl = block_scan_1024(pc, s_cnt);
+  const int n_pairs = s_cnt[1023];
+  const bool quads_ok = s_bad[0] == 0;
+  const bool pairs_ok = s_bad[1] == 0;
+  if (quads_ok) {
+    int q = q_incl - qc;
+    for (int i = b0; i < b1; ++i) {
+      if (i < PX[i] && i < PY[i] && i < PXY[i]) {
+        quads[4 * q + 0] = i;
+        quads[4 * q + 1] = PX[i];
+        quads[4 * q + 2] = PY[i];
+        quads[4 * q + 3] = PXY[i];
+        ++q;
+      }
+    }
+  }
+  if (pairs_ok) {
+    int q = p_incl - pc;
+    for (int i = b0; i < b1; ++i) {
+      if (i < PX[i]) {
+        pairs[2 * q + 0] = i;
+        pairs[2 * q + 1] = PX[i];
+        ++q;
+      }
+    }
+  }
+  if (tid == 0) {
+    info[0] = (quads_ok && 4 * n_quads == N) ? 1 : 0;
+    info[1] = n_quads;
+    info[2] = (pairs_ok && 2 * n_pairs == N) ? 1 : 0;
+    info[3] = n_pairs;
+  }
+}
+
+__global__ __launch_bounds__(1024) void scan_quads_kernel(const int* __restrict__ partner,
+                                                          int N, int* __restrict__ quads,
+                                                          int* __restrict__ pairs,
+                                                          int* __restrict__ info) {
+  scan_quads_block(partner, N, quads, pairs, info);
+}
+
+// One workgroup item of the symmetric R1: scan quads qgrp*8 .. qgrp*8+7 (one
+// per wave), bins t_base .. t_base + 64K - 1, column-quad group grp.  The
+// chunk buffers hold, per quadrant column, its four mirror columns:
+// lds2[((j * 4) + m) * Z + z] for column quad j and mirror m.
+template <int K>
+__device__ __forceinline__ void r1_sym_block(
+    const float* __restrict__ vol, const float* __restrict__ mp,
+    const int* __restrict__ quads, int n_quads, float* __restrict__ dst,
+    const ConfGeom& g, int t_base, int grp, int qgrp, int n_groups, int reps_per_chunk,
+    int chunks_per_group, float2* __restrict__ lds2) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int q = qgrp * kWaves + wave;
+  const bool quad_ok = q < n_quads;
+  const int HX = g.X / 2, HY = g.Y / 2;
+  const int n_reps = HX * HY;
+  const int chunk_vox = reps_per_chunk * 4 * g.Z;
+
+  float tc2[K], itc4[K], acc[4][K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int t = t_base + 64 * k + lane;
+    const float tc = bin_distance(t, g.w, g.off);
+    const float qq = tc * tc;
+    tc2[k] = (t < g.T) ? qq : -1.f;
+    itc4[k] = 1.f / (qq * qq);
+#pragma unroll
+    for (int m = 0; m < 4; ++m) acc[m][k] = 0.f;
+  }
+  int sidx[4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) sidx[m] = quad_ok ? quads[4 * q + m] : 0;
+  const int s0 = uniform_i32(sidx[0]);
+  const float wx = uniform_f32(mp[s0]);
+  const float wy = uniform_f32(mp[g.N + s0]);
+  const float wz = uniform_f32(mp[2 * g.N + s0]);
+  float c0 = (wz - g.oz) * g.s;
+  c0 = c0 - 0.5f;
+  const float delta = guard_delta(g, wz, c0, 0.f);
+  const float one_m_delta = 1.f - delta;
+  float dlo, dhi;
+  depth_range(g, wz, 0.f, (float)(g.Z - 1), &dlo, &dhi);
+
+  const int total_chunks = (n_reps + reps_per_chunk - 1) / reps_per_chunk;
+  const int chunk_first = grp * chunks_per_group;
+  const int chunk_end = min(chunk_first + chunks_per_group, total_chunks);
+  const int nthreads = kWaves * 64;
+
+  // Pair (v[z], v[z+1] - v[z]) of element e of a chunk (rep-major, then
+  // mirror, then z).
+  auto load_pair = [&](int c, int e) -> float2 {
+    const int jr = e / (4 * g.Z);
+    const int m = (e / g.Z) & 3;
+    const int z = e - (jr * 4 + m) * g.Z;
+    const int r = c * reps_per_chunk + jr;
+    if (r >= n_reps) return make_float2(0.f, 0.f);
+    const int rx = r / HY, ry = r - rx * HY;
+    const int xi = (m & 1) ? g.X - 1 - rx : rx;
+    const int yi = (m & 2) ? g.Y - 1 - ry : ry;
+    const float* col = vol + ((size_t)xi * g.Y + yi) * g.Z;
+    const float v0 = col[z];
+    const float v1 = (z == g.Z - 1) ? v0 : col[z + 1];
+    return make_float2(v0, v1 - v0);
+  };
+
+  if (chunk_first < chunk_end)
+    for (int e = threadIdx.x; e < chunk_vox; e += nthreads) lds2[e] = load_pair(chunk_first, e);
+  __syncthreads();
+
+  constexpr int kPref = kChunkVoxels / (kWaves * 64);
+  for (int c = chunk_first; c < chunk_end; ++c) {
+    const int buf = (c - chunk_first) & 1;
+    const float2* cur = lds2 + buf * chunk_vox;
+    float2* nxt = lds2 + (buf ^ 1) * chunk_vox;
+    const bool has_next = c + 1 < chunk_end;
+    float2 pref[kPref];
+    if (has_next) {
+#pragma unroll
+      for (int r = 0; r < kPref; ++r) {
+        const int e = threadIdx.x + r * nthreads;
+        pref[r] = e < chunk_vox ? load_pair(c + 1, e) : make_float2(0.f, 0.f);
+      }
+    }
+
+    if (quad_ok) {
+      // Lane j describes column quad j of the chunk.
+      const int r = c * reps_per_chunk + lane;
+      const bool rvalid = lane < reps_per_chunk && r < n_reps;
+      const int rr = rvalid ? r : 0;
+      const int rx = rr / HY, ry = rr - rx * HY;
+      const float x0 = cell_centre(rx, g.ox, g.szx, g.X);
+      const float x1 = cell_centre(g.X - 1 - rx, g.ox, g.szx, g.X);
+      const float y0 = cell_centre(ry, g.oy, g.szy, g.Y);
+      const float y1 = cell_centre(g.Y - 1 - ry, g.oy, g.szy, g.Y);
+      const float dxa = wx - x0, dxb = wx - x1, dya = wy - y0, dyb = wy - y1;
+      const float A = dxa * dxa, B = dxb * dxb, C = dya * dya, D = dyb * dyb;
+      unsigned packed = 0u;
+      bool any = false;
+#pragma unroll
+      for (int f = 0; f < 4; ++f) {
+        const float dx2 = (f & 1) ? B : A;
+        const float dy2 = (f & 2) ? D : C;
+        int tlo, thi;
+        bin_range(g, dx2 + dy2, dlo, dhi, &tlo, &thi);
+        const bool hit = rvalid && thi >= t_base && tlo <= t_base + 64 * K - 1 && tlo <= thi;
+        const int klo = (max(tlo, t_base) - t_base) >> 6;
+        const int khi = (min(thi, t_base + 64 * K - 1) - t_base) >> 6;
+        const unsigned bits = hit ? (((2u << khi) - 1u) & ~((1u << klo) - 1u)) : 0u;
+        packed |= bits << (8 * f);
+        any = any || hit;
+      }
+      unsigned long long mask = __ballot(any);
+      while (mask) {
+        const int j = __builtin_ctzll(mask);
+        mask &= mask - 1;
+        const unsigned wm = (unsigned)__builtin_amdgcn_readlane((int)packed, j);
+        const float cA = lane_value(A, j), cB = lane_value(B, j);
+        const float cC = lane_value(C, j), cD = lane_value(D, j);
+        const float2* quad = cur + j * 4 * g.Z;
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+          const unsigned fm = (wm >> (8 * f)) & 0xffu;
+          const float dx2 = (f & 1) ? cB : cA;
+          const float dy2 = (f & 2) ? cD : cC;
+#pragma unroll
+          for (int k = 0; k < K; ++k) {
+            if (!((fm >> k) & 1u)) continue;
+            const float d2 = (tc2[k] - dx2) - dy2;  // volume_cuda.cu:40
+            float al;
+            bool nr;
+            int zi = decide_fast(g, d2, c0, delta, one_m_delta, &al, &nr);
+            if (__ballot(nr)) {
+              if (nr) zi = decide_exact(g, d2, wz, 0, &al);
+            }
+            const bool act = d2 > 0.f && (unsigned)zi <= (unsigned)(g.Z - 2);
+            const int zs = act ? zi : 0;
+            const float qv = d2 * itc4[k];
+            const float wgt = qv * qv;
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+              const float2 v = quad[(m ^ f) * g.Z + zs];
+              const float f1 = fmaf(al, v.y, v.x);
+              const float nv = fmaf(f1, wgt, acc[m][k]);
+              acc[m][k] = act ? nv : acc[m][k];
+            }
+          }
+        }
+      }
+    }
+
+    if (has_next) {
+#pragma unroll
+      for (int r = 0; r < kPref; ++r) {
+        const int e = threadIdx.x + r * nthreads;
+        if (e < chunk_vox) nxt[e] = pref[r];
+      }
+    }
+    __syncthreads();
+  }
+
+  if (quad_ok) {
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      float* row = dst + ((size_t)grp * g.N + sidx[m]) * g.T;
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const int t = t_base + 64 * k + lane;
+        if (t < g.T) row[t] = acc[m][k];
+      }
+    }
+  }
+}
+
+// Two workgroups per CU (their chunk buffers take 128 KiB of LDS), so the
+// compiler is held to 128 VGPRs.  blockIdx.y is the bin pass.
+template <int K>
+__global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(4)))
+void r1_sym_kernel(
+    const float* __restrict__ vol, const float* __restrict__ mp,
+    const int* __restrict__ quads, const int* __restrict__ info, float* __restrict__ dst,
+    ConfGeom g, int n_groups, int reps_per_chunk, int chunks_per_group) {
+  if (!info[0]) return;
+  extern __shared__ float2 lds2[];
+  const int n_quads = info[1];
+  const int qgrp = (int)(blockIdx.x / n_groups);
+  if (qgrp * kWaves >= n_quads) return;  // whole workgroup idle
+  r1_sym_block<K>(vol, mp, quads, n_quads, dst, g, (int)blockIdx.y * 64 * K,
+                  (int)(blockIdx.x % n_groups), qgrp, n_groups, reps_per_chunk,
+                  chunks_per_group, lds2);
+}
+
+// Sum of partials in a fixed order where the number of parts depends on
+// which path ran (info[0] set: the symmetric one).
+template <typename OutT>
+__global__ void sum_partials_flag_kernel(const float* __restrict__ part,
+                                         const int* __restrict__ info, int n_sym, int n_gen,
+                                         size_t n, OutT* __restrict__ out) {
+  const int n_parts = info[0] ? n_sym : n_gen;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (size_t)gridDim.x * blockDim.x) {
+    float v = part[e];
+    for (int p = 1; p < n_parts; ++p) v += part[(size_t)p * n + e];
+    store_elem(out + e, v);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -717,18 +1082,196 @@ __device__ __forceinline__ void a1_item(
   }
 }
 
+// `skip` (optional): a device flag; when it is set the launch does nothing
+// (the x-mirror kernel below has done the work).
 template <bool SAFE>
 __global__ __launch_bounds__(kA1Waves * 64) void a1_transpose_kernel(
     const float* __restrict__ Kf, const float* __restrict__ tc2t,
     const float* __restrict__ itc4t, const float* __restrict__ mp,
     float* __restrict__ dst, ConfGeom g, int tiles_x, int tiles_y, int n_slabs,
-    int n_ranges, int t_stride, int n_items) {
+    int n_ranges, int t_stride, int n_items, const int* __restrict__ skip) {
+  if (skip && *skip) return;
   __shared__ float acc_all[kA1Waves][(kSlab + 2) * 64];
   const int wave = threadIdx.x >> 6;
   const int item = blockIdx.x * kA1Waves + wave;
   if (item >= n_items) return;  // whole wave exits; no barrier follows
   a1_item<SAFE>(Kf, tc2t, itc4t, mp, dst, g, item, tiles_x, tiles_y, n_slabs, n_ranges,
                 t_stride, acc_all[wave]);
+}
+
+// ---------------------------------------------------------------------------
+// x-mirror A1
+// ---------------------------------------------------------------------------
+// When the voxel grid is antisymmetric in x and the scan points come in
+// x-mirror pairs s0 = (wx, wy, wz), s1 = (-wx, wy, wz), the samples
+// (s_m, c_(m^f)) for m = 0, 1 share one decision, where c_0 = (x, y) and
+// c_1 = (-x, y): family f has dx^2 = (wx - x_f)^2.  A lane owns the column
+// pair (c_0, c_1) of an 8x8 tile in the half xi < X/2 and keeps both columns'
+// accumulators as float2 rows in LDS, so one decision updates two voxels
+// pairs: component n of the row gets the weight of scan s_(n^f).
+constexpr int kSlabS = 16;  // x-mirror A1 z-slab depth
+
+template <bool SAFE>
+__device__ __forceinline__ void a1_sym_item(
+    const float* __restrict__ Kf, const float* __restrict__ tc2t,
+    const float* __restrict__ itc4t, const float* __restrict__ mp,
+    const int* __restrict__ pairs, int n_pairs,
+    float* __restrict__ dst,  // [R][X][Y][Z] partials, or [X][Y][Z]
+    const ConfGeom& g, int item, int tiles_x, int tiles_y, int n_slabs, int n_ranges,
+    int t_stride, float2* __restrict__ acc) {
+  const int lane = threadIdx.x & 63;
+  const int tx = item % tiles_x;
+  item /= tiles_x;
+  const int ty = item % tiles_y;
+  item /= tiles_y;
+  const int slab = item % n_slabs;
+  const int range = item / n_slabs;
+
+  const int HX = g.X / 2;
+  const int xi = tx * 8 + (lane >> 3);
+  const int yi = ty * 8 + (lane & 7);
+  const bool cvalid = xi < HX && yi < g.Y;
+  const int xm = g.X - 1 - xi;
+  const float x0 = cell_centre(xi, g.ox, g.szx, g.X);
+  const float x1 = cell_centre(xm, g.ox, g.szx, g.X);
+  const float y = cell_centre(yi, g.oy, g.szy, g.Y);
+
+  const int z0 = slab * kSlabS;
+  const int rmin = (z0 == 0) ? 1 : 0;
+  const int rmax = min(kSlabS, (g.Z - 2) - (z0 - 1));
+  const int rspan = rmax - rmin;
+  const int zshift = z0 - 1 + rmin;
+  const float zlo = (float)max(z0 - 1, 0);
+  const float zhi = (float)min(z0 + kSlabS, g.Z - 1);
+
+  for (int r = 0; r < kSlabS + 2; ++r) acc[r * 64 + lane] = make_float2(0.f, 0.f);
+  float2 tot[kSlabS];
+#pragma unroll
+  for (int r = 0; r < kSlabS; ++r) tot[r] = make_float2(0.f, 0.f);
+
+  const int per = (n_pairs + n_ranges - 1) / n_ranges;
+  const int p_begin = range * per;
+  const int p_end = min(n_pairs, p_begin + per);
+  float2* my = acc + rmin * 64 + lane;
+
+  for (int pi = p_begin; pi < p_end; ++pi) {
+    if (pi > p_begin && (pi - p_begin) % (kFlush / 2) == 0) {
+#pragma unroll
+      for (int r = 0; r < kSlabS; ++r) {
+        const float2 a = acc[(r + 1) * 64 + lane];
+        tot[r].x += a.x;
+        tot[r].y += a.y;
+        acc[(r + 1) * 64 + lane] = make_float2(0.f, 0.f);
+      }
+    }
+    const int s0 = uniform_i32(pairs[2 * pi]);
+    const int s1 = uniform_i32(pairs[2 * pi + 1]);
+    const float wx = mp[s0], wy = mp[g.N + s0], wz = mp[2 * g.N + s0];
+    float c0 = (wz - g.oz) * g.s;
+    c0 = c0 - 0.5f;
+    const float cp = c0 - (float)zshift;
+    const float delta = guard_delta(g, wz, c0, cp);
+    const float one_m_delta = 1.f - delta;
+    const float dxa = wx - x0, dxb = wx - x1, dy = wy - y;
+    const float dy2 = dy * dy;
+    float dlo, dhi;
+    depth_range(g, wz, zlo, zhi, &dlo, &dhi);
+    const float* K0 = Kf + (size_t)s0 * t_stride;
+    const float* K1 = Kf + (size_t)s1 * t_stride;
+#pragma unroll
+    for (int f = 0; f < 2; ++f) {
+      const float dx2 = cvalid ? (f ? dxb * dxb : dxa * dxa) : INFINITY;
+      int tlo, thi;
+      bin_range(g, dx2 + dy2, dlo, dhi, &tlo, &thi);
+      if (!cvalid || rspan < 0 || tlo > thi) {
+        tlo = 0x7fffffff;
+        thi = -1;
+      }
+      const int t_begin = uniform_i32(wave_min_i32(tlo));
+      const int t_end = uniform_i32(wave_max_i32(thi));
+      for (int tb = t_begin & ~63; tb <= t_end; tb += 64) {
+        // component n of a row takes scan s_(n^f)
+        const float kreg0 = (f ? K1 : K0)[tb + lane];
+        const float kreg1 = (f ? K0 : K1)[tb + lane];
+        const float treg = tc2t[tb + lane];
+        const float ireg = SAFE ? itc4t[tb + lane] : 0.f;
+        const int u0 = max(t_begin - tb, 0);
+        const int u1 = min(t_end - tb, 63);
+        for (int u = u0; u <= u1; u += 2) {
+          const int ub = min(u + 1, u1);
+          const bool hb = u + 1 <= u1;
+          const float d2a = (lane_value(treg, u) - dx2) - dy2;
+          const float d2b = (lane_value(treg, ub) - dx2) - dy2;
+          float ala, alb;
+          bool na, nb;
+          int ra = decide_fast(g, d2a, cp, delta, one_m_delta, &ala, &na);
+          int rb = decide_fast(g, d2b, cp, delta, one_m_delta, &alb, &nb);
+          if (__ballot(na || nb)) {
+            if (na) ra = decide_exact(g, d2a, wz, zshift, &ala);
+            if (nb) rb = decide_exact(g, d2b, wz, zshift, &alb);
+          }
+          const bool acta = d2a > 0.f && (unsigned)ra <= (unsigned)rspan;
+          const bool actb = hb && d2b > 0.f && (unsigned)rb <= (unsigned)rspan;
+          if (acta) {
+            const float base = SAFE ? sq(d2a * lane_value(ireg, u)) : d2a * d2a;
+            const float w0 = base * lane_value(kreg0, u), w1 = base * lane_value(kreg1, u);
+            float2* pa = my + ra * 64;
+            float2 a0 = pa[0], a1 = pa[64];
+            const float om = 1.f - ala;
+            a0.x = fmaf(w0, om, a0.x);
+            a0.y = fmaf(w1, om, a0.y);
+            a1.x = fmaf(w0, ala, a1.x);
+            a1.y = fmaf(w1, ala, a1.y);
+            pa[0] = a0;
+            pa[64] = a1;
+          }
+          if (actb) {
+            const float base = SAFE ? sq(d2b * lane_value(ireg, ub)) : d2b * d2b;
+            const float w0 = base * lane_value(kreg0, ub), w1 = base * lane_value(kreg1, ub);
+            float2* pb = my + rb * 64;
+            float2 b0 = pb[0], b1 = pb[64];
+            const float om = 1.f - alb;
+            b0.x = fmaf(w0, om, b0.x);
+            b0.y = fmaf(w1, om, b0.y);
+            b1.x = fmaf(w0, alb, b1.x);
+            b1.y = fmaf(w1, alb, b1.y);
+            pb[0] = b0;
+            pb[64] = b1;
+          }
+        }
+      }
+    }
+  }
+
+  if (cvalid) {
+    float* out0 = dst + (((size_t)range * g.X + xi) * g.Y + yi) * g.Z;
+    float* out1 = dst + (((size_t)range * g.X + xm) * g.Y + yi) * g.Z;
+#pragma unroll
+    for (int r = 0; r < kSlabS; ++r) {
+      const int z = z0 + r;
+      if (z < g.Z) {
+        const float2 a = acc[(r + 1) * 64 + lane];
+        out0[z] = tot[r].x + a.x;
+        out1[z] = tot[r].y + a.y;
+      }
+    }
+  }
+}
+
+template <bool SAFE>
+__global__ __launch_bounds__(kA1Waves * 64) void a1_sym_kernel(
+    const float* __restrict__ Kf, const float* __restrict__ tc2t,
+    const float* __restrict__ itc4t, const float* __restrict__ mp,
+    const int* __restrict__ pairs, const int* __restrict__ info, float* __restrict__ dst,
+    ConfGeom g, int tiles_x, int tiles_y, int n_slabs, int n_ranges, int t_stride,
+    int n_items) {
+  if (!info[2]) return;
+  __shared__ float2 acc_all[kA1Waves][(kSlabS + 2) * 64];
+  const int wave = threadIdx.x >> 6;
+  const int item = blockIdx.x * kA1Waves + wave;
+  if (item >= n_items) return;  // whole wave exits; no barrier follows
+  a1_sym_item<SAFE>(Kf, tc2t, itc4t, mp, pairs, info[3], dst, g, item, tiles_x, tiles_y,
+                    n_slabs, n_ranges, t_stride, acc_all[wave]);
 }
 
 int launch_grid(size_t n) {
@@ -793,6 +1336,47 @@ struct R1Plan {
   size_t ws;
 };
 
+// Workspace of the scan-mirror detection: partner indices [3][N], quads
+// [N/4][4], x-pairs [N/2][2] and the info block, from byte `base`.
+struct MirrorLayout {
+  size_t partner_off, quads_off, pairs_off, info_off, end;
+};
+
+MirrorLayout mirror_layout(int64_t N, size_t base) {
+  auto align = [](size_t v) { return (v + 255) & ~(size_t)255; };
+  MirrorLayout m;
+  m.partner_off = align(base);
+  m.quads_off = m.partner_off + align(3 * (size_t)N * sizeof(int));
+  m.pairs_off = m.quads_off + align((size_t)N * sizeof(int));
+  m.info_off = m.pairs_off + align((size_t)N * sizeof(int));
+  m.end = m.info_off + 256;
+  return m;
+}
+
+// Plan of the mirror-symmetric R1 (used when the device finds scan quads).
+struct R1SymPlan {
+  bool possible;  // sizes allow it (X, Y even, N a multiple of 4, Z small enough)
+  int K, passes, n_groups, reps_per_chunk, chunks_per_group;
+  MirrorLayout ml;
+  size_t staging_off, ws;
+};
+
+// Cell centres of one axis, computed as the kernels compute them, are
+// antisymmetric: centre(n-1-i) == -centre(i) for every i.
+bool axis_antisymmetric(float o, float sz, int64_t n) {
+  if (n < 2 || (n & 1)) return false;
+  for (int64_t i = 0; i < n / 2; ++i) {
+    float a = ((float)i + 0.5f) * sz;
+    a = a / (float)n;
+    a = o + a;
+    float b = ((float)(n - 1 - i) + 0.5f) * sz;
+    b = b / (float)n;
+    b = o + b;
+    if (!(a == -b)) return false;
+  }
+  return true;
+}
+
 R1Plan plan_r1(int64_t X, int64_t Y, int64_t Z, int64_t N, int64_t T) {
   R1Plan p;
   const int64_t bins_pass = T <= 64 ? 64 : T <= 128 ? 128 : T <= 256 ? 256 : T <= 512 ? 512 : 1024;
@@ -816,20 +1400,78 @@ R1Plan plan_r1(int64_t X, int64_t Y, int64_t Z, int64_t N, int64_t T) {
   return p;
 }
 
+R1SymPlan plan_r1_sym(int64_t X, int64_t Y, int64_t Z, int64_t N, int64_t T,
+                      const R1Plan& gen) {
+  R1SymPlan p;
+  memset(&p, 0, sizeof(p));
+  p.possible = X >= 2 && Y >= 2 && !(X & 1) && !(Y & 1) && N >= 4 && !(N & 3) &&
+               Z >= 2 && 4 * Z <= kChunkVoxels && N <= (1 << 24);
+  if (!p.possible) return p;
+  p.K = gen.K < 8 ? gen.K : 8;
+  p.passes = (int)((T + 64 * p.K - 1) / (64 * p.K));
+  const int64_t n_reps = (X / 2) * (Y / 2);
+  int64_t rpc = kChunkVoxels / (4 * Z);
+  if (rpc > 64) rpc = 64;
+  p.reps_per_chunk = (int)rpc;
+  const int64_t total_chunks = (n_reps + rpc - 1) / rpc;
+  const int64_t quad_blocks = (N / 4 + kWaves - 1) / kWaves;
+  const int64_t target = 2 * (int64_t)device_cu_count();
+  int64_t G = 1;
+  if (quad_blocks < target) G = (target + quad_blocks - 1) / quad_blocks;
+  if (G > total_chunks) G = total_chunks;
+  p.chunks_per_group = (int)((total_chunks + G - 1) / G);
+  p.n_groups = (int)((total_chunks + p.chunks_per_group - 1) / p.chunks_per_group);
+  p.ml = mirror_layout(N, 0);
+  p.staging_off = p.ml.end;
+  const int parts = p.n_groups > gen.n_groups ? p.n_groups : gen.n_groups;
+  p.ws = p.staging_off + (size_t)parts * N * T * sizeof(float);
+  return p;
+}
+
 size_t r1_workspace(int64_t X, int64_t Y, int64_t Z, int64_t N, int64_t T) {
   if (X <= 0 || Y <= 0 || Z < 2 || N <= 0 || T <= 0) return 0;
-  return plan_r1(X, Y, Z, N, T).ws;
+  const R1Plan p = plan_r1(X, Y, Z, N, T);
+  const R1SymPlan q = plan_r1_sym(X, Y, Z, N, T, p);
+  return q.possible && q.ws > p.ws ? q.ws : p.ws;
 }
 
 template <int K>
 void launch_r1(const R1Plan& p, const float* vol, const float* mp, float* dst,
-               const ConfGeom& g, int t_base, hipStream_t st) {
+               const ConfGeom& g, int t_base, hipStream_t st, const int* skip) {
   const int scan_groups = (g.N + kWaves - 1) / kWaves;
   const int blocks = scan_groups * p.n_groups;
   const size_t lds = 2 * (size_t)p.chunk_cols * g.Z * sizeof(float2);
   hipLaunchKernelGGL(r1_forward_kernel<K>, dim3(blocks), dim3(kWaves * 64), lds, st,
                      vol, mp, dst, g, t_base, p.n_groups, p.chunk_cols,
+                     p.chunks_per_group, skip);
+}
+
+template <int K>
+void launch_r1_sym(const R1SymPlan& p, const float* vol, const float* mp, const int* quads,
+                   const int* info, float* dst, const ConfGeom& g, hipStream_t st) {
+  const int quad_blocks = (g.N / 4 + kWaves - 1) / kWaves;
+  const int blocks = quad_blocks * p.n_groups;
+  const size_t lds = 2 * (size_t)p.reps_per_chunk * 4 * g.Z * sizeof(float2);
+  hipLaunchKernelGGL(r1_sym_kernel<K>, dim3(blocks, p.passes), dim3(kWaves * 64), lds, st,
+                     vol, mp, quads, info, dst, g, p.n_groups, p.reps_per_chunk,
                      p.chunks_per_group);
+}
+
+// Finds the scan-point quads and x-pairs on the device (no host
+// synchronisation).
+int launch_mirror_detect(const MirrorLayout& ml, const float* mp, int N, void* ws,
+                         hipStream_t st) {
+  int* partner = (int*)((char*)ws + ml.partner_off);
+  int* quads = (int*)((char*)ws + ml.quads_off);
+  int* pairs = (int*)((char*)ws + ml.pairs_off);
+  int* info = (int*)((char*)ws + ml.info_off);
+  hipLaunchKernelGGL(scan_mirror_kernel, dim3((N + 255) / 256), dim3(256), 0, st, mp, N,
+                     partner);
+  int rc = check_launch("scan_mirror_kernel");
+  if (rc) return rc;
+  hipLaunchKernelGGL(scan_quads_kernel, dim3(1), dim3(1024), 0, st, partner, N, quads, pairs,
+                     info);
+  return check_launch("scan_quads_kernel");
 }
 
 template <typename OutT>
@@ -877,28 +1519,68 @@ extern "C" int totri_volume_confocal_forward(
   TOTRI_CHECK_ARG(Z <= kChunkVoxels, "volume depth Z=%lld exceeds %d", (long long)Z,
                   kChunkVoxels);
   const R1Plan p = plan_r1(X, Y, Z, N, T);
+  const R1SymPlan q = plan_r1_sym(X, Y, Z, N, T, p);
   const size_t need = r1_workspace(X, Y, Z, N, T);
-  const bool direct = p.n_groups == 1 && transient_dtype == TOTRI_F32;
+  // The symmetric path needs the volume grid antisymmetric in x and y; the
+  // scan-point quads are then looked for on the device.
+  const bool sym = q.possible &&
+                   axis_antisymmetric(volume_origin[0], volume_size[0], X) &&
+                   axis_antisymmetric(volume_origin[1], volume_size[1], Y);
+  const bool direct = !sym && p.n_groups == 1 && transient_dtype == TOTRI_F32;
   if (!direct && workspace_bytes < need)
     return fail(TOTRI_EWORKSPACE, "workspace too small: need %zu bytes, got %zu",
                 need, workspace_bytes);
   if (!direct) TOTRI_CHECK_ARG(workspace != nullptr, "null workspace");
   const ConfGeom g = make_geom(X, Y, Z, N, T, volume_origin, volume_size,
                                bin_width, bin_offset);
+  const int* info = nullptr;
   float* dst = direct ? (float*)transient : (float*)workspace;
+  if (sym) {
+    rc = launch_mirror_detect(q.ml, measurement_points, (int)N, workspace, st);
+    if (rc) return rc;
+    info = (const int*)((char*)workspace + q.ml.info_off);
+    const int* quads = (const int*)((char*)workspace + q.ml.quads_off);
+    dst = (float*)((char*)workspace + q.staging_off);
+    switch (q.K) {
+      case 1: launch_r1_sym<1>(q, volume, measurement_points, quads, info, dst, g, st); break;
+      case 2: launch_r1_sym<2>(q, volume, measurement_points, quads, info, dst, g, st); break;
+      case 4: launch_r1_sym<4>(q, volume, measurement_points, quads, info, dst, g, st); break;
+      default: launch_r1_sym<8>(q, volume, measurement_points, quads, info, dst, g, st); break;
+    }
+    rc = check_launch("r1_sym_kernel");
+    if (rc) return rc;
+  }
   for (int pass = 0; pass < p.passes; ++pass) {
     const int t_base = pass * 64 * p.K;
     switch (p.K) {
-      case 1: launch_r1<1>(p, volume, measurement_points, dst, g, t_base, st); break;
-      case 2: launch_r1<2>(p, volume, measurement_points, dst, g, t_base, st); break;
-      case 4: launch_r1<4>(p, volume, measurement_points, dst, g, t_base, st); break;
-      case 8: launch_r1<8>(p, volume, measurement_points, dst, g, t_base, st); break;
-      default: launch_r1<16>(p, volume, measurement_points, dst, g, t_base, st); break;
+      case 1: launch_r1<1>(p, volume, measurement_points, dst, g, t_base, st, info); break;
+      case 2: launch_r1<2>(p, volume, measurement_points, dst, g, t_base, st, info); break;
+      case 4: launch_r1<4>(p, volume, measurement_points, dst, g, t_base, st, info); break;
+      case 8: launch_r1<8>(p, volume, measurement_points, dst, g, t_base, st, info); break;
+      default: launch_r1<16>(p, volume, measurement_points, dst, g, t_base, st, info); break;
     }
     rc = check_launch("r1_forward_kernel");
     if (rc) return rc;
   }
   if (direct) return TOTRI_OK;
+  if (sym) {
+    const dim3 grid(launch_grid(n)), block(256);
+    switch (transient_dtype) {
+      case TOTRI_F32:
+        hipLaunchKernelGGL(sum_partials_flag_kernel<float>, grid, block, 0, st, dst, info,
+                           q.n_groups, p.n_groups, n, (float*)transient);
+        break;
+      case TOTRI_BF16:
+        hipLaunchKernelGGL(sum_partials_flag_kernel<__hip_bfloat16>, grid, block, 0, st, dst,
+                           info, q.n_groups, p.n_groups, n, (__hip_bfloat16*)transient);
+        break;
+      default:
+        hipLaunchKernelGGL(sum_partials_flag_kernel<__half>, grid, block, 0, st, dst, info,
+                           q.n_groups, p.n_groups, n, (__half*)transient);
+        break;
+    }
+    return check_launch("sum_partials_flag_kernel");
+  }
   switch (transient_dtype) {
     case TOTRI_F32: return finish_r1(p, dst, n, (float*)transient, st);
     case TOTRI_BF16: return finish_r1(p, dst, n, (__hip_bfloat16*)transient, st);
@@ -959,14 +1641,59 @@ A1Plan plan_a1(int64_t X, int64_t Y, int64_t Z, int64_t N, int64_t T, float w,
 }  // namespace
 }  // namespace totri_hip
 
+namespace totri_hip {
+namespace {
+
+// Plan of the x-mirror A1 (used when the device finds x-pairs of scan
+// points).  Workspace: tables | partial volumes (the larger count of the two
+// paths) | mirror detection.
+struct A1SymPlan {
+  bool possible;  // sizes allow it (X even, N even)
+  int tiles_x, tiles_y, n_slabs, n_ranges;
+  size_t partial_bytes;
+  MirrorLayout ml;
+  size_t ws;
+};
+
+A1SymPlan plan_a1_sym(int64_t X, int64_t Y, int64_t Z, int64_t N, const A1Plan& gen) {
+  A1SymPlan p;
+  memset(&p, 0, sizeof(p));
+  p.possible = X >= 2 && !(X & 1) && N >= 2 && !(N & 1) && N <= (1 << 24);
+  if (!p.possible) return p;
+  p.tiles_x = (int)((X / 2 + 7) / 8);
+  p.tiles_y = (int)((Y + 7) / 8);
+  p.n_slabs = (int)((Z + kSlabS - 1) / kSlabS);
+  const int64_t items = (int64_t)p.tiles_x * p.tiles_y * p.n_slabs;
+  const int64_t target = 16 * (int64_t)device_cu_count();
+  int64_t R = 1;
+  if (items < target) R = (target + items - 1) / items;
+  if (R > N / 2) R = N / 2;
+  if (R < 1) R = 1;
+  p.n_ranges = (int)R;
+  const int parts = p.n_ranges > gen.n_ranges ? p.n_ranges : gen.n_ranges;
+  p.partial_bytes = parts > 1 ? (size_t)parts * X * Y * Z * sizeof(float) : 0;
+  p.ml = mirror_layout(N, gen.tables_bytes + p.partial_bytes);
+  p.ws = p.ml.end;
+  return p;
+}
+
+size_t a1_workspace(int64_t X, int64_t Y, int64_t Z, int64_t N, int64_t T) {
+  if (X <= 0 || Y <= 0 || Z < 2 || N <= 0 || T <= 0) return 0;
+  // The size does not depend on the geometry, only on the safe flag's
+  // table layout, which is the same in both forms.
+  const A1Plan p = plan_a1(X, Y, Z, N, T, 1.f, 1.f);
+  const A1SymPlan q = plan_a1_sym(X, Y, Z, N, p);
+  const size_t gen = p.tables_bytes + p.partial_bytes;
+  return q.possible && q.ws > gen ? q.ws : gen;
+}
+
+}  // namespace
+}  // namespace totri_hip
+
 extern "C" size_t totri_volume_confocal_grad_volume_workspace(int64_t X, int64_t Y,
                                                               int64_t Z, int64_t N,
                                                               int64_t T) {
-  if (X <= 0 || Y <= 0 || Z < 2 || N <= 0 || T <= 0) return 0;
-  // The plan's size does not depend on the geometry, only on the safe flag's
-  // table layout, which is the same in both forms.
-  A1Plan p = plan_a1(X, Y, Z, N, T, 1.f, 1.f);
-  return p.tables_bytes + p.partial_bytes;
+  return a1_workspace(X, Y, Z, N, T);
 }
 
 extern "C" int totri_volume_confocal_grad_volume(
@@ -985,7 +1712,11 @@ extern "C" int totri_volume_confocal_grad_volume(
   TOTRI_CHECK_ARG(transient != nullptr && measurement_points != nullptr,
                   "null transient or measurement points");
   const A1Plan p = plan_a1(X, Y, Z, N, T, bin_width, bin_offset);
-  const size_t need = p.tables_bytes + p.partial_bytes;
+  const A1SymPlan q = plan_a1_sym(X, Y, Z, N, p);
+  // The x-mirror path needs the voxel grid antisymmetric in x; the scan
+  // pairs are then looked for on the device.
+  const bool sym = q.possible && axis_antisymmetric(volume_origin[0], volume_size[0], X);
+  const size_t need = a1_workspace(X, Y, Z, N, T);
   if (workspace_bytes < need)
     return fail(TOTRI_EWORKSPACE, "workspace too small: need %zu bytes, got %zu",
                 need, workspace_bytes);
@@ -1014,351 +1745,53 @@ extern "C" int totri_volume_confocal_grad_volume(
   }
   rc = check_launch("a1_tables_kernel");
   if (rc) return rc;
-  float* dst = p.n_ranges > 1 ? partial : volume;
+  const int* info = nullptr;
+  const bool staged = p.n_ranges > 1 || (sym && q.n_ranges > 1);
+  float* dst = staged ? partial : volume;
+  if (sym) {
+    rc = launch_mirror_detect(q.ml, measurement_points, (int)N, workspace, st);
+    if (rc) return rc;
+    info = (const int*)((char*)workspace + q.ml.info_off);
+    const int* pairs = (const int*)((char*)workspace + q.ml.pairs_off);
+    const int items = q.tiles_x * q.tiles_y * q.n_slabs * q.n_ranges;
+    const int blocks = (items + kA1Waves - 1) / kA1Waves;
+    if (p.safe)
+      hipLaunchKernelGGL(a1_sym_kernel<true>, dim3(blocks), dim3(kA1Waves * 64), 0, st, Kf,
+                         tc2, itc4, measurement_points, pairs, info, dst, g, q.tiles_x,
+                         q.tiles_y, q.n_slabs, q.n_ranges, p.t_stride, items);
+    else
+      hipLaunchKernelGGL(a1_sym_kernel<false>, dim3(blocks), dim3(kA1Waves * 64), 0, st, Kf,
+                         tc2, itc4, measurement_points, pairs, info, dst, g, q.tiles_x,
+                         q.tiles_y, q.n_slabs, q.n_ranges, p.t_stride, items);
+    rc = check_launch("a1_sym_kernel");
+    if (rc) return rc;
+  }
+  const int* skip = info ? info + 2 : nullptr;
   const int items = p.tiles_x * p.tiles_y * p.n_slabs * p.n_ranges;
   const int blocks = (items + kA1Waves - 1) / kA1Waves;
   if (p.safe)
     hipLaunchKernelGGL(a1_transpose_kernel<true>, dim3(blocks), dim3(kA1Waves * 64), 0, st,
                        Kf, tc2, itc4, measurement_points, dst, g, p.tiles_x, p.tiles_y,
-                       p.n_slabs, p.n_ranges, p.t_stride, items);
+                       p.n_slabs, p.n_ranges, p.t_stride, items, skip);
   else
     hipLaunchKernelGGL(a1_transpose_kernel<false>, dim3(blocks), dim3(kA1Waves * 64), 0,
                        st, Kf, tc2, itc4, measurement_points, dst, g, p.tiles_x,
-                       p.tiles_y, p.n_slabs, p.n_ranges, p.t_stride, items);
+                       p.tiles_y, p.n_slabs, p.n_ranges, p.t_stride, items, skip);
   rc = check_launch("a1_transpose_kernel");
   if (rc) return rc;
-  if (p.n_ranges > 1) {
-    hipLaunchKernelGGL(sum_partials_kernel<float>, dim3(launch_grid(nv)), dim3(256), 0,
-                       st, partial, p.n_ranges, nv, volume);
-    rc = check_launch("sum_partials_kernel");
+  if (!staged) return TOTRI_OK;
+  if (sym) {
+    // info[2] says which path ran; sum_partials_flag_kernel reads info[0]
+    hipLaunchKernelGGL(sum_partials_flag_kernel<float>, dim3(launch_grid(nv)), dim3(256), 0,
+                       st, partial, info + 2, q.n_ranges, p.n_ranges, nv, volume);
+    return check_launch("sum_partials_flag_kernel");
   }
-  return rc;
+  hipLaunchKernelGGL(sum_partials_kernel<float>, dim3(launch_grid(nv)), dim3(256), 0, st,
+                     partial, p.n_ranges, nv, volume);
+  return check_launch("sum_partials_kernel");
 }
 
-// ---------------------------------------------------------------------------
-// Batched step: many R1 / A1 jobs in one set of persistent launches
-// ---------------------------------------------------------------------------
-namespace totri_hip {
-namespace {
-
-constexpr int kBatchMax = TOTRI_BATCH_MAX_JOBS;
-constexpr int kClassA1 = 5;  // classes 0..4: R1 with K = 1, 2, 4, 8, 16
-constexpr int kClasses = 7;  // 5: A1, 6: A1 in the safe (tiny radius) form
-constexpr unsigned kBatchMagic = 0x74427431u;
-
-int r1_class(int K) { return K == 1 ? 0 : K == 2 ? 1 : K == 4 ? 2 : K == 8 ? 3 : 4; }
-
-// One job as the kernels see it: geometry, plan and buffer addresses.
-struct BatchJob {
-  ConfGeom g;
-  int kind;
-  int K, passes, scan_groups, n_groups, chunk_cols, chunks_per_group;
-  int tiles_x, tiles_y, n_slabs, n_ranges, t_stride, safe;
-  const float* vol_in;
-  const float* mp;
-  const float* tr_in;
-  float* r1_dst;
-  float* tc2;
-  float* itc4;
-  float* kf;
-  float* a1_dst;
-  const float* fin_src;
-  float* fin_dst;
-  long long fin_n;
-  int fin_parts;
-  int pad_;
-};
-
-// One step.  The work-queue heads come first, in a 256-byte block of their
-// own, and are zero in every table that batch_prepare writes, so uploading
-// the table resets them.  Per class: the jobs with work and the prefix sums
-// of their item counts; likewise the A1 table elements and the partial-sum
-// elements of the finishing pass.
-struct BatchTable {
-  int counters[kClasses];
-  unsigned magic;
-  int n_jobs;
-  int pad_[64 - kClasses - 2];
-  int cls_n[kClasses];
-  int cls_jobs[kClasses][kBatchMax];
-  int cls_prefix[kClasses][kBatchMax + 1];
-  int tab_n;
-  int tab_jobs[kBatchMax];
-  long long tab_prefix[kBatchMax + 1];
-  int fin_n;
-  int fin_jobs[kBatchMax];
-  long long fin_prefix[kBatchMax + 1];
-  BatchJob jobs[kBatchMax];
-};
-
-// Largest slot j in [0, n) with prefix[j] <= e (prefix is non-decreasing and
-// every listed slot is non-empty, so this is the slot that holds e).
-template <typename IT>
-__device__ __forceinline__ int find_slot(const IT* prefix, int n, IT e) {
-  int lo = 0, hi = n - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (prefix[mid] <= e) lo = mid; else hi = mid - 1;
-  }
-  return lo;
-}
-
-// Persistent R1 over the class-`cls` jobs (all with K windows).  Each
-// workgroup takes the next item from the class queue until it is empty; the
-// item index goes through LDS slot 0, which every thread has read before
-// r1_block refills the chunk buffers.
-// The table is read through a read-only view (job fields become scalar
-// loads) and the queue heads are updated through `counters`, which points at
-// the same table's counter block (the view never reads it).
-template <int K>
-__global__ __launch_bounds__(kWaves * 64) void r1_batch_kernel(
-    const BatchTable* __restrict__ tab, int* __restrict__ counters, int cls) {
-  extern __shared__ float2 lds2[];
-  const int n = tab->cls_n[cls];
-  const int total = tab->cls_prefix[cls][n];
-  int* slot0 = reinterpret_cast<int*>(lds2);
-  for (;;) {
-    __syncthreads();
-    if (threadIdx.x == 0) slot0[0] = atomicAdd(&counters[cls], 1);
-    __syncthreads();
-    const int item = uniform_i32(slot0[0]);
-    __syncthreads();
-    if (item >= total) break;
-    const int j = find_slot(tab->cls_prefix[cls], n, item);
-    const BatchJob* jb = &tab->jobs[tab->cls_jobs[cls][j]];
-    const ConfGeom g = jb->g;
-    const int n_groups = jb->n_groups, scan_groups = jb->scan_groups;
-    int local = item - tab->cls_prefix[cls][j];
-    const int grp = local % n_groups;
-    local /= n_groups;
-    const int sgrp = local % scan_groups;
-    const int pass = local / scan_groups;
-    r1_block<K>(jb->vol_in, jb->mp, jb->r1_dst, g, pass * 64 * K, grp, sgrp, n_groups,
-                jb->chunk_cols, jb->chunks_per_group, lds2);
-  }
-}
-
-// Persistent A1 over the class-`cls` jobs: every wave takes its own items.
-template <bool SAFE>
-__global__ __launch_bounds__(kA1Waves * 64) void a1_batch_kernel(
-    const BatchTable* __restrict__ tab, int* __restrict__ counters, int cls) {
-  __shared__ float acc_all[kA1Waves][(kSlab + 2) * 64];
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const int n = tab->cls_n[cls];
-  const int total = tab->cls_prefix[cls][n];
-  for (;;) {
-    int item = 0;
-    if (lane == 0) item = atomicAdd(&counters[cls], 1);
-    item = uniform_i32(__shfl(item, 0, 64));
-    if (item >= total) break;
-    const int j = find_slot(tab->cls_prefix[cls], n, item);
-    const BatchJob* jb = &tab->jobs[tab->cls_jobs[cls][j]];
-    const ConfGeom g = jb->g;
-    a1_item<SAFE>(jb->kf, jb->tc2, jb->itc4, jb->mp, jb->a1_dst, g,
-                  item - tab->cls_prefix[cls][j], jb->tiles_x, jb->tiles_y, jb->n_slabs,
-                  jb->n_ranges, jb->t_stride, acc_all[wave]);
-  }
-}
-
-__global__ __launch_bounds__(256) void a1_tables_batch_kernel(const BatchTable* tab) {
-  const int n = tab->tab_n;
-  const long long total = tab->tab_prefix[n];
-  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
-       e += (long long)gridDim.x * blockDim.x) {
-    const int j = find_slot(tab->tab_prefix, n, e);
-    const BatchJob* jb = &tab->jobs[tab->tab_jobs[j]];
-    a1_table_elem(jb->tr_in, jb->g, jb->safe, jb->t_stride,
-                  (size_t)(e - tab->tab_prefix[j]), jb->tc2, jb->itc4, jb->kf);
-  }
-}
-
-// Partial sums in the same fixed order as sum_partials_kernel.
-__global__ __launch_bounds__(256) void finish_batch_kernel(const BatchTable* tab) {
-  const int n = tab->fin_n;
-  const long long total = tab->fin_prefix[n];
-  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
-       e += (long long)gridDim.x * blockDim.x) {
-    const int j = find_slot(tab->fin_prefix, n, e);
-    const BatchJob* jb = &tab->jobs[tab->fin_jobs[j]];
-    const size_t le = (size_t)(e - tab->fin_prefix[j]);
-    const size_t m = (size_t)jb->fin_n;
-    float v = jb->fin_src[le];
-    for (int p = 1; p < jb->fin_parts; ++p) v += jb->fin_src[(size_t)p * m + le];
-    jb->fin_dst[le] = v;
-  }
-}
-
-// Validates one job and fills its kernel-side description; *need receives
-// the workspace it needs.  With check_buffers false only the sizes are
-// validated (the workspace query, made before buffers exist).
-int plan_job(const totri_confocal_job& j, BatchJob* out, size_t* need, bool check_buffers) {
-  *need = 0;
-  TOTRI_CHECK_ARG(j.kind == TOTRI_JOB_RENDER || j.kind == TOTRI_JOB_BACKPROJECT,
-                  "unknown job kind %d", j.kind);
-  int rc = check_common(j.X, j.Y, j.Z, j.N, j.T, j.volume_origin, j.volume_size,
-                        j.bin_width, j.bin_offset, TOTRI_F32);
-  if (rc) return rc;
-  TOTRI_CHECK_ARG(j.X >= 1 && j.Y >= 1 && j.Z >= 2 && j.N >= 1 && j.T >= 1,
-                  "batched jobs need X, Y, N, T >= 1 and Z >= 2");
-  TOTRI_CHECK_ARG(j.Z <= kChunkVoxels, "volume depth Z=%lld exceeds %d", (long long)j.Z,
-                  kChunkVoxels);
-  if (check_buffers)
-    TOTRI_CHECK_ARG(j.volume && j.measurement_points && j.transient, "null job buffer");
-  BatchJob b;
-  memset(&b, 0, sizeof(b));
-  b.g = make_geom(j.X, j.Y, j.Z, j.N, j.T, j.volume_origin, j.volume_size, j.bin_width,
-                  j.bin_offset);
-  b.kind = j.kind;
-  b.mp = j.measurement_points;
-  float* ws = (float*)j.workspace;
-  if (j.kind == TOTRI_JOB_RENDER) {
-    const R1Plan p = plan_r1(j.X, j.Y, j.Z, j.N, j.T);
-    b.K = p.K;
-    b.passes = p.passes;
-    b.scan_groups = (int)((j.N + kWaves - 1) / kWaves);
-    b.n_groups = p.n_groups;
-    b.chunk_cols = p.chunk_cols;
-    b.chunks_per_group = p.chunks_per_group;
-    b.vol_in = j.volume;
-    *need = p.n_groups > 1 ? p.ws : 0;
-    b.r1_dst = p.n_groups > 1 ? ws : j.transient;
-    if (p.n_groups > 1) {
-      b.fin_src = ws;
-      b.fin_dst = j.transient;
-      b.fin_n = (long long)j.N * j.T;
-      b.fin_parts = p.n_groups;
-    }
-  } else {
-    const A1Plan p = plan_a1(j.X, j.Y, j.Z, j.N, j.T, j.bin_width, j.bin_offset);
-    b.tiles_x = p.tiles_x;
-    b.tiles_y = p.tiles_y;
-    b.n_slabs = p.n_slabs;
-    b.n_ranges = p.n_ranges;
-    b.t_stride = p.t_stride;
-    b.safe = p.safe ? 1 : 0;
-    b.tr_in = j.transient;
-    *need = p.tables_bytes + p.partial_bytes;
-    b.tc2 = ws;
-    b.itc4 = ws ? ws + p.t_stride : nullptr;
-    b.kf = ws ? ws + 2 * (size_t)p.t_stride : nullptr;
-    float* partial = ws ? (float*)((char*)ws + p.tables_bytes) : nullptr;
-    b.a1_dst = p.n_ranges > 1 ? partial : j.volume;
-    if (p.n_ranges > 1) {
-      b.fin_src = partial;
-      b.fin_dst = j.volume;
-      b.fin_n = (long long)j.X * j.Y * j.Z;
-      b.fin_parts = p.n_ranges;
-    }
-  }
-  if (*need > 0 && check_buffers) {
-    TOTRI_CHECK_ARG(j.workspace != nullptr, "null job workspace");
-    if (j.workspace_bytes < *need)
-      return fail(TOTRI_EWORKSPACE, "job workspace too small: need %zu bytes, got %zu",
-                  *need, j.workspace_bytes);
-  }
-  *out = b;
-  return TOTRI_OK;
-}
-
-long long job_items(const BatchJob& b) {
-  if (b.kind == TOTRI_JOB_RENDER) return (long long)b.passes * b.scan_groups * b.n_groups;
-  return (long long)b.tiles_x * b.tiles_y * b.n_slabs * b.n_ranges;
-}
-
-}  // namespace
-}  // namespace totri_hip
-
-extern "C" size_t totri_confocal_batch_table_bytes(void) { return sizeof(BatchTable); }
-
-extern "C" size_t totri_confocal_job_workspace(const totri_confocal_job* job) {
-  if (!job) return 0;
-  BatchJob b;
-  size_t need = 0;
-  if (plan_job(*job, &b, &need, false) != TOTRI_OK) return 0;
-  return need;
-}
-
-extern "C" int totri_confocal_batch_prepare(const totri_confocal_job* jobs, int n_jobs,
-                                            void* host_table, size_t table_bytes) {
-  TOTRI_CHECK_ARG(host_table != nullptr, "null host table");
-  TOTRI_CHECK_ARG(table_bytes >= sizeof(BatchTable), "table needs %zu bytes, got %zu",
-                  sizeof(BatchTable), table_bytes);
-  TOTRI_CHECK_ARG(n_jobs >= 0 && n_jobs <= kBatchMax, "n_jobs=%d outside [0, %d]", n_jobs,
-                  kBatchMax);
-  TOTRI_CHECK_ARG(n_jobs == 0 || jobs != nullptr, "null jobs");
-  BatchTable* t = (BatchTable*)host_table;
-  memset(t, 0, sizeof(BatchTable));
-  for (int k = 0; k < n_jobs; ++k) {
-    size_t need = 0;
-    int rc = plan_job(jobs[k], &t->jobs[k], &need, true);
-    if (rc) {
-      memset(t, 0, sizeof(BatchTable));
-      return rc;
-    }
-  }
-  long long cls_total[kClasses] = {0};
-  for (int k = 0; k < n_jobs; ++k) {
-    const BatchJob& b = t->jobs[k];
-    const long long items = job_items(b);
-    const int c = b.kind == TOTRI_JOB_RENDER ? r1_class(b.K) : kClassA1 + b.safe;
-    if (items > 0) {
-      TOTRI_CHECK_ARG(cls_total[c] + items < (1ll << 30), "too many work items in one step");
-      const int m = t->cls_n[c]++;
-      t->cls_jobs[c][m] = k;
-      t->cls_prefix[c][m] = (int)cls_total[c];
-      cls_total[c] += items;
-      t->cls_prefix[c][m + 1] = (int)cls_total[c];
-    }
-    if (b.kind == TOTRI_JOB_BACKPROJECT) {
-      const int m = t->tab_n++;
-      t->tab_jobs[m] = k;
-      t->tab_prefix[m + 1] = t->tab_prefix[m] + (long long)b.g.N * b.t_stride;
-    }
-    if (b.fin_parts > 1) {
-      const int m = t->fin_n++;
-      t->fin_jobs[m] = k;
-      t->fin_prefix[m + 1] = t->fin_prefix[m] + b.fin_n;
-    }
-  }
-  t->n_jobs = n_jobs;
-  t->magic = kBatchMagic;
-  return TOTRI_OK;
-}
-
-extern "C" int totri_confocal_batch_run(const void* host_table, void* device_table,
-                                        size_t table_bytes, void* stream) {
-  TOTRI_CHECK_ARG(host_table != nullptr && device_table != nullptr, "null table");
-  TOTRI_CHECK_ARG(table_bytes >= sizeof(BatchTable), "table needs %zu bytes, got %zu",
-                  sizeof(BatchTable), table_bytes);
-  TOTRI_CHECK_ARG(((const BatchTable*)host_table)->magic == kBatchMagic,
-                  "host table was not written by totri_confocal_batch_prepare");
-  hipStream_t st = (hipStream_t)stream;
-  TOTRI_CHECK_HIP(hipMemcpyAsync(device_table, host_table, sizeof(BatchTable),
-                                 hipMemcpyHostToDevice, st));
-  const BatchTable* tab = (const BatchTable*)device_table;
-  int* ctr = ((BatchTable*)device_table)->counters;
-  const int cu = device_cu_count();
-  const size_t r1_lds = 2 * (size_t)kChunkVoxels * sizeof(float2);
-  hipLaunchKernelGGL(a1_tables_batch_kernel, dim3(4 * cu), dim3(256), 0, st, tab);
-  int rc = check_launch("a1_tables_batch_kernel");
-  if (rc) return rc;
-  hipLaunchKernelGGL(r1_batch_kernel<16>, dim3(2 * cu), dim3(kWaves * 64), r1_lds, st, tab, ctr, 4);
-  hipLaunchKernelGGL(r1_batch_kernel<8>, dim3(2 * cu), dim3(kWaves * 64), r1_lds, st, tab, ctr, 3);
-  hipLaunchKernelGGL(r1_batch_kernel<4>, dim3(2 * cu), dim3(kWaves * 64), r1_lds, st, tab, ctr, 2);
-  hipLaunchKernelGGL(r1_batch_kernel<2>, dim3(2 * cu), dim3(kWaves * 64), r1_lds, st, tab, ctr, 1);
-  hipLaunchKernelGGL(r1_batch_kernel<1>, dim3(2 * cu), dim3(kWaves * 64), r1_lds, st, tab, ctr, 0);
-  rc = check_launch("r1_batch_kernel");
-  if (rc) return rc;
-  hipLaunchKernelGGL(a1_batch_kernel<false>, dim3(4 * cu), dim3(kA1Waves * 64), 0, st, tab,
-                     ctr, kClassA1);
-  hipLaunchKernelGGL(a1_batch_kernel<true>, dim3(4 * cu), dim3(kA1Waves * 64), 0, st, tab,
-                     ctr, kClassA1 + 1);
-  rc = check_launch("a1_batch_kernel");
-  if (rc) return rc;
-  hipLaunchKernelGGL(finish_batch_kernel, dim3(4 * cu), dim3(256), 0, st, tab);
-  return check_launch("finish_batch_kernel");
-}
+#include "confocal_batch.inc"
 
 namespace totri_hip {
 namespace {
