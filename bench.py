@@ -90,6 +90,22 @@ def make_inputs(torch, dev):
     return mp, vol, tgt_vol
 
 
+def mirror_info(torch, mp):
+    """[quads ok, n quads, x-pairs ok, n pairs] of the device-side scan
+    mirror detection that selects the symmetric kernels."""
+    from totrilib_amd import _lib
+    lib = _lib.load()
+    N = mp.shape[1]
+    ws = torch.empty(lib.totri_debug_mirror_workspace(N), dtype=torch.uint8, device=mp.device)
+    info = torch.zeros(4, dtype=torch.int32, device=mp.device)
+    _lib.check(lib.totri_debug_mirror_info(mp.data_ptr(), N, ws.data_ptr(), ws.numel(),
+                                           info.data_ptr(),
+                                           torch.cuda.current_stream().cuda_stream),
+               "totri_debug_mirror_info")
+    torch.cuda.synchronize()
+    return info.cpu().tolist()
+
+
 def cpu_baseline(mp, vol, T, w):
     """Oracle FP32 restatement (same formulas, brute force as the reference
     kernels) on host cores: R1 on a seeded subset of scan points, A1 on a
@@ -196,10 +212,15 @@ def confocal_main(args):
     fm = flop_model(mp_np, res, T, w)
     value = pairs * args.steps / wall
     r1_avg, a1_avg = float(r1_ms.mean()), float(a1_ms.mean())
+    # which kernels ran: the mirror-symmetric ones when the scan points form
+    # mirror quads / pairs (the box is centred at x = y = 0)
+    info = mirror_info(torch, mp)
+    r1_name = "r1_sym_kernel" if info[0] else "r1_forward_kernel"
+    a1_name = "a1_sym_kernel" if info[2] else "a1_transpose_kernel"
     if r1_avg >= a1_avg:
-        dom, flops, dur = "r1_forward_kernel", fm["f_r1"] * pairs, r1_avg
+        dom, flops, dur = r1_name, fm["f_r1"] * pairs, r1_avg
     else:
-        dom, flops, dur = "a1_transpose_kernel", fm["f_a1"] * pairs, a1_avg
+        dom, flops, dur = a1_name, fm["f_a1"] * pairs, a1_avg
     achieved = flops / (dur * 1e-3)
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic.json")
@@ -237,7 +258,8 @@ def confocal_main(args):
                                "box x,y in [-1,1], z in [0.5,2.5] m",
                    "voxels": res ** 3, "scan_points": N, "bins": T,
                    "pairs_per_step": pairs},
-        "kernels_ms": {"r1_forward_avg": r1_avg, "a1_backward_avg": a1_avg},
+        "kernels_ms": {"r1_forward_avg": r1_avg, "a1_backward_avg": a1_avg,
+                       "r1_kernel": r1_name, "a1_kernel": a1_name},
         "flop_model": fm,
         "roofline": {"bound": "valu", "kernel": dom, "achieved": achieved / 1e12,
                      "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",

@@ -305,6 +305,24 @@ int totri_debug_confocal_samples(const float* measurement_points, int64_t N,
                                  float bin_offset, int32_t* zi_out,
                                  float* alpha_out, float* d2_out, void* stream);
 
+/*
+ * Test hook with no counterpart in the reference: runs the scan-mirror
+ * detection that selects the mirror-symmetric R1 / A1 kernels and writes
+ * four ints to info_out (device memory):
+ *   [0] 1 if all N points form quads (wx, wy), (-wx, wy), (wx, -wy), (-wx, -wy)
+ *       with equal wz -- R1 shares decisions when the voxel grid is also
+ *       antisymmetric in x and y
+ *   [1] number of quads
+ *   [2] 1 if all points form x-mirror pairs -- A1 shares decisions when the
+ *       voxel grid is antisymmetric in x
+ *   [3] number of pairs
+ * workspace: at least totri_debug_mirror_workspace(N) bytes of device memory.
+ */
+size_t totri_debug_mirror_workspace(int64_t N);
+int totri_debug_mirror_info(const float* measurement_points, int64_t N,
+                            void* workspace, size_t workspace_bytes,
+                            int32_t* info_out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

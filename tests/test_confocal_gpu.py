@@ -40,6 +40,22 @@ def gpu_a1(G, mp, res, w, off, origin=ORIGIN, size=SIZE):
     return out.cpu().numpy().astype(np.float64)
 
 
+def mirror_info(mp):
+    """[quads ok, n quads, x-pairs ok, n pairs] from the device detection."""
+    from totrilib_amd import _lib
+    lib = _lib.load()
+    m = torch.from_numpy(np.ascontiguousarray(mp)).to(dev())
+    N = m.shape[1]
+    ws = torch.empty(lib.totri_debug_mirror_workspace(N), dtype=torch.uint8, device=dev())
+    info = torch.full((4,), -7, dtype=torch.int32, device=dev())
+    _lib.check(lib.totri_debug_mirror_info(m.data_ptr(), N, ws.data_ptr(), ws.numel(),
+                                           info.data_ptr(),
+                                           torch.cuda.current_stream().cuda_stream),
+               "debug_mirror_info")
+    torch.cuda.synchronize()
+    return info.cpu().tolist()
+
+
 def gpu_decisions(idx, mp, X, Y, Z, T, w, off):
     from totrilib_amd import _lib
     lib = _lib.load()
@@ -299,10 +315,12 @@ def test_r1_mirror_quads_and_general_path(scan):
         mp = np.stack([rng.uniform(-1, 1, 64), rng.uniform(-1, 1, 64),
                        np.zeros(64)]).astype(np.float32)
     w = 0.0045
+    info = mirror_info(mp)
+    assert info[:2] == ([1, 16] if scan == "mirror" else [0, info[1]])
     got = gpu_r1(vol, mp, T, w, 0.0)
     ref = O.r1_forward(vol, mp, T, ORIGIN, SIZE, w, 0.0, mode=O.PINNED64)
     e = rel_l2(got, ref)
-    print(f"{scan}: rel-L2 vs FP64 pinned {e:.2e}")
+    print(f"{scan}: mirror info {info}, rel-L2 vs FP64 pinned {e:.2e}")
     assert e < TOL
     assert np.count_nonzero(got) > 0
 
@@ -323,8 +341,10 @@ def test_a1_mirror_pairs_and_general_path(scan):
                        np.zeros(64)]).astype(np.float32)
     G = rng.standard_normal((mp.shape[1], T)).astype(np.float32)
     w = 0.0045
+    info = mirror_info(mp)
+    assert info[2:] == ([1, 32] if scan == "mirror" else [0, info[3]])
     got = gpu_a1(G, mp, (X, Y, Z), w, 0.0)
     ref = O.a1_transpose(G, mp, X, Y, Z, ORIGIN, SIZE, w, 0.0, mode=O.PINNED64)
     e = rel_l2(got, ref)
-    print(f"{scan}: rel-L2 vs FP64 pinned {e:.2e}")
+    print(f"{scan}: mirror info {info}, rel-L2 vs FP64 pinned {e:.2e}")
     assert e < TOL

@@ -84,6 +84,8 @@ SIGNATURES: dict[str, tuple] = {
     "totri_confocal_batch_table_bytes": (_c_sz, []),
     "totri_confocal_batch_prepare": (_c_int, [_c_p, _c_int, _c_p, _c_sz]),
     "totri_confocal_batch_run": (_c_int, [_c_p, _c_p, _c_sz, _c_p]),
+    "totri_debug_mirror_workspace": (_c_sz, [_c_i64]),
+    "totri_debug_mirror_info": (_c_int, [_c_p, _c_i64, _c_p, _c_sz, _c_p, _c_p]),
     "totri_debug_confocal_samples": (
         _c_int,
         [_c_p, _c_i64, _c_p, _c_i64, _c_i64, _c_i64, _c_i64, _c_i64, _F3, _F3,

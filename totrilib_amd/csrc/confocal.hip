@@ -403,45 +403,53 @@ __global__ __launch_bounds__(kWaves * 64) void r1_forward_kernel(
 // kernel otherwise.  Results equal the general kernel's up to the FP32
 // summation order over columns.
 
-// For each scan point, the index of its x-, y- and xy-mirror image (the
-// first match; -1 if none), by comparing against every point.  One 256-thread
-// workgroup handles points block*256 .. block*256+255.
-__device__ __forceinline__ void scan_mirror_block(const float* __restrict__ mp, int N,
-                                                  int* __restrict__ partner, int block) {
-  __shared__ float sx[256], sy[256], sz[256];
-  const int i = block * 256 + threadIdx.x;
+// For each scan point, the index of its x-, y- and xy-mirror image: the
+// smallest matching index, or N and above if there is none (partner[3][N]
+// starts at INT_MAX, see mirror_init_kernel).  Work is split into wave items
+// (block of 64 points, segment of the candidates); a wave compares its
+// points with every candidate of its segment (uniform, scalar loads) and
+// merges its first matches with atomicMin.
+constexpr int kMirrorPts = 64;
+
+__host__ __device__ inline int mirror_segments(int N) {
+  const int blocks = (N + kMirrorPts - 1) / kMirrorPts;
+  int segs = 4096 / (blocks > 0 ? blocks : 1);
+  return segs < 1 ? 1 : (segs > 64 ? 64 : segs);
+}
+
+__device__ __forceinline__ void scan_mirror_item(const float* __restrict__ mp, int N,
+                                                 int* __restrict__ partner, int item) {
+  const int segs = mirror_segments(N);
+  const int blk = item / segs, seg = item - blk * segs;
+  const int per = (N + segs - 1) / segs;
+  const int j0 = seg * per, j1 = min(N, j0 + per);
+  const int i = blk * kMirrorPts + (int)(threadIdx.x & 63);
   const int ii = i < N ? i : 0;
   const float wx = mp[ii], wy = mp[N + ii], wz = mp[2 * N + ii];
-  int px = -1, py = -1, pxy = -1;
-  for (int j0 = 0; j0 < N; j0 += 256) {
-    __syncthreads();
-    const int jl = j0 + threadIdx.x;
-    if (jl < N) {
-      sx[threadIdx.x] = mp[jl];
-      sy[threadIdx.x] = mp[N + jl];
-      sz[threadIdx.x] = mp[2 * N + jl];
-    }
-    __syncthreads();
-    const int nj = min(256, N - j0);
-    for (int jj = 0; jj < nj; ++jj) {
-      const float x = sx[jj], y = sy[jj];
-      const bool zs = sz[jj] == wz;
-      const bool xm = x == -wx, xs = x == wx, ym = y == -wy, ys = y == wy;
-      if (px < 0 && zs && xm && ys) px = j0 + jj;
-      if (py < 0 && zs && xs && ym) py = j0 + jj;
-      if (pxy < 0 && zs && xm && ym) pxy = j0 + jj;
-    }
+  int px = 0x7fffffff, py = 0x7fffffff, pxy = 0x7fffffff;
+  for (int j = j0; j < j1; ++j) {
+    const float x = mp[j], y = mp[N + j], z = mp[2 * N + j];
+    const bool zs = z == wz;
+    const bool xm = x == -wx, xs = x == wx, ym = y == -wy, ys = y == wy;
+    if (zs && xm && ys) px = min(px, j);
+    if (zs && xs && ym) py = min(py, j);
+    if (zs && xm && ym) pxy = min(pxy, j);
   }
   if (i < N) {
-    partner[i] = px;
-    partner[N + i] = py;
-    partner[2 * N + i] = pxy;
+    if (px != 0x7fffffff) atomicMin(&partner[i], px);
+    if (py != 0x7fffffff) atomicMin(&partner[N + i], py);
+    if (pxy != 0x7fffffff) atomicMin(&partner[2 * N + i], pxy);
   }
 }
 
-__global__ __launch_bounds__(256) void scan_mirror_kernel(const float* __restrict__ mp, int N,
-                                                          int* __restrict__ partner) {
-  scan_mirror_block(mp, N, partner, (int)blockIdx.x);
+__global__ void mirror_init_kernel(int* __restrict__ partner, int n) {
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x)
+    partner[e] = 0x7fffffff;
+}
+
+__global__ __launch_bounds__(64) void scan_mirror_kernel(const float* __restrict__ mp, int N,
+                                                         int* __restrict__ partner) {
+  scan_mirror_item(mp, N, partner, (int)blockIdx.x);
 }
 
 // Inclusive scan of one int per thread over a 1024-thread workgroup.
@@ -479,15 +487,17 @@ __device__ __forceinline__ void scan_quads_block(const int* __restrict__ partner
   const int* PX = partner;
   const int* PY = partner + N;
   const int* PXY = partner + 2 * N;
+  // an index outside [0, N) means "no mirror image"
   auto quad_ok = [&](int i) {
-    const int px = PX[i], py = PY[i], pxy = PXY[i];
-    return px >= 0 && py >= 0 && pxy >= 0 && px != i && py != i && pxy != i && px != py &&
-           px != pxy && py != pxy && PX[px] == i && PY[py] == i && PX[pxy] == py &&
-           PY[pxy] == px;
+    const unsigned px = PX[i], py = PY[i], pxy = PXY[i];
+    if (px >= (unsigned)N || py >= (unsigned)N || pxy >= (unsigned)N) return false;
+    return (int)px != i && (int)py != i && (int)pxy != i && px != py && px != pxy &&
+           py != pxy && PX[px] == i && PY[py] == i && PX[pxy] == (int)py &&
+           PY[pxy] == (int)px;
   };
   auto pair_ok = [&](int i) {
-    const int px = PX[i];
-    return px >= 0 && px != i && PX[px] == i;
+    const unsigned px = PX[i];
+    return px < (unsigned)N && (int)px != i && PX[px] == i;
   };
   int qc = 0, pc = 0, qbad = 0, pbad = 0;
   for (int i = b0; i < b1; ++i) {
@@ -1465,8 +1475,10 @@ int launch_mirror_detect(const MirrorLayout& ml, const float* mp, int N, void* w
   int* quads = (int*)((char*)ws + ml.quads_off);
   int* pairs = (int*)((char*)ws + ml.pairs_off);
   int* info = (int*)((char*)ws + ml.info_off);
-  hipLaunchKernelGGL(scan_mirror_kernel, dim3((N + 255) / 256), dim3(256), 0, st, mp, N,
-                     partner);
+  hipLaunchKernelGGL(mirror_init_kernel, dim3(launch_grid((size_t)3 * N)), dim3(256), 0, st,
+                     partner, 3 * N);
+  const int items = ((N + kMirrorPts - 1) / kMirrorPts) * mirror_segments(N);
+  hipLaunchKernelGGL(scan_mirror_kernel, dim3(items), dim3(64), 0, st, mp, N, partner);
   int rc = check_launch("scan_mirror_kernel");
   if (rc) return rc;
   hipLaunchKernelGGL(scan_quads_kernel, dim3(1), dim3(1024), 0, st, partner, N, quads, pairs,
@@ -1942,4 +1954,26 @@ extern "C" int totri_volume_confocal_grad_measurement_points(
   hipLaunchKernelGGL(sum_partials_kernel<float>, dim3(launch_grid(n)), dim3(256), 0, st, part,
                      p.passes * p.r.n_groups, n, measurement_points_grad);
   return check_launch("sum_partials_kernel");
+}
+
+extern "C" size_t totri_debug_mirror_workspace(int64_t N) {
+  if (N <= 0) return 0;
+  return mirror_layout(N, 0).end;
+}
+
+extern "C" int totri_debug_mirror_info(const float* measurement_points, int64_t N,
+                                       void* workspace, size_t workspace_bytes,
+                                       int32_t* info_out, void* stream) {
+  TOTRI_CHECK_ARG(N >= 1 && N <= (1 << 24), "N=%lld outside [1, 2^24]", (long long)N);
+  TOTRI_CHECK_ARG(measurement_points && workspace && info_out, "null argument");
+  const MirrorLayout ml = mirror_layout(N, 0);
+  if (workspace_bytes < ml.end)
+    return fail(TOTRI_EWORKSPACE, "workspace too small: need %zu bytes, got %zu", ml.end,
+                workspace_bytes);
+  hipStream_t st = (hipStream_t)stream;
+  int rc = launch_mirror_detect(ml, measurement_points, (int)N, workspace, st);
+  if (rc) return rc;
+  TOTRI_CHECK_HIP(hipMemcpyAsync(info_out, (char*)workspace + ml.info_off, 4 * sizeof(int),
+                                 hipMemcpyDeviceToDevice, st));
+  return TOTRI_OK;
 }
