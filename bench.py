@@ -146,9 +146,11 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 256 MiB device-to-device copy (a known-traffic "
                          "kernel that calibrates FETCH_SIZE/WRITE_SIZE counters)")
-    ap.add_argument("--workload", choices=["confocal", "serve"], default="confocal",
+    ap.add_argument("--workload", choices=["confocal", "serve", "kernels"],
+                    default="confocal",
                     help="confocal: the render + gradient step (default); serve: the "
-                         "request mix through the serving engine")
+                         "request mix through the serving engine; kernels: each other "
+                         "operation of the path at a stated size")
     ap.add_argument("--requests", type=int, default=48, help="serve: requests per run")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -157,6 +159,8 @@ def main():
         sys.exit(2)
     if args.workload == "serve":
         serve_main(args)
+    elif args.workload == "kernels":
+        kernels_main(args)
     else:
         confocal_main(args)
 
@@ -420,6 +424,129 @@ def serve_main(args):
         "speedup_vs_per_request": value / base_value,
         "rates": rates,
     }
+    print(json.dumps(line))
+
+
+# ---------------------------------------------------------------------------
+# The other operations of the path, one at a time
+# ---------------------------------------------------------------------------
+def kernels_main(args):
+    """Times each remaining operation at a stated size (HIP events around the
+    call on its stream, after a warm-up call) and reports its algorithmic
+    rate: FLOP per unit in the rocprof-compute convention, counted from the
+    reference formulas, and the bytes it must move."""
+    import torch
+    from tests.cases import sparse_albedo, wall_grid
+    from totrilib_amd.reconstruct import (BackprojectionConfocalVelten,
+                                          BackprojectionExhaustiveArellano,
+                                          BackprojectionExhaustiveVelten)
+    from totrilib_amd.render import VolumeRenderConfocal
+    from totrilib_amd.render.volume import VolumeRenderExhaustive
+    from totrilib_amd.types import BinDef, VolumeDef
+    from totrilib_amd.util import Rbf2VolumeGaussian
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    gen = torch.Generator(device=dev).manual_seed(9)
+    reps = max(args.steps // 4, 3)
+
+    def timeit(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    rows = []
+
+    def row(op, kernel, workload, ms, units, unit, flop_per_unit, nbytes):
+        flops = units * flop_per_unit
+        rows.append(dict(op=op, kernel=kernel, workload=workload, ms=ms, units=units, unit=unit,
+                         units_per_s=units / (ms * 1e-3), flop_per_unit=flop_per_unit,
+                         tflops=flops / (ms * 1e-3) / 1e12,
+                         frac_fp32=flops / (ms * 1e-3) / PEAK_FP32,
+                         gbps=nbytes / (ms * 1e-3) / 1e9))
+        print(f"[kernels] {op}: {ms:.3f} ms", file=sys.stderr, flush=True)
+
+    box = VolumeDef((-1.0, -1.0, 0.5), (1.0, 1.0, 2.5), (64, 64, 64))
+    # A2: Velten confocal, 64^3 voxels, 32x32 scan, 512 bins of 0.01 m round trip
+    n, T, R = 32, 512, 64
+    S = n * n
+    scan = torch.from_numpy(wall_grid(n)).T.contiguous()[None].to(dev)        # [1, S, 3]
+    tr = torch.rand((1, T, S), device=dev, generator=gen)
+    ms = timeit(lambda: BackprojectionConfocalVelten.apply(tr, scan, box, BinDef(0.01, 0.0, T)))
+    row("A2 Velten confocal backprojection", "velten_confocal_kernel",
+        "64^3 voxels, 32x32 scan, 512 bins", ms, R ** 3 * S, "voxel-scan pairs", 14,
+        4 * (T * S + R ** 3))
+    # A3: Velten exhaustive, 64^3, 16x16 scan points x 16 laser points, 512 bins
+    Sx, L = 256, 16
+    sp = torch.from_numpy(wall_grid(16)).T.contiguous()[None].to(dev)         # [1, 256, 3]
+    lp = torch.from_numpy(wall_grid(4)).T.contiguous()[None].to(dev)          # [1, 16, 3]
+    trx = torch.rand((1, T, Sx, L), device=dev, generator=gen)
+    ms = timeit(lambda: BackprojectionExhaustiveVelten.apply(trx, sp, lp, box, BinDef(0.01, 0.0, T)))
+    row("A3 Velten exhaustive backprojection", "velten_exhaustive_kernel",
+        "64^3 voxels, 256 scan x 16 laser points, 512 bins", ms, R ** 3 * Sx * L,
+        "voxel-scan-laser triples", 8, 4 * (T * Sx * L + R ** 3))
+    # A4: Arellano, same sizes, 5 % of the bins nonzero
+    tra = trx * (torch.rand(trx.shape, device=dev, generator=gen) < 0.05)
+    nz = int(torch.count_nonzero(tra))
+    sp3, lp3 = sp.transpose(1, 2).contiguous(), lp.transpose(1, 2).contiguous()
+    ms = timeit(lambda: BackprojectionExhaustiveArellano.apply(tra, sp3, lp3, box, BinDef(0.01, 0.0, T)))
+    row("A4 Arellano backprojection", "arellano_kernel",
+        "64^3 voxels, 256 scan x 16 laser points, 512 bins, 5% nonzero", ms, 64 * 64 * nz,
+        "column-bin pairs (nonzero bins)", 30, 4 * (T * Sx * L + R ** 3))
+    # R2 + A5: exhaustive render and its adjoint, 64^3 (5 % occupied), 256 x 16, 512 bins
+    vol = torch.from_numpy(sparse_albedo((R, R, R), 0.05, 3)).permute(2, 1, 0).contiguous()[None].to(dev)
+    nnz = int(torch.count_nonzero(vol))
+    v = vol.clone().requires_grad_(True)
+    ms = timeit(lambda: VolumeRenderExhaustive.apply(vol, sp3, lp3, box, BinDef(0.01, 0.0, T)))
+    row("R2 exhaustive render", "r2_forward_kernel",
+        "64^3 voxels (5% occupied), 256 scan x 16 laser points, 512 bins", ms, nnz * Sx * L,
+        "occupied-voxel-scan-laser triples", 40, 4 * (T * Sx * L + R ** 3))
+    out = VolumeRenderExhaustive.apply(v, sp3, lp3, box, BinDef(0.01, 0.0, T))
+    G = torch.randn(out.shape, device=dev, generator=gen)
+    ms = timeit(lambda: torch.autograd.grad(out, v, G, retain_graph=True))
+    row("A5 exhaustive render adjoint", "r2_adjoint_kernel",
+        "64^3 voxels, 256 scan x 16 laser points, 512 bins", ms, R ** 3 * Sx * L,
+        "voxel-scan-laser triples", 40, 4 * (T * Sx * L + R ** 3))
+    # S1 + S2: 20,000 RBFs -> 128^3, truncated at 3 sigma
+    M, RR = 20000, 128
+    rbf = torch.cat([torch.rand((1, M, 2), device=dev, generator=gen) * 2 - 1,
+                     torch.rand((1, M, 1), device=dev, generator=gen) * 2 + 0.5,
+                     torch.rand((1, M, 1), device=dev, generator=gen) * 0.04 + 0.01,
+                     torch.rand((1, M, 1), device=dev, generator=gen)], 2)
+    dv = 2.0 / RR
+    vd = VolumeDef((-1 + dv / 2, -1 + dv / 2, 0.5 + dv / 2), (1 - dv / 2, 1 - dv / 2, 2.5 - dv / 2),
+                   (RR, RR, RR))
+    sig = rbf[0, :, 3].double()
+    in_support = float(torch.sum(4.0 / 3.0 * np.pi * (3 * sig) ** 3) / dv ** 3)
+    ms = timeit(lambda: Rbf2VolumeGaussian.apply(rbf, vd, 3.0))
+    row("S1 RBF splat", "splat_forward_kernel", "20,000 RBFs -> 128^3, 3 sigma", ms, in_support,
+        "voxel-RBF pairs within 3 sigma", 14, 4 * (2 * RR ** 3 + 5 * M))
+    r = rbf.clone().requires_grad_(True)
+    V = Rbf2VolumeGaussian.apply(r, vd, 3.0)
+    GV = torch.randn(V.shape, device=dev, generator=gen)
+    ms = timeit(lambda: torch.autograd.grad(V, r, GV, retain_graph=True))
+    row("S2 RBF splat gradient", "splat_grad_kernel", "20,000 RBFs -> 128^3, 3 sigma", ms,
+        in_support, "voxel-RBF pairs within 3 sigma", 25, 4 * (4 * RR ** 3 + 10 * M))
+    # A6: scan-point gradient, 128^3 / 64x64 / 1024 bins
+    mp = torch.from_numpy(wall_grid(64)).to(dev).requires_grad_(True)
+    alb = torch.from_numpy(sparse_albedo((128, 128, 128), 0.03, 1)).to(dev)
+    w = WORKLOAD["w_round_trip"] / 2
+    trc = VolumeRenderConfocal.apply(alb, mp, ORIGIN, SIZE, 1024, w, 0.0)
+    Gc = torch.randn(trc.shape, device=dev, generator=gen)
+    ms = timeit(lambda: torch.autograd.grad(trc, mp, Gc, retain_graph=True))
+    fm = flop_model(mp.detach().cpu().numpy(), 128, 1024, w)
+    row("A6 scan-point gradient", "a6_kernel", "128^3 voxels, 64x64 scan, 1024 bins", ms,
+        fm["a_fwd"] * 128 ** 3 * 4096, "active samples", 30, 4 * (2 * 4096 * 1024 + 128 ** 3))
+    line = {"metric": "per-operation time and algorithmic rate", "value": None, "unit": "ms",
+            "n_gpus": 1, "steps": reps, "warmup": 1, "higher_is_better": False,
+            "dtype": "f32", "data": "synthetic (seeded)", "rows": rows,
+            "peak_fp32_tflops": PEAK_FP32 / 1e12, "peak_hbm_gbps": PEAK_HBM / 1e9}
     print(json.dumps(line))
 
 

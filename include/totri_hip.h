@@ -162,8 +162,9 @@ int totri_backprojection_exhaustive_velten_forward(
  *   written (the reference accumulates into a zeroed volume).  As in the
  *   reference, the interpolation weight between neighbouring voxels is 0.
  */
-size_t totri_backprojection_exhaustive_arellano_workspace(int64_t B, int64_t T, int64_t S,
-                                                          int64_t L);
+size_t totri_backprojection_exhaustive_arellano_workspace(int64_t B, int64_t T,
+                                                         int64_t S, int64_t L, int64_t D,
+                                                         int64_t H, int64_t W);
 int totri_backprojection_exhaustive_arellano_forward(
     const float* transient, int64_t B, int64_t T, int64_t S, int64_t L,
     const float* scan_points, const float* laser_points, const float* scan_origin,

@@ -61,20 +61,30 @@ def test_velten_exhaustive_matches_oracle():
     assert np.array_equal(got.astype(np.float64), ref), rel_l2(got, ref)
 
 
-def test_arellano_matches_oracle():
+@pytest.mark.parametrize("B,res,S,L", [(1, (40, 20, 24), 4, 3), (2, (300, 4, 6), 5, 2)])
+def test_arellano_matches_oracle(B, res, S, L):
+    """Voxel decisions bit for bit: with a transient of ones every voxel holds
+    its exact hit count, whatever the summation order.  Sums of a sparse
+    random transient (positive bins are listed and split over waves, then
+    partial volumes are added in order) agree with the sequential FP32 oracle
+    to rounding.  D = 300 exercises the global-memory column path."""
     from totrilib_amd.reconstruct import BackprojectionExhaustiveArellano
     from totrilib_amd.types import BinDef, VolumeDef
     rng = np.random.default_rng(53)
-    B, T, S, L = 1, 256, 4, 3
-    res = (40, 20, 24)
+    T = 256
     tr = (rng.random((B, T, S, L)) * (rng.random((B, T, S, L)) < 0.3)).astype(np.float32)
     scan = np.concatenate([rng.uniform(-1, 1, (B, 2, S)), np.zeros((B, 1, S))], 1).astype(np.float32)
     laser = np.concatenate([rng.uniform(-1, 1, (B, 2, L)), np.zeros((B, 1, L))], 1).astype(np.float32)
     vd, bd = VolumeDef(VS, VE, res), BinDef(0.025, 0.0, T)
+    ones = np.ones_like(tr)
+    got1 = BackprojectionExhaustiveArellano.apply(t(ones), t(scan), t(laser), vd, bd).cpu().numpy()
+    ref1 = O.arellano(ones, scan, laser, None, None, res, VS, VE, 0.025, 0.0, mode=O.FP32)
+    assert np.count_nonzero(ref1) > 100
+    assert np.array_equal(got1.astype(np.float64), ref1)
     got = BackprojectionExhaustiveArellano.apply(t(tr), t(scan), t(laser), vd, bd).cpu().numpy()
     ref = O.arellano(tr, scan, laser, None, None, res, VS, VE, 0.025, 0.0, mode=O.FP32)
-    assert np.count_nonzero(ref) > 100
-    assert np.array_equal(got.astype(np.float64), ref), rel_l2(got, ref)
+    assert np.array_equal(got != 0, ref != 0)
+    assert rel_l2(got, ref) < 1e-6
 
 
 def test_exhaustive_render_and_adjoint_match_oracle():

@@ -58,7 +58,7 @@ SIGNATURES: dict[str, tuple] = {
         _c_int,
         [_c_p, _c_i64, _c_i64, _c_i64, _c_i64, _c_p, _c_p, _c_p, _c_i64, _c_p, _c_i64,
          _c_p, _c_i64, _c_i64, _c_i64, _F3, _F3, _c_f, _c_f, _c_p, _c_sz, _c_p]),
-    "totri_backprojection_exhaustive_arellano_workspace": (_c_sz, [_c_i64] * 4),
+    "totri_backprojection_exhaustive_arellano_workspace": (_c_sz, [_c_i64] * 7),
     "totri_backprojection_exhaustive_arellano_forward": (
         _c_int,
         [_c_p, _c_i64, _c_i64, _c_i64, _c_i64, _c_p, _c_p, _c_p, _c_i64, _c_p, _c_i64,

@@ -15,14 +15,17 @@
 // rows of a group of scan points in LDS and every lane gathers its bin from
 // there.  Sums run over scan (and laser) points in the reference's order.
 //
-// A4 (scatter): one lane per volume column (x, y); the column's D voxels are
-// accumulated in LDS slots owned by that lane (row-major [D][64]), so the
-// reference's global atomics become plain read-modify-writes.  Scan/laser
-// pairs and bins are uniform across the wave; a bin's transient value is
-// read with one scalar load.
+// A4 (scatter): the positive bins of the transient are first listed in
+// (scan, laser, bin) order (stream compaction) and the list is split into
+// chunks.  A wave owns 64 volume columns (x, y) and one chunk; the columns'
+// D voxels are accumulated in LDS slots owned by their lanes (row-major
+// [D][64]), so the reference's global atomics become plain read-modify-
+// writes; list entries are uniform across the wave.  Chunks write partial
+// volumes that are summed in chunk order.
 #include <math.h>
 
 #include "common.h"
+#include "compact.h"
 
 namespace totri_hip {
 namespace {
@@ -178,18 +181,57 @@ __global__ __launch_bounds__(256) void velten_exhaustive_kernel(
 // ---------------------------------------------------------------------------
 // A4: Arellano ellipsoid backprojection (backprojection_cuda.cu:10-115).
 // ---------------------------------------------------------------------------
+// Geometry of one scan/laser pair (backprojection_cuda.cu:51-74), shared by
+// all of the pair's bins.
+struct EllPair {
+  float sd, ld, len, nx, ny, nz, cx, cy, cz;
+};
+
+__device__ __forceinline__ EllPair ell_pair(const float* sb, const float* lb,
+                                            const float* sob, const float* lob, int S,
+                                            int L, int So, int Lo, int s, int l) {
+  EllPair e;
+  const float spx = sb[s], spy = sb[S + s], spz = sb[2 * S + s];
+  const float lpx = lb[l], lpy = lb[L + l], lpz = lb[2 * L + l];
+  e.sd = So > 0 ? dist3(spx, spy, spz, sob[s], sob[So + s], sob[2 * So + s]) : 0.f;
+  e.ld = Lo > 0 ? dist3(lpx, lpy, lpz, lob[l], lob[Lo + l], lob[2 * Lo + l]) : 0.f;
+  e.cx = 0.5f * (spx + lpx);
+  e.cy = 0.5f * (spy + lpy);
+  e.cz = 0.5f * (spz + lpz);
+  float nx = spx - lpx, ny = spy - lpy, nz = spz - lpz;                      // :62-66
+  float ls = nx * nx;
+  ls = ls + ny * ny;
+  ls = ls + nz * nz;
+  e.len = sqrt_rn(ls);                                                       // :67
+  if (e.len > 1.e-6f) {                                                      // :68-70
+    const float inv = 1.f / e.len;
+    nx = inv * nx;
+    ny = inv * ny;
+    nz = inv * nz;
+  }
+  e.nx = nx;
+  e.ny = ny;
+  e.nz = nz;
+  return e;
+}
+
+// One wave: columns tile*64 .. tile*64+63 of batch b, list chunk `chunk` of
+// n_chunks.  dst is the chunk's partial volume [B][D][H][W] (written).
 template <bool IN_LDS>
 __global__ __launch_bounds__(64) void arellano_kernel(
     const float* __restrict__ trT,     // [B][S][L][T]
+    const int* __restrict__ list,      // [B][S*L*T] positive bins, in order
+    const int* __restrict__ totals,    // [B]
     const float* __restrict__ scan,    // [B][3][S]
     const float* __restrict__ laser,   // [B][3][L]
     const float* __restrict__ sorig,   // [B][3][So]
     const float* __restrict__ lorig,   // [B][3][Lo]
-    float* __restrict__ vol,           // [B][D][H][W], written
-    BoxGeom g, int S, int L, int So, int Lo) {
+    float* __restrict__ dst,           // [n_chunks][B][D][H][W]
+    BoxGeom g, int S, int L, int So, int Lo, int n_chunks) {
   extern __shared__ float colacc[];    // [D][64] when IN_LDS
   const int lane = threadIdx.x;
-  const int b = blockIdx.y;
+  const int chunk = blockIdx.y;
+  const int b = blockIdx.z;
   const int ncol = g.H * g.W;
   const int v = blockIdx.x * 64 + lane;
   const bool ok = v < ncol;
@@ -197,97 +239,115 @@ __global__ __launch_bounds__(64) void arellano_kernel(
   const int xi = vv % g.W, yi = vv / g.W;                                    // :27-28
   const float xf = inclusive_coord(xi, g.W, g.sx, g.ex);                     // :29-30
   const float yf = inclusive_coord(yi, g.H, g.sy, g.ey);                     // :31-32
-  float* gcol = vol + (size_t)b * g.D * ncol + vv;                           // stride ncol
+  float* gcol = dst + ((size_t)chunk * g.B + b) * g.D * ncol + vv;           // stride ncol
   if (IN_LDS) {
     for (int z = 0; z < g.D; ++z) colacc[z * 64 + lane] = 0.f;
   } else if (ok) {
     for (int z = 0; z < g.D; ++z) gcol[(size_t)z * ncol] = 0.f;
   }
+  const int n_elem = S * L * g.T;
+  const int total = totals[b];
+  const int per = (total + n_chunks - 1) / n_chunks;
+  const int k0 = chunk * per, k1 = min(total, k0 + per);
+  const int* lb_list = list + (size_t)b * n_elem;
+  const float* tb = trT + (size_t)b * n_elem;
   const float* sb = scan + (size_t)b * 3 * S;
   const float* lb = laser + (size_t)b * 3 * L;
   const float* sob = sorig + (size_t)b * 3 * So;
   const float* lob = lorig + (size_t)b * 3 * Lo;
-  for (int s = 0; s < S; ++s) {
-    const float spx = sb[s], spy = sb[S + s], spz = sb[2 * S + s];
-    const float sd = So > 0 ? dist3(spx, spy, spz, sob[s], sob[So + s], sob[2 * So + s]) : 0.f;
-    for (int l = 0; l < L; ++l) {
-      const float lpx = lb[l], lpy = lb[L + l], lpz = lb[2 * L + l];
-      const float ld = Lo > 0 ? dist3(lpx, lpy, lpz, lob[l], lob[Lo + l], lob[2 * Lo + l]) : 0.f;
-      const float cx = 0.5f * (spx + lpx), cy = 0.5f * (spy + lpy), cz = 0.5f * (spz + lpz);
-      float nx = spx - lpx, ny = spy - lpy, nz = spz - lpz;                  // :62-66
-      float ls = nx * nx;
-      ls = ls + ny * ny;
-      ls = ls + nz * nz;
-      const float len = sqrt_rn(ls);                                         // :67
-      if (len > 1.e-6f) {                                                    // :68-70
-        const float inv = 1.f / len;
-        nx = inv * nx;
-        ny = inv * ny;
-        nz = inv * nz;
-      }
-      const float rsx = xf - cx, rsy = yf - cy, rsz = g.sz - cz;             // :72
-      const float rex = xf - cx, rey = yf - cy, rez = g.ez - cz;             // :74
-      float dps = rsx * nx;
-      dps = dps + rsy * ny;
-      dps = dps + rsz * nz;
-      float dpe = rex * nx;
-      dpe = dpe + rey * ny;
-      dpe = dpe + rez * nz;
-      const float* row = trT + ((size_t)(b * S + s) * L + l) * g.T;
-      for (int ti = 0; ti < g.T; ++ti) {                                     // :75
-        const float value = row[ti];
-        if (!(value > 0.f)) continue;                                        // :77
-        float tc = (float)ti * g.w;                                          // :79-80
-        tc = tc + g.off;
-        tc = tc - sd;
-        tc = tc - ld;
-        if (!(tc > len)) continue;                                           // :81
-        const float r0 = tc / 2.f;                                           // :83
-        float r1s = r0 * r0;
-        r1s = r1s - (len * len) / 4.f;                                       // :84
-        const float r1 = sqrt_rn(r1s);
-        const float sc = r1 / r0;                                            // :85
-        const float ks = (sc - 1.f) * dps;                                   // :87
-        const float stx = rsx + ks * nx, sty = rsy + ks * ny, stz = rsz + ks * nz;
-        const float ke = (sc - 1.f) * dpe;                                   // :88
-        const float etx = rex + ke * nx, ety = rey + ke * ny, etz = rez + ke * nz;
-        float dx = etx - stx, dy = ety - sty, dz = etz - stz;                // :89
-        float rl = dx * dx;
-        rl = rl + dy * dy;
-        rl = rl + dz * dz;
-        const float rlen = sqrt_rn(rl);                                      // :90
-        const float ir = 1.f / rlen;                                         // :91
-        dx = ir * dx;
-        dy = ir * dy;
-        dz = ir * dz;
-        float v0 = dx * stx;                                                 // :93
-        v0 = v0 + dy * sty;
-        v0 = v0 + dz * stz;
-        float ss = stx * stx;
-        ss = ss + sty * sty;
-        ss = ss + stz * stz;
-        float v1 = v0 * v0 - (ss - r1 * r1);                                 // :94
-        if (!(v1 > 0.f) || !ok) continue;                                    // :95
-        v1 = sqrt_rn(v1);
-        float d = -v0 - v1;
-        if (d < 0.f) d = -v0 + v1;                                           // :97-100
-        float bf = d / rlen;
-        bf = bf * (float)g.D;
-        bf = bf + 0.5f;
-        const int bini = (int)bf;                                            // :101-102
-        // :103-109: the interpolation weight is int - int = 0, so only bini
-        // receives the value.
-        if (bini >= 0 && bini < g.D) {
-          if (IN_LDS)
-            colacc[bini * 64 + lane] += value;
-          else
-            gcol[(size_t)bini * ncol] += value;
-        }
-      }
+  int cur_pair = -1;
+  EllPair e;
+  float rsx = 0.f, rsy = 0.f, rsz = 0.f, rex = 0.f, rey = 0.f, rez = 0.f, dps = 0.f, dpe = 0.f;
+  for (int k = k0; k < k1; ++k) {
+    const int idx = uniform_i32(lb_list[k]);
+    const int pair = idx / g.T;                                              // s * L + l
+    const int ti = idx - pair * g.T;
+    if (pair != cur_pair) {
+      cur_pair = pair;
+      const int s = pair / L, l = pair - s * L;
+      e = ell_pair(sb, lb, sob, lob, S, L, So, Lo, s, l);
+      rsx = xf - e.cx;                                                       // :72
+      rsy = yf - e.cy;
+      rsz = g.sz - e.cz;
+      rex = xf - e.cx;                                                       // :74
+      rey = yf - e.cy;
+      rez = g.ez - e.cz;
+      dps = rsx * e.nx;
+      dps = dps + rsy * e.ny;
+      dps = dps + rsz * e.nz;
+      dpe = rex * e.nx;
+      dpe = dpe + rey * e.ny;
+      dpe = dpe + rez * e.nz;
+    }
+    const float value = tb[idx];                                             // > 0 (:77)
+    float tc = (float)ti * g.w;                                              // :79-80
+    tc = tc + g.off;
+    tc = tc - e.sd;
+    tc = tc - e.ld;
+    if (!(tc > e.len)) continue;                                             // :81
+    const float r0 = tc / 2.f;                                               // :83
+    float r1s = r0 * r0;
+    r1s = r1s - (e.len * e.len) / 4.f;                                       // :84
+    const float r1 = sqrt_rn(r1s);
+    const float sc = r1 / r0;                                                // :85
+    const float ks = (sc - 1.f) * dps;                                       // :87
+    const float stx = rsx + ks * e.nx, sty = rsy + ks * e.ny, stz = rsz + ks * e.nz;
+    const float ke = (sc - 1.f) * dpe;                                       // :88
+    const float etx = rex + ke * e.nx, ety = rey + ke * e.ny, etz = rez + ke * e.nz;
+    float dx = etx - stx, dy = ety - sty, dz = etz - stz;                    // :89
+    float rl = dx * dx;
+    rl = rl + dy * dy;
+    rl = rl + dz * dz;
+    const float rlen = sqrt_rn(rl);                                          // :90
+    const float ir = 1.f / rlen;                                             // :91
+    dx = ir * dx;
+    dy = ir * dy;
+    dz = ir * dz;
+    float v0 = dx * stx;                                                     // :93
+    v0 = v0 + dy * sty;
+    v0 = v0 + dz * stz;
+    float ss = stx * stx;
+    ss = ss + sty * sty;
+    ss = ss + stz * stz;
+    float v1 = v0 * v0 - (ss - r1 * r1);                                     // :94
+    if (!(v1 > 0.f) || !ok) continue;                                        // :95
+    v1 = sqrt_rn(v1);
+    float d = -v0 - v1;
+    if (d < 0.f) d = -v0 + v1;                                               // :97-100
+    float bf = d / rlen;
+    bf = bf * (float)g.D;
+    bf = bf + 0.5f;
+    const int bini = (int)bf;                                                // :101-102
+    // :103-109: the interpolation weight is int - int = 0, so only bini
+    // receives the value.
+    if (bini >= 0 && bini < g.D) {
+      if (IN_LDS)
+        colacc[bini * 64 + lane] += value;
+      else
+        gcol[(size_t)bini * ncol] += value;
     }
   }
   if (IN_LDS && ok)
     for (int z = 0; z < g.D; ++z) gcol[(size_t)z * ncol] = colacc[z * 64 + lane];
+}
+
+__global__ void zero_kernel(float* __restrict__ out, size_t n) {
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (size_t)gridDim.x * blockDim.x)
+    out[e] = 0.f;
+}
+
+size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+// out[e] = sum over parts p of part[p * n + e], in part order.
+__global__ void sum_parts_kernel(const float* __restrict__ part, int n_parts, size_t n,
+                                 float* __restrict__ out) {
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (size_t)gridDim.x * blockDim.x) {
+    float v = part[e];
+    for (int p = 1; p < n_parts; ++p) v += part[(size_t)p * n + e];
+    out[e] = v;
+  }
 }
 
 BoxGeom box_geom(int64_t B, int64_t D, int64_t H, int64_t W, int64_t T,
@@ -420,9 +480,46 @@ extern "C" int totri_backprojection_exhaustive_velten_forward(
   return check_launch("velten_exhaustive_kernel");
 }
 
-extern "C" size_t totri_backprojection_exhaustive_arellano_workspace(int64_t B, int64_t T,
-                                                                     int64_t S, int64_t L) {
-  return (size_t)B * T * S * L * sizeof(float);
+namespace totri_hip {
+namespace {
+
+// Arellano plan: chunks of the positive-bin list per column tile, enough
+// waves to fill the device, partial volumes capped at 1 GiB.  Workspace:
+// transposed transient | compaction | partial volumes (when chunks > 1).
+struct ArellanoPlan {
+  int chunks;
+  size_t tr_bytes;
+  CompactLayout cl;
+  size_t part_off, ws;
+};
+
+ArellanoPlan plan_arellano(int64_t B, int64_t T, int64_t S, int64_t L, int64_t D, int64_t H,
+                           int64_t W) {
+  ArellanoPlan p;
+  const int64_t n = T * S * L;
+  const int64_t V = D * H * W;
+  const int64_t tiles = ((H * W + 63) / 64) * (B > 0 ? B : 1);
+  const int64_t target = 8 * (int64_t)device_cu_count();
+  int64_t C = tiles > 0 ? (target + tiles - 1) / tiles : 1;
+  if (C > 64) C = 64;
+  const int64_t cap = (int64_t)1 << 28;  // floats of partial volumes
+  if (B * V > 0 && C * B * V > cap) C = cap / (B * V);
+  if (C < 1) C = 1;
+  p.chunks = (int)C;
+  p.tr_bytes = align256((size_t)B * n * sizeof(float));
+  p.cl = compact_layout(B, n, p.tr_bytes);
+  p.part_off = p.cl.end;
+  p.ws = p.part_off + (C > 1 ? align256((size_t)C * B * V * sizeof(float)) : 0);
+  return p;
+}
+
+}  // namespace
+}  // namespace totri_hip
+
+extern "C" size_t totri_backprojection_exhaustive_arellano_workspace(
+    int64_t B, int64_t T, int64_t S, int64_t L, int64_t D, int64_t H, int64_t W) {
+  if (B <= 0 || T <= 0 || S <= 0 || L <= 0 || D <= 0 || H <= 0 || W <= 0) return 0;
+  return plan_arellano(B, T, S, L, D, H, W).ws;
 }
 
 extern "C" int totri_backprojection_exhaustive_arellano_forward(
@@ -438,31 +535,43 @@ extern "C" int totri_backprojection_exhaustive_arellano_forward(
   TOTRI_CHECK_ARG(volume_start && volume_end, "null volume box");
   hipStream_t st = (hipStream_t)stream;
   const size_t ncol = (size_t)H * W;
-  if (B == 0 || ncol == 0 || D == 0) return TOTRI_OK;
+  const size_t nv = (size_t)B * D * ncol;
+  if (nv == 0) return TOTRI_OK;
   TOTRI_CHECK_ARG(volume != nullptr, "null volume");
-  const BoxGeom g = box_geom(B, D, H, W, T, volume_start, volume_end, bin_width, bin_offset);
-  const size_t need = totri_backprojection_exhaustive_arellano_workspace(B, T, S, L);
-  if (T > 0 && S > 0 && L > 0) {
-    TOTRI_CHECK_ARG(transient && scan_points && laser_points, "null input");
-    TOTRI_CHECK_ARG((So == 0 || scan_origin) && (Lo == 0 || laser_origin), "null origin");
-    if (workspace_bytes < need)
-      return fail(TOTRI_EWORKSPACE, "workspace too small: need %zu bytes, got %zu", need,
-                  workspace_bytes);
-    TOTRI_CHECK_ARG(workspace != nullptr, "null workspace");
-    rc = launch_transpose(transient, (float*)workspace, (int)B, (int)T, (int)(S * L), st);
-    if (rc) return rc;
-  } else {
-    S = 0;  // nothing to backproject: the kernel only zero-fills
+  if (T == 0 || S == 0 || L == 0) {
+    // nothing to backproject: the written volume is zero
+    hipLaunchKernelGGL(zero_kernel, dim3(grid1d(nv)), dim3(256), 0, st, volume, nv);
+    return check_launch("zero_kernel");
   }
-  const dim3 grid((unsigned)((ncol + 63) / 64), (unsigned)B);
+  TOTRI_CHECK_ARG(transient && scan_points && laser_points, "null input");
+  TOTRI_CHECK_ARG((So == 0 || scan_origin) && (Lo == 0 || laser_origin), "null origin");
+  const ArellanoPlan p = plan_arellano(B, T, S, L, D, H, W);
+  if (workspace_bytes < p.ws)
+    return fail(TOTRI_EWORKSPACE, "workspace too small: need %zu bytes, got %zu", p.ws,
+                workspace_bytes);
+  TOTRI_CHECK_ARG(workspace != nullptr, "null workspace");
+  const BoxGeom g = box_geom(B, D, H, W, T, volume_start, volume_end, bin_width, bin_offset);
+  float* trT = (float*)workspace;
+  rc = launch_transpose(transient, trT, (int)B, (int)T, (int)(S * L), st);
+  if (rc) return rc;
+  rc = launch_compact(trT, (int)B, (int)(T * S * L), p.cl, workspace, Positive(), st);
+  if (rc) return rc;
+  const int* totals = (const int*)((char*)workspace + p.cl.totals_off);
+  const int* list = (const int*)((char*)workspace + p.cl.list_off);
+  float* dst = p.chunks > 1 ? (float*)((char*)workspace + p.part_off) : volume;
+  const dim3 grid((unsigned)((ncol + 63) / 64), (unsigned)p.chunks, (unsigned)B);
   if (D <= 256) {
     hipLaunchKernelGGL(arellano_kernel<true>, grid, dim3(64), (size_t)D * 64 * sizeof(float),
-                       st, (const float*)workspace, scan_points, laser_points, scan_origin,
-                       laser_origin, volume, g, (int)S, (int)L, (int)So, (int)Lo);
+                       st, trT, list, totals, scan_points, laser_points, scan_origin,
+                       laser_origin, dst, g, (int)S, (int)L, (int)So, (int)Lo, p.chunks);
   } else {
-    hipLaunchKernelGGL(arellano_kernel<false>, grid, dim3(64), 0, st, (const float*)workspace,
-                       scan_points, laser_points, scan_origin, laser_origin, volume, g, (int)S,
-                       (int)L, (int)So, (int)Lo);
+    hipLaunchKernelGGL(arellano_kernel<false>, grid, dim3(64), 0, st, trT, list, totals,
+                       scan_points, laser_points, scan_origin, laser_origin, dst, g, (int)S,
+                       (int)L, (int)So, (int)Lo, p.chunks);
   }
-  return check_launch("arellano_kernel");
+  rc = check_launch("arellano_kernel");
+  if (rc || p.chunks == 1) return rc;
+  hipLaunchKernelGGL(sum_parts_kernel, dim3(grid1d(nv)), dim3(256), 0, st, dst, p.chunks, nv,
+                     volume);
+  return check_launch("sum_parts_kernel");
 }

@@ -22,6 +22,7 @@
 #include <math.h>
 
 #include "common.h"
+#include "compact.h"
 
 namespace totri_hip {
 namespace {
@@ -100,67 +101,6 @@ __device__ __forceinline__ Span voxel_span(const ExGeom& g, float occ, float xf,
 }
 
 // ---- compaction of occupied voxels (index order) -------------------------
-constexpr int kCompactBlock = 1024;
-
-__global__ void count_nonzero_kernel(const float* __restrict__ vol, int nvox,
-                                     int* __restrict__ counts, int nblocks) {
-  const int b = blockIdx.y;
-  const int base = blockIdx.x * kCompactBlock;
-  int c = 0;
-  for (int e = threadIdx.x; e < kCompactBlock; e += blockDim.x) {
-    const int v = base + e;
-    if (v < nvox && vol[(size_t)b * nvox + v] != 0.f) ++c;
-  }
-  __shared__ int red[256];
-  red[threadIdx.x] = c;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) counts[(size_t)b * nblocks + blockIdx.x] = red[0];
-}
-
-// Exclusive scan of the per-block counts of one batch (one workgroup per b).
-__global__ void scan_counts_kernel(int* __restrict__ counts, int nblocks,
-                                   int* __restrict__ totals) {
-  const int b = blockIdx.x;
-  int* c = counts + (size_t)b * nblocks;
-  if (threadIdx.x == 0) {
-    int run = 0;
-    for (int k = 0; k < nblocks; ++k) {
-      const int v = c[k];
-      c[k] = run;
-      run += v;
-    }
-    totals[b] = run;
-  }
-}
-
-__global__ void scatter_nonzero_kernel(const float* __restrict__ vol, int nvox,
-                                       const int* __restrict__ offsets, int nblocks,
-                                       int* __restrict__ list) {
-  const int b = blockIdx.y;
-  const int base = blockIdx.x * kCompactBlock;
-  __shared__ int wave_tot[4];
-  int out = offsets[(size_t)b * nblocks + blockIdx.x];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int e0 = 0; e0 < kCompactBlock; e0 += blockDim.x) {
-    const int v = base + e0 + threadIdx.x;
-    const bool nz = v < nvox && vol[(size_t)b * nvox + v] != 0.f;
-    const unsigned long long m = __ballot(nz);
-    const int before = __popcll(m & ((1ull << lane) - 1ull));
-    if (lane == 0) wave_tot[wave] = __popcll(m);
-    __syncthreads();
-    int wbase = 0;
-    for (int k = 0; k < wave; ++k) wbase += wave_tot[k];
-    const int step = wave_tot[0] + wave_tot[1] + wave_tot[2] + wave_tot[3];
-    if (nz) list[(size_t)b * nvox + out + wbase + before] = v;
-    out += step;
-    __syncthreads();
-  }
-}
-
 // ---- R2 -------------------------------------------------------------------
 __global__ __launch_bounds__(kR2Waves * 64) void r2_forward_kernel(
     const float* __restrict__ vol, const int* __restrict__ list,
@@ -294,10 +234,7 @@ using namespace totri_hip;
 
 extern "C" size_t totri_volume_render_exhaustive_forward_workspace(int64_t B, int64_t D,
                                                                    int64_t H, int64_t W) {
-  const size_t nvox = (size_t)D * H * W;
-  const size_t nblocks = (nvox + kCompactBlock - 1) / kCompactBlock;
-  return align256((size_t)B * nblocks * sizeof(int)) + align256((size_t)B * sizeof(int)) +
-         align256((size_t)B * nvox * sizeof(int));
+  return compact_layout(B, D * H * W, 0).end;
 }
 
 extern "C" int totri_volume_render_exhaustive_forward(
@@ -322,29 +259,17 @@ extern "C" int totri_volume_render_exhaustive_forward(
                 workspace_bytes);
   TOTRI_CHECK_ARG(workspace != nullptr, "null workspace");
   TOTRI_CHECK_ARG(nvox == 0 || volume != nullptr, "null volume");
-  const int nblocks = (int)((nvox + kCompactBlock - 1) / kCompactBlock);
+  const CompactLayout cl = compact_layout(B, (int64_t)nvox, 0);
   char* ws = (char*)workspace;
-  int* counts = (int*)ws;
-  int* totals = (int*)(ws + align256((size_t)B * nblocks * sizeof(int)));
-  int* list = (int*)(ws + align256((size_t)B * nblocks * sizeof(int)) +
-                     align256((size_t)B * sizeof(int)));
+  const int* totals = (const int*)(ws + cl.totals_off);
+  const int* list = (const int*)(ws + cl.list_off);
   if (nvox > 0) {
-    hipLaunchKernelGGL(count_nonzero_kernel, dim3(nblocks, (unsigned)B), dim3(256), 0, st,
-                       volume, (int)nvox, counts, nblocks);
-    rc = check_launch("count_nonzero_kernel");
-    if (rc) return rc;
-    hipLaunchKernelGGL(scan_counts_kernel, dim3((unsigned)B), dim3(64), 0, st, counts,
-                       nblocks, totals);
-    rc = check_launch("scan_counts_kernel");
-    if (rc) return rc;
-    hipLaunchKernelGGL(scatter_nonzero_kernel, dim3(nblocks, (unsigned)B), dim3(256), 0, st,
-                       volume, (int)nvox, counts, nblocks, list);
-    rc = check_launch("scatter_nonzero_kernel");
+    rc = launch_compact(volume, (int)B, (int)nvox, cl, workspace, NonZero(), st);
     if (rc) return rc;
   } else {
-    hipLaunchKernelGGL(scan_counts_kernel, dim3((unsigned)B), dim3(64), 0, st, counts, 0,
-                       totals);
-    rc = check_launch("scan_counts_kernel");
+    hipLaunchKernelGGL(compact_scan_kernel, dim3((unsigned)B), dim3(64), 0, st,
+                       (int*)(ws + cl.counts_off), 0, (int*)(ws + cl.totals_off));
+    rc = check_launch("compact_scan_kernel");
     if (rc) return rc;
   }
   hipLaunchKernelGGL(r2_forward_kernel, dim3((unsigned)(S * L), (unsigned)B),

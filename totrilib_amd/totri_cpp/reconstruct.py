@@ -128,8 +128,8 @@ def backprojection_exhaustive_arellano_forward(transient, scan_points, laser_poi
     lib = _lib.load()
     out = OutputSlot(volume)
     with torch.cuda.device(dev):
-        ws, wsp = workspace(lib.totri_backprojection_exhaustive_arellano_workspace(B, T, S, L),
-                            dev)
+        ws, wsp = workspace(
+            lib.totri_backprojection_exhaustive_arellano_workspace(B, T, S, L, D, H, W), dev)
         rc = lib.totri_backprojection_exhaustive_arellano_forward(
             tr.data_ptr(), B, T, S, L, sp.data_ptr(), lp.data_ptr(), so.data_ptr(), So,
             lo.data_ptr(), Lo, out.ptr(), D, H, W, _lib.f3(volume_start), _lib.f3(volume_end),
