@@ -670,31 +670,41 @@ __device__ __forceinline__ void r1_sym_block(
         const float cA = lane_value(A, j), cB = lane_value(B, j);
         const float cC = lane_value(C, j), cD = lane_value(D, j);
         const float2* quad = cur + j * 4 * g.Z;
+        // Families 2h and 2h+1 share dy^2; both are evaluated for a window
+        // where either is active (two independent decisions per pass; a
+        // sample outside its family's range fails the activity test).
 #pragma unroll
-        for (int f = 0; f < 4; ++f) {
-          const unsigned fm = (wm >> (8 * f)) & 0xffu;
-          const float dx2 = (f & 1) ? cB : cA;
-          const float dy2 = (f & 2) ? cD : cC;
+        for (int h = 0; h < 2; ++h) {
+          const int fa = 2 * h, fb = 2 * h + 1;
+          const unsigned wmask = ((wm >> (8 * fa)) | (wm >> (8 * fb))) & 0xffu;
+          const float dy2 = h ? cD : cC;
 #pragma unroll
           for (int k = 0; k < K; ++k) {
-            if (!((fm >> k) & 1u)) continue;
-            const float d2 = (tc2[k] - dx2) - dy2;  // volume_cuda.cu:40
-            float al;
-            bool nr;
-            int zi = decide_fast(g, d2, c0, delta, one_m_delta, &al, &nr);
-            if (__ballot(nr)) {
-              if (nr) zi = decide_exact(g, d2, wz, 0, &al);
+            if (!((wmask >> k) & 1u)) continue;
+            const float d2a = (tc2[k] - cA) - dy2;  // volume_cuda.cu:40
+            const float d2b = (tc2[k] - cB) - dy2;
+            float ala, alb;
+            bool na, nb;
+            int za = decide_fast(g, d2a, c0, delta, one_m_delta, &ala, &na);
+            int zb = decide_fast(g, d2b, c0, delta, one_m_delta, &alb, &nb);
+            if (__ballot(na || nb)) {
+              if (na) za = decide_exact(g, d2a, wz, 0, &ala);
+              if (nb) zb = decide_exact(g, d2b, wz, 0, &alb);
             }
-            const bool act = d2 > 0.f && (unsigned)zi <= (unsigned)(g.Z - 2);
-            const int zs = act ? zi : 0;
-            const float qv = d2 * itc4[k];
-            const float wgt = qv * qv;
+            const bool acta = d2a > 0.f && (unsigned)za <= (unsigned)(g.Z - 2);
+            const bool actb = d2b > 0.f && (unsigned)zb <= (unsigned)(g.Z - 2);
+            const int zsa = acta ? za : 0, zsb = actb ? zb : 0;
+            const float qa = d2a * itc4[k], qb = d2b * itc4[k];
+            const float wa = qa * qa, wb = qb * qb;
 #pragma unroll
             for (int m = 0; m < 4; ++m) {
-              const float2 v = quad[(m ^ f) * g.Z + zs];
-              const float f1 = fmaf(al, v.y, v.x);
-              const float nv = fmaf(f1, wgt, acc[m][k]);
-              acc[m][k] = act ? nv : acc[m][k];
+              const float2 va = quad[(m ^ fa) * g.Z + zsa];
+              const float2 vb = quad[(m ^ fb) * g.Z + zsb];
+              const float fva = fmaf(ala, va.y, va.x);
+              const float fvb = fmaf(alb, vb.y, vb.x);
+              const float na1 = fmaf(fva, wa, acc[m][k]);
+              const float nb1 = fmaf(fvb, wb, acta ? na1 : acc[m][k]);
+              acc[m][k] = actb ? nb1 : (acta ? na1 : acc[m][k]);
             }
           }
         }
