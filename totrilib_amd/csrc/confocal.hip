@@ -127,12 +127,15 @@ __device__ __forceinline__ void depth_range(const ConfGeom& g, float wz, float z
 // candidate floor (returned) and alpha.  `near` tells whether the exact
 // sequence must confirm it (see confocal_decide).
 __device__ __forceinline__ int decide_fast(const ConfGeom& g, float d2, float cfast,
-                                           float delta, float one_m_delta, float* alpha,
+                                           float delta, float half_m_delta, float* alpha,
                                            bool* near) {
   const float zh = fmaf(sqrt_fast(d2), g.s, cfast);
   const float al = __builtin_amdgcn_fractf(zh);
   *alpha = al;
-  *near = d2 > 0.f && (al < delta || al > one_m_delta);
+  // frac within delta of an integer: |frac - 1/2| > 1/2 - delta (one compare;
+  // the rounding of frac - 1/2 is far inside delta's margin).  d2 <= 0 gives
+  // a NaN frac, which compares false (such samples are inactive anyway).
+  *near = fabsf(al - 0.5f) > half_m_delta;
   return floor_to_int(zh);
 }
 
@@ -158,14 +161,14 @@ __device__ __forceinline__ int decide_exact(const ConfGeom& g, float d2, float w
 // test d2 > 0 themselves.
 __device__ __forceinline__ int confocal_decide(const ConfGeom& g, float d2, float wz,
                                                float cfast, int zshift, float delta,
-                                               float one_m_delta, float* alpha) {
+                                               float half_m_delta, float* alpha) {
   const float depth = sqrt_fast(d2);
   const float zh = fmaf(depth, g.s, cfast);
   // alpha = zh - floor(zh) (exact; v_fract only clamps results that would
   // round to 1.0, which happens for zh in (-1, 0), a rejected range anyway).
   float al = __builtin_amdgcn_fractf(zh);
   int r = floor_to_int(zh);
-  if (d2 > 0.f && (al < delta || al > one_m_delta)) {
+  if ((d2 > 0.f) & (fabsf(al - 0.5f) > half_m_delta)) {
     const float de = sqrt_rn(d2);
     const float z = wz + de;
     const float zs = (z - g.oz) * g.s;
@@ -225,7 +228,7 @@ __device__ __forceinline__ void r1_block(
   float c0 = (wz - g.oz) * g.s;
   c0 = c0 - 0.5f;
   const float delta = guard_delta(g, wz, c0, 0.f);
-  const float one_m_delta = 1.f - delta;
+  const float half_m_delta = 0.5f - delta;
   float dlo, dhi;
   depth_range(g, wz, 0.f, (float)(g.Z - 1), &dlo, &dhi);
 
@@ -328,14 +331,14 @@ __device__ __forceinline__ void r1_block(
           const float d2b = (same_x ? a[k] : tc2[k] - cdx2b) - cdy2b;
           float ala, alb;
           bool na, nb;
-          int za = decide_fast(g, d2a, c0, delta, one_m_delta, &ala, &na);
-          int zb = decide_fast(g, d2b, c0, delta, one_m_delta, &alb, &nb);
+          int za = decide_fast(g, d2a, c0, delta, half_m_delta, &ala, &na);
+          int zb = decide_fast(g, d2b, c0, delta, half_m_delta, &alb, &nb);
           if (__ballot(na || nb)) {  // rare: confirm near-boundary samples exactly
             if (na) za = decide_exact(g, d2a, wz, 0, &ala);
             if (nb) zb = decide_exact(g, d2b, wz, 0, &alb);
           }
-          const bool acta = d2a > 0.f && (unsigned)za <= (unsigned)(g.Z - 2);
-          const bool actb = d2b > 0.f && (unsigned)zb <= (unsigned)(g.Z - 2);
+          const bool acta = (d2a > 0.f) & ((unsigned)za <= (unsigned)(g.Z - 2));
+          const bool actb = (d2b > 0.f) & ((unsigned)zb <= (unsigned)(g.Z - 2));
           // Inactive lanes read slot 0 and keep their accumulator unchanged
           // (the discarded candidate may be NaN or inf).  Selects measure
           // faster here than exec-masked updates.
@@ -590,7 +593,7 @@ __device__ __forceinline__ void r1_sym_block(
   float c0 = (wz - g.oz) * g.s;
   c0 = c0 - 0.5f;
   const float delta = guard_delta(g, wz, c0, 0.f);
-  const float one_m_delta = 1.f - delta;
+  const float half_m_delta = 0.5f - delta;
   float dlo, dhi;
   depth_range(g, wz, 0.f, (float)(g.Z - 1), &dlo, &dhi);
 
@@ -685,14 +688,14 @@ __device__ __forceinline__ void r1_sym_block(
             const float d2b = (tc2[k] - cB) - dy2;
             float ala, alb;
             bool na, nb;
-            int za = decide_fast(g, d2a, c0, delta, one_m_delta, &ala, &na);
-            int zb = decide_fast(g, d2b, c0, delta, one_m_delta, &alb, &nb);
+            int za = decide_fast(g, d2a, c0, delta, half_m_delta, &ala, &na);
+            int zb = decide_fast(g, d2b, c0, delta, half_m_delta, &alb, &nb);
             if (__ballot(na || nb)) {
               if (na) za = decide_exact(g, d2a, wz, 0, &ala);
               if (nb) zb = decide_exact(g, d2b, wz, 0, &alb);
             }
-            const bool acta = d2a > 0.f && (unsigned)za <= (unsigned)(g.Z - 2);
-            const bool actb = d2b > 0.f && (unsigned)zb <= (unsigned)(g.Z - 2);
+            const bool acta = (d2a > 0.f) & ((unsigned)za <= (unsigned)(g.Z - 2));
+            const bool actb = (d2b > 0.f) & ((unsigned)zb <= (unsigned)(g.Z - 2));
             const int zsa = acta ? za : 0, zsb = actb ? zb : 0;
             const float qa = d2a * itc4[k], qb = d2b * itc4[k];
             const float wa = qa * qa, wb = qb * qb;
@@ -813,7 +816,7 @@ __global__ __launch_bounds__(kWaves * 64) void a6_kernel(
   float c0 = (wz - g.oz) * g.s;
   c0 = c0 - 0.5f;
   const float delta = guard_delta(g, wz, c0, 0.f);
-  const float one_m_delta = 1.f - delta;
+  const float half_m_delta = 0.5f - delta;
   float dlo, dhi;
   depth_range(g, wz, 0.f, (float)(g.Z - 1), &dlo, &dhi);
 
@@ -866,8 +869,8 @@ __global__ __launch_bounds__(kWaves * 64) void a6_kernel(
         if (!((wmask >> k) & 1u)) continue;
         const float d2 = a[k] - cdy2;  // volume_cuda.cu:222
         float al;
-        const int zi = confocal_decide(g, d2, wz, c0, 0, delta, one_m_delta, &al);
-        if (d2 > 0.f && (unsigned)zi <= (unsigned)(g.Z - 2)) {
+        const int zi = confocal_decide(g, d2, wz, c0, 0, delta, half_m_delta, &al);
+        if ((d2 > 0.f) & ((unsigned)zi <= (unsigned)(g.Z - 2))) {
           const float2 v = column[zi];
           const float f1 = fmaf(al, v.y, v.x);
           const float i8 = itc4[k] * itc4[k];
@@ -1026,7 +1029,7 @@ __device__ __forceinline__ void a1_item(
     c0 = c0 - 0.5f;
     const float cp = c0 - (float)zshift;
     const float delta = guard_delta(g, wz, c0, cp);
-    const float one_m_delta = 1.f - delta;
+    const float half_m_delta = 0.5f - delta;
     const float dx = wx - x, dy = wy - y;
     const float dx2 = cvalid ? dx * dx : INFINITY, dy2 = dy * dy;
     float dlo, dhi;
@@ -1061,15 +1064,15 @@ __device__ __forceinline__ void a1_item(
         const float d2b = (lane_value(treg, ub) - dx2) - dy2;
         float ala, alb;
         bool na, nb;
-        int ra = decide_fast(g, d2a, cp, delta, one_m_delta, &ala, &na);
-        int rb = decide_fast(g, d2b, cp, delta, one_m_delta, &alb, &nb);
+        int ra = decide_fast(g, d2a, cp, delta, half_m_delta, &ala, &na);
+        int rb = decide_fast(g, d2b, cp, delta, half_m_delta, &alb, &nb);
         if (__ballot(na || nb)) {  // rare: confirm near-boundary samples exactly
           if (na) ra = decide_exact(g, d2a, wz, zshift, &ala);
           if (nb) rb = decide_exact(g, d2b, wz, zshift, &alb);
         }
         // dx2 is +inf for lanes without a column, so d2 > 0 also checks that.
-        const bool acta = d2a > 0.f && (unsigned)ra <= (unsigned)rspan;
-        const bool actb = d2b > 0.f && (unsigned)rb <= (unsigned)rspan;
+        const bool acta = (d2a > 0.f) & ((unsigned)ra <= (unsigned)rspan);
+        const bool actb = (d2b > 0.f) & ((unsigned)rb <= (unsigned)rspan);
         // Each lane owns its slots and a wave's LDS accesses complete in
         // order, so plain read-modify-writes are exact.  (LDS float atomics
         // measure ~25x slower than this on gfx950.)  Updates run under the
@@ -1191,7 +1194,7 @@ __device__ __forceinline__ void a1_sym_item(
     c0 = c0 - 0.5f;
     const float cp = c0 - (float)zshift;
     const float delta = guard_delta(g, wz, c0, cp);
-    const float one_m_delta = 1.f - delta;
+    const float half_m_delta = 0.5f - delta;
     const float dxa = wx - x0, dxb = wx - x1, dy = wy - y;
     const float dy2 = dy * dy;
     float dlo, dhi;
@@ -1224,14 +1227,14 @@ __device__ __forceinline__ void a1_sym_item(
           const float d2b = (lane_value(treg, ub) - dx2) - dy2;
           float ala, alb;
           bool na, nb;
-          int ra = decide_fast(g, d2a, cp, delta, one_m_delta, &ala, &na);
-          int rb = decide_fast(g, d2b, cp, delta, one_m_delta, &alb, &nb);
+          int ra = decide_fast(g, d2a, cp, delta, half_m_delta, &ala, &na);
+          int rb = decide_fast(g, d2b, cp, delta, half_m_delta, &alb, &nb);
           if (__ballot(na || nb)) {
             if (na) ra = decide_exact(g, d2a, wz, zshift, &ala);
             if (nb) rb = decide_exact(g, d2b, wz, zshift, &alb);
           }
-          const bool acta = d2a > 0.f && (unsigned)ra <= (unsigned)rspan;
-          const bool actb = hb && d2b > 0.f && (unsigned)rb <= (unsigned)rspan;
+          const bool acta = (d2a > 0.f) & ((unsigned)ra <= (unsigned)rspan);
+          const bool actb = hb & (d2b > 0.f) & ((unsigned)rb <= (unsigned)rspan);
           if (acta) {
             const float base = SAFE ? sq(d2a * lane_value(ireg, u)) : d2a * d2a;
             const float w0 = base * lane_value(kreg0, u), w1 = base * lane_value(kreg1, u);
@@ -1844,7 +1847,7 @@ __global__ void confocal_samples_kernel(const float* __restrict__ mp,
   const float a = tc * tc - dx * dx;
   const float d2 = a - dy * dy;
   float al;
-  const int zi = confocal_decide(g, d2, wz, c0, 0, delta, 1.f - delta, &al);
+  const int zi = confocal_decide(g, d2, wz, c0, 0, delta, 0.5f - delta, &al);
   d2_out[k] = d2;
   zi_out[k] = d2 > 0.f ? zi : -1000000;
   alpha_out[k] = d2 > 0.f ? al : 0.f;
