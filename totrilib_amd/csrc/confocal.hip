@@ -1281,17 +1281,19 @@ __device__ __forceinline__ void a1_sym_item(
   }
 }
 
+constexpr int kA1SymWaves = 1;  // waves per x-mirror A1 workgroup (independent items)
+
 template <bool SAFE>
-__global__ __launch_bounds__(kA1Waves * 64) void a1_sym_kernel(
+__global__ __launch_bounds__(kA1SymWaves * 64) void a1_sym_kernel(
     const float* __restrict__ Kf, const float* __restrict__ tc2t,
     const float* __restrict__ itc4t, const float* __restrict__ mp,
     const int* __restrict__ pairs, const int* __restrict__ info, float* __restrict__ dst,
     ConfGeom g, int tiles_x, int tiles_y, int n_slabs, int n_ranges, int t_stride,
     int n_items) {
   if (!info[2]) return;
-  __shared__ float2 acc_all[kA1Waves][(kSlabS + 2) * 64];
+  __shared__ float2 acc_all[kA1SymWaves][(kSlabS + 2) * 64];
   const int wave = threadIdx.x >> 6;
-  const int item = blockIdx.x * kA1Waves + wave;
+  const int item = blockIdx.x * kA1SymWaves + wave;
   if (item >= n_items) return;  // whole wave exits; no barrier follows
   a1_sym_item<SAFE>(Kf, tc2t, itc4t, mp, pairs, info[3], dst, g, item, tiles_x, tiles_y,
                     n_slabs, n_ranges, t_stride, acc_all[wave]);
@@ -1779,13 +1781,13 @@ extern "C" int totri_volume_confocal_grad_volume(
     info = (const int*)((char*)workspace + q.ml.info_off);
     const int* pairs = (const int*)((char*)workspace + q.ml.pairs_off);
     const int items = q.tiles_x * q.tiles_y * q.n_slabs * q.n_ranges;
-    const int blocks = (items + kA1Waves - 1) / kA1Waves;
+    const int blocks = (items + kA1SymWaves - 1) / kA1SymWaves;
     if (p.safe)
-      hipLaunchKernelGGL(a1_sym_kernel<true>, dim3(blocks), dim3(kA1Waves * 64), 0, st, Kf,
+      hipLaunchKernelGGL(a1_sym_kernel<true>, dim3(blocks), dim3(kA1SymWaves * 64), 0, st, Kf,
                          tc2, itc4, measurement_points, pairs, info, dst, g, q.tiles_x,
                          q.tiles_y, q.n_slabs, q.n_ranges, p.t_stride, items);
     else
-      hipLaunchKernelGGL(a1_sym_kernel<false>, dim3(blocks), dim3(kA1Waves * 64), 0, st, Kf,
+      hipLaunchKernelGGL(a1_sym_kernel<false>, dim3(blocks), dim3(kA1SymWaves * 64), 0, st, Kf,
                          tc2, itc4, measurement_points, pairs, info, dst, g, q.tiles_x,
                          q.tiles_y, q.n_slabs, q.n_ranges, p.t_stride, items);
     rc = check_launch("a1_sym_kernel");
