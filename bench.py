@@ -217,10 +217,12 @@ def confocal_main(args):
     value = pairs * args.steps / wall
     r1_avg, a1_avg = float(r1_ms.mean()), float(a1_ms.mean())
     # which kernels ran: the mirror-symmetric ones when the scan points form
-    # mirror quads / pairs (the box is centred at x = y = 0)
+    # mirror quads / pairs (the box is centred at x = y = 0; with w > 0 and
+    # offset 0 the A1 quad walk applies)
     info = mirror_info(torch, mp)
     r1_name = "r1_sym_kernel" if info[0] else "r1_forward_kernel"
-    a1_name = "a1_sym_kernel" if info[2] else "a1_transpose_kernel"
+    a1_name = ("a1_quad_kernel" if info[0] else
+               "a1_sym_kernel" if info[2] else "a1_transpose_kernel")
     if r1_avg >= a1_avg:
         dom, flops, dur = r1_name, fm["f_r1"] * pairs, r1_avg
     else:

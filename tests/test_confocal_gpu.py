@@ -325,26 +325,50 @@ def test_r1_mirror_quads_and_general_path(scan):
     assert np.count_nonzero(got) > 0
 
 
-@pytest.mark.parametrize("scan", ["mirror", "perturbed", "random"])
+@pytest.mark.parametrize("scan", ["quads", "quads_safe", "quads_negoff", "xpairs", "odd_y",
+                                  "perturbed", "random"])
 def test_a1_mirror_pairs_and_general_path(scan):
-    """x-mirror pairs of scan points on a box centred at x = 0 take the
-    mirror kernel (one decision per two samples); otherwise the general
-    kernel runs.  Both match the oracle's exact adjoint."""
+    """Path selection of A1, each checked against the oracle's exact adjoint.
+
+    - Mirror quads of scan points on a box centred at x = y = 0, with w > 0
+      and offset >= 0, take the xy-mirror walk (one decision per four
+      samples): "quads", and "quads_safe" whose offset 0.5 mm makes 1/tc^8
+      overflow-prone, so the table takes its safe form.
+    - A negative offset (tc < 0 for the first bins) fails the walk's
+      monotonicity condition, and x-pairs without y-mirrors or an odd Y allow
+      only the x-mirror kernel (one decision per two samples).
+    - A broken pair or a random scan takes the general kernel.
+    """
     rng = np.random.default_rng(33)
     X, Y, Z, T = 30, 26, 44, 600
     mp = wall_grid(8)
-    if scan == "perturbed":
+    w, off = 0.0045, 0.0
+    if scan == "quads_safe":
+        off = 0.0005
+    elif scan == "quads_negoff":
+        off = -0.05
+    elif scan == "xpairs":
+        # move one x-pair in y: x-pairs stay, the y mirror of that pair is lost
+        k = 9
+        j = int(np.flatnonzero((mp[0] == -mp[0, k]) & (mp[1] == mp[1, k]))[0])
+        mp[1, [k, j]] = np.nextafter(mp[1, k], np.float32(2))
+    elif scan == "odd_y":
+        Y = 25
+    elif scan == "perturbed":
         mp[1, 9] = np.nextafter(mp[1, 9], np.float32(2))   # breaks one x-pair
         mp[0, 9] = np.nextafter(mp[0, 9], np.float32(2))
     elif scan == "random":
         mp = np.stack([rng.uniform(-1, 1, 64), rng.uniform(-1, 1, 64),
                        np.zeros(64)]).astype(np.float32)
     G = rng.standard_normal((mp.shape[1], T)).astype(np.float32)
-    w = 0.0045
     info = mirror_info(mp)
-    assert info[2:] == ([1, 32] if scan == "mirror" else [0, info[3]])
-    got = gpu_a1(G, mp, (X, Y, Z), w, 0.0)
-    ref = O.a1_transpose(G, mp, X, Y, Z, ORIGIN, SIZE, w, 0.0, mode=O.PINNED64)
+    expect = {"quads": [1, 16, 1, 32], "quads_safe": [1, 16, 1, 32],
+              "quads_negoff": [1, 16, 1, 32], "odd_y": [1, 16, 1, 32],
+              "xpairs": [0, info[1], 1, 32]}.get(scan, [0, info[1], 0, info[3]])
+    assert info == expect
+    got = gpu_a1(G, mp, (X, Y, Z), w, off)
+    ref = O.a1_transpose(G, mp, X, Y, Z, ORIGIN, SIZE, w, off, mode=O.PINNED64)
     e = rel_l2(got, ref)
     print(f"{scan}: mirror info {info}, rel-L2 vs FP64 pinned {e:.2e}")
     assert e < TOL
+    assert np.count_nonzero(got) > 0

@@ -770,6 +770,22 @@ __global__ void sum_partials_flag_kernel(const float* __restrict__ part,
   }
 }
 
+// A1 partial sums: the xy-mirror kernel ran when quad_on and info[0], else the
+// x-mirror kernel when info[2], else the general kernel; each has its own
+// partial count.
+__global__ void sum_partials_a1_kernel(const float* __restrict__ part,
+                                       const int* __restrict__ info, int quad_on, int n_quad,
+                                       int n_pair, int n_gen, size_t n,
+                                       float* __restrict__ out) {
+  const int n_parts = (quad_on && info[0]) ? n_quad : info[2] ? n_pair : n_gen;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (size_t)gridDim.x * blockDim.x) {
+    float v = part[e];
+    for (int p = 1; p < n_parts; ++p) v += part[(size_t)p * n + e];
+    out[e] = v;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // A6: gradient of <G, R1(v; w)> w.r.t. the wall points w (volume_cuda.cu:188-265)
 // ---------------------------------------------------------------------------
@@ -1289,14 +1305,239 @@ __global__ __launch_bounds__(kA1SymWaves * 64) void a1_sym_kernel(
     const float* __restrict__ itc4t, const float* __restrict__ mp,
     const int* __restrict__ pairs, const int* __restrict__ info, float* __restrict__ dst,
     ConfGeom g, int tiles_x, int tiles_y, int n_slabs, int n_ranges, int t_stride,
-    int n_items) {
-  if (!info[2]) return;
+    int n_items, int quad_on) {
+  if (!info[2] || (quad_on && info[0])) return;
   __shared__ float2 acc_all[kA1SymWaves][(kSlabS + 2) * 64];
   const int wave = threadIdx.x >> 6;
   const int item = blockIdx.x * kA1SymWaves + wave;
   if (item >= n_items) return;  // whole wave exits; no barrier follows
   a1_sym_item<SAFE>(Kf, tc2t, itc4t, mp, pairs, info[3], dst, g, item, tiles_x, tiles_y,
                     n_slabs, n_ranges, t_stride, acc_all[wave]);
+}
+
+// ---------------------------------------------------------------------------
+// xy-mirror A1, row-synchronous
+// ---------------------------------------------------------------------------
+// When the grid is antisymmetric in x and y and the scan points come in
+// mirror quads s_m (m = bit0: x mirror, bit1: y mirror), the samples
+// (s_m, c_(m^f)) share one decision, where c_n is the lane's column mirrored
+// by n: family f has dx^2 = (wx - x_(f&1))^2 and dy^2 = (wy - y_(f>>1))^2.
+// A lane owns the four mirror columns of one column of the quadrant
+// x < X/2, y < Y/2 and keeps their accumulators as float4 rows; component n
+// takes scan s_(n^f).
+//
+// With tc >= 0 (w > 0, off >= 0) and s > 0 the row floor(z_hat) of a lane's
+// bins never decreases with the bin index.  The wave walks the slab's rows in
+// order; in each row every lane walks its own bins until its row moves on, so
+// lanes do not share a bin window.  The row's two partial sums stay in
+// registers; the lower one joins the LDS row once per row, the upper one is
+// carried into the next row.  Table values for the lane's next bin are
+// loaded one step ahead.
+constexpr int kSlabQ = 8;  // xy-mirror A1 z-slab depth
+
+// True when `p` holds on any active lane of the wave.
+__device__ __forceinline__ bool wave_any(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0; }
+
+// base[off / 4] for a wave-uniform base and a 32-bit byte offset (lets the
+// compiler use the scalar-base load form).
+__device__ __forceinline__ float load_at(const float* __restrict__ base, uint32_t off) {
+  return *(const float*)((const char*)base + off);
+}
+
+// v[i] for a wave-uniform i in [0, 4).
+__device__ __forceinline__ int pick4(int i, int v0, int v1, int v2, int v3) {
+  return i == 0 ? v0 : i == 1 ? v1 : i == 2 ? v2 : v3;
+}
+
+template <bool SAFE>
+__device__ __forceinline__ void a1_quad_item(
+    const float* __restrict__ Kf, const float* __restrict__ tc2t,
+    const float* __restrict__ itc4t, const float* __restrict__ mp,
+    const int* __restrict__ quads, int n_quads,
+    float* __restrict__ dst,  // [R][X][Y][Z] partials, or [X][Y][Z]
+    const ConfGeom& g, int item, int tiles_x, int tiles_y, int n_slabs, int n_ranges,
+    int t_stride, float4* __restrict__ acc) {
+  const int lane = threadIdx.x & 63;
+  const int tx = item % tiles_x;
+  item /= tiles_x;
+  const int ty = item % tiles_y;
+  item /= tiles_y;
+  const int slab = item % n_slabs;
+  const int range = item / n_slabs;
+
+  const int xi = tx * 8 + (lane >> 3);
+  const int yi = ty * 8 + (lane & 7);
+  const bool cvalid = xi < g.X / 2 && yi < g.Y / 2;
+  const int xm = g.X - 1 - xi, ym = g.Y - 1 - yi;
+  const float x0 = cell_centre(xi, g.ox, g.szx, g.X);
+  const float x1 = cell_centre(xm, g.ox, g.szx, g.X);
+  const float y0 = cell_centre(yi, g.oy, g.szy, g.Y);
+  const float y1 = cell_centre(ym, g.oy, g.szy, g.Y);
+
+  const int z0 = slab * kSlabQ;
+  const int rmin = (z0 == 0) ? 1 : 0;
+  const int rmax = min(kSlabQ, (g.Z - 2) - (z0 - 1));
+  const int rspan = rmax - rmin;
+  const int zshift = z0 - 1 + rmin;
+  const float zlo = (float)max(z0 - 1, 0);
+  const float zhi = (float)min(z0 + kSlabQ, g.Z - 1);
+
+  const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (int r = 0; r < kSlabQ + 2; ++r) acc[r * 64 + lane] = zero4;
+  float4 tot[kSlabQ];
+#pragma unroll
+  for (int r = 0; r < kSlabQ; ++r) tot[r] = zero4;
+
+  const int per = (n_quads + n_ranges - 1) / n_ranges;
+  const int q_begin = range * per;
+  const int q_end = min(n_quads, q_begin + per);
+  float4* my = acc + rmin * 64 + lane;
+
+  for (int qi = q_begin; qi < q_end; ++qi) {
+    if (qi > q_begin && (qi - q_begin) % (kFlush / 4) == 0) {
+#pragma unroll
+      for (int r = 0; r < kSlabQ; ++r) {
+        const float4 a = acc[(r + 1) * 64 + lane];
+        tot[r].x += a.x;
+        tot[r].y += a.y;
+        tot[r].z += a.z;
+        tot[r].w += a.w;
+        acc[(r + 1) * 64 + lane] = zero4;
+      }
+    }
+    const int s0 = uniform_i32(quads[4 * qi]);
+    const int s1 = uniform_i32(quads[4 * qi + 1]);
+    const int s2 = uniform_i32(quads[4 * qi + 2]);
+    const int s3 = uniform_i32(quads[4 * qi + 3]);
+    const float wx = mp[s0], wy = mp[g.N + s0], wz = mp[2 * g.N + s0];
+    float c0 = (wz - g.oz) * g.s;
+    c0 = c0 - 0.5f;
+    const float cp = c0 - (float)zshift;
+    const float delta = guard_delta(g, wz, c0, cp);
+    const float half_m_delta = 0.5f - delta;
+    const float dxa = wx - x0, dxb = wx - x1, dya = wy - y0, dyb = wy - y1;
+    float dlo, dhi;
+    depth_range(g, wz, zlo, zhi, &dlo, &dhi);
+#pragma unroll 1
+    for (int f = 0; f < 4; ++f) {
+      const float dx2 = cvalid ? ((f & 1) ? dxb * dxb : dxa * dxa) : INFINITY;
+      const float dy2 = (f & 2) ? dyb * dyb : dya * dya;
+      int tlo, thi;
+      bin_range(g, dx2 + dy2, dlo, dhi, &tlo, &thi);
+      if (!cvalid || rspan < 0 || tlo > thi) {
+        tlo = 0;
+        thi = -1;
+      }
+      if (!wave_any(tlo <= thi)) continue;
+      // component n takes scan s_(n^f)
+      const float* k0p = Kf + (size_t)pick4(f, s0, s1, s2, s3) * t_stride;
+      const float* k1p = Kf + (size_t)pick4(f ^ 1, s0, s1, s2, s3) * t_stride;
+      const float* k2p = Kf + (size_t)pick4(f ^ 2, s0, s1, s2, s3) * t_stride;
+      const float* k3p = Kf + (size_t)pick4(f ^ 3, s0, s1, s2, s3) * t_stride;
+      int t = tlo;
+      // decision of bin t; -1 for d2 <= 0 (before the ray reaches depth 0).
+      // tc^2 is recomputed with the tables' rounding.
+      auto decide_bin = [&](int tt, float* d2o, float* alo) -> int {
+        const float tc = bin_distance(tt, g.w, g.off);
+        const float d2 = (tc * tc - dx2) - dy2;
+        bool nr;
+        int r = decide_fast(g, d2, cp, delta, half_m_delta, alo, &nr);
+        if (wave_any(nr))
+          if (nr) r = decide_exact(g, d2, wz, zshift, alo);
+        *d2o = d2;
+        return d2 > 0.f ? r : -1;
+      };
+      float d2, al;
+      int r = decide_bin(t, &d2, &al);
+      int rr = uniform_i32(wave_min_i32(t <= thi ? max(r, 0) : 0x7fffffff));
+      float4 carry = zero4;
+      while (rr <= rspan) {
+        if (!wave_any(t <= thi)) break;
+        float4 lo = zero4, hi = zero4;
+        while (true) {
+          const bool act = (t <= thi) & (r <= rr);
+          if (!wave_any(act)) break;
+          if (act) {
+            // this bin's table values; the next bin's decision covers their latency
+            const uint32_t off = (uint32_t)t * 4u;
+            const float k0 = load_at(k0p, off), k1 = load_at(k1p, off);
+            const float k2 = load_at(k2p, off), k3 = load_at(k3p, off);
+            const float ic = SAFE ? load_at(itc4t, off) : 0.f;
+            // (past thi the lane stops; the decision is then unused)
+            float d2n, aln;
+            const int rn = decide_bin(t + 1, &d2n, &aln);
+            if (r == rr) {
+              const float base = SAFE ? sq(d2 * ic) : d2 * d2;
+              const float w0 = base * k0, w1 = base * k1;
+              const float w2 = base * k2, w3 = base * k3;
+              const float om = 1.f - al;
+              lo.x = fmaf(w0, om, lo.x);
+              lo.y = fmaf(w1, om, lo.y);
+              lo.z = fmaf(w2, om, lo.z);
+              lo.w = fmaf(w3, om, lo.w);
+              hi.x = fmaf(w0, al, hi.x);
+              hi.y = fmaf(w1, al, hi.y);
+              hi.z = fmaf(w2, al, hi.z);
+              hi.w = fmaf(w3, al, hi.w);
+            }
+            ++t;
+            r = rn;
+            al = aln;
+            d2 = d2n;
+          }
+        }
+        float4* row = my + rr * 64;
+        float4 a = *row;
+        a.x += carry.x + lo.x;
+        a.y += carry.y + lo.y;
+        a.z += carry.z + lo.z;
+        a.w += carry.w + lo.w;
+        *row = a;
+        carry = hi;
+        ++rr;
+      }
+      float4* row = my + rr * 64;
+      float4 a = *row;
+      a.x += carry.x;
+      a.y += carry.y;
+      a.z += carry.z;
+      a.w += carry.w;
+      *row = a;
+    }
+  }
+
+  if (cvalid) {
+    float* out00 = dst + (((size_t)range * g.X + xi) * g.Y + yi) * g.Z;
+    float* out10 = dst + (((size_t)range * g.X + xm) * g.Y + yi) * g.Z;
+    float* out01 = dst + (((size_t)range * g.X + xi) * g.Y + ym) * g.Z;
+    float* out11 = dst + (((size_t)range * g.X + xm) * g.Y + ym) * g.Z;
+#pragma unroll
+    for (int r = 0; r < kSlabQ; ++r) {
+      const int z = z0 + r;
+      if (z < g.Z) {
+        const float4 a = acc[(r + 1) * 64 + lane];
+        out00[z] = tot[r].x + a.x;
+        out10[z] = tot[r].y + a.y;
+        out01[z] = tot[r].z + a.z;
+        out11[z] = tot[r].w + a.w;
+      }
+    }
+  }
+}
+
+template <bool SAFE>
+__global__ __launch_bounds__(64) void a1_quad_kernel(
+    const float* __restrict__ Kf, const float* __restrict__ tc2t,
+    const float* __restrict__ itc4t, const float* __restrict__ mp,
+    const int* __restrict__ quads, const int* __restrict__ info, float* __restrict__ dst,
+    ConfGeom g, int tiles_x, int tiles_y, int n_slabs, int n_ranges, int t_stride,
+    int n_items) {
+  if (!info[0]) return;
+  __shared__ float4 acc[(kSlabQ + 2) * 64];
+  const int item = blockIdx.x;
+  if (item >= n_items) return;
+  a1_quad_item<SAFE>(Kf, tc2t, itc4t, mp, quads, info[1], dst, g, item, tiles_x, tiles_y,
+                     n_slabs, n_ranges, t_stride, acc);
 }
 
 int launch_grid(size_t n) {
@@ -1674,9 +1915,44 @@ namespace {
 // Plan of the x-mirror A1 (used when the device finds x-pairs of scan
 // points).  Workspace: tables | partial volumes (the larger count of the two
 // paths) | mirror detection.
+// Plan of the xy-mirror A1 (scan points in mirror quads).
+struct A1QuadPlan {
+  bool possible;  // sizes allow it (X, Y even, N a multiple of 4)
+  int tiles_x, tiles_y, n_slabs, n_ranges;
+};
+
+A1QuadPlan plan_a1_quad(int64_t X, int64_t Y, int64_t Z, int64_t N) {
+  A1QuadPlan p;
+  memset(&p, 0, sizeof(p));
+  p.possible = X >= 2 && Y >= 2 && !(X & 1) && !(Y & 1) && N >= 4 && !(N & 3) &&
+               N <= (1 << 24);
+  if (!p.possible) return p;
+  p.tiles_x = (int)((X / 2 + 7) / 8);
+  p.tiles_y = (int)((Y / 2 + 7) / 8);
+  p.n_slabs = (int)((Z + kSlabQ - 1) / kSlabQ);
+  const int64_t items = (int64_t)p.tiles_x * p.tiles_y * p.n_slabs;
+  // two single-wave workgroups per wave slot: item costs differ with the
+  // geometry, and a second round evens out the tail
+  const int64_t target = 32 * (int64_t)device_cu_count();
+  int64_t R = 1;
+  if (items < target) R = (target + items - 1) / items;
+  if (R > N / 4) R = N / 4;
+  if (R < 1) R = 1;
+  p.n_ranges = (int)R;
+  return p;
+}
+
+// The xy-mirror walk needs each lane's row to grow with the bin index:
+// tc = t w + off >= 0 for every bin and a positive z scale.
+bool a1_quad_geometry(const float* origin, const float* size, int64_t Y, float w, float off) {
+  return w > 0.f && isfinite(w) && off >= 0.f && isfinite(off) && size[2] > 0.f &&
+         isfinite(size[2]) && axis_antisymmetric(origin[1], size[1], Y);
+}
+
 struct A1SymPlan {
   bool possible;  // sizes allow it (X even, N even)
   int tiles_x, tiles_y, n_slabs, n_ranges;
+  A1QuadPlan quad;
   size_t partial_bytes;
   MirrorLayout ml;
   size_t ws;
@@ -1697,7 +1973,9 @@ A1SymPlan plan_a1_sym(int64_t X, int64_t Y, int64_t Z, int64_t N, const A1Plan& 
   if (R > N / 2) R = N / 2;
   if (R < 1) R = 1;
   p.n_ranges = (int)R;
-  const int parts = p.n_ranges > gen.n_ranges ? p.n_ranges : gen.n_ranges;
+  p.quad = plan_a1_quad(X, Y, Z, N);
+  int parts = p.n_ranges > gen.n_ranges ? p.n_ranges : gen.n_ranges;
+  if (p.quad.possible && p.quad.n_ranges > parts) parts = p.quad.n_ranges;
   p.partial_bytes = parts > 1 ? (size_t)parts * X * Y * Z * sizeof(float) : 0;
   p.ml = mirror_layout(N, gen.tables_bytes + p.partial_bytes);
   p.ws = p.ml.end;
@@ -1743,6 +2021,9 @@ extern "C" int totri_volume_confocal_grad_volume(
   // The x-mirror path needs the voxel grid antisymmetric in x; the scan
   // pairs are then looked for on the device.
   const bool sym = q.possible && axis_antisymmetric(volume_origin[0], volume_size[0], X);
+  // The xy-mirror path also needs the y antisymmetry and a monotone walk.
+  const bool quad = sym && q.quad.possible &&
+                    a1_quad_geometry(volume_origin, volume_size, Y, bin_width, bin_offset);
   const size_t need = a1_workspace(X, Y, Z, N, T);
   if (workspace_bytes < need)
     return fail(TOTRI_EWORKSPACE, "workspace too small: need %zu bytes, got %zu",
@@ -1773,23 +2054,39 @@ extern "C" int totri_volume_confocal_grad_volume(
   rc = check_launch("a1_tables_kernel");
   if (rc) return rc;
   const int* info = nullptr;
-  const bool staged = p.n_ranges > 1 || (sym && q.n_ranges > 1);
+  const bool staged =
+      p.n_ranges > 1 || (sym && q.n_ranges > 1) || (quad && q.quad.n_ranges > 1);
   float* dst = staged ? partial : volume;
   if (sym) {
     rc = launch_mirror_detect(q.ml, measurement_points, (int)N, workspace, st);
     if (rc) return rc;
     info = (const int*)((char*)workspace + q.ml.info_off);
     const int* pairs = (const int*)((char*)workspace + q.ml.pairs_off);
+    if (quad) {
+      const int* quads = (const int*)((char*)workspace + q.ml.quads_off);
+      const A1QuadPlan& qd = q.quad;
+      const int items = qd.tiles_x * qd.tiles_y * qd.n_slabs * qd.n_ranges;
+      if (p.safe)
+        hipLaunchKernelGGL(a1_quad_kernel<true>, dim3(items), dim3(64), 0, st, Kf, tc2, itc4,
+                           measurement_points, quads, info, dst, g, qd.tiles_x, qd.tiles_y,
+                           qd.n_slabs, qd.n_ranges, p.t_stride, items);
+      else
+        hipLaunchKernelGGL(a1_quad_kernel<false>, dim3(items), dim3(64), 0, st, Kf, tc2, itc4,
+                           measurement_points, quads, info, dst, g, qd.tiles_x, qd.tiles_y,
+                           qd.n_slabs, qd.n_ranges, p.t_stride, items);
+      rc = check_launch("a1_quad_kernel");
+      if (rc) return rc;
+    }
     const int items = q.tiles_x * q.tiles_y * q.n_slabs * q.n_ranges;
     const int blocks = (items + kA1SymWaves - 1) / kA1SymWaves;
     if (p.safe)
       hipLaunchKernelGGL(a1_sym_kernel<true>, dim3(blocks), dim3(kA1SymWaves * 64), 0, st, Kf,
                          tc2, itc4, measurement_points, pairs, info, dst, g, q.tiles_x,
-                         q.tiles_y, q.n_slabs, q.n_ranges, p.t_stride, items);
+                         q.tiles_y, q.n_slabs, q.n_ranges, p.t_stride, items, (int)quad);
     else
       hipLaunchKernelGGL(a1_sym_kernel<false>, dim3(blocks), dim3(kA1SymWaves * 64), 0, st, Kf,
                          tc2, itc4, measurement_points, pairs, info, dst, g, q.tiles_x,
-                         q.tiles_y, q.n_slabs, q.n_ranges, p.t_stride, items);
+                         q.tiles_y, q.n_slabs, q.n_ranges, p.t_stride, items, (int)quad);
     rc = check_launch("a1_sym_kernel");
     if (rc) return rc;
   }
@@ -1808,10 +2105,11 @@ extern "C" int totri_volume_confocal_grad_volume(
   if (rc) return rc;
   if (!staged) return TOTRI_OK;
   if (sym) {
-    // info[2] says which path ran; sum_partials_flag_kernel reads info[0]
-    hipLaunchKernelGGL(sum_partials_flag_kernel<float>, dim3(launch_grid(nv)), dim3(256), 0,
-                       st, partial, info + 2, q.n_ranges, p.n_ranges, nv, volume);
-    return check_launch("sum_partials_flag_kernel");
+    // info[0] and info[2] say which path ran
+    hipLaunchKernelGGL(sum_partials_a1_kernel, dim3(launch_grid(nv)), dim3(256), 0, st,
+                       partial, info, (int)quad, q.quad.n_ranges, q.n_ranges, p.n_ranges, nv,
+                       volume);
+    return check_launch("sum_partials_a1_kernel");
   }
   hipLaunchKernelGGL(sum_partials_kernel<float>, dim3(launch_grid(nv)), dim3(256), 0, st,
                      partial, p.n_ranges, nv, volume);
