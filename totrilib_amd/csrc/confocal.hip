@@ -1531,13 +1531,101 @@ __global__ __launch_bounds__(64) void a1_quad_kernel(
     const float* __restrict__ itc4t, const float* __restrict__ mp,
     const int* __restrict__ quads, const int* __restrict__ info, float* __restrict__ dst,
     ConfGeom g, int tiles_x, int tiles_y, int n_slabs, int n_ranges, int t_stride,
-    int n_items) {
+    int n_items, const int* __restrict__ order) {
   if (!info[0]) return;
   __shared__ float4 acc[(kSlabQ + 2) * 64];
-  const int item = blockIdx.x;
-  if (item >= n_items) return;
+  if ((int)blockIdx.x >= n_items) return;
+  const int item = order[blockIdx.x];
   a1_quad_item<SAFE>(Kf, tc2t, itc4t, mp, quads, info[1], dst, g, item, tiles_x, tiles_y,
                      n_slabs, n_ranges, t_stride, acc);
+}
+
+// Dispatch order of the xy-mirror A1 items.  Item costs differ by a factor of
+// several (deep slabs see more bins per voxel row than shallow ones; tiles
+// near the scan's centre more than those at its edge), and workgroups start
+// in index order as slots free up, so the costliest items go first
+// (longest-processing-time order).  The order changes scheduling only: every
+// item writes its own part of the partial volumes.
+//
+// Cost estimate of one item, in bins: for 16 quads spread over the item's
+// scan range and each family, the widest of the slab's bin ranges at the
+// tile's four corner columns.
+__global__ __launch_bounds__(64) void a1_quad_cost_kernel(
+    const float* __restrict__ mp, const int* __restrict__ quads, const int* __restrict__ info,
+    ConfGeom g, int tiles_x, int tiles_y, int n_slabs, int n_ranges, int n_items,
+    float* __restrict__ cost) {
+  if (!info[0]) return;
+  int item = blockIdx.x;
+  if (item >= n_items) return;
+  const int lane = threadIdx.x;
+  const int tx = item % tiles_x;
+  item /= tiles_x;
+  const int ty = item % tiles_y;
+  item /= tiles_y;
+  const int slab = item % n_slabs;
+  const int range = item / n_slabs;
+  const int n_quads = info[1];
+  const int per = (n_quads + n_ranges - 1) / n_ranges;
+  const int q_begin = range * per;
+  const int cnt = min(n_quads, q_begin + per) - q_begin;
+  const int k = lane >> 2, f = lane & 3;
+  const int samples = min(cnt, 16);
+  float bins = 0.f;
+  if (k < samples) {
+    const int q = q_begin + (int)(((long long)k * cnt) / samples);
+    const int s0 = quads[4 * q];
+    const float wx = mp[s0], wy = mp[g.N + s0], wz = mp[2 * g.N + s0];
+    const int z0 = slab * kSlabQ;
+    float dlo, dhi;
+    depth_range(g, wz, (float)max(z0 - 1, 0), (float)min(z0 + kSlabQ, g.Z - 1), &dlo, &dhi);
+    int widest = 0;
+    for (int c = 0; c < 4; ++c) {
+      const int xi = min(tx * 8 + ((c & 1) ? 7 : 0), g.X / 2 - 1);
+      const int yi = min(ty * 8 + ((c & 2) ? 7 : 0), g.Y / 2 - 1);
+      const float x = cell_centre((f & 1) ? g.X - 1 - xi : xi, g.ox, g.szx, g.X);
+      const float y = cell_centre((f & 2) ? g.Y - 1 - yi : yi, g.oy, g.szy, g.Y);
+      int tlo, thi;
+      bin_range(g, (wx - x) * (wx - x) + (wy - y) * (wy - y), dlo, dhi, &tlo, &thi);
+      widest = max(widest, thi - tlo + 1);
+    }
+    bins = (float)widest;
+  }
+  const float total = wave_sum_f32(bins);
+  if (lane == 0) cost[blockIdx.x] = samples > 0 ? total * ((float)cnt / (float)samples) : 0.f;
+}
+
+// Counting sort of the items into 256 cost buckets, costliest first (one
+// workgroup; the order within a bucket is immaterial).
+__global__ __launch_bounds__(1024) void a1_quad_order_kernel(const float* __restrict__ cost,
+                                                             const int* __restrict__ info,
+                                                             int n, int* __restrict__ order) {
+  if (!info[0]) return;
+  __shared__ int hist[256];
+  __shared__ float red[1024];
+  const int tid = threadIdx.x;
+  float m = 0.f;
+  for (int i = tid; i < n; i += 1024) m = fmaxf(m, cost[i]);
+  red[tid] = m;
+  if (tid < 256) hist[tid] = 0;
+  __syncthreads();
+  for (int h = 512; h > 0; h >>= 1) {
+    if (tid < h) red[tid] = fmaxf(red[tid], red[tid + h]);
+    __syncthreads();
+  }
+  const float scale = red[0] > 0.f ? 255.f / red[0] : 0.f;
+  auto bucket = [&](int i) { return 255 - min(255, max(0, (int)(cost[i] * scale))); };
+  for (int i = tid; i < n; i += 1024) atomicAdd(&hist[bucket(i)], 1);
+  __syncthreads();
+  if (tid == 0) {
+    int run = 0;
+    for (int b = 0; b < 256; ++b) {
+      const int c = hist[b];
+      hist[b] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
+  for (int i = tid; i < n; i += 1024) order[atomicAdd(&hist[bucket(i)], 1)] = i;
 }
 
 int launch_grid(size_t n) {
@@ -1955,6 +2043,7 @@ struct A1SymPlan {
   A1QuadPlan quad;
   size_t partial_bytes;
   MirrorLayout ml;
+  size_t cost_off, order_off;  // xy-mirror item costs (float) and dispatch order (int)
   size_t ws;
 };
 
@@ -1979,6 +2068,14 @@ A1SymPlan plan_a1_sym(int64_t X, int64_t Y, int64_t Z, int64_t N, const A1Plan& 
   p.partial_bytes = parts > 1 ? (size_t)parts * X * Y * Z * sizeof(float) : 0;
   p.ml = mirror_layout(N, gen.tables_bytes + p.partial_bytes);
   p.ws = p.ml.end;
+  if (p.quad.possible) {
+    const size_t items = (size_t)p.quad.tiles_x * p.quad.tiles_y * p.quad.n_slabs *
+                         p.quad.n_ranges;
+    const size_t bytes = (items * sizeof(int) + 255) & ~(size_t)255;
+    p.cost_off = p.ws;
+    p.order_off = p.cost_off + bytes;
+    p.ws = p.order_off + bytes;
+  }
   return p;
 }
 
@@ -2064,16 +2161,23 @@ extern "C" int totri_volume_confocal_grad_volume(
     const int* pairs = (const int*)((char*)workspace + q.ml.pairs_off);
     if (quad) {
       const int* quads = (const int*)((char*)workspace + q.ml.quads_off);
+      float* cost = (float*)((char*)workspace + q.cost_off);
+      int* order = (int*)((char*)workspace + q.order_off);
       const A1QuadPlan& qd = q.quad;
       const int items = qd.tiles_x * qd.tiles_y * qd.n_slabs * qd.n_ranges;
+      hipLaunchKernelGGL(a1_quad_cost_kernel, dim3(items), dim3(64), 0, st, measurement_points,
+                         quads, info, g, qd.tiles_x, qd.tiles_y, qd.n_slabs, qd.n_ranges, items,
+                         cost);
+      hipLaunchKernelGGL(a1_quad_order_kernel, dim3(1), dim3(1024), 0, st, cost, info, items,
+                         order);
       if (p.safe)
         hipLaunchKernelGGL(a1_quad_kernel<true>, dim3(items), dim3(64), 0, st, Kf, tc2, itc4,
                            measurement_points, quads, info, dst, g, qd.tiles_x, qd.tiles_y,
-                           qd.n_slabs, qd.n_ranges, p.t_stride, items);
+                           qd.n_slabs, qd.n_ranges, p.t_stride, items, order);
       else
         hipLaunchKernelGGL(a1_quad_kernel<false>, dim3(items), dim3(64), 0, st, Kf, tc2, itc4,
                            measurement_points, quads, info, dst, g, qd.tiles_x, qd.tiles_y,
-                           qd.n_slabs, qd.n_ranges, p.t_stride, items);
+                           qd.n_slabs, qd.n_ranges, p.t_stride, items, order);
       rc = check_launch("a1_quad_kernel");
       if (rc) return rc;
     }
