@@ -1434,9 +1434,8 @@ __device__ __forceinline__ void a1_quad_item(
       const float* k1p = Kf + (size_t)pick4(f ^ 1, s0, s1, s2, s3) * t_stride;
       const float* k2p = Kf + (size_t)pick4(f ^ 2, s0, s1, s2, s3) * t_stride;
       const float* k3p = Kf + (size_t)pick4(f ^ 3, s0, s1, s2, s3) * t_stride;
-      int t = tlo;
-      // decision of bin t; -1 for d2 <= 0 (before the ray reaches depth 0).
-      // tc^2 is recomputed with the tables' rounding.
+      // decision of bin tt (unspecified for d2 <= 0; the walk never meets
+      // such a bin).  tc^2 is recomputed with the tables' rounding.
       auto decide_bin = [&](int tt, float* d2o, float* alo) -> int {
         const float tc = bin_distance(tt, g.w, g.off);
         const float d2 = (tc * tc - dx2) - dy2;
@@ -1445,12 +1444,47 @@ __device__ __forceinline__ void a1_quad_item(
         if (wave_any(nr))
           if (nr) r = decide_exact(g, d2, wz, zshift, alo);
         *d2o = d2;
-        return d2 > 0.f ? r : -1;
+        return r;
       };
+      // The walk starts at the lane's first bin with d2 > 0 and row >= 0:
+      // estimated from the real-valued inverse (depth of row 0's start), then
+      // settled by exact decisions (backward while the previous bin also
+      // qualifies, forward while this one does not).  Every later bin then
+      // has d2 > 0 and a row no lower.
+      const float rho2 = dx2 + dy2;
+      const float depth0 = fmaxf(((float)zshift + 0.5f) / g.s + g.oz - wz, 0.f);
+      const float te = (sqrtf(rho2 + depth0 * depth0) - g.off) / g.w;
+      int t = tlo;
+      if (te > (float)tlo + 1.f) t = te < (float)thi ? (int)te - 1 : thi;
       float d2, al;
       int r = decide_bin(t, &d2, &al);
-      int rr = uniform_i32(wave_min_i32(t <= thi ? max(r, 0) : 0x7fffffff));
+      bool back = (t <= thi) & (t > tlo) & (d2 > 0.f) & (r >= 0);
+      while (wave_any(back)) {
+        if (back) {
+          float d2p, alp;
+          const int rp = decide_bin(t - 1, &d2p, &alp);
+          back = (d2p > 0.f) & (rp >= 0);
+          if (back) {
+            --t;
+            r = rp;
+            al = alp;
+            d2 = d2p;
+            back = t > tlo;
+          }
+        }
+      }
+      bool fwd = (t <= thi) & !((d2 > 0.f) & (r >= 0));
+      while (wave_any(fwd)) {
+        if (fwd) {
+          ++t;
+          r = decide_bin(t, &d2, &al);
+          fwd = (t <= thi) & !((d2 > 0.f) & (r >= 0));
+        }
+      }
+      int rr = uniform_i32(wave_min_i32(t <= thi ? r : 0x7fffffff));
       float4 carry = zero4;
+      // At the start of row rr every live lane's row is >= rr, so a lane
+      // that is live with row <= rr is on row rr.
       while (rr <= rspan) {
         if (!wave_any(t <= thi)) break;
         float4 lo = zero4, hi = zero4;
@@ -1466,20 +1500,16 @@ __device__ __forceinline__ void a1_quad_item(
             // (past thi the lane stops; the decision is then unused)
             float d2n, aln;
             const int rn = decide_bin(t + 1, &d2n, &aln);
-            if (r == rr) {
-              const float base = SAFE ? sq(d2 * ic) : d2 * d2;
-              const float w0 = base * k0, w1 = base * k1;
-              const float w2 = base * k2, w3 = base * k3;
-              const float om = 1.f - al;
-              lo.x = fmaf(w0, om, lo.x);
-              lo.y = fmaf(w1, om, lo.y);
-              lo.z = fmaf(w2, om, lo.z);
-              lo.w = fmaf(w3, om, lo.w);
-              hi.x = fmaf(w0, al, hi.x);
-              hi.y = fmaf(w1, al, hi.y);
-              hi.z = fmaf(w2, al, hi.z);
-              hi.w = fmaf(w3, al, hi.w);
-            }
+            const float base = SAFE ? sq(d2 * ic) : d2 * d2;
+            const float wl = base * (1.f - al), wh = base * al;
+            lo.x = fmaf(k0, wl, lo.x);
+            lo.y = fmaf(k1, wl, lo.y);
+            lo.z = fmaf(k2, wl, lo.z);
+            lo.w = fmaf(k3, wl, lo.w);
+            hi.x = fmaf(k0, wh, hi.x);
+            hi.y = fmaf(k1, wh, hi.y);
+            hi.z = fmaf(k2, wh, hi.z);
+            hi.w = fmaf(k3, wh, hi.w);
             ++t;
             r = rn;
             al = aln;
