@@ -42,6 +42,7 @@ constexpr int kWaves = 8;            // scan points per R1 workgroup
 constexpr int kChunkVoxels = 4096;   // voxels per R1 LDS chunk buffer (32 KiB of pairs)
 constexpr int kSlab = 32;            // A1 z-slab depth
 constexpr int kFlush = 32;           // A1 scan points per first-level sum
+constexpr int kSlabQ = 8;            // xy-mirror A1 z-slab depth
 
 struct ConfGeom {
   int X, Y, Z, N, T;
@@ -770,18 +771,35 @@ __global__ void sum_partials_flag_kernel(const float* __restrict__ part,
   }
 }
 
-// A1 partial sums: the xy-mirror kernel ran when quad_on and info[0], else the
-// x-mirror kernel when info[2], else the general kernel; each has its own
-// partial count.
+// Boundary rows of the xy-mirror A1 for element e of an [X][Y][Z] volume of
+// n voxels: voxel z = k * kSlabQ (k >= 1) receives the alpha part of slab
+// k - 1's top sample row, summed over the n_parts scan ranges in order.
+__device__ __forceinline__ float a1_boundary(const float* __restrict__ bnd, size_t e,
+                                             size_t n, int Z, int n_slabs, int n_parts) {
+  const int z = (int)(e % (size_t)Z);
+  if (z == 0 || z % kSlabQ) return 0.f;
+  const size_t ncol = n / (size_t)Z, col = e / (size_t)Z;
+  const int sb = z / kSlabQ - 1;
+  float b = bnd[col * n_slabs + sb];
+  for (int p = 1; p < n_parts; ++p) b += bnd[((size_t)p * ncol + col) * n_slabs + sb];
+  return b;
+}
+
+// A1 partial sums: the xy-mirror kernel ran when quad_on and info[0] (its
+// boundary rows are added too), else the x-mirror kernel when info[2], else
+// the general kernel; each has its own partial count.
 __global__ void sum_partials_a1_kernel(const float* __restrict__ part,
+                                       const float* __restrict__ bnd,
                                        const int* __restrict__ info, int quad_on, int n_quad,
-                                       int n_pair, int n_gen, size_t n,
+                                       int n_pair, int n_gen, size_t n, int Z, int n_slabs,
                                        float* __restrict__ out) {
-  const int n_parts = (quad_on && info[0]) ? n_quad : info[2] ? n_pair : n_gen;
+  const bool quad = quad_on && info[0];
+  const int n_parts = quad ? n_quad : info[2] ? n_pair : n_gen;
   for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
        e += (size_t)gridDim.x * blockDim.x) {
     float v = part[e];
     for (int p = 1; p < n_parts; ++p) v += part[(size_t)p * n + e];
+    if (quad) v += a1_boundary(bnd, e, n, Z, n_slabs, n_parts);
     out[e] = v;
   }
 }
@@ -1333,7 +1351,6 @@ __global__ __launch_bounds__(kA1SymWaves * 64) void a1_sym_kernel(
 // registers; the lower one joins the LDS row once per row, the upper one is
 // carried into the next row.  Table values for the lane's next bin are
 // loaded one step ahead.
-constexpr int kSlabQ = 8;  // xy-mirror A1 z-slab depth
 
 // True when `p` holds on any active lane of the wave.
 __device__ __forceinline__ bool wave_any(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0; }
@@ -1354,7 +1371,8 @@ __device__ __forceinline__ void a1_quad_item(
     const float* __restrict__ Kf, const float* __restrict__ tc2t,
     const float* __restrict__ itc4t, const float* __restrict__ mp,
     const int* __restrict__ quads, int n_quads,
-    float* __restrict__ dst,  // [R][X][Y][Z] partials, or [X][Y][Z]
+    float* __restrict__ dst,  // [R][X][Y][Z] partial volumes
+    float* __restrict__ bnd,  // [R][X][Y][n_slabs] boundary rows
     const ConfGeom& g, int item, int tiles_x, int tiles_y, int n_slabs, int n_ranges,
     int t_stride, float4* __restrict__ acc) {
   const int lane = threadIdx.x & 63;
@@ -1374,35 +1392,38 @@ __device__ __forceinline__ void a1_quad_item(
   const float y0 = cell_centre(yi, g.oy, g.szy, g.Y);
   const float y1 = cell_centre(ym, g.oy, g.szy, g.Y);
 
+  // The slab walks the sample rows floor(z_hat) = z0 .. z0 + rspan (at most
+  // kSlabQ of them, never past Z - 2).  LDS row k holds voxel z0 + k; row
+  // kSlabQ takes the alpha part of the slab's top sample row, which belongs
+  // to the next slab's first voxel: it goes to the boundary buffer and the
+  // partial-sum pass adds it there, so no slab walks its neighbour's rows.
   const int z0 = slab * kSlabQ;
-  const int rmin = (z0 == 0) ? 1 : 0;
-  const int rmax = min(kSlabQ, (g.Z - 2) - (z0 - 1));
-  const int rspan = rmax - rmin;
-  const int zshift = z0 - 1 + rmin;
-  const float zlo = (float)max(z0 - 1, 0);
+  const int rspan = min(kSlabQ, (g.Z - 1) - z0) - 1;
+  const int zshift = z0;
+  const float zlo = (float)z0;
   const float zhi = (float)min(z0 + kSlabQ, g.Z - 1);
 
   const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
-  for (int r = 0; r < kSlabQ + 2; ++r) acc[r * 64 + lane] = zero4;
-  float4 tot[kSlabQ];
+  for (int r = 0; r <= kSlabQ; ++r) acc[r * 64 + lane] = zero4;
+  float4 tot[kSlabQ + 1];
 #pragma unroll
-  for (int r = 0; r < kSlabQ; ++r) tot[r] = zero4;
+  for (int r = 0; r <= kSlabQ; ++r) tot[r] = zero4;
 
   const int per = (n_quads + n_ranges - 1) / n_ranges;
   const int q_begin = range * per;
   const int q_end = min(n_quads, q_begin + per);
-  float4* my = acc + rmin * 64 + lane;
+  float4* my = acc + lane;
 
   for (int qi = q_begin; qi < q_end; ++qi) {
     if (qi > q_begin && (qi - q_begin) % (kFlush / 4) == 0) {
 #pragma unroll
-      for (int r = 0; r < kSlabQ; ++r) {
-        const float4 a = acc[(r + 1) * 64 + lane];
+      for (int r = 0; r <= kSlabQ; ++r) {
+        const float4 a = acc[r * 64 + lane];
         tot[r].x += a.x;
         tot[r].y += a.y;
         tot[r].z += a.z;
         tot[r].w += a.w;
-        acc[(r + 1) * 64 + lane] = zero4;
+        acc[r * 64 + lane] = zero4;
       }
     }
     const int s0 = uniform_i32(quads[4 * qi]);
@@ -1526,13 +1547,18 @@ __device__ __forceinline__ void a1_quad_item(
         carry = hi;
         ++rr;
       }
-      float4* row = my + rr * 64;
-      float4 a = *row;
-      a.x += carry.x;
-      a.y += carry.y;
-      a.z += carry.z;
-      a.w += carry.w;
-      *row = a;
+      // the alpha part of the last walked row; when no row was walked (rr
+      // started past the slab, or at INT_MAX with no live lane) the carry is
+      // zero and rr may lie outside the LDS block, so nothing is written
+      if (rr <= rspan + 1) {
+        float4* row = my + rr * 64;
+        float4 a = *row;
+        a.x += carry.x;
+        a.y += carry.y;
+        a.z += carry.z;
+        a.w += carry.w;
+        *row = a;
+      }
     }
   }
 
@@ -1545,12 +1571,20 @@ __device__ __forceinline__ void a1_quad_item(
     for (int r = 0; r < kSlabQ; ++r) {
       const int z = z0 + r;
       if (z < g.Z) {
-        const float4 a = acc[(r + 1) * 64 + lane];
+        const float4 a = acc[r * 64 + lane];
         out00[z] = tot[r].x + a.x;
         out10[z] = tot[r].y + a.y;
         out01[z] = tot[r].z + a.z;
         out11[z] = tot[r].w + a.w;
       }
+    }
+    if (z0 + kSlabQ < g.Z) {
+      const float4 a = acc[kSlabQ * 64 + lane];
+      float* b = bnd + (size_t)range * g.X * g.Y * n_slabs + slab;
+      b[((size_t)xi * g.Y + yi) * n_slabs] = tot[kSlabQ].x + a.x;
+      b[((size_t)xm * g.Y + yi) * n_slabs] = tot[kSlabQ].y + a.y;
+      b[((size_t)xi * g.Y + ym) * n_slabs] = tot[kSlabQ].z + a.z;
+      b[((size_t)xm * g.Y + ym) * n_slabs] = tot[kSlabQ].w + a.w;
     }
   }
 }
@@ -1560,14 +1594,14 @@ __global__ __launch_bounds__(64) void a1_quad_kernel(
     const float* __restrict__ Kf, const float* __restrict__ tc2t,
     const float* __restrict__ itc4t, const float* __restrict__ mp,
     const int* __restrict__ quads, const int* __restrict__ info, float* __restrict__ dst,
-    ConfGeom g, int tiles_x, int tiles_y, int n_slabs, int n_ranges, int t_stride,
-    int n_items, const int* __restrict__ order) {
+    float* __restrict__ bnd, ConfGeom g, int tiles_x, int tiles_y, int n_slabs, int n_ranges,
+    int t_stride, int n_items, const int* __restrict__ order) {
   if (!info[0]) return;
-  __shared__ float4 acc[(kSlabQ + 2) * 64];
+  __shared__ float4 acc[(kSlabQ + 1) * 64];
   if ((int)blockIdx.x >= n_items) return;
   const int item = order[blockIdx.x];
-  a1_quad_item<SAFE>(Kf, tc2t, itc4t, mp, quads, info[1], dst, g, item, tiles_x, tiles_y,
-                     n_slabs, n_ranges, t_stride, acc);
+  a1_quad_item<SAFE>(Kf, tc2t, itc4t, mp, quads, info[1], dst, bnd, g, item, tiles_x,
+                     tiles_y, n_slabs, n_ranges, t_stride, acc);
 }
 
 // Dispatch order of the xy-mirror A1 items.  Item costs differ by a factor of
@@ -1607,7 +1641,7 @@ __global__ __launch_bounds__(64) void a1_quad_cost_kernel(
     const float wx = mp[s0], wy = mp[g.N + s0], wz = mp[2 * g.N + s0];
     const int z0 = slab * kSlabQ;
     float dlo, dhi;
-    depth_range(g, wz, (float)max(z0 - 1, 0), (float)min(z0 + kSlabQ, g.Z - 1), &dlo, &dhi);
+    depth_range(g, wz, (float)z0, (float)min(z0 + kSlabQ, g.Z - 1), &dlo, &dhi);
     int widest = 0;
     for (int c = 0; c < 4; ++c) {
       const int xi = min(tx * 8 + ((c & 1) ? 7 : 0), g.X / 2 - 1);
@@ -2074,6 +2108,7 @@ struct A1SymPlan {
   size_t partial_bytes;
   MirrorLayout ml;
   size_t cost_off, order_off;  // xy-mirror item costs (float) and dispatch order (int)
+  size_t bnd_off;              // xy-mirror boundary rows [R][X][Y][n_slabs]
   size_t ws;
 };
 
@@ -2095,7 +2130,10 @@ A1SymPlan plan_a1_sym(int64_t X, int64_t Y, int64_t Z, int64_t N, const A1Plan& 
   p.quad = plan_a1_quad(X, Y, Z, N);
   int parts = p.n_ranges > gen.n_ranges ? p.n_ranges : gen.n_ranges;
   if (p.quad.possible && p.quad.n_ranges > parts) parts = p.quad.n_ranges;
-  p.partial_bytes = parts > 1 ? (size_t)parts * X * Y * Z * sizeof(float) : 0;
+  // the xy-mirror path always writes partial volumes (its boundary rows are
+  // added in the partial-sum pass)
+  p.partial_bytes =
+      (parts > 1 || p.quad.possible) ? (size_t)parts * X * Y * Z * sizeof(float) : 0;
   p.ml = mirror_layout(N, gen.tables_bytes + p.partial_bytes);
   p.ws = p.ml.end;
   if (p.quad.possible) {
@@ -2104,7 +2142,9 @@ A1SymPlan plan_a1_sym(int64_t X, int64_t Y, int64_t Z, int64_t N, const A1Plan& 
     const size_t bytes = (items * sizeof(int) + 255) & ~(size_t)255;
     p.cost_off = p.ws;
     p.order_off = p.cost_off + bytes;
-    p.ws = p.order_off + bytes;
+    p.bnd_off = p.order_off + bytes;
+    p.ws = p.bnd_off + (((size_t)p.quad.n_ranges * X * Y * p.quad.n_slabs * sizeof(float) +
+                         255) & ~(size_t)255);
   }
   return p;
 }
@@ -2181,8 +2221,7 @@ extern "C" int totri_volume_confocal_grad_volume(
   rc = check_launch("a1_tables_kernel");
   if (rc) return rc;
   const int* info = nullptr;
-  const bool staged =
-      p.n_ranges > 1 || (sym && q.n_ranges > 1) || (quad && q.quad.n_ranges > 1);
+  const bool staged = p.n_ranges > 1 || (sym && q.n_ranges > 1) || quad;
   float* dst = staged ? partial : volume;
   if (sym) {
     rc = launch_mirror_detect(q.ml, measurement_points, (int)N, workspace, st);
@@ -2193,6 +2232,7 @@ extern "C" int totri_volume_confocal_grad_volume(
       const int* quads = (const int*)((char*)workspace + q.ml.quads_off);
       float* cost = (float*)((char*)workspace + q.cost_off);
       int* order = (int*)((char*)workspace + q.order_off);
+      float* bnd = (float*)((char*)workspace + q.bnd_off);
       const A1QuadPlan& qd = q.quad;
       const int items = qd.tiles_x * qd.tiles_y * qd.n_slabs * qd.n_ranges;
       hipLaunchKernelGGL(a1_quad_cost_kernel, dim3(items), dim3(64), 0, st, measurement_points,
@@ -2202,12 +2242,12 @@ extern "C" int totri_volume_confocal_grad_volume(
                          order);
       if (p.safe)
         hipLaunchKernelGGL(a1_quad_kernel<true>, dim3(items), dim3(64), 0, st, Kf, tc2, itc4,
-                           measurement_points, quads, info, dst, g, qd.tiles_x, qd.tiles_y,
-                           qd.n_slabs, qd.n_ranges, p.t_stride, items, order);
+                           measurement_points, quads, info, dst, bnd, g, qd.tiles_x,
+                           qd.tiles_y, qd.n_slabs, qd.n_ranges, p.t_stride, items, order);
       else
         hipLaunchKernelGGL(a1_quad_kernel<false>, dim3(items), dim3(64), 0, st, Kf, tc2, itc4,
-                           measurement_points, quads, info, dst, g, qd.tiles_x, qd.tiles_y,
-                           qd.n_slabs, qd.n_ranges, p.t_stride, items, order);
+                           measurement_points, quads, info, dst, bnd, g, qd.tiles_x,
+                           qd.tiles_y, qd.n_slabs, qd.n_ranges, p.t_stride, items, order);
       rc = check_launch("a1_quad_kernel");
       if (rc) return rc;
     }
@@ -2241,7 +2281,8 @@ extern "C" int totri_volume_confocal_grad_volume(
   if (sym) {
     // info[0] and info[2] say which path ran
     hipLaunchKernelGGL(sum_partials_a1_kernel, dim3(launch_grid(nv)), dim3(256), 0, st,
-                       partial, info, (int)quad, q.quad.n_ranges, q.n_ranges, p.n_ranges, nv,
+                       partial, (const float*)((char*)workspace + q.bnd_off), info, (int)quad,
+                       q.quad.n_ranges, q.n_ranges, p.n_ranges, nv, (int)Z, q.quad.n_slabs,
                        volume);
     return check_launch("sum_partials_a1_kernel");
   }
