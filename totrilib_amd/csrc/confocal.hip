@@ -88,17 +88,21 @@ __device__ __forceinline__ void bin_range(const ConfGeom& g, float rho2, float d
     *hi = g.T - 1;
     return;
   }
+  // Hardware square root and reciprocal (a few ulp, i.e. a few parts in
+  // 1e7): the relative margin of 1e-5 on the radii and the 2 bins on the
+  // indices are far wider, up to T of order 1e7.
   const float dl = fmaxf(dlo, 0.f);
-  const float r_lo = sqrtf(rho2 + dl * dl) * (1.f - 1e-5f) - 1e-6f;
-  const float r_hi = sqrtf(rho2 + dhi * dhi) * (1.f + 1e-5f) + 1e-6f;
+  const float r_lo = sqrt_fast(rho2 + dl * dl) * (1.f - 1e-5f) - 1e-6f;
+  const float r_hi = sqrt_fast(rho2 + dhi * dhi) * (1.f + 1e-5f) + 1e-6f;
+  const float iw = __builtin_amdgcn_rcpf(g.w);
   const float top = (float)g.T + 1.f;
   // Positive radii tc in [r_lo, r_hi] and negative radii tc in [-r_hi, -r_lo]
   // give two bin intervals; each is clipped to [0, T-1] on its own and the
   // result is the hull of the non-empty ones.
-  const float pa = fminf(fmaxf((r_lo - g.off) / g.w - 2.f, -1.f), top);
-  const float pb = fminf(fmaxf((r_hi - g.off) / g.w + 2.f, -1.f), top);
-  const float na = fminf(fmaxf((-r_hi - g.off) / g.w - 2.f, -1.f), top);
-  const float nb = fminf(fmaxf((-r_lo - g.off) / g.w + 2.f, -1.f), top);
+  const float pa = fminf(fmaxf((r_lo - g.off) * iw - 2.f, -1.f), top);
+  const float pb = fminf(fmaxf((r_hi - g.off) * iw + 2.f, -1.f), top);
+  const float na = fminf(fmaxf((-r_hi - g.off) * iw - 2.f, -1.f), top);
+  const float nb = fminf(fmaxf((-r_lo - g.off) * iw + 2.f, -1.f), top);
   int plo = max((int)floorf(pa), 0), phi = min((int)ceilf(pb), g.T - 1);
   int nlo = max((int)floorf(na), 0), nhi = min((int)ceilf(nb), g.T - 1);
   const bool pe = plo <= phi, ne = nlo <= nhi;
@@ -1474,7 +1478,7 @@ __device__ __forceinline__ void a1_quad_item(
       // has d2 > 0 and a row no lower.
       const float rho2 = dx2 + dy2;
       const float depth0 = fmaxf(((float)zshift + 0.5f) / g.s + g.oz - wz, 0.f);
-      const float te = (sqrtf(rho2 + depth0 * depth0) - g.off) / g.w;
+      const float te = (sqrt_fast(rho2 + depth0 * depth0) - g.off) * __builtin_amdgcn_rcpf(g.w);
       int t = tlo;
       if (te > (float)tlo + 1.f) t = te < (float)thi ? (int)te - 1 : thi;
       float d2, al;
