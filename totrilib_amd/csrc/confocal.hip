@@ -1510,32 +1510,44 @@ __device__ __forceinline__ void a1_quad_item(
       float4 carry = zero4;
       // At the start of row rr every live lane's row is >= rr, so a lane
       // that is live with row <= rr is on row rr.
+      // Table values of the lane's current bin, loaded one step ahead (right
+      // after the previous bin used them), so a whole step covers their
+      // latency.  The index stops at T - 1.
+      const uint32_t tmax4 = (uint32_t)(g.T - 1) * 4u;
+      uint32_t off = min((uint32_t)t * 4u, tmax4);
+      float k0 = load_at(k0p, off), k1 = load_at(k1p, off);
+      float k2 = load_at(k2p, off), k3 = load_at(k3p, off);
+      float ic = SAFE ? load_at(itc4t, off) : 0.f;
+      // Per row: L = carry + sum of K d2^2, H = sum of K d2^2 alpha; the row's
+      // voxel gets L - H (its (1 - alpha) parts plus the carry) and H is
+      // carried into the next row.
       while (rr <= rspan) {
         if (!wave_any(t <= thi)) break;
-        float4 lo = zero4, hi = zero4;
+        float4 L = carry, H = zero4;
         while (true) {
           const bool act = (t <= thi) & (r <= rr);
           if (!wave_any(act)) break;
           if (act) {
-            // this bin's table values; the next bin's decision covers their latency
-            const uint32_t off = (uint32_t)t * 4u;
-            const float k0 = load_at(k0p, off), k1 = load_at(k1p, off);
-            const float k2 = load_at(k2p, off), k3 = load_at(k3p, off);
-            const float ic = SAFE ? load_at(itc4t, off) : 0.f;
             // (past thi the lane stops; the decision is then unused)
             float d2n, aln;
             const int rn = decide_bin(t + 1, &d2n, &aln);
             const float base = SAFE ? sq(d2 * ic) : d2 * d2;
-            const float wl = base * (1.f - al), wh = base * al;
-            lo.x = fmaf(k0, wl, lo.x);
-            lo.y = fmaf(k1, wl, lo.y);
-            lo.z = fmaf(k2, wl, lo.z);
-            lo.w = fmaf(k3, wl, lo.w);
-            hi.x = fmaf(k0, wh, hi.x);
-            hi.y = fmaf(k1, wh, hi.y);
-            hi.z = fmaf(k2, wh, hi.z);
-            hi.w = fmaf(k3, wh, hi.w);
+            const float wh = base * al;
+            L.x = fmaf(k0, base, L.x);
+            L.y = fmaf(k1, base, L.y);
+            L.z = fmaf(k2, base, L.z);
+            L.w = fmaf(k3, base, L.w);
+            H.x = fmaf(k0, wh, H.x);
+            H.y = fmaf(k1, wh, H.y);
+            H.z = fmaf(k2, wh, H.z);
+            H.w = fmaf(k3, wh, H.w);
             ++t;
+            off = min(off + 4u, tmax4);
+            k0 = load_at(k0p, off);
+            k1 = load_at(k1p, off);
+            k2 = load_at(k2p, off);
+            k3 = load_at(k3p, off);
+            if (SAFE) ic = load_at(itc4t, off);
             r = rn;
             al = aln;
             d2 = d2n;
@@ -1543,12 +1555,12 @@ __device__ __forceinline__ void a1_quad_item(
         }
         float4* row = my + rr * 64;
         float4 a = *row;
-        a.x += carry.x + lo.x;
-        a.y += carry.y + lo.y;
-        a.z += carry.z + lo.z;
-        a.w += carry.w + lo.w;
+        a.x += L.x - H.x;
+        a.y += L.y - H.y;
+        a.z += L.z - H.z;
+        a.w += L.w - H.w;
         *row = a;
-        carry = hi;
+        carry = H;
         ++rr;
       }
       // the alpha part of the last walked row; when no row was walked (rr
