@@ -124,8 +124,8 @@ def test_admission_respects_device_budget():
     from totrilib_amd.serve import BackprojectRequest, ServingEngine, request_pairs
     reqs = make_requests(seed=2)
     # above the largest single request (a backprojection's scan-range
-    # partial volumes take up to 64 MiB), below the mix's total
-    budget = 96 << 20
+    # partial volumes take up to 128 MiB), below the mix's total
+    budget = 128 << 20
     eng = ServingEngine(device_budget=budget, host_budget=64 << 20, lanes=2)
     peak = 0
     tickets = [eng.submit(r) for r in reqs]

@@ -2099,12 +2099,15 @@ A1QuadPlan plan_a1_quad(int64_t X, int64_t Y, int64_t Z, int64_t N) {
   p.tiles_y = (int)((Y / 2 + 7) / 8);
   p.n_slabs = (int)((Z + kSlabQ - 1) / kSlabQ);
   const int64_t items = (int64_t)p.tiles_x * p.tiles_y * p.n_slabs;
-  // two single-wave workgroups per wave slot: item costs differ with the
-  // geometry, and a second round evens out the tail
-  const int64_t target = 32 * (int64_t)device_cu_count();
+  // four single-wave workgroups per wave slot: item costs differ by several
+  // times with the geometry, and with only one or two rounds the slots that
+  // draw two costly items set the kernel's time.  Partial volumes then take
+  // at most 64 x CU x 2048 voxels of floats (128 MiB on 256 CUs) at any
+  // volume size.  Ranges keep at least 4 quads.
+  const int64_t target = 64 * (int64_t)device_cu_count();
   int64_t R = 1;
   if (items < target) R = (target + items - 1) / items;
-  if (R > N / 4) R = N / 4;
+  if (R > N / 16) R = N / 16;
   if (R < 1) R = 1;
   p.n_ranges = (int)R;
   return p;
