@@ -743,7 +743,8 @@ __device__ __forceinline__ void r1_sym_block(
 }
 
 // Two workgroups per CU (their chunk buffers take 128 KiB of LDS), so the
-// compiler is held to 128 VGPRs.  blockIdx.y is the bin pass.
+// compiler is held to 128 VGPRs.  blockIdx.y counts the bin passes from
+// the last one.
 template <int K>
 __global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(4)))
 void r1_sym_kernel(
@@ -755,7 +756,10 @@ void r1_sym_kernel(
   const int n_quads = info[1];
   const int qgrp = (int)(blockIdx.x / n_groups);
   if (qgrp * kWaves >= n_quads) return;  // whole workgroup idle
-  r1_sym_block<K>(vol, mp, quads, n_quads, dst, g, (int)blockIdx.y * 64 * K,
+  // later bin passes first: they usually reach more of the volume (the first
+  // bins end short of a box placed away from the wall), so the costlier
+  // workgroups start in the first round
+  r1_sym_block<K>(vol, mp, quads, n_quads, dst, g, (int)(gridDim.y - 1 - blockIdx.y) * 64 * K,
                   (int)(blockIdx.x % n_groups), qgrp, n_groups, reps_per_chunk,
                   chunks_per_group, lds2);
 }
@@ -1849,7 +1853,9 @@ R1SymPlan plan_r1_sym(int64_t X, int64_t Y, int64_t Z, int64_t N, int64_t T,
   p.reps_per_chunk = (int)rpc;
   const int64_t total_chunks = (n_reps + rpc - 1) / rpc;
   const int64_t quad_blocks = (N / 4 + kWaves - 1) / kWaves;
-  const int64_t target = 2 * (int64_t)device_cu_count();
+  // four workgroups per CU and pass (two run at a time): smaller items
+  // shorten the tail of the last round
+  const int64_t target = 4 * (int64_t)device_cu_count();
   int64_t G = 1;
   if (quad_blocks < target) G = (target + quad_blocks - 1) / quad_blocks;
   if (G > total_chunks) G = total_chunks;
