@@ -575,7 +575,13 @@ __device__ __forceinline__ void r1_sym_block(
   const bool quad_ok = q < n_quads;
   const int HX = g.X / 2, HY = g.Y / 2;
   const int n_reps = HX * HY;
-  const int chunk_vox = reps_per_chunk * 4 * g.Z;
+  // A chunk holds reps_per_chunk column quads as [rep][z][m] pairs with
+  // z = 0 .. Z, where row z = Z is (0, 0): a sample's four mirror pairs sit
+  // side by side, and an inactive sample reads the zero row.
+  const int zrows = g.Z + 1;
+  const int rep_pairs = 4 * zrows;
+  const int chunk_vox = reps_per_chunk * rep_pairs;  // pairs
+  const int chunk_items = reps_per_chunk * zrows;    // (rep, z) rows of 4 pairs
 
   float tc2[K], itc4[K], acc[4][K];
 #pragma unroll
@@ -584,7 +590,8 @@ __device__ __forceinline__ void r1_sym_block(
     const float tc = bin_distance(t, g.w, g.off);
     const float qq = tc * tc;
     tc2[k] = (t < g.T) ? qq : -1.f;
-    itc4[k] = 1.f / (qq * qq);
+    // finite, so that an inactive sample (d2 clamped to 0) has weight 0
+    itc4[k] = qq > 0.f ? fminf(1.f / (qq * qq), 3.4028235e38f) : 0.f;
 #pragma unroll
     for (int m = 0; m < 4; ++m) acc[m][k] = 0.f;
   }
@@ -607,39 +614,61 @@ __device__ __forceinline__ void r1_sym_block(
   const int chunk_end = min(chunk_first + chunks_per_group, total_chunks);
   const int nthreads = kWaves * 64;
 
-  // Pair (v[z], v[z+1] - v[z]) of element e of a chunk (rep-major, then
-  // mirror, then z).
-  auto load_pair = [&](int c, int e) -> float2 {
-    const int jr = e / (4 * g.Z);
-    const int m = (e / g.Z) & 3;
-    const int z = e - (jr * 4 + m) * g.Z;
+  // Row `it` of chunk c: the pairs (v[z], v[z+1] - v[z]) of the four mirror
+  // columns m = 0..3 of one column quad at one z, as two float4 (m = 0, 1 and
+  // m = 2, 3).  Consecutive rows are consecutive z of the same columns, so a
+  // wave's loads run along four columns.
+  auto load_row = [&](int c, int it, float4* lo, float4* hi) {
+    const int jr = it / zrows;
+    const int z = it - jr * zrows;
     const int r = c * reps_per_chunk + jr;
-    if (r >= n_reps) return make_float2(0.f, 0.f);
-    const int rx = r / HY, ry = r - rx * HY;
-    const int xi = (m & 1) ? g.X - 1 - rx : rx;
-    const int yi = (m & 2) ? g.Y - 1 - ry : ry;
-    const float* col = vol + ((size_t)xi * g.Y + yi) * g.Z;
-    const float v0 = col[z];
-    const float v1 = (z == g.Z - 1) ? v0 : col[z + 1];
-    return make_float2(v0, v1 - v0);
+    float2 p[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) p[m] = make_float2(0.f, 0.f);
+    if (r < n_reps && z < g.Z) {
+      const int rx = r / HY, ry = r - rx * HY;
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        const int xi = (m & 1) ? g.X - 1 - rx : rx;
+        const int yi = (m & 2) ? g.Y - 1 - ry : ry;
+        const float* col = vol + ((size_t)xi * g.Y + yi) * g.Z;
+        const float v0 = col[z];
+        const float v1 = (z == g.Z - 1) ? v0 : col[z + 1];
+        p[m] = make_float2(v0, v1 - v0);
+      }
+    }
+    *lo = make_float4(p[0].x, p[0].y, p[1].x, p[1].y);
+    *hi = make_float4(p[2].x, p[2].y, p[3].x, p[3].y);
   };
 
-  if (chunk_first < chunk_end)
-    for (int e = threadIdx.x; e < chunk_vox; e += nthreads) lds2[e] = load_pair(chunk_first, e);
+  if (chunk_first < chunk_end) {
+    float4* dst4 = reinterpret_cast<float4*>(lds2);
+    for (int it = threadIdx.x; it < chunk_items; it += nthreads) {
+      float4 lo, hi;
+      load_row(chunk_first, it, &lo, &hi);
+      dst4[2 * it] = lo;
+      dst4[2 * it + 1] = hi;
+    }
+  }
   __syncthreads();
 
-  constexpr int kPref = kChunkVoxels / (kWaves * 64);
+  constexpr int kPref = kChunkVoxels / (4 * kWaves * 64);  // rows per thread
   for (int c = chunk_first; c < chunk_end; ++c) {
     const int buf = (c - chunk_first) & 1;
     const float2* cur = lds2 + buf * chunk_vox;
-    float2* nxt = lds2 + (buf ^ 1) * chunk_vox;
+    float4* nxt4 = reinterpret_cast<float4*>(lds2 + (buf ^ 1) * chunk_vox);
     const bool has_next = c + 1 < chunk_end;
-    float2 pref[kPref];
+    float4 pref_lo[kPref], pref_hi[kPref];
     if (has_next) {
 #pragma unroll
       for (int r = 0; r < kPref; ++r) {
-        const int e = threadIdx.x + r * nthreads;
-        pref[r] = e < chunk_vox ? load_pair(c + 1, e) : make_float2(0.f, 0.f);
+        const int it = threadIdx.x + r * nthreads;
+        if (it < chunk_items) {
+          load_row(c + 1, it, &pref_lo[r], &pref_hi[r]);
+        } else {
+          pref_lo[r] = make_float4(0.f, 0.f, 0.f, 0.f);
+          pref_hi[r] = pref_lo[r];
+        }
       }
     }
 
@@ -677,7 +706,7 @@ __device__ __forceinline__ void r1_sym_block(
         const unsigned wm = (unsigned)__builtin_amdgcn_readlane((int)packed, j);
         const float cA = lane_value(A, j), cB = lane_value(B, j);
         const float cC = lane_value(C, j), cD = lane_value(D, j);
-        const float2* quad = cur + j * 4 * g.Z;
+        const float2* quad = cur + j * rep_pairs;
         // Families 2h and 2h+1 share dy^2; both are evaluated for a window
         // where either is active (two independent decisions per pass; a
         // sample outside its family's range fails the activity test).
@@ -691,28 +720,31 @@ __device__ __forceinline__ void r1_sym_block(
             if (!((wmask >> k) & 1u)) continue;
             const float d2a = (tc2[k] - cA) - dy2;  // volume_cuda.cu:40
             const float d2b = (tc2[k] - cB) - dy2;
+            // Decisions and weights use d2 clamped at 0: identical for an
+            // active sample (d2 > 0), finite for an inactive one, which then
+            // reads the zero row: its term is 0 * 0 + 0 times a finite weight.
+            const float d2ca = fmaxf(d2a, 0.f), d2cb = fmaxf(d2b, 0.f);
             float ala, alb;
             bool na, nb;
-            int za = decide_fast(g, d2a, c0, delta, half_m_delta, &ala, &na);
-            int zb = decide_fast(g, d2b, c0, delta, half_m_delta, &alb, &nb);
+            int za = decide_fast(g, d2ca, c0, delta, half_m_delta, &ala, &na);
+            int zb = decide_fast(g, d2cb, c0, delta, half_m_delta, &alb, &nb);
             if (__ballot(na || nb)) {
-              if (na) za = decide_exact(g, d2a, wz, 0, &ala);
-              if (nb) zb = decide_exact(g, d2b, wz, 0, &alb);
+              if (na) za = decide_exact(g, d2ca, wz, 0, &ala);
+              if (nb) zb = decide_exact(g, d2cb, wz, 0, &alb);
             }
             const bool acta = (d2a > 0.f) & ((unsigned)za <= (unsigned)(g.Z - 2));
             const bool actb = (d2b > 0.f) & ((unsigned)zb <= (unsigned)(g.Z - 2));
-            const int zsa = acta ? za : 0, zsb = actb ? zb : 0;
-            const float qa = d2a * itc4[k], qb = d2b * itc4[k];
+            const float2* pa = quad + 4 * (acta ? za : g.Z);
+            const float2* pb = quad + 4 * (actb ? zb : g.Z);
+            const float qa = d2ca * itc4[k], qb = d2cb * itc4[k];
             const float wa = qa * qa, wb = qb * qb;
 #pragma unroll
             for (int m = 0; m < 4; ++m) {
-              const float2 va = quad[(m ^ fa) * g.Z + zsa];
-              const float2 vb = quad[(m ^ fb) * g.Z + zsb];
+              const float2 va = pa[m ^ fa];
+              const float2 vb = pb[m ^ fb];
               const float fva = fmaf(ala, va.y, va.x);
               const float fvb = fmaf(alb, vb.y, vb.x);
-              const float na1 = fmaf(fva, wa, acc[m][k]);
-              const float nb1 = fmaf(fvb, wb, acta ? na1 : acc[m][k]);
-              acc[m][k] = actb ? nb1 : (acta ? na1 : acc[m][k]);
+              acc[m][k] = fmaf(fvb, wb, fmaf(fva, wa, acc[m][k]));
             }
           }
         }
@@ -722,8 +754,11 @@ __device__ __forceinline__ void r1_sym_block(
     if (has_next) {
 #pragma unroll
       for (int r = 0; r < kPref; ++r) {
-        const int e = threadIdx.x + r * nthreads;
-        if (e < chunk_vox) nxt[e] = pref[r];
+        const int it = threadIdx.x + r * nthreads;
+        if (it < chunk_items) {
+          nxt4[2 * it] = pref_lo[r];
+          nxt4[2 * it + 1] = pref_hi[r];
+        }
       }
     }
     __syncthreads();
@@ -1843,12 +1878,12 @@ R1SymPlan plan_r1_sym(int64_t X, int64_t Y, int64_t Z, int64_t N, int64_t T,
   R1SymPlan p;
   memset(&p, 0, sizeof(p));
   p.possible = X >= 2 && Y >= 2 && !(X & 1) && !(Y & 1) && N >= 4 && !(N & 3) &&
-               Z >= 2 && 4 * Z <= kChunkVoxels && N <= (1 << 24);
+               Z >= 2 && 4 * (Z + 1) <= kChunkVoxels && N <= (1 << 24);
   if (!p.possible) return p;
   p.K = gen.K < 8 ? gen.K : 8;
   p.passes = (int)((T + 64 * p.K - 1) / (64 * p.K));
   const int64_t n_reps = (X / 2) * (Y / 2);
-  int64_t rpc = kChunkVoxels / (4 * Z);
+  int64_t rpc = kChunkVoxels / (4 * (Z + 1));  // column quads plus a zero row each
   if (rpc > 64) rpc = 64;
   p.reps_per_chunk = (int)rpc;
   const int64_t total_chunks = (n_reps + rpc - 1) / rpc;
@@ -1891,7 +1926,7 @@ void launch_r1_sym(const R1SymPlan& p, const float* vol, const float* mp, const 
                    const int* info, float* dst, const ConfGeom& g, hipStream_t st) {
   const int quad_blocks = (g.N / 4 + kWaves - 1) / kWaves;
   const int blocks = quad_blocks * p.n_groups;
-  const size_t lds = 2 * (size_t)p.reps_per_chunk * 4 * g.Z * sizeof(float2);
+  const size_t lds = 2 * (size_t)p.reps_per_chunk * 4 * (g.Z + 1) * sizeof(float2);
   hipLaunchKernelGGL(r1_sym_kernel<K>, dim3(blocks, p.passes), dim3(kWaves * 64), lds, st,
                      vol, mp, quads, info, dst, g, p.n_groups, p.reps_per_chunk,
                      p.chunks_per_group);
