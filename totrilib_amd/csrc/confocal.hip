@@ -1644,19 +1644,27 @@ __device__ __forceinline__ void a1_quad_item(
   }
 }
 
+// Persistent: one single-wave workgroup per wave slot takes items from
+// `counter` (zeroed by a1_quad_order_kernel) in the cost order, so no slot
+// waits for the dispatcher between items.
 template <bool SAFE>
 __global__ __launch_bounds__(64) void a1_quad_kernel(
     const float* __restrict__ Kf, const float* __restrict__ tc2t,
     const float* __restrict__ itc4t, const float* __restrict__ mp,
     const int* __restrict__ quads, const int* __restrict__ info, float* __restrict__ dst,
     float* __restrict__ bnd, ConfGeom g, int tiles_x, int tiles_y, int n_slabs, int n_ranges,
-    int t_stride, int n_items, const int* __restrict__ order) {
+    int t_stride, int n_items, const int* __restrict__ order, int* __restrict__ counter) {
   if (!info[0]) return;
   __shared__ float4 acc[(kSlabQ + 1) * 64];
-  if ((int)blockIdx.x >= n_items) return;
-  const int item = order[blockIdx.x];
-  a1_quad_item<SAFE>(Kf, tc2t, itc4t, mp, quads, info[1], dst, bnd, g, item, tiles_x,
-                     tiles_y, n_slabs, n_ranges, t_stride, acc);
+  const int lane = threadIdx.x;
+  for (;;) {
+    int k = 0;
+    if (lane == 0) k = atomicAdd(counter, 1);
+    k = uniform_i32(__shfl(k, 0, 64));
+    if (k >= n_items) break;
+    a1_quad_item<SAFE>(Kf, tc2t, itc4t, mp, quads, info[1], dst, bnd, g, order[k], tiles_x,
+                       tiles_y, n_slabs, n_ranges, t_stride, acc);
+  }
 }
 
 // Dispatch order of the xy-mirror A1 items.  Item costs differ by a factor of
@@ -1717,11 +1725,13 @@ __global__ __launch_bounds__(64) void a1_quad_cost_kernel(
 // workgroup; the order within a bucket is immaterial).
 __global__ __launch_bounds__(1024) void a1_quad_order_kernel(const float* __restrict__ cost,
                                                              const int* __restrict__ info,
-                                                             int n, int* __restrict__ order) {
+                                                             int n, int* __restrict__ order,
+                                                             int* __restrict__ counter) {
   if (!info[0]) return;
   __shared__ int hist[256];
   __shared__ float red[1024];
   const int tid = threadIdx.x;
+  if (tid == 0) *counter = 0;  // work queue head of the A1 kernel that follows
   float m = 0.f;
   for (int i = tid; i < n; i += 1024) m = fmaxf(m, cost[i]);
   red[tid] = m;
@@ -2168,6 +2178,7 @@ struct A1SymPlan {
   size_t partial_bytes;
   MirrorLayout ml;
   size_t cost_off, order_off;  // xy-mirror item costs (float) and dispatch order (int)
+  size_t counter_off;          // xy-mirror work-queue head
   size_t bnd_off;              // xy-mirror boundary rows [R][X][Y][n_slabs]
   size_t ws;
 };
@@ -2202,7 +2213,8 @@ A1SymPlan plan_a1_sym(int64_t X, int64_t Y, int64_t Z, int64_t N, const A1Plan& 
     const size_t bytes = (items * sizeof(int) + 255) & ~(size_t)255;
     p.cost_off = p.ws;
     p.order_off = p.cost_off + bytes;
-    p.bnd_off = p.order_off + bytes;
+    p.counter_off = p.order_off + bytes;
+    p.bnd_off = p.counter_off + 256;
     p.ws = p.bnd_off + (((size_t)p.quad.n_ranges * X * Y * p.quad.n_slabs * sizeof(float) +
                          255) & ~(size_t)255);
   }
@@ -2293,21 +2305,26 @@ extern "C" int totri_volume_confocal_grad_volume(
       float* cost = (float*)((char*)workspace + q.cost_off);
       int* order = (int*)((char*)workspace + q.order_off);
       float* bnd = (float*)((char*)workspace + q.bnd_off);
+      int* counter = (int*)((char*)workspace + q.counter_off);
       const A1QuadPlan& qd = q.quad;
       const int items = qd.tiles_x * qd.tiles_y * qd.n_slabs * qd.n_ranges;
       hipLaunchKernelGGL(a1_quad_cost_kernel, dim3(items), dim3(64), 0, st, measurement_points,
                          quads, info, g, qd.tiles_x, qd.tiles_y, qd.n_slabs, qd.n_ranges, items,
                          cost);
       hipLaunchKernelGGL(a1_quad_order_kernel, dim3(1), dim3(1024), 0, st, cost, info, items,
-                         order);
+                         order, counter);
+      // one single-wave workgroup per wave slot (16 per CU with 9 KiB of LDS each)
+      const int waves = items < 16 * device_cu_count() ? items : 16 * device_cu_count();
       if (p.safe)
-        hipLaunchKernelGGL(a1_quad_kernel<true>, dim3(items), dim3(64), 0, st, Kf, tc2, itc4,
+        hipLaunchKernelGGL(a1_quad_kernel<true>, dim3(waves), dim3(64), 0, st, Kf, tc2, itc4,
                            measurement_points, quads, info, dst, bnd, g, qd.tiles_x,
-                           qd.tiles_y, qd.n_slabs, qd.n_ranges, p.t_stride, items, order);
+                           qd.tiles_y, qd.n_slabs, qd.n_ranges, p.t_stride, items, order,
+                           counter);
       else
-        hipLaunchKernelGGL(a1_quad_kernel<false>, dim3(items), dim3(64), 0, st, Kf, tc2, itc4,
+        hipLaunchKernelGGL(a1_quad_kernel<false>, dim3(waves), dim3(64), 0, st, Kf, tc2, itc4,
                            measurement_points, quads, info, dst, bnd, g, qd.tiles_x,
-                           qd.tiles_y, qd.n_slabs, qd.n_ranges, p.t_stride, items, order);
+                           qd.tiles_y, qd.n_slabs, qd.n_ranges, p.t_stride, items, order,
+                           counter);
       rc = check_launch("a1_quad_kernel");
       if (rc) return rc;
     }
