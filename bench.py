@@ -543,7 +543,7 @@ def kernels_main(args):
     Gc = torch.randn(trc.shape, device=dev, generator=gen)
     ms = timeit(lambda: torch.autograd.grad(trc, mp, Gc, retain_graph=True))
     fm = flop_model(mp.detach().cpu().numpy(), 128, 1024, w)
-    row("A6 scan-point gradient", "a6_kernel", "128^3 voxels, 64x64 scan, 1024 bins", ms,
+    row("A6 scan-point gradient", "a6_sym_kernel", "128^3 voxels, 64x64 scan, 1024 bins", ms,
         fm["a_fwd"] * 128 ** 3 * 4096, "active samples", 30, 4 * (2 * 4096 * 1024 + 128 ** 3))
     line = {"metric": "per-operation time and algorithmic rate", "value": None, "unit": "ms",
             "n_gpus": 1, "steps": reps, "warmup": 1, "higher_is_better": False,

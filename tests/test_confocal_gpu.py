@@ -278,6 +278,29 @@ def test_a6_matches_oracle():
     assert e64 < TOL
 
 
+@pytest.mark.parametrize("scan", ["mirror", "perturbed"])
+def test_a6_mirror_quads_and_general_path(scan):
+    """Mirror quads of scan points on a box centred at x = y = 0 take the
+    symmetric A6 (one decision per four scan points, gradients with the
+    mirror signs); one point moved by an ulp takes the general kernel.  Both
+    match the oracle."""
+    rng = np.random.default_rng(23)
+    X, Y, Z, T = 20, 18, 24, 700
+    vol = rng.random((X, Y, Z)).astype(np.float32)
+    mp = wall_grid(6)
+    if scan == "perturbed":
+        mp[0, 7] = np.nextafter(mp[0, 7], np.float32(2))
+    w, off = 0.004, 0.001
+    G = rng.standard_normal((mp.shape[1], T)).astype(np.float32)
+    info = mirror_info(mp)
+    assert info[:2] == ([1, 9] if scan == "mirror" else [0, info[1]])
+    got = gpu_a6(vol, mp, G, w, off)
+    ref64 = O.a6(vol, mp, G, ORIGIN, SIZE, w, off, mode=O.PINNED64)
+    e64 = rel_l2(got, ref64)
+    print(f"{scan}: A6 rel-L2 vs FP64 pinned {e64:.2e}")
+    assert e64 < TOL
+
+
 def test_autograd_volume_and_points():
     """VolumeRenderConfocal.apply: forward R1, backward A1 (albedo) and A6
     (wall points), as the reference's autograd Function."""

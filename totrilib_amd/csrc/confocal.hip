@@ -777,6 +777,257 @@ __device__ __forceinline__ void r1_sym_block(
   }
 }
 
+// One workgroup item of the symmetric A6: the walk of r1_sym_block (same
+// staging, culling and decisions), accumulating per scan point s_m of the
+// wave's quad and bin the three partial derivatives of A6 (a6_kernel).  The
+// sample of family f for s_m reads column c_(m^f) and has offsets
+// (sigma_x(m) dx_f, sigma_y(m) dy_f) with dx_f = wx - x_f, dy_f = wy - y_f
+// (sigma = -1 on a mirrored axis), so the shared decision serves four scan
+// points whose gradients differ only in those signs and in the gathered
+// pair.  The wave then contracts its bins with G[s_m][t] and writes one
+// (gx, gy, gz) per scan point to part[pass][grp][3][N].
+template <int K, typename GT>
+__device__ __forceinline__ void a6_sym_block(
+    const float* __restrict__ vol, const float* __restrict__ mp,
+    const int* __restrict__ quads, int n_quads, const GT* __restrict__ G,
+    float* __restrict__ part, const ConfGeom& g, int t_base, int pass, int grp, int qgrp,
+    int n_groups, int reps_per_chunk, int chunks_per_group, float2* __restrict__ lds2) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int q = qgrp * kWaves + wave;
+  const bool quad_ok = q < n_quads;
+  const int HX = g.X / 2, HY = g.Y / 2;
+  const int n_reps = HX * HY;
+  // A chunk holds reps_per_chunk column quads as [rep][z][m] pairs with
+  // z = 0 .. Z, where row z = Z is (0, 0): a sample's four mirror pairs sit
+  // side by side, and an inactive sample reads the zero row.
+  const int zrows = g.Z + 1;
+  const int rep_pairs = 4 * zrows;
+  const int chunk_vox = reps_per_chunk * rep_pairs;  // pairs
+  const int chunk_items = reps_per_chunk * zrows;    // (rep, z) rows of 4 pairs
+
+  float tc2[K], itc4[K], gx[4][K], gy[4][K], gz[4][K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int t = t_base + 64 * k + lane;
+    const float tc = bin_distance(t, g.w, g.off);
+    const float qq = tc * tc;
+    tc2[k] = (t < g.T) ? qq : -1.f;
+    // finite, so that an inactive sample (d2 clamped to 0) has weight 0
+    itc4[k] = qq > 0.f ? fminf(1.f / (qq * qq), 3.4028235e38f) : 0.f;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) gx[m][k] = gy[m][k] = gz[m][k] = 0.f;
+  }
+  int sidx[4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) sidx[m] = quad_ok ? quads[4 * q + m] : 0;
+  const int s0 = uniform_i32(sidx[0]);
+  const float wx = uniform_f32(mp[s0]);
+  const float wy = uniform_f32(mp[g.N + s0]);
+  const float wz = uniform_f32(mp[2 * g.N + s0]);
+  float c0 = (wz - g.oz) * g.s;
+  c0 = c0 - 0.5f;
+  const float delta = guard_delta(g, wz, c0, 0.f);
+  const float half_m_delta = 0.5f - delta;
+  float dlo, dhi;
+  depth_range(g, wz, 0.f, (float)(g.Z - 1), &dlo, &dhi);
+
+  const int total_chunks = (n_reps + reps_per_chunk - 1) / reps_per_chunk;
+  const int chunk_first = grp * chunks_per_group;
+  const int chunk_end = min(chunk_first + chunks_per_group, total_chunks);
+  const int nthreads = kWaves * 64;
+
+  // Row `it` of chunk c: the pairs (v[z], v[z+1] - v[z]) of the four mirror
+  // columns m = 0..3 of one column quad at one z, as two float4 (m = 0, 1 and
+  // m = 2, 3).  Consecutive rows are consecutive z of the same columns, so a
+  // wave's loads run along four columns.
+  auto load_row = [&](int c, int it, float4* lo, float4* hi) {
+    const int jr = it / zrows;
+    const int z = it - jr * zrows;
+    const int r = c * reps_per_chunk + jr;
+    float2 p[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) p[m] = make_float2(0.f, 0.f);
+    if (r < n_reps && z < g.Z) {
+      const int rx = r / HY, ry = r - rx * HY;
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        const int xi = (m & 1) ? g.X - 1 - rx : rx;
+        const int yi = (m & 2) ? g.Y - 1 - ry : ry;
+        const float* col = vol + ((size_t)xi * g.Y + yi) * g.Z;
+        const float v0 = col[z];
+        const float v1 = (z == g.Z - 1) ? v0 : col[z + 1];
+        p[m] = make_float2(v0, v1 - v0);
+      }
+    }
+    *lo = make_float4(p[0].x, p[0].y, p[1].x, p[1].y);
+    *hi = make_float4(p[2].x, p[2].y, p[3].x, p[3].y);
+  };
+
+  if (chunk_first < chunk_end) {
+    float4* dst4 = reinterpret_cast<float4*>(lds2);
+    for (int it = threadIdx.x; it < chunk_items; it += nthreads) {
+      float4 lo, hi;
+      load_row(chunk_first, it, &lo, &hi);
+      dst4[2 * it] = lo;
+      dst4[2 * it + 1] = hi;
+    }
+  }
+  __syncthreads();
+
+  constexpr int kPref = kChunkVoxels / (4 * kWaves * 64);  // rows per thread
+  for (int c = chunk_first; c < chunk_end; ++c) {
+    const int buf = (c - chunk_first) & 1;
+    const float2* cur = lds2 + buf * chunk_vox;
+    float4* nxt4 = reinterpret_cast<float4*>(lds2 + (buf ^ 1) * chunk_vox);
+    const bool has_next = c + 1 < chunk_end;
+    float4 pref_lo[kPref], pref_hi[kPref];
+    if (has_next) {
+#pragma unroll
+      for (int r = 0; r < kPref; ++r) {
+        const int it = threadIdx.x + r * nthreads;
+        if (it < chunk_items) {
+          load_row(c + 1, it, &pref_lo[r], &pref_hi[r]);
+        } else {
+          pref_lo[r] = make_float4(0.f, 0.f, 0.f, 0.f);
+          pref_hi[r] = pref_lo[r];
+        }
+      }
+    }
+
+    if (quad_ok) {
+      // Lane j describes column quad j of the chunk.
+      const int r = c * reps_per_chunk + lane;
+      const bool rvalid = lane < reps_per_chunk && r < n_reps;
+      const int rr = rvalid ? r : 0;
+      const int rx = rr / HY, ry = rr - rx * HY;
+      const float x0 = cell_centre(rx, g.ox, g.szx, g.X);
+      const float x1 = cell_centre(g.X - 1 - rx, g.ox, g.szx, g.X);
+      const float y0 = cell_centre(ry, g.oy, g.szy, g.Y);
+      const float y1 = cell_centre(g.Y - 1 - ry, g.oy, g.szy, g.Y);
+      const float dxa = wx - x0, dxb = wx - x1, dya = wy - y0, dyb = wy - y1;
+      const float A = dxa * dxa, B = dxb * dxb, C = dya * dya, D = dyb * dyb;
+      unsigned packed = 0u;
+      bool any = false;
+#pragma unroll
+      for (int f = 0; f < 4; ++f) {
+        const float dx2 = (f & 1) ? B : A;
+        const float dy2 = (f & 2) ? D : C;
+        int tlo, thi;
+        bin_range(g, dx2 + dy2, dlo, dhi, &tlo, &thi);
+        const bool hit = rvalid && thi >= t_base && tlo <= t_base + 64 * K - 1 && tlo <= thi;
+        const int klo = (max(tlo, t_base) - t_base) >> 6;
+        const int khi = (min(thi, t_base + 64 * K - 1) - t_base) >> 6;
+        const unsigned bits = hit ? (((2u << khi) - 1u) & ~((1u << klo) - 1u)) : 0u;
+        packed |= bits << (8 * f);
+        any = any || hit;
+      }
+      unsigned long long mask = __ballot(any);
+      while (mask) {
+        const int j = __builtin_ctzll(mask);
+        mask &= mask - 1;
+        const unsigned wm = (unsigned)__builtin_amdgcn_readlane((int)packed, j);
+        const float cA = lane_value(A, j), cB = lane_value(B, j);
+        const float cC = lane_value(C, j), cD = lane_value(D, j);
+        const float cxa = lane_value(dxa, j), cxb = lane_value(dxb, j);
+        const float cya = lane_value(dya, j), cyb = lane_value(dyb, j);
+        const float2* quad = cur + j * rep_pairs;
+        // Families 2h and 2h+1 share dy^2; both are evaluated for a window
+        // where either is active (two independent decisions per pass; a
+        // sample outside its family's range fails the activity test).
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int fa = 2 * h, fb = 2 * h + 1;
+          const unsigned wmask = ((wm >> (8 * fa)) | (wm >> (8 * fb))) & 0xffu;
+          const float dy2 = h ? cD : cC;
+          const float cy = h ? cyb : cya;
+#pragma unroll
+          for (int k = 0; k < K; ++k) {
+            if (!((wmask >> k) & 1u)) continue;
+            const float d2a = (tc2[k] - cA) - dy2;  // volume_cuda.cu:40
+            const float d2b = (tc2[k] - cB) - dy2;
+            // Decisions and weights use d2 clamped at 0: identical for an
+            // active sample (d2 > 0), finite for an inactive one, which then
+            // reads the zero row: its term is 0 * 0 + 0 times a finite weight.
+            const float d2ca = fmaxf(d2a, 0.f), d2cb = fmaxf(d2b, 0.f);
+            float ala, alb;
+            bool na, nb;
+            int za = decide_fast(g, d2ca, c0, delta, half_m_delta, &ala, &na);
+            int zb = decide_fast(g, d2cb, c0, delta, half_m_delta, &alb, &nb);
+            if (__ballot(na || nb)) {
+              if (na) za = decide_exact(g, d2ca, wz, 0, &ala);
+              if (nb) zb = decide_exact(g, d2cb, wz, 0, &alb);
+            }
+            const bool acta = (d2a > 0.f) & ((unsigned)za <= (unsigned)(g.Z - 2));
+            const bool actb = (d2b > 0.f) & ((unsigned)zb <= (unsigned)(g.Z - 2));
+            const float2* pa = quad + 4 * (acta ? za : g.Z);
+            const float2* pb = quad + 4 * (actb ? zb : g.Z);
+            // Shared per decision, with u = d2 / tc^8: f2 = (depth / tc^2)^4
+            // = u d2, the first term's factor 4 u, and f2 / depth (an
+            // inactive sample has d2 = 0 and a finite 1 / max(depth, tiny)).
+            const float i8 = fminf(itc4[k] * itc4[k], 3.4028235e38f);
+            const float ua = d2ca * i8, ub = d2cb * i8;
+            const float f2a = ua * d2ca, f2b = ub * d2cb;
+            const float Aa = 4.f * ua, Ab = 4.f * ub;
+            const float Ba = f2a * __builtin_amdgcn_rcpf(fmaxf(sqrt_fast(d2ca), 1e-30f));
+            const float Bb = f2b * __builtin_amdgcn_rcpf(fmaxf(sqrt_fast(d2cb), 1e-30f));
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+              // d/dw of scan s_m (a6_kernel): -dx (4 f1 u + s dv f2 / depth),
+              // likewise y, and s dv f2 for z; sample offsets carry m's signs
+              const float sx = (m & 1) ? 1.f : -1.f, sy = (m & 2) ? 1.f : -1.f;
+              const float2 va = pa[m ^ fa];
+              const float2 vb = pb[m ^ fb];
+              const float sdva = g.s * va.y, sdvb = g.s * vb.y;
+              const float cxya = fmaf(fmaf(ala, va.y, va.x), Aa, sdva * Ba);
+              const float cxyb = fmaf(fmaf(alb, vb.y, vb.x), Ab, sdvb * Bb);
+              gx[m][k] = fmaf(sx * cxb, cxyb, fmaf(sx * cxa, cxya, gx[m][k]));
+              gy[m][k] = fmaf(sy * cy, cxyb, fmaf(sy * cy, cxya, gy[m][k]));
+              gz[m][k] = fmaf(sdvb, f2b, fmaf(sdva, f2a, gz[m][k]));
+            }
+          }
+        }
+      }
+    }
+
+    if (has_next) {
+#pragma unroll
+      for (int r = 0; r < kPref; ++r) {
+        const int it = threadIdx.x + r * nthreads;
+        if (it < chunk_items) {
+          nxt4[2 * it] = pref_lo[r];
+          nxt4[2 * it + 1] = pref_hi[r];
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  if (quad_ok) {
+    float* o = part + ((size_t)pass * n_groups + grp) * 3 * g.N;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      float sxs = 0.f, sys = 0.f, szs = 0.f;
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const int t = t_base + 64 * k + lane;
+        const float gt = t < g.T ? load_elem(G + (size_t)sidx[m] * g.T + t) : 0.f;
+        sxs = fmaf(gx[m][k], gt, sxs);
+        sys = fmaf(gy[m][k], gt, sys);
+        szs = fmaf(gz[m][k], gt, szs);
+      }
+      sxs = wave_sum_f32(sxs);
+      sys = wave_sum_f32(sys);
+      szs = wave_sum_f32(szs);
+      if (lane == 0) {
+        o[sidx[m]] = sxs;
+        o[g.N + sidx[m]] = sys;
+        o[2 * g.N + sidx[m]] = szs;
+      }
+    }
+  }
+}
+
 // Two workgroups per CU (their chunk buffers take 128 KiB of LDS), so the
 // compiler is held to 128 VGPRs.  blockIdx.y counts the bin passes from
 // the last one.
@@ -797,6 +1048,26 @@ void r1_sym_kernel(
   r1_sym_block<K>(vol, mp, quads, n_quads, dst, g, (int)(gridDim.y - 1 - blockIdx.y) * 64 * K,
                   (int)(blockIdx.x % n_groups), qgrp, n_groups, reps_per_chunk,
                   chunks_per_group, lds2);
+}
+
+// Symmetric A6, all bin passes in one launch (blockIdx.y counts them from the
+// last one, as in r1_sym_kernel).
+template <int K, typename GT>
+__global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(4)))
+void a6_sym_kernel(
+    const float* __restrict__ vol, const float* __restrict__ mp,
+    const int* __restrict__ quads, const int* __restrict__ info, const GT* __restrict__ G,
+    float* __restrict__ part, ConfGeom g, int n_groups, int reps_per_chunk,
+    int chunks_per_group) {
+  if (!info[0]) return;
+  extern __shared__ float2 lds2[];
+  const int n_quads = info[1];
+  const int qgrp = (int)(blockIdx.x / n_groups);
+  if (qgrp * kWaves >= n_quads) return;  // whole workgroup idle
+  const int pass = (int)(gridDim.y - 1 - blockIdx.y);
+  a6_sym_block<K, GT>(vol, mp, quads, n_quads, G, part, g, pass * 64 * K, pass,
+                      (int)(blockIdx.x % n_groups), qgrp, n_groups, reps_per_chunk,
+                      chunks_per_group, lds2);
 }
 
 // Sum of partials in a fixed order where the number of parts depends on
@@ -864,7 +1135,8 @@ __global__ __launch_bounds__(kWaves * 64) void a6_kernel(
     const float* __restrict__ vol, const float* __restrict__ mp,
     const GT* __restrict__ G, float* __restrict__ part,  // [parts][3][N]
     ConfGeom g, int t_base, int pass, int n_groups, int chunk_cols,
-    int chunks_per_group) {
+    int chunks_per_group, const int* __restrict__ skip) {
+  if (skip && *skip) return;  // the mirror kernel has done the work
   extern __shared__ float2 lds2[];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
@@ -2434,12 +2706,12 @@ namespace {
 
 template <int K, typename GT>
 void launch_a6(const R1Plan& p, const float* vol, const float* mp, const GT* G, float* part,
-               const ConfGeom& g, int t_base, int pass, hipStream_t st) {
+               const ConfGeom& g, int t_base, int pass, hipStream_t st, const int* skip) {
   const int scan_groups = (g.N + kWaves - 1) / kWaves;
   const size_t lds = (size_t)p.chunk_cols * g.Z * sizeof(float2);
   hipLaunchKernelGGL((a6_kernel<K, GT>), dim3(scan_groups * p.n_groups), dim3(kWaves * 64),
                      lds, st, vol, mp, G, part, g, t_base, pass, p.n_groups, p.chunk_cols,
-                     p.chunks_per_group);
+                     p.chunks_per_group, skip);
 }
 
 struct A6Plan {
@@ -2457,22 +2729,87 @@ A6Plan plan_a6(int64_t X, int64_t Y, int64_t Z, int64_t N, int64_t T) {
   return p;
 }
 
+// Plan of the symmetric A6: the symmetric R1's grid (quad blocks x column
+// groups, the same chunks) with K <= 4 windows (three accumulators per scan
+// point and window).  Workspace: mirror detection | partials of the larger
+// of the two paths.
+struct A6SymPlan {
+  bool possible;
+  int K, passes, n_groups, reps_per_chunk, chunks_per_group;
+  MirrorLayout ml;
+  size_t part_off, ws;
+};
+
+A6SymPlan plan_a6_sym(int64_t X, int64_t Y, int64_t Z, int64_t N, int64_t T,
+                      const A6Plan& gen) {
+  A6SymPlan p;
+  memset(&p, 0, sizeof(p));
+  const R1SymPlan r = plan_r1_sym(X, Y, Z, N, T, gen.r);
+  p.possible = r.possible;
+  if (!p.possible) return p;
+  p.K = r.K < 4 ? r.K : 4;
+  p.passes = (int)((T + 64 * p.K - 1) / (64 * p.K));
+  p.n_groups = r.n_groups;
+  p.reps_per_chunk = r.reps_per_chunk;
+  p.chunks_per_group = r.chunks_per_group;
+  p.ml = mirror_layout(N, 0);
+  p.part_off = (p.ml.end + 255) & ~(size_t)255;
+  const size_t parts_s = (size_t)p.passes * p.n_groups;
+  const size_t parts_g = (size_t)gen.passes * gen.r.n_groups;
+  p.ws = p.part_off + (parts_s > parts_g ? parts_s : parts_g) * 3 * N * sizeof(float);
+  return p;
+}
+
+size_t a6_workspace(int64_t X, int64_t Y, int64_t Z, int64_t N, int64_t T) {
+  const A6Plan p = plan_a6(X, Y, Z, N, T);
+  const A6SymPlan q = plan_a6_sym(X, Y, Z, N, T, p);
+  return q.possible && q.ws > p.ws ? q.ws : p.ws;
+}
+
 template <typename GT>
 int run_a6(const A6Plan& p, const float* vol, const float* mp, const void* Gv, float* part,
-           const ConfGeom& g, hipStream_t st) {
+           const ConfGeom& g, hipStream_t st, const int* skip) {
   const GT* G = (const GT*)Gv;
   for (int pass = 0; pass < p.passes; ++pass) {
     const int t_base = pass * 64 * p.K;
     switch (p.K) {
-      case 1: launch_a6<1, GT>(p.r, vol, mp, G, part, g, t_base, pass, st); break;
-      case 2: launch_a6<2, GT>(p.r, vol, mp, G, part, g, t_base, pass, st); break;
-      case 4: launch_a6<4, GT>(p.r, vol, mp, G, part, g, t_base, pass, st); break;
-      default: launch_a6<8, GT>(p.r, vol, mp, G, part, g, t_base, pass, st); break;
+      case 1: launch_a6<1, GT>(p.r, vol, mp, G, part, g, t_base, pass, st, skip); break;
+      case 2: launch_a6<2, GT>(p.r, vol, mp, G, part, g, t_base, pass, st, skip); break;
+      case 4: launch_a6<4, GT>(p.r, vol, mp, G, part, g, t_base, pass, st, skip); break;
+      default: launch_a6<8, GT>(p.r, vol, mp, G, part, g, t_base, pass, st, skip); break;
     }
     int rc = check_launch("a6_kernel");
     if (rc) return rc;
   }
   return TOTRI_OK;
+}
+
+template <typename GT>
+int run_a6_sym(const A6SymPlan& q, const float* vol, const float* mp, const int* quads,
+               const int* info, const void* Gv, float* part, const ConfGeom& g,
+               hipStream_t st) {
+  const GT* G = (const GT*)Gv;
+  const int quad_blocks = (g.N / 4 + kWaves - 1) / kWaves;
+  const dim3 grid(quad_blocks * q.n_groups, q.passes);
+  const size_t lds = 2 * (size_t)q.reps_per_chunk * 4 * (g.Z + 1) * sizeof(float2);
+  switch (q.K) {
+    case 1:
+      hipLaunchKernelGGL((a6_sym_kernel<1, GT>), grid, dim3(kWaves * 64), lds, st, vol, mp,
+                         quads, info, G, part, g, q.n_groups, q.reps_per_chunk,
+                         q.chunks_per_group);
+      break;
+    case 2:
+      hipLaunchKernelGGL((a6_sym_kernel<2, GT>), grid, dim3(kWaves * 64), lds, st, vol, mp,
+                         quads, info, G, part, g, q.n_groups, q.reps_per_chunk,
+                         q.chunks_per_group);
+      break;
+    default:
+      hipLaunchKernelGGL((a6_sym_kernel<4, GT>), grid, dim3(kWaves * 64), lds, st, vol, mp,
+                         quads, info, G, part, g, q.n_groups, q.reps_per_chunk,
+                         q.chunks_per_group);
+      break;
+  }
+  return check_launch("a6_sym_kernel");
 }
 
 }  // namespace
@@ -2481,7 +2818,7 @@ int run_a6(const A6Plan& p, const float* vol, const float* mp, const void* Gv, f
 extern "C" size_t totri_volume_confocal_grad_measurement_points_workspace(
     int64_t X, int64_t Y, int64_t Z, int64_t N, int64_t T) {
   if (X <= 0 || Y <= 0 || Z < 2 || N <= 0 || T <= 0) return 0;
-  return plan_a6(X, Y, Z, N, T).ws;
+  return a6_workspace(X, Y, Z, N, T);
 }
 
 extern "C" int totri_volume_confocal_grad_measurement_points(
@@ -2502,20 +2839,46 @@ extern "C" int totri_volume_confocal_grad_measurement_points(
   TOTRI_CHECK_ARG(Z <= kChunkVoxels, "volume depth Z=%lld exceeds %d", (long long)Z,
                   kChunkVoxels);
   const A6Plan p = plan_a6(X, Y, Z, N, T);
-  if (workspace_bytes < p.ws)
-    return fail(TOTRI_EWORKSPACE, "workspace too small: need %zu bytes, got %zu", p.ws,
+  const A6SymPlan q = plan_a6_sym(X, Y, Z, N, T, p);
+  // The mirror path needs the voxel grid antisymmetric in x and y; the scan
+  // quads are then looked for on the device.
+  const bool sym = q.possible && axis_antisymmetric(volume_origin[0], volume_size[0], X) &&
+                   axis_antisymmetric(volume_origin[1], volume_size[1], Y);
+  const size_t need = a6_workspace(X, Y, Z, N, T);
+  if (workspace_bytes < need)
+    return fail(TOTRI_EWORKSPACE, "workspace too small: need %zu bytes, got %zu", need,
                 workspace_bytes);
   TOTRI_CHECK_ARG(workspace != nullptr, "null workspace");
   const ConfGeom g = make_geom(X, Y, Z, N, T, volume_origin, volume_size, bin_width,
                                bin_offset);
-  float* part = (float*)workspace;
+  float* part = (float*)((char*)workspace + (sym ? q.part_off : 0));
+  const int* info = nullptr;
+  if (sym) {
+    rc = launch_mirror_detect(q.ml, measurement_points, (int)N, workspace, st);
+    if (rc) return rc;
+    info = (const int*)((char*)workspace + q.ml.info_off);
+    const int* quads = (const int*)((char*)workspace + q.ml.quads_off);
+    switch (transient_dtype) {
+      case TOTRI_F32: rc = run_a6_sym<float>(q, volume, measurement_points, quads, info, transient_grad, part, g, st); break;
+      case TOTRI_BF16: rc = run_a6_sym<__hip_bfloat16>(q, volume, measurement_points, quads, info, transient_grad, part, g, st); break;
+      default: rc = run_a6_sym<__half>(q, volume, measurement_points, quads, info, transient_grad, part, g, st); break;
+    }
+    if (rc) return rc;
+  }
   switch (transient_dtype) {
-    case TOTRI_F32: rc = run_a6<float>(p, volume, measurement_points, transient_grad, part, g, st); break;
-    case TOTRI_BF16: rc = run_a6<__hip_bfloat16>(p, volume, measurement_points, transient_grad, part, g, st); break;
-    default: rc = run_a6<__half>(p, volume, measurement_points, transient_grad, part, g, st); break;
+    case TOTRI_F32: rc = run_a6<float>(p, volume, measurement_points, transient_grad, part, g, st, info); break;
+    case TOTRI_BF16: rc = run_a6<__hip_bfloat16>(p, volume, measurement_points, transient_grad, part, g, st, info); break;
+    default: rc = run_a6<__half>(p, volume, measurement_points, transient_grad, part, g, st, info); break;
   }
   if (rc) return rc;
   const size_t n = (size_t)3 * N;
+  if (sym) {
+    // info[0] says which path ran
+    hipLaunchKernelGGL(sum_partials_flag_kernel<float>, dim3(launch_grid(n)), dim3(256), 0, st,
+                       part, info, q.passes * q.n_groups, p.passes * p.r.n_groups, n,
+                       measurement_points_grad);
+    return check_launch("sum_partials_flag_kernel");
+  }
   hipLaunchKernelGGL(sum_partials_kernel<float>, dim3(launch_grid(n)), dim3(256), 0, st, part,
                      p.passes * p.r.n_groups, n, measurement_points_grad);
   return check_launch("sum_partials_kernel");
