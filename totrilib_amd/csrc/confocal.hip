@@ -1226,7 +1226,9 @@ __global__ __launch_bounds__(kWaves * 64) void a6_kernel(
           const float f2 = (d2 * d2) * i8;
           const float depth = sqrt_fast(d2);
           const float sdv = g.s * v.y;
-          const float cxy = 4.f * f1 * d2 * i8 + sdv * f2 / depth;
+          // f2 / depth with the hardware reciprocal (about 1 ulp) instead of
+          // an IEEE division; the term only enters a gradient sum
+          const float cxy = 4.f * f1 * d2 * i8 + sdv * (f2 * __builtin_amdgcn_rcpf(depth));
           gx[k] = fmaf(-cdx, cxy, gx[k]);
           gy[k] = fmaf(-cdy, cxy, gy[k]);
           gz[k] = fmaf(sdv, f2, gz[k]);
