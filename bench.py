@@ -360,11 +360,15 @@ def serve_main(args):
     eng = ServingEngine(device_budget=SERVE["device_budget"], host_budget=SERVE["host_budget"],
                         lanes=2)
     eng.run(small, [0.0] * len(small), on_done=lambda tk: None)
+    # the same engine with arrival-order admission, for comparison
+    eng_fifo = ServingEngine(device_budget=SERVE["device_budget"],
+                             host_budget=SERVE["host_budget"], lanes=2, policy="fifo")
+    eng_fifo.run(small, [0.0] * len(small), on_done=lambda tk: None)
 
     base_t, base_lat = per_request_baseline(torch, dev, reqs)
     print(f"[serve] per-request baseline {base_t:.2f} s", file=sys.stderr, flush=True)
 
-    def serve(arrivals):
+    def serve(arrivals, eng=eng):
         steps0 = eng.steps
         jobs0 = len(eng.jobs_per_step)
         t0 = time.perf_counter()
@@ -387,10 +391,14 @@ def serve_main(args):
         arr = np.cumsum(rng.exponential(1.0 / lam, len(reqs)))
         arr -= arr[0]
         r = serve(arr)
+        rf = serve(arr, eng_fifo)
         rates.append(dict(load=load, rate_rps=lam, p50_ms=r["p50_ms"], p99_ms=r["p99_ms"],
                           pairs_per_s=pairs / r["span"], steps=r["steps"],
-                          mean_jobs_per_step=r["mean_jobs_per_step"]))
-        print(f"[serve] load {load}: p50 {r['p50_ms']:.1f} ms p99 {r['p99_ms']:.1f} ms",
+                          mean_jobs_per_step=r["mean_jobs_per_step"],
+                          fifo_p50_ms=rf["p50_ms"], fifo_p99_ms=rf["p99_ms"],
+                          fifo_pairs_per_s=pairs / rf["span"]))
+        print(f"[serve] load {load}: p50 {r['p50_ms']:.1f} ms p99 {r['p99_ms']:.1f} ms "
+              f"(fifo p50 {rf['p50_ms']:.1f} p99 {rf['p99_ms']:.1f})",
               file=sys.stderr, flush=True)
 
     value = pairs / sat["span"]
@@ -416,7 +424,8 @@ def serve_main(args):
                                "trip, volume 64^3|128^3|192^3, occupancy U(1%,5%); inputs "
                                "in pinned host memory, results copied back",
                    "requests": len(reqs), "pairs": pairs,
-                   "device_budget_bytes": SERVE["device_budget"], "lanes": 2},
+                   "device_budget_bytes": SERVE["device_budget"], "lanes": 2,
+                   "policy": "sjf (fifo legs in rates for comparison)"},
         "saturated": sat,
         "per_request_baseline": {"value": base_value, "unit": "pairs/s",
                                  "p50_ms": float(np.percentile(base_lat, 50) * 1e3),

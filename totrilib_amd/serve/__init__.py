@@ -5,8 +5,9 @@ from totrilib_amd.serve.engine import (
     RenderRequest,
     ServingEngine,
     Ticket,
+    request_cost,
     request_pairs,
 )
 
 __all__ = ["Arena", "TensorArena", "ServingEngine", "RenderRequest", "BackprojectRequest",
-           "Ticket", "request_pairs"]
+           "Ticket", "request_cost", "request_pairs"]

@@ -7,13 +7,20 @@ each with its own scan, bin count and volume size.
 
 How a request moves through the engine:
 
-* submit() puts it in a FIFO queue.
-* Admission (every poll()) takes queued requests in order while their
-  device memory -- input copies, output, scratch -- fits in the fixed-size
-  HBM arena, and their host result buffer fits in the pinned arena.  Inputs
-  that already live on the device are used in place (resident data).
+* submit() puts it in a queue.
+* Admission (every poll()) takes queued requests in priority order while
+  their device memory -- input copies, output, scratch -- fits in the
+  fixed-size HBM arena, and their host result buffer fits in the pinned
+  arena.  Inputs that already live on the device are used in place
+  (resident data).  The default policy, "sjf", orders by estimated service
+  time minus time waited (shortest job first with aging, so a large request
+  overtakes once it has waited longer than its extra cost); "fifo" keeps
+  arrival order.
 * A step starts whenever a lane is idle and admitted requests wait: up to
-  TOTRI_BATCH_MAX_JOBS of them are copied in (on the lane's stream), run by
+  TOTRI_BATCH_MAX_JOBS of them, in the same order (under "sjf" only those at
+  most `group_ratio` times as costly as the first, so a short job does not
+  wait for a long one in the same step), are copied in (on the lane's
+  stream), run by
   one batched step of persistent kernels (totri_confocal_batch_run), and
   copied out into pinned host memory.  A lane's step is a HIP graph captured
   once and replayed for every step, since its launches do not depend on the
@@ -97,6 +104,18 @@ def request_pairs(req) -> int:
     return X * Y * Z * N
 
 
+# Nominal rate for the shortest-job-first key, in voxel x scan-point x bin
+# triples per second: the active samples per pair grow with the bin count
+# for a fixed path length.  The key compares this estimate with time waited.
+_TRIPLES_PER_S = 8e14
+
+
+def request_cost(req) -> float:
+    """Estimated service time of a request in seconds (ordering only)."""
+    X, Y, Z, N, T = req.dims()
+    return X * Y * Z * N * T / _TRIPLES_PER_S
+
+
 @dataclasses.dataclass
 class Ticket:
     """Handle of a submitted request."""
@@ -106,6 +125,7 @@ class Ticket:
     t_admit: Optional[float] = None
     t_done: Optional[float] = None
     error: Optional[str] = None
+    cost: float = 0.0
     _host_out: Optional[torch.Tensor] = None
     _release: Optional[object] = None
 
@@ -167,11 +187,19 @@ class ServingEngine:
     host_budget    bytes of pinned host memory for results (one allocation)
     lanes          steps that may be queued on the device at once
     use_graphs     replay each lane's step as a captured HIP graph
+    policy         "sjf" (shortest job first with aging) or "fifo"
+    group_ratio    under "sjf", a step takes only jobs at most this many times
+                   as costly as its first (None: no limit)
     """
 
     def __init__(self, device_budget: int = 16 << 30, host_budget: int = 4 << 30,
                  lanes: int = 2, max_jobs: int = _lib.TOTRI_BATCH_MAX_JOBS,
-                 use_graphs: bool = True, device: Optional[torch.device] = None):
+                 use_graphs: bool = True, device: Optional[torch.device] = None,
+                 policy: str = "sjf", group_ratio: Optional[float] = 4.0):
+        if policy not in ("sjf", "fifo"):
+            raise ValueError(f"policy must be 'sjf' or 'fifo', not {policy!r}")
+        self.policy = policy
+        self.group_ratio = group_ratio
         if not torch.cuda.is_available():
             raise RuntimeError("ServingEngine needs a GPU")
         self.lib = _lib.load()
@@ -194,6 +222,7 @@ class ServingEngine:
     def submit(self, request, t_submit: Optional[float] = None) -> Ticket:
         self._validate(request)
         tk = Ticket(request, self._next_id, time.perf_counter() if t_submit is None else t_submit)
+        tk.cost = request_cost(request)
         self._next_id += 1
         self.queue.append(tk)
         return tk
@@ -238,27 +267,36 @@ class ServingEngine:
         sizes.append(ws)
         return sizes, out, ws
 
+    def _key(self, tk: Ticket, now: float):
+        """Priority of a ticket (smaller first)."""
+        if self.policy == "fifo":
+            return tk.id
+        return tk.cost - (now - tk.t_submit)
+
     def _admit(self, now: float) -> None:
-        while self.queue:
-            tk = self.queue[0]
+        if not self.queue:
+            return
+        taken = set()
+        for tk in sorted(self.queue, key=lambda t: self._key(t, now)):
             req = tk.request
             sizes, out_bytes, ws = self._footprint(req)
             if not self.dev.arena.fits(sizes) or out_bytes > self.host.arena.capacity:
-                self.queue.popleft()
+                taken.add(tk.id)
                 tk.error = "request exceeds the engine's memory budget"
                 tk.t_done = now
                 continue
             blocks = self.dev.arena.alloc_many(sizes)
             if blocks is None:
-                return
+                break
             hb = self.host.arena.alloc(out_bytes)
             if hb is None:
                 for b in blocks:
                     self.dev.arena.free(b)
-                return
-            self.queue.popleft()
+                break
+            taken.add(tk.id)
             tk.t_admit = now
             self.admitted.append(self._make_job(tk, blocks, hb, ws))
+        self.queue = deque(tk for tk in self.queue if tk.id not in taken)
 
     def _make_job(self, tk: Ticket, blocks: List[int], hb: int, ws: int) -> _Job:
         req = tk.request
@@ -302,10 +340,17 @@ class ServingEngine:
         return job
 
     # -- steps --------------------------------------------------------------
-    def _launch(self, lane: _Lane) -> None:
+    def _launch(self, lane: _Lane, now: float) -> None:
         jobs = []
-        while self.admitted and len(jobs) < self.max_jobs:
-            jobs.append(self.admitted.popleft())
+        for j in sorted(self.admitted, key=lambda j: self._key(j.ticket, now)):
+            if len(jobs) == self.max_jobs:
+                break
+            if (self.policy == "sjf" and self.group_ratio is not None and jobs
+                    and j.ticket.cost > self.group_ratio * jobs[0].ticket.cost):
+                continue
+            jobs.append(j)
+        chosen = set(id(j) for j in jobs)
+        self.admitted = deque(j for j in self.admitted if id(j) not in chosen)
         arr = (_lib.ConfocalJob * len(jobs))(*[j.cjob for j in jobs])
         rc = self.lib.totri_confocal_batch_prepare(arr, len(jobs), lane.host_table.data_ptr(),
                                                    self.table_bytes)
@@ -365,7 +410,7 @@ class ServingEngine:
             if not self.admitted:
                 break
             if lane.event is None:
-                self._launch(lane)
+                self._launch(lane, now)
         return self.pending()
 
     def pending(self) -> int:
