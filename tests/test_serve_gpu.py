@@ -142,3 +142,27 @@ def test_admission_respects_device_budget():
         assert torch.equal(g, single_request(r))
     assert eng.dev.arena.used == 0 and eng.host.arena.used == 0
     assert sum(request_pairs(r) for r in reqs) > 0
+
+
+def test_shortest_job_first_order():
+    """With one lane, "sjf" runs a short request submitted after a long one in
+    an earlier step (the long one is more than 4x as costly, so they do not
+    share a step); "fifo" puts both into one step in arrival order."""
+    from totrilib_amd.serve import ServingEngine, request_cost
+    reqs = make_requests(seed=3)
+    small, big = reqs[0], reqs[3]          # 24x20x40 render, 64^3 backprojection
+    assert request_cost(big) > 4 * request_cost(small)
+    for policy in ("sjf", "fifo"):
+        eng = ServingEngine(device_budget=1 << 30, host_budget=256 << 20, lanes=1,
+                            policy=policy)
+        tb = eng.submit(big)
+        ts = eng.submit(small)
+        eng.drain()
+        got = collect([tb, ts])
+        assert torch.equal(got[0], single_request(big))
+        assert torch.equal(got[1], single_request(small))
+        if policy == "sjf":
+            assert eng.jobs_per_step == [1, 1]
+            assert ts.t_done < tb.t_done
+        else:
+            assert eng.jobs_per_step == [2]
