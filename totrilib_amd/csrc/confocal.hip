@@ -2134,7 +2134,9 @@ bool axis_antisymmetric(float o, float sz, int64_t n) {
   return true;
 }
 
-R1Plan plan_r1(int64_t X, int64_t Y, int64_t Z, int64_t N, int64_t T) {
+// wg_per_cu: workgroups per CU to aim for when the scan points alone are too
+// few (column groups make up the rest).
+R1Plan plan_r1(int64_t X, int64_t Y, int64_t Z, int64_t N, int64_t T, int wg_per_cu = 2) {
   R1Plan p;
   const int64_t bins_pass = T <= 64 ? 64 : T <= 128 ? 128 : T <= 256 ? 256 : T <= 512 ? 512 : 1024;
   p.K = (int)(bins_pass / 64);
@@ -2144,7 +2146,7 @@ R1Plan plan_r1(int64_t X, int64_t Y, int64_t Z, int64_t N, int64_t T) {
   const int64_t n_cols = X * Y;
   p.total_chunks = (int)((n_cols + p.chunk_cols - 1) / p.chunk_cols);
   const int64_t scan_groups = (N + kWaves - 1) / kWaves;
-  const int64_t target = 2 * (int64_t)device_cu_count();
+  const int64_t target = wg_per_cu * (int64_t)device_cu_count();
   int64_t G = 1;
   if (scan_groups < target) G = (target + scan_groups - 1) / scan_groups;
   if (G > p.total_chunks) G = p.total_chunks;
@@ -2724,7 +2726,9 @@ struct A6Plan {
 
 A6Plan plan_a6(int64_t X, int64_t Y, int64_t Z, int64_t N, int64_t T) {
   A6Plan p;
-  p.r = plan_r1(X, Y, Z, N, T);
+  // two rounds of workgroups: A6's per-scan-point work is uneven, and a
+  // second round shortens the tail
+  p.r = plan_r1(X, Y, Z, N, T, 4);
   p.K = p.r.K < 8 ? p.r.K : 8;
   p.passes = (int)((T + 64 * p.K - 1) / (64 * p.K));
   p.ws = (size_t)p.passes * p.r.n_groups * 3 * N * sizeof(float);
