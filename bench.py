@@ -493,6 +493,18 @@ def kernels_main(args):
     row("A2 Velten confocal backprojection", "velten_confocal_kernel",
         "64^3 voxels, 32x32 scan, 512 bins", ms, R ** 3 * S, "voxel-scan pairs", 14,
         4 * (T * S + R ** 3))
+    # The fitting step after it: blob proposals from the positive residual (A2 at half
+    # resolution, second-derivative filter, normalise, square, 5 draws)
+    from totrilib_amd.fitting import sample_blob_positions, volume_error_map
+    model = 0.9 * tr[0]
+    scan0 = scan[0]
+    gdraw = torch.Generator(device=dev).manual_seed(11)
+    ms = timeit(lambda: sample_blob_positions(
+        volume_error_map(tr[0], model, scan0, box, BinDef(0.01, 0.0, T)), box, 5, gdraw))
+    Rh = R // 2
+    row("Blob proposals (fitting step)", "velten_confocal_kernel",
+        "32^3 half-resolution map of a 64^3 box, 32x32 scan, 512 bins, 5 draws", ms,
+        Rh ** 3 * S, "voxel-scan pairs", 14, 4 * (2 * T * S + 4 * Rh ** 3))
     # A3: Velten exhaustive, 64^3, 16x16 scan points x 16 laser points, 512 bins
     Sx, L = 256, 16
     sp = torch.from_numpy(wall_grid(16)).T.contiguous()[None].to(dev)         # [1, 256, 3]
