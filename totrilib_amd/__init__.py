@@ -29,6 +29,9 @@ _ALIASES = {
     "totri.util.format": "totrilib_amd.util.format",
     "totri.data": "totrilib_amd.data",
     "totri.data.util": "totrilib_amd.data.util",
+    "totri.data.pfm": "totrilib_amd.data.pfm",
+    "totri.data.otooleconfocal2018": "totrilib_amd.data.otooleconfocal2018",
+    "totri.data.veltennc2012": "totrilib_amd.data.veltennc2012",
     "totri_cpp": "totrilib_amd.totri_cpp",
     "totri_cpp.render": "totrilib_amd.totri_cpp.render",
     "totri_cpp.reconstruct": "totrilib_amd.totri_cpp.reconstruct",
