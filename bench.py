@@ -566,6 +566,26 @@ def kernels_main(args):
     fm = flop_model(mp.detach().cpu().numpy(), 128, 1024, w)
     row("A6 scan-point gradient", "a6_sym_kernel", "128^3 voxels, 64x64 scan, 1024 bins", ms,
         fm["a_fwd"] * 128 ** 3 * 4096, "active samples", 30, 4 * (2 * 4096 * 1024 + 128 ** 3))
+    # f-k migration: 1024 bins on a 64 x 64 wall; the spectrum is [2048, 64, 64] complex
+    from totrilib_amd import _lib
+    from totrilib_amd.reconstruct import FkMigration
+    from totrilib_amd.reconstruct.fk_migration import _stolt
+    Df, Hf, Wf = 1024, 64, 64
+    npts = 2 * Df * Hf * Wf
+    trf = torch.rand((1, Df, Hf, Wf), device=dev, generator=gen)
+    fkm = FkMigration(Df * 0.0025, 2.0, 2.0)
+    ms = timeit(lambda: fkm(trf))
+    # two FFTs at 5 N log2 N, about 80 per Stolt element; bytes with one pass per FFT:
+    # prepare 4+16, FFT 32+32, Stolt 8+8, magnitude 8+4 per (bin, wall point)
+    row("f-k migration", "fk_stolt_kernel + hipFFT",
+        "1024 bins, 64x64 wall, spectrum 2048x64x64 complex", ms, npts, "spectrum points",
+        10 * np.log2(npts) + 80, 112 * Df * Hf * Wf)
+    spec = torch.randn((1, 2 * Df, Hf, Wf), dtype=torch.complex64, device=dev, generator=gen)
+    gout = torch.zeros_like(spec[0])
+    ms = timeit(lambda: _stolt(spec, gout, Df * 0.0025, 2.0, 2.0, _lib.TOTRI_FK_UNSHIFTED, True))
+    row("f-k Stolt interpolation", "fk_stolt_kernel",
+        "spectrum 2048x64x64 complex, z > 0 half written", ms, npts // 2, "output points",
+        80, 16 * Df * Hf * Wf)
     line = {"metric": "per-operation time and algorithmic rate", "value": None, "unit": "ms",
             "n_gpus": 1, "steps": reps, "warmup": 1, "higher_is_better": False,
             "dtype": "f32", "data": "synthetic (seeded)", "rows": rows,

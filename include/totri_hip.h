@@ -290,6 +290,47 @@ int totri_confocal_batch_run(const void* host_table, void* device_table,
                              size_t table_bytes, void* stream);
 
 /*
+ * f-k migration.  The reference's totri.reconstruct.FkMigration is pure
+ * PyTorch (totri/reconstruct/fk_migration.py:5-120); these three entry points
+ * are its non-FFT steps, and the host layer runs the two 3-D FFTs (hipFFT)
+ * between them.  Complex buffers are interleaved complex64 (re, im).
+ *
+ * totri_fk_prepare replaces fk_migration.py:108-112:
+ *   transient [D][H][W] FP32;  padded [2D][H][W] complex, written:
+ *   sqrt(transient[d]) * linspace(0, 1, D)[d] for d < D, 0 for d >= D.
+ */
+int totri_fk_prepare(const float* transient, int64_t D, int64_t H, int64_t W,
+                     void* padded, void* stream);
+
+/*
+ * totri_fk_stolt replaces StoltInterpolation (fk_migration.py:21-85) and, with
+ * layout TOTRI_FK_UNSHIFTED, the fftshift / ifftshift around it (:114-118).
+ *   spectrum, out  [C][D2][H][W] complex, D2 even; they must not overlap
+ *   depth, height, width  StoltInterpolation's constructor arguments
+ *                         (height, width > 0)
+ *   layout  TOTRI_FK_SHIFTED: spectrum and out in fftshift order (what
+ *           StoltInterpolation.forward takes and returns);
+ *           TOTRI_FK_UNSHIFTED: spectrum as fftn returns it and out in the
+ *           order ifftn takes
+ *   out_half_zeroed  nonzero: out already holds zeros where the weights are
+ *           zero (depth index < D2/2 in fftshift order, >= D2/2 unshifted),
+ *           and that half is not written
+ */
+#define TOTRI_FK_SHIFTED 0
+#define TOTRI_FK_UNSHIFTED 1
+int totri_fk_stolt(const void* spectrum, int64_t C, int64_t D2, int64_t H, int64_t W,
+                   double depth, double height, double width, int layout,
+                   int out_half_zeroed, void* out, void* stream);
+
+/*
+ * totri_fk_magnitude replaces fk_migration.py:119:
+ *   volume [C][2D][H][W] complex;  out [C][D][H][W] FP32, written:
+ *   |volume[c][d][h][w]| for d < D.
+ */
+int totri_fk_magnitude(const void* volume, int64_t C, int64_t D, int64_t H, int64_t W,
+                       float* out, void* stream);
+
+/*
  * Test hook with no counterpart in the reference: evaluates the per-sample
  * decision function shared by the R1 and A1 kernels for n samples listed as
  * idx[n][4] = (scan point i, bin t < T, xi, yi) and writes

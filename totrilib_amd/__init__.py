@@ -24,6 +24,7 @@ _ALIASES = {
     "totri.render.volume": "totrilib_amd.render.volume",
     "totri.reconstruct": "totrilib_amd.reconstruct",
     "totri.reconstruct.backprojection": "totrilib_amd.reconstruct.backprojection",
+    "totri.reconstruct.fk_migration": "totrilib_amd.reconstruct.fk_migration",
     "totri.util": "totrilib_amd.util",
     "totri.util.rbf2volume": "totrilib_amd.util.rbf2volume",
     "totri.util.format": "totrilib_amd.util.format",

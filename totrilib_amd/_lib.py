@@ -22,6 +22,9 @@ TOTRI_F32 = 0
 TOTRI_BF16 = 1
 TOTRI_F16 = 2
 
+TOTRI_FK_SHIFTED = 0
+TOTRI_FK_UNSHIFTED = 1
+
 _c_i64 = ctypes.c_int64
 _c_p = ctypes.c_void_p
 _c_f = ctypes.c_float
@@ -84,6 +87,12 @@ SIGNATURES: dict[str, tuple] = {
     "totri_confocal_batch_table_bytes": (_c_sz, []),
     "totri_confocal_batch_prepare": (_c_int, [_c_p, _c_int, _c_p, _c_sz]),
     "totri_confocal_batch_run": (_c_int, [_c_p, _c_p, _c_sz, _c_p]),
+    "totri_fk_prepare": (_c_int, [_c_p, _c_i64, _c_i64, _c_i64, _c_p, _c_p]),
+    "totri_fk_stolt": (
+        _c_int,
+        [_c_p, _c_i64, _c_i64, _c_i64, _c_i64, ctypes.c_double, ctypes.c_double,
+         ctypes.c_double, _c_int, _c_int, _c_p, _c_p]),
+    "totri_fk_magnitude": (_c_int, [_c_p, _c_i64, _c_i64, _c_i64, _c_i64, _c_p, _c_p]),
     "totri_debug_mirror_workspace": (_c_sz, [_c_i64]),
     "totri_debug_mirror_info": (_c_int, [_c_p, _c_i64, _c_p, _c_sz, _c_p, _c_p]),
     "totri_debug_confocal_samples": (
