@@ -575,17 +575,20 @@ def kernels_main(args):
     trf = torch.rand((1, Df, Hf, Wf), device=dev, generator=gen)
     fkm = FkMigration(Df * 0.0025, 2.0, 2.0)
     ms = timeit(lambda: fkm(trf))
-    # two FFTs at 5 N log2 N, about 80 per Stolt element; bytes with one pass per FFT:
-    # prepare 4+16, FFT 32+32, Stolt 8+8, magnitude 8+4 per (bin, wall point)
+    # FLOP: real-to-complex FFT 2.5 N log2 N, inverse 5 N log2 N, about 80 per Stolt
+    # element.  Bytes per (bin, wall point), one pass per FFT: prepare 4+4 (half the
+    # padded rows), forward 8+8, Stolt 8+8, inverse 16+16, magnitude 8+4
     row("f-k migration", "fk_stolt_kernel + hipFFT",
-        "1024 bins, 64x64 wall, spectrum 2048x64x64 complex", ms, npts, "spectrum points",
-        10 * np.log2(npts) + 80, 112 * Df * Hf * Wf)
-    spec = torch.randn((1, 2 * Df, Hf, Wf), dtype=torch.complex64, device=dev, generator=gen)
-    gout = torch.zeros_like(spec[0])
-    ms = timeit(lambda: _stolt(spec, gout, Df * 0.0025, 2.0, 2.0, _lib.TOTRI_FK_UNSHIFTED, True))
+        "1024 bins, 64x64 wall, spectrum 2048x64x64", ms, npts, "spectrum points",
+        7.5 * np.log2(npts) + 80, 84 * Df * Hf * Wf)
+    spec = torch.randn((1, 2 * Df, Hf, Wf // 2 + 1), dtype=torch.complex64, device=dev,
+                       generator=gen)
+    gout = torch.zeros((2 * Df, Hf, Wf), dtype=torch.complex64, device=dev)
+    ms = timeit(lambda: _stolt(spec, gout, Df * 0.0025, 2.0, 2.0,
+                               _lib.TOTRI_FK_UNSHIFTED_ONESIDED, True))
     row("f-k Stolt interpolation", "fk_stolt_kernel",
-        "spectrum 2048x64x64 complex, z > 0 half written", ms, npts // 2, "output points",
-        80, 16 * Df * Hf * Wf)
+        "one-sided spectrum 2048x64x33 -> 2048x64x64, z > 0 half written", ms, npts // 2,
+        "output points", 80, 16 * Df * Hf * Wf)
     line = {"metric": "per-operation time and algorithmic rate", "value": None, "unit": "ms",
             "n_gpus": 1, "steps": reps, "warmup": 1, "higher_is_better": False,
             "dtype": "f32", "data": "synthetic (seeded)", "rows": rows,

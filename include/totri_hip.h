@@ -296,11 +296,13 @@ int totri_confocal_batch_run(const void* host_table, void* device_table,
  * between them.  Complex buffers are interleaved complex64 (re, im).
  *
  * totri_fk_prepare replaces fk_migration.py:108-112:
- *   transient [D][H][W] FP32;  padded [2D][H][W] complex, written:
+ *   transient [D][H][W] FP32;  padded [2D][H][W] FP32 (the reference's
+ *   complex input has a zero imaginary part), written:
  *   sqrt(transient[d]) * linspace(0, 1, D)[d] for d < D, 0 for d >= D.
+ *   pad_zeroed  nonzero: rows d >= D already hold zeros and are not written
  */
 int totri_fk_prepare(const float* transient, int64_t D, int64_t H, int64_t W,
-                     void* padded, void* stream);
+                     float* padded, int pad_zeroed, void* stream);
 
 /*
  * totri_fk_stolt replaces StoltInterpolation (fk_migration.py:21-85) and, with
@@ -311,13 +313,17 @@ int totri_fk_prepare(const float* transient, int64_t D, int64_t H, int64_t W,
  *   layout  TOTRI_FK_SHIFTED: spectrum and out in fftshift order (what
  *           StoltInterpolation.forward takes and returns);
  *           TOTRI_FK_UNSHIFTED: spectrum as fftn returns it and out in the
- *           order ifftn takes
+ *           order ifftn takes;
+ *           TOTRI_FK_UNSHIFTED_ONESIDED: as TOTRI_FK_UNSHIFTED, but the
+ *           spectrum is [C][D2][H][W/2+1], the rfftn of a real input (the
+ *           other half follows from F[-k] = conj F[k])
  *   out_half_zeroed  nonzero: out already holds zeros where the weights are
  *           zero (depth index < D2/2 in fftshift order, >= D2/2 unshifted),
  *           and that half is not written
  */
 #define TOTRI_FK_SHIFTED 0
 #define TOTRI_FK_UNSHIFTED 1
+#define TOTRI_FK_UNSHIFTED_ONESIDED 2
 int totri_fk_stolt(const void* spectrum, int64_t C, int64_t D2, int64_t H, int64_t W,
                    double depth, double height, double width, int layout,
                    int out_half_zeroed, void* out, void* stream);
@@ -329,6 +335,22 @@ int totri_fk_stolt(const void* spectrum, int64_t C, int64_t D2, int64_t H, int64
  */
 int totri_fk_magnitude(const void* volume, int64_t C, int64_t D, int64_t H, int64_t W,
                        float* out, void* stream);
+
+/*
+ * totri_fk_migration replaces FkMigration.forward (fk_migration.py:93-120)
+ * for one transient, FFTs included, when D, H and W are powers of two with
+ * H * W <= 16384 and 2D <= 4096:
+ *   transient [D][H][W] FP32;  out [D][H][W] FP32, written (the reference's
+ *   volume[0] for a batch of one)
+ *   depth, height, width  as totri_fk_stolt
+ * totri_fk_migration_workspace returns the scratch bytes it needs, or 0 for
+ * sizes it does not take (the host layer then runs the three kernels above
+ * around hipFFT).  The workspace needs no initial contents.
+ */
+size_t totri_fk_migration_workspace(int64_t D, int64_t H, int64_t W);
+int totri_fk_migration(const float* transient, int64_t D, int64_t H, int64_t W,
+                       double depth, double height, double width, float* out,
+                       void* workspace, size_t workspace_bytes, void* stream);
 
 /*
  * Test hook with no counterpart in the reference: evaluates the per-sample

@@ -79,10 +79,19 @@ def test_fk_host_checks_without_gpu():
     assert rc == _lib.TOTRI_EINVAL and "even" in _lib.last_error()
     rc = lib.totri_fk_stolt(None, 1, 8, 4, 4, 1.0, 1.0, 0.0, 0, 0, None, None)
     assert rc == _lib.TOTRI_EINVAL and "positive" in _lib.last_error()
-    rc = lib.totri_fk_stolt(None, 1, 8, 4, 4, 1.0, 1.0, 1.0, 2, 0, None, None)
+    rc = lib.totri_fk_stolt(None, 1, 8, 4, 4, 1.0, 1.0, 1.0, 3, 0, None, None)
     assert rc == _lib.TOTRI_EINVAL and "layout" in _lib.last_error()
-    assert lib.totri_fk_prepare(None, 0, 4, 4, None, None) == _lib.TOTRI_OK
+    assert lib.totri_fk_prepare(None, 0, 4, 4, None, 0, None) == _lib.TOTRI_OK
     assert lib.totri_fk_magnitude(None, 1, 4, 0, 4, None, None) == _lib.TOTRI_OK
+    # the native FFT path takes powers of two up to H * W = 16384 and 2D = 4096
+    assert lib.totri_fk_migration_workspace(1024, 64, 64) > 0
+    assert lib.totri_fk_migration_workspace(2048, 128, 128) > 0
+    for shape in [(96, 16, 16), (64, 12, 16), (4096, 4, 4), (64, 256, 128)]:
+        assert lib.totri_fk_migration_workspace(*shape) == 0
+    rc = lib.totri_fk_migration(None, 96, 16, 16, 1.0, 1.0, 1.0, None, None, 0, None)
+    assert rc == _lib.TOTRI_EINVAL and "powers of two" in _lib.last_error()
+    rc = lib.totri_fk_migration(None, 64, 16, 16, 1.0, 1.0, 1.0, None, None, 0, None)
+    assert rc == _lib.TOTRI_EINVAL and "null" in _lib.last_error()
     if not torch.cuda.is_available():
         from totrilib_amd.reconstruct import FkMigration
         with pytest.raises(RuntimeError):

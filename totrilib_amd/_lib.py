@@ -24,6 +24,7 @@ TOTRI_F16 = 2
 
 TOTRI_FK_SHIFTED = 0
 TOTRI_FK_UNSHIFTED = 1
+TOTRI_FK_UNSHIFTED_ONESIDED = 2
 
 _c_i64 = ctypes.c_int64
 _c_p = ctypes.c_void_p
@@ -87,12 +88,17 @@ SIGNATURES: dict[str, tuple] = {
     "totri_confocal_batch_table_bytes": (_c_sz, []),
     "totri_confocal_batch_prepare": (_c_int, [_c_p, _c_int, _c_p, _c_sz]),
     "totri_confocal_batch_run": (_c_int, [_c_p, _c_p, _c_sz, _c_p]),
-    "totri_fk_prepare": (_c_int, [_c_p, _c_i64, _c_i64, _c_i64, _c_p, _c_p]),
+    "totri_fk_prepare": (_c_int, [_c_p, _c_i64, _c_i64, _c_i64, _c_p, _c_int, _c_p]),
     "totri_fk_stolt": (
         _c_int,
         [_c_p, _c_i64, _c_i64, _c_i64, _c_i64, ctypes.c_double, ctypes.c_double,
          ctypes.c_double, _c_int, _c_int, _c_p, _c_p]),
     "totri_fk_magnitude": (_c_int, [_c_p, _c_i64, _c_i64, _c_i64, _c_i64, _c_p, _c_p]),
+    "totri_fk_migration_workspace": (_c_sz, [_c_i64] * 3),
+    "totri_fk_migration": (
+        _c_int,
+        [_c_p, _c_i64, _c_i64, _c_i64, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+         _c_p, _c_p, _c_sz, _c_p]),
     "totri_debug_mirror_workspace": (_c_sz, [_c_i64]),
     "totri_debug_mirror_info": (_c_int, [_c_p, _c_i64, _c_p, _c_sz, _c_p, _c_p]),
     "totri_debug_confocal_samples": (
