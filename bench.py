@@ -575,12 +575,12 @@ def kernels_main(args):
     trf = torch.rand((1, Df, Hf, Wf), device=dev, generator=gen)
     fkm = FkMigration(Df * 0.0025, 2.0, 2.0)
     ms = timeit(lambda: fkm(trf))
-    # FLOP: real-to-complex FFT 2.5 N log2 N, inverse 5 N log2 N, about 80 per Stolt
-    # element.  Bytes per (bin, wall point), one pass per FFT: prepare 4+4 (half the
-    # padded rows), forward 8+8, Stolt 8+8, inverse 16+16, magnitude 8+4
-    row("f-k migration", "fk_stolt_kernel + hipFFT",
+    # FLOP: the two 3-D FFTs at 5 N log2 N each, about 80 per Stolt element.  Bytes per
+    # (bin, wall point) the five kernels must move (W/2+1 = 33 of 64 columns one-sided):
+    # slices 4 + 4.1, depth 4.1 + 8.3, Stolt 8.3 + 8, inverse depth 8 + 8, slices 8 + 4
+    row("f-k migration", "fk_depth_fwd_kernel, fk_depth_inv_kernel, fk_stolt_kernel",
         "1024 bins, 64x64 wall, spectrum 2048x64x64", ms, npts, "spectrum points",
-        7.5 * np.log2(npts) + 80, 84 * Df * Hf * Wf)
+        10 * np.log2(npts) + 80, 65 * Df * Hf * Wf)
     spec = torch.randn((1, 2 * Df, Hf, Wf // 2 + 1), dtype=torch.complex64, device=dev,
                        generator=gen)
     gout = torch.zeros((2 * Df, Hf, Wf), dtype=torch.complex64, device=dev)

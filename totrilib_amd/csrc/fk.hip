@@ -63,6 +63,16 @@ __device__ __forceinline__ float fk_unnormalize(float c, int size) {
   return a / 2.f;
 }
 
+// Workgroup order that keeps neighbouring work on one XCD: workgroups are
+// dealt to the 8 XCDs in turn, so workgroup b runs on XCD b mod 8; the
+// first 8 * floor(n / 8) are renumbered so that each XCD gets a contiguous
+// range (the rest keep their numbers).
+__device__ __forceinline__ int fk_xcd_order(int b, int n) {
+  const int full = n & ~7;
+  if (b >= full) return b;
+  return (b & 7) * (full >> 3) + (b >> 3);
+}
+
 struct FkStoltGeom {
   int C, D2, H, W;
   float ax, ay;  // (W depth / (D2 width))^2, (H depth / (D2 height))^2
@@ -211,45 +221,124 @@ __global__ __launch_bounds__(kFkBlock) void fk_magnitude_kernel(
 //                        its D nonzero rows, keeping the D output rows used
 //   fk_slice_inv_kernel  inverse 2-D FFT of each kept slice and |.| / N
 // Each FFT runs in LDS as radix-2 decimation in frequency (natural order in,
-// bit-reversed order out, read back through the bit reversal), with
-// twiddles exp(-2 pi i k / N) computed in double.
+// bit-reversed order out, read back through the bit reversal), three stages
+// per pass in registers, with twiddles exp(-2 pi i k / N) computed in double.
 // ---------------------------------------------------------------------------
 constexpr int kFkMaxPlane = 16384;    // H * W of a slice held in LDS
 constexpr int kFkMaxDepth2 = 4096;    // 2D of a column held in LDS
-constexpr int kFkColumnLds = 16384;   // float2 of columns per depth workgroup (128 KiB)
-constexpr int kFkSliceThreads = 256;
-constexpr int kFkDepthThreads = 512;
+constexpr int kFkColumnLds = 17408;   // float2 of columns per depth workgroup (136 KiB)
+constexpr int kFkSliceThreads = 512;
+constexpr int kFkDepthThreads = 1024;
 
 __device__ __forceinline__ int fk_bitrev(int i, int logn) {
   return logn == 0 ? 0 : (int)(__builtin_bitreverse32((unsigned)i) >> (32 - logn));
 }
 
+// Multiply by W_8^r = exp(-2 pi i r / 8) (conjugated for the inverse), r < 4.
+template <bool INV, int R>
+__device__ __forceinline__ float2 fk_rot8(float2 b) {
+  constexpr float c = 0.70710678118654752f;
+  if (R == 0) return b;
+  if (R == 2) return INV ? make_float2(-b.y, b.x) : make_float2(b.y, -b.x);
+  if (R == 1)
+    return INV ? make_float2(c * (b.x - b.y), c * (b.x + b.y))
+               : make_float2(c * (b.x + b.y), c * (b.y - b.x));
+  // R == 3: W_8^3 = W_8 * W_8^2
+  const float2 t = fk_rot8<INV, 1>(b);
+  return INV ? make_float2(-t.y, t.x) : make_float2(t.y, -t.x);
+}
+
+// The K stages of one pass on a thread's 2^K registers (see fk_dif_pass).
+// Stage st = lq + l pairs v[m] and v[m + 2^l] (bit l of m clear); its twiddle
+// W_2S^(i mod S), i = j + m q, S = 2^st, is b_l * W_(2^(l+1))^(m mod 2^l) with
+// b_l = W_2S^j = tw[j << (logn - 1 - st)]: one table read per stage, the
+// rest multiplications by eighth roots of unity.
+template <bool INV, int K>
+__device__ __forceinline__ void fk_dif_stages(float2 (&v)[1 << K], int j, int lq, int logn,
+                                              const float2* tw) {
+  constexpr int G = 1 << K;
+#pragma unroll
+  for (int l = K - 1; l >= 0; --l) {
+    const int h = 1 << l;
+    float2 b = tw[j << (logn - 1 - (lq + l))];
+    if (INV) b.y = -b.y;
+#pragma unroll
+    for (int m = 0; m < G; ++m) {
+      if (m & h) continue;
+      const int r = (m & (h - 1)) << (2 - l);  // eighths: W_(2h)^(m mod h) = W_8^(r)
+      float2 w;
+      if (l == 0 || r == 0) w = b;
+      else if (r == 1) w = fk_rot8<INV, 1>(b);
+      else if (r == 2) w = fk_rot8<INV, 2>(b);
+      else w = fk_rot8<INV, 3>(b);
+      const float2 a = v[m], c = v[m + h];
+      const float dr = a.x - c.x, di = a.y - c.y;
+      v[m] = make_float2(a.x + c.x, a.y + c.y);
+      v[m + h] = make_float2(dr * w.x - di * w.y, dr * w.y + di * w.x);
+    }
+  }
+}
+
+// One LDS pass of K consecutive radix-2 decimation-in-frequency stages
+// (spans 2^s, 2^(s-1), ..., 2^(s-K+1)).  Stage span S pairs elements i and
+// i + S with i mod 2S < S: a - c is multiplied by W_2S^(i mod S) =
+// tw[(i mod S) * N / 2S].  The 2^K elements base + j + m q (m < 2^K,
+// q = 2^(s-K+1), j < q, base a multiple of 2^(s+1)) only meet each other in
+// these K stages, so a thread loads them, runs the K stages in registers and
+// stores them back.  tf_fast puts consecutive lanes on consecutive
+// transforms (for transforms that are adjacent in memory).
+template <bool INV, int K>
+__device__ __forceinline__ void fk_dif_pass(float2* buf, int logn, int s, int n_tf,
+                                            int log_ntf, int tf_stride, int el_stride,
+                                            const float2* tw, bool tf_fast) {
+  constexpr int G = 1 << K;
+  const int lq = s - K + 1;
+  const int q = 1 << lq;
+  const int lgpt = logn - K;  // log2 of the groups per transform
+  const int total = n_tf << lgpt;
+  const int step = q * el_stride;
+  for (int b = threadIdx.x; b < total; b += blockDim.x) {
+    int t, gg;
+    if (tf_fast) {
+      t = b & ((1 << log_ntf) - 1);
+      gg = b >> log_ntf;
+    } else {
+      t = b >> lgpt;
+      gg = b & ((1 << lgpt) - 1);
+    }
+    const int j = gg & (q - 1);
+    const int base = (gg >> lq) << (s + 1);
+    float2* p = buf + t * tf_stride + (base + j) * el_stride;
+    float2 v[G];
+#pragma unroll
+    for (int m = 0; m < G; ++m) v[m] = p[m * step];
+    fk_dif_stages<INV, K>(v, j, lq, logn, tw);
+#pragma unroll
+    for (int m = 0; m < G; ++m) p[m * step] = v[m];
+  }
+}
+
 // In-place radix-2 DIF FFT of n_tf transforms of length 2^logn in LDS:
 // element e of transform t is buf[t * tf_stride + e * el_stride].  The result
 // is in bit-reversed order.  tw[k] = exp(-2 pi i k / 2^logn), k < 2^(logn-1),
-// conjugated for the inverse.  Every stage ends with a barrier.
+// conjugated for the inverse.  Three stages per LDS pass; every pass ends
+// with a barrier.  tf_fast needs n_tf = 2^log_ntf.  s_first >= 0 starts at
+// the stage of span 2^s_first (the earlier stages done by the caller).
 template <bool INV>
 __device__ void fk_fft_dif(float2* buf, int logn, int n_tf, int tf_stride, int el_stride,
-                           const float2* tw) {
-  if (logn == 0) return;
-  const int half = 1 << (logn - 1);
-  const int total = n_tf * half;
-  for (int s = logn - 1; s >= 0; --s) {
-    const int span = 1 << s;
-    const int tws = half >> s;
-    for (int b = threadIdx.x; b < total; b += blockDim.x) {
-      const int t = b >> (logn - 1);
-      const int bb = b & (half - 1);
-      const int j = bb & (span - 1);
-      const int i0 = ((bb >> s) << (s + 1)) + j;
-      float2* p = buf + t * tf_stride;
-      const float2 a = p[i0 * el_stride];
-      const float2 c = p[(i0 + span) * el_stride];
-      float2 w = tw[j * tws];
-      if (INV) w.y = -w.y;
-      const float dr = a.x - c.x, di = a.y - c.y;
-      p[i0 * el_stride] = make_float2(a.x + c.x, a.y + c.y);
-      p[(i0 + span) * el_stride] = make_float2(dr * w.x - di * w.y, dr * w.y + di * w.x);
+                           const float2* tw, bool tf_fast = false, int log_ntf = 0,
+                           int s_first = -1) {
+  int s = s_first < 0 ? logn - 1 : s_first;
+  while (s >= 0) {
+    if (s >= 2) {
+      fk_dif_pass<INV, 3>(buf, logn, s, n_tf, log_ntf, tf_stride, el_stride, tw, tf_fast);
+      s -= 3;
+    } else if (s == 1) {
+      fk_dif_pass<INV, 2>(buf, logn, s, n_tf, log_ntf, tf_stride, el_stride, tw, tf_fast);
+      s -= 2;
+    } else {
+      fk_dif_pass<INV, 1>(buf, logn, s, n_tf, log_ntf, tf_stride, el_stride, tw, tf_fast);
+      s -= 1;
     }
     __syncthreads();
   }
@@ -302,7 +391,7 @@ __global__ __launch_bounds__(kFkSliceThreads) void fk_slice_fwd_kernel(
     buf[(e >> logW) * ls + (e & (W - 1))] = make_float2(sqrtf(td[e]) * ramp, 0.f);
   __syncthreads();
   fk_fft_dif<false>(buf, logW, H, ls, 1, tw_w);  // rows: over w
-  fk_fft_dif<false>(buf, logH, W, 1, ls, tw_h);  // columns: over h
+  fk_fft_dif<false>(buf, logH, W, 1, ls, tw_h, true, logW);  // columns: over h
   float2* Sd = S + (size_t)d * H * Wh;
   for (int e = threadIdx.x; e < H * Wh; e += blockDim.x) {
     const int ky = e / Wh, kx = e - ky * Wh;
@@ -310,50 +399,76 @@ __global__ __launch_bounds__(kFkSliceThreads) void fk_slice_fwd_kernel(
   }
 }
 
-// Columns c0 .. c0 + nb - 1 of a [rows][ncol] array, one transform each.
+// Columns c0 .. c0 + nb - 1 of src [Z/2][ncol] (rows Z/2 .. Z-1 are zero),
+// one length-Z transform each, column c at buf[c * (Z + 1) + e].  For Z >= 8
+// the first three stages run on the registers loaded from global memory
+// (the zero half is never stored), then the rest in LDS.
+template <bool INV>
+__device__ void fk_depth_transform(const float2* __restrict__ src, int ncol, int c0, int nb,
+                                   float2* buf, int logZ, const float2* tw) {
+  const int Z = 1 << logZ, D = Z / 2, ts = Z + 1;
+  if (logZ >= 3) {
+    const int q = Z >> 3;
+    for (int b = threadIdx.x; b < nb * q; b += blockDim.x) {
+      const int j = b / nb, c = b - j * nb;
+      float2 v[8];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) v[m] = src[(size_t)(j + m * q) * ncol + c0 + c];
+#pragma unroll
+      for (int m = 4; m < 8; ++m) v[m] = make_float2(0.f, 0.f);
+      fk_dif_stages<INV, 3>(v, j, logZ - 3, logZ, tw);
+      float2* p = buf + c * ts + j;
+#pragma unroll
+      for (int m = 0; m < 8; ++m) p[m * q] = v[m];
+    }
+    __syncthreads();
+    if (logZ > 3) fk_fft_dif<INV>(buf, logZ, nb, ts, 1, tw, false, 0, logZ - 4);
+  } else {
+    for (int e = threadIdx.x; e < nb * D; e += blockDim.x) {
+      const int d = e / nb, c = e - d * nb;
+      buf[c * ts + d] = src[(size_t)d * ncol + c0 + c];
+      buf[c * ts + D + d] = make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    fk_fft_dif<INV>(buf, logZ, nb, ts, 1, tw);
+  }
+}
+
 __global__ __launch_bounds__(kFkDepthThreads) void fk_depth_fwd_kernel(
     const float2* __restrict__ S,   // [D][ncol]
     float2* __restrict__ F,         // [2D][ncol]
-    const float2* __restrict__ twZ, int D, int logZ, int ncol, int cb) {
+    const float2* __restrict__ twZ, int logZ, int ncol, int cb) {
   extern __shared__ float2 lds2[];
   const int Z = 1 << logZ;
-  const int c0 = blockIdx.x * cb, nb = min(cb, ncol - c0);
+  const int c0 = fk_xcd_order(blockIdx.x, gridDim.x) * cb, nb = min(cb, ncol - c0);
   float2* buf = lds2;
-  float2* tw = buf + cb * Z;
+  float2* tw = buf + cb * (Z + 1);
   fk_stage_twiddles(tw, twZ, Z / 2);
-  for (int e = threadIdx.x; e < nb * D; e += blockDim.x) {
-    const int d = e / nb, c = e - d * nb;
-    buf[c * Z + d] = S[(size_t)d * ncol + c0 + c];
-    buf[c * Z + D + d] = make_float2(0.f, 0.f);
-  }
   __syncthreads();
-  fk_fft_dif<false>(buf, logZ, nb, Z, 1, tw);
+  fk_depth_transform<false>(S, ncol, c0, nb, buf, logZ, tw);
+#pragma unroll 4
   for (int e = threadIdx.x; e < nb * Z; e += blockDim.x) {
     const int kz = e / nb, c = e - kz * nb;
-    F[(size_t)kz * ncol + c0 + c] = buf[c * Z + fk_bitrev(kz, logZ)];
+    F[(size_t)kz * ncol + c0 + c] = buf[c * (Z + 1) + fk_bitrev(kz, logZ)];
   }
 }
 
 __global__ __launch_bounds__(kFkDepthThreads) void fk_depth_inv_kernel(
     const float2* __restrict__ G,   // [D][ncol]: rows 0 .. D-1 of the spectrum, the rest zero
     float2* __restrict__ V,         // [D][ncol]: output rows 0 .. D-1
-    const float2* __restrict__ twZ, int D, int logZ, int ncol, int cb) {
+    const float2* __restrict__ twZ, int logZ, int ncol, int cb) {
   extern __shared__ float2 lds2[];
-  const int Z = 1 << logZ;
-  const int c0 = blockIdx.x * cb, nb = min(cb, ncol - c0);
+  const int Z = 1 << logZ, D = Z / 2;
+  const int c0 = fk_xcd_order(blockIdx.x, gridDim.x) * cb, nb = min(cb, ncol - c0);
   float2* buf = lds2;
-  float2* tw = buf + cb * Z;
+  float2* tw = buf + cb * (Z + 1);
   fk_stage_twiddles(tw, twZ, Z / 2);
-  for (int e = threadIdx.x; e < nb * D; e += blockDim.x) {
-    const int d = e / nb, c = e - d * nb;
-    buf[c * Z + d] = G[(size_t)d * ncol + c0 + c];
-    buf[c * Z + D + d] = make_float2(0.f, 0.f);
-  }
   __syncthreads();
-  fk_fft_dif<true>(buf, logZ, nb, Z, 1, tw);
+  fk_depth_transform<true>(G, ncol, c0, nb, buf, logZ, tw);
+#pragma unroll 4
   for (int e = threadIdx.x; e < nb * D; e += blockDim.x) {
     const int n = e / nb, c = e - n * nb;
-    V[(size_t)n * ncol + c0 + c] = buf[c * Z + fk_bitrev(n, logZ)];
+    V[(size_t)n * ncol + c0 + c] = buf[c * (Z + 1) + fk_bitrev(n, logZ)];
   }
 }
 
@@ -373,7 +488,7 @@ __global__ __launch_bounds__(kFkSliceThreads) void fk_slice_inv_kernel(
   for (int e = threadIdx.x; e < HW; e += blockDim.x)
     buf[(e >> logW) * ls + (e & (W - 1))] = V[off + e];
   __syncthreads();
-  fk_fft_dif<true>(buf, logH, W, 1, ls, tw_h);   // columns: over ky
+  fk_fft_dif<true>(buf, logH, W, 1, ls, tw_h, true, logW);   // columns: over ky
   fk_fft_dif<true>(buf, logW, H, ls, 1, tw_w);   // rows: over kx
   for (int e = threadIdx.x; e < HW; e += blockDim.x) {
     const float2 v = buf[fk_bitrev(e >> logW, logH) * ls + fk_bitrev(e & (W - 1), logW)];
@@ -404,7 +519,7 @@ FkNativePlan plan_fk_native(int64_t D, int64_t H, int64_t W) {
   if (!p.ok) return p;
   p.logZ = p.logD + 1;
   const int64_t Z = 2 * D, Wh = W / 2 + 1;
-  const int cb = (int)std::max<int64_t>(1, std::min<int64_t>(16, kFkColumnLds / Z));
+  const int cb = (int)std::max<int64_t>(1, std::min<int64_t>(16, kFkColumnLds / (Z + 1)));
   p.cb_fwd = p.cb_inv = cb;
   auto align = [](size_t b) { return (b + 255) & ~(size_t)255; };
   const size_t ntw = (size_t)(W / 2 + H / 2 + Z / 2);
@@ -415,7 +530,7 @@ FkNativePlan plan_fk_native(int64_t D, int64_t H, int64_t W) {
   const size_t b_elems = std::max<size_t>((size_t)Z * H * Wh, (size_t)D * H * W);  // F, then V
   p.bytes = p.b_off + align(b_elems * sizeof(float2));
   p.slice_lds = ((size_t)H * (W + 1) + (W / 2 + 1) + (H / 2 + 1)) * sizeof(float2);
-  p.depth_lds = ((size_t)cb * Z + Z / 2) * sizeof(float2);
+  p.depth_lds = ((size_t)cb * (Z + 1) + Z / 2) * sizeof(float2);
   return p;
 }
 
@@ -557,7 +672,7 @@ extern "C" int totri_fk_migration(const float* transient, int64_t D, int64_t H, 
   if ((rc = check_launch("fk_slice_fwd_kernel"))) return rc;
   const int ncol_half = Hi * (Wi / 2 + 1), ncol = Hi * Wi;
   hipLaunchKernelGGL(fk_depth_fwd_kernel, dim3((unsigned)((ncol_half + p.cb_fwd - 1) / p.cb_fwd)),
-                     dim3(kFkDepthThreads), p.depth_lds, st, A, B, twZ, Di, p.logZ, ncol_half,
+                     dim3(kFkDepthThreads), p.depth_lds, st, A, B, twZ, p.logZ, ncol_half,
                      p.cb_fwd);
   if ((rc = check_launch("fk_depth_fwd_kernel"))) return rc;
   // Stolt: one-sided spectrum B -> nonzero half of G (rows 0 .. D-1 unshifted) in A
@@ -578,8 +693,8 @@ extern "C" int totri_fk_migration(const float* transient, int64_t D, int64_t H, 
                      0, st, (const float2*)B, A, g, n_rows);
   if ((rc = check_launch("fk_stolt_kernel"))) return rc;
   hipLaunchKernelGGL(fk_depth_inv_kernel, dim3((unsigned)((ncol + p.cb_inv - 1) / p.cb_inv)),
-                     dim3(kFkDepthThreads), p.depth_lds, st, (const float2*)A, B, twZ, Di,
-                     p.logZ, ncol, p.cb_inv);
+                     dim3(kFkDepthThreads), p.depth_lds, st, (const float2*)A, B, twZ, p.logZ,
+                     ncol, p.cb_inv);
   if ((rc = check_launch("fk_depth_inv_kernel"))) return rc;
   const float scale = ldexpf(1.f, -(p.logZ + p.logH + p.logW));
   hipLaunchKernelGGL(fk_slice_inv_kernel, dim3((unsigned)Di), dim3(kFkSliceThreads),
