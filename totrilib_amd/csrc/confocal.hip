@@ -1683,9 +1683,21 @@ __device__ __forceinline__ int pick4(int i, int v0, int v1, int v2, int v3) {
   return i == 0 ? v0 : i == 1 ? v1 : i == 2 ? v2 : v3;
 }
 
-template <bool SAFE>
+// Component n of v ^ f: (v[0^f], v[1^f], v[2^f], v[3^f]) for a wave-uniform f.
+__device__ __forceinline__ float4 xor_perm4(float4 v, int f) {
+  if (f & 1) v = make_float4(v.y, v.x, v.w, v.z);
+  if (f & 2) v = make_float4(v.z, v.w, v.x, v.y);
+  return v;
+}
+
+// PACKED reads the quad's four table rows interleaved (Kq, one 16-byte load
+// per step) and sums in table order (component m for scan s_m), permuting
+// into the family's order when a row is flushed; otherwise it reads the four
+// rows of Kf.  Both add the same values in the same order.
+template <bool SAFE, bool PACKED = false>
 __device__ __forceinline__ void a1_quad_item(
-    const float* __restrict__ Kf, const float* __restrict__ tc2t,
+    const float* __restrict__ Kf, const float4* __restrict__ Kq,
+    const float* __restrict__ tc2t,
     const float* __restrict__ itc4t, const float* __restrict__ mp,
     const int* __restrict__ quads, int n_quads,
     float* __restrict__ dst,  // [R][X][Y][Z] partial volumes
@@ -1767,11 +1779,12 @@ __device__ __forceinline__ void a1_quad_item(
         thi = -1;
       }
       if (!wave_any(tlo <= thi)) continue;
-      // component n takes scan s_(n^f)
-      const float* k0p = Kf + (size_t)pick4(f, s0, s1, s2, s3) * t_stride;
-      const float* k1p = Kf + (size_t)pick4(f ^ 1, s0, s1, s2, s3) * t_stride;
-      const float* k2p = Kf + (size_t)pick4(f ^ 2, s0, s1, s2, s3) * t_stride;
-      const float* k3p = Kf + (size_t)pick4(f ^ 3, s0, s1, s2, s3) * t_stride;
+      // component n takes scan s_(n^f) (PACKED: component m takes s_m)
+      const float* k0p = PACKED ? nullptr : Kf + (size_t)pick4(f, s0, s1, s2, s3) * t_stride;
+      const float* k1p = PACKED ? nullptr : Kf + (size_t)pick4(f ^ 1, s0, s1, s2, s3) * t_stride;
+      const float* k2p = PACKED ? nullptr : Kf + (size_t)pick4(f ^ 2, s0, s1, s2, s3) * t_stride;
+      const float* k3p = PACKED ? nullptr : Kf + (size_t)pick4(f ^ 3, s0, s1, s2, s3) * t_stride;
+      const char* kqp = PACKED ? (const char*)(Kq + (size_t)qi * t_stride) : nullptr;
       // decision of bin tt (unspecified for d2 <= 0; the walk never meets
       // such a bin).  tc^2 is recomputed with the tables' rounding.
       auto decide_bin = [&](int tt, float* d2o, float* alo) -> int {
@@ -1828,8 +1841,19 @@ __device__ __forceinline__ void a1_quad_item(
       // latency.  The index stops at T - 1.
       const uint32_t tmax4 = (uint32_t)(g.T - 1) * 4u;
       uint32_t off = min((uint32_t)t * 4u, tmax4);
-      float k0 = load_at(k0p, off), k1 = load_at(k1p, off);
-      float k2 = load_at(k2p, off), k3 = load_at(k3p, off);
+      float k0, k1, k2, k3;
+      if (PACKED) {
+        const float4 kq = *(const float4*)(kqp + 4u * off);
+        k0 = kq.x;
+        k1 = kq.y;
+        k2 = kq.z;
+        k3 = kq.w;
+      } else {
+        k0 = load_at(k0p, off);
+        k1 = load_at(k1p, off);
+        k2 = load_at(k2p, off);
+        k3 = load_at(k3p, off);
+      }
       float ic = SAFE ? load_at(itc4t, off) : 0.f;
       // Per row: L = carry + sum of K d2^2, H = sum of K d2^2 alpha; the row's
       // voxel gets L - H (its (1 - alpha) parts plus the carry) and H is
@@ -1856,10 +1880,18 @@ __device__ __forceinline__ void a1_quad_item(
             H.w = fmaf(k3, wh, H.w);
             ++t;
             off = min(off + 4u, tmax4);
-            k0 = load_at(k0p, off);
-            k1 = load_at(k1p, off);
-            k2 = load_at(k2p, off);
-            k3 = load_at(k3p, off);
+            if (PACKED) {
+              const float4 kq = *(const float4*)(kqp + 4u * off);
+              k0 = kq.x;
+              k1 = kq.y;
+              k2 = kq.z;
+              k3 = kq.w;
+            } else {
+              k0 = load_at(k0p, off);
+              k1 = load_at(k1p, off);
+              k2 = load_at(k2p, off);
+              k3 = load_at(k3p, off);
+            }
             if (SAFE) ic = load_at(itc4t, off);
             r = rn;
             al = aln;
@@ -1868,12 +1900,16 @@ __device__ __forceinline__ void a1_quad_item(
         }
         float4* row = my + rr * 64;
         float4 a = *row;
+        if (PACKED) {
+          L = xor_perm4(L, f);
+          H = xor_perm4(H, f);
+        }
         a.x += L.x - H.x;
         a.y += L.y - H.y;
         a.z += L.z - H.z;
         a.w += L.w - H.w;
         *row = a;
-        carry = H;
+        carry = PACKED ? xor_perm4(H, f) : H;  // back to the sums' order
         ++rr;
       }
       // the alpha part of the last walked row; when no row was walked (rr
@@ -1882,6 +1918,7 @@ __device__ __forceinline__ void a1_quad_item(
       if (rr <= rspan + 1) {
         float4* row = my + rr * 64;
         float4 a = *row;
+        if (PACKED) carry = xor_perm4(carry, f);
         a.x += carry.x;
         a.y += carry.y;
         a.z += carry.z;
@@ -1923,7 +1960,7 @@ __device__ __forceinline__ void a1_quad_item(
 // waits for the dispatcher between items.
 template <bool SAFE>
 __global__ __launch_bounds__(64) void a1_quad_kernel(
-    const float* __restrict__ Kf, const float* __restrict__ tc2t,
+    const float4* __restrict__ Kq, const float* __restrict__ tc2t,
     const float* __restrict__ itc4t, const float* __restrict__ mp,
     const int* __restrict__ quads, const int* __restrict__ info, float* __restrict__ dst,
     float* __restrict__ bnd, ConfGeom g, int tiles_x, int tiles_y, int n_slabs, int n_ranges,
@@ -1936,8 +1973,25 @@ __global__ __launch_bounds__(64) void a1_quad_kernel(
     if (lane == 0) k = atomicAdd(counter, 1);
     k = uniform_i32(__shfl(k, 0, 64));
     if (k >= n_items) break;
-    a1_quad_item<SAFE>(Kf, tc2t, itc4t, mp, quads, info[1], dst, bnd, g, order[k], tiles_x,
-                       tiles_y, n_slabs, n_ranges, t_stride, acc);
+    a1_quad_item<SAFE, true>(nullptr, Kq, tc2t, itc4t, mp, quads, info[1], dst, bnd, g,
+                             order[k], tiles_x, tiles_y, n_slabs, n_ranges, t_stride, acc);
+  }
+}
+
+// Kq[q][t] = (K[s0][t], K[s1][t], K[s2][t], K[s3][t]) for the device-found
+// quads q < info[1] (s_m = quads[4q + m]): the four rows one xy-mirror walk
+// reads, interleaved.
+__global__ __launch_bounds__(256) void a1_quad_table_kernel(
+    const float* __restrict__ Kf, const int* __restrict__ quads, const int* __restrict__ info,
+    int t_stride, int max_quads, float4* __restrict__ Kq) {
+  if (!info[0]) return;
+  const size_t n = (size_t)min(info[1], max_quads) * t_stride;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const size_t qi = e / t_stride, t = e - qi * t_stride;
+    const int* qs = quads + 4 * qi;
+    Kq[e] = make_float4(Kf[(size_t)qs[0] * t_stride + t], Kf[(size_t)qs[1] * t_stride + t],
+                        Kf[(size_t)qs[2] * t_stride + t], Kf[(size_t)qs[3] * t_stride + t]);
   }
 }
 
@@ -2456,6 +2510,7 @@ struct A1SymPlan {
   size_t cost_off, order_off;  // xy-mirror item costs (float) and dispatch order (int)
   size_t counter_off;          // xy-mirror work-queue head
   size_t bnd_off;              // xy-mirror boundary rows [R][X][Y][n_slabs]
+  size_t kq_off;               // xy-mirror interleaved quad tables [N/4][t_stride] float4
   size_t ws;
 };
 
@@ -2491,20 +2546,25 @@ A1SymPlan plan_a1_sym(int64_t X, int64_t Y, int64_t Z, int64_t N, const A1Plan& 
     p.order_off = p.cost_off + bytes;
     p.counter_off = p.order_off + bytes;
     p.bnd_off = p.counter_off + 256;
-    p.ws = p.bnd_off + (((size_t)p.quad.n_ranges * X * Y * p.quad.n_slabs * sizeof(float) +
-                         255) & ~(size_t)255);
+    p.kq_off = p.bnd_off + (((size_t)p.quad.n_ranges * X * Y * p.quad.n_slabs * sizeof(float) +
+                             255) & ~(size_t)255);
+    p.ws = p.kq_off + (((size_t)(N / 4) * gen.t_stride * sizeof(float4) + 255) & ~(size_t)255);
   }
   return p;
 }
 
-size_t a1_workspace(int64_t X, int64_t Y, int64_t Z, int64_t N, int64_t T) {
+// with_kq: room for the interleaved quad tables, which the single-request
+// xy-mirror path reads (the batched step reads the per-scan tables).
+size_t a1_workspace(int64_t X, int64_t Y, int64_t Z, int64_t N, int64_t T,
+                    bool with_kq = true) {
   if (X <= 0 || Y <= 0 || Z < 2 || N <= 0 || T <= 0) return 0;
   // The size does not depend on the geometry, only on the safe flag's
   // table layout, which is the same in both forms.
   const A1Plan p = plan_a1(X, Y, Z, N, T, 1.f, 1.f);
   const A1SymPlan q = plan_a1_sym(X, Y, Z, N, p);
   const size_t gen = p.tables_bytes + p.partial_bytes;
-  return q.possible && q.ws > gen ? q.ws : gen;
+  const size_t sym = (!with_kq && q.quad.possible) ? q.kq_off : q.ws;
+  return q.possible && sym > gen ? sym : gen;
 }
 
 }  // namespace
@@ -2582,6 +2642,10 @@ extern "C" int totri_volume_confocal_grad_volume(
       int* order = (int*)((char*)workspace + q.order_off);
       float* bnd = (float*)((char*)workspace + q.bnd_off);
       int* counter = (int*)((char*)workspace + q.counter_off);
+      float4* Kq = (float4*)((char*)workspace + q.kq_off);
+      hipLaunchKernelGGL(a1_quad_table_kernel,
+                         dim3(launch_grid((size_t)(N / 4) * p.t_stride)), dim3(256), 0, st, Kf,
+                         quads, info, p.t_stride, (int)(N / 4), Kq);
       const A1QuadPlan& qd = q.quad;
       const int items = qd.tiles_x * qd.tiles_y * qd.n_slabs * qd.n_ranges;
       hipLaunchKernelGGL(a1_quad_cost_kernel, dim3(items), dim3(64), 0, st, measurement_points,
@@ -2592,12 +2656,12 @@ extern "C" int totri_volume_confocal_grad_volume(
       // one single-wave workgroup per wave slot (16 per CU with 9 KiB of LDS each)
       const int waves = items < 16 * device_cu_count() ? items : 16 * device_cu_count();
       if (p.safe)
-        hipLaunchKernelGGL(a1_quad_kernel<true>, dim3(waves), dim3(64), 0, st, Kf, tc2, itc4,
+        hipLaunchKernelGGL(a1_quad_kernel<true>, dim3(waves), dim3(64), 0, st, Kq, tc2, itc4,
                            measurement_points, quads, info, dst, bnd, g, qd.tiles_x,
                            qd.tiles_y, qd.n_slabs, qd.n_ranges, p.t_stride, items, order,
                            counter);
       else
-        hipLaunchKernelGGL(a1_quad_kernel<false>, dim3(waves), dim3(64), 0, st, Kf, tc2, itc4,
+        hipLaunchKernelGGL(a1_quad_kernel<false>, dim3(waves), dim3(64), 0, st, Kq, tc2, itc4,
                            measurement_points, quads, info, dst, bnd, g, qd.tiles_x,
                            qd.tiles_y, qd.n_slabs, qd.n_ranges, p.t_stride, items, order,
                            counter);
