@@ -1678,11 +1678,6 @@ __device__ __forceinline__ float load_at(const float* __restrict__ base, uint32_
   return *(const float*)((const char*)base + off);
 }
 
-// v[i] for a wave-uniform i in [0, 4).
-__device__ __forceinline__ int pick4(int i, int v0, int v1, int v2, int v3) {
-  return i == 0 ? v0 : i == 1 ? v1 : i == 2 ? v2 : v3;
-}
-
 // Component n of v ^ f: (v[0^f], v[1^f], v[2^f], v[3^f]) for a wave-uniform f.
 __device__ __forceinline__ float4 xor_perm4(float4 v, int f) {
   if (f & 1) v = make_float4(v.y, v.x, v.w, v.z);
@@ -1690,14 +1685,12 @@ __device__ __forceinline__ float4 xor_perm4(float4 v, int f) {
   return v;
 }
 
-// PACKED reads the quad's four table rows interleaved (Kq, one 16-byte load
-// per step) and sums in table order (component m for scan s_m), permuting
-// into the family's order when a row is flushed; otherwise it reads the four
-// rows of Kf.  Both add the same values in the same order.
-template <bool SAFE, bool PACKED = false>
+// The quad's four table rows are read interleaved (Kq[q][t], one 16-byte
+// load per step) and summed in table order (component m for scan s_m); the
+// sums are permuted into the family's order when a row is flushed.
+template <bool SAFE>
 __device__ __forceinline__ void a1_quad_item(
-    const float* __restrict__ Kf, const float4* __restrict__ Kq,
-    const float* __restrict__ tc2t,
+    const float4* __restrict__ Kq, const float* __restrict__ tc2t,
     const float* __restrict__ itc4t, const float* __restrict__ mp,
     const int* __restrict__ quads, int n_quads,
     float* __restrict__ dst,  // [R][X][Y][Z] partial volumes
@@ -1779,12 +1772,9 @@ __device__ __forceinline__ void a1_quad_item(
         thi = -1;
       }
       if (!wave_any(tlo <= thi)) continue;
-      // component n takes scan s_(n^f) (PACKED: component m takes s_m)
-      const float* k0p = PACKED ? nullptr : Kf + (size_t)pick4(f, s0, s1, s2, s3) * t_stride;
-      const float* k1p = PACKED ? nullptr : Kf + (size_t)pick4(f ^ 1, s0, s1, s2, s3) * t_stride;
-      const float* k2p = PACKED ? nullptr : Kf + (size_t)pick4(f ^ 2, s0, s1, s2, s3) * t_stride;
-      const float* k3p = PACKED ? nullptr : Kf + (size_t)pick4(f ^ 3, s0, s1, s2, s3) * t_stride;
-      const char* kqp = PACKED ? (const char*)(Kq + (size_t)qi * t_stride) : nullptr;
+      // component n of the LDS rows takes scan s_(n^f); the sums' component m
+      // takes s_m
+      const char* kqp = (const char*)(Kq + (size_t)qi * t_stride);
       // decision of bin tt (unspecified for d2 <= 0; the walk never meets
       // such a bin).  tc^2 is recomputed with the tables' rounding.
       auto decide_bin = [&](int tt, float* d2o, float* alo) -> int {
@@ -1841,19 +1831,7 @@ __device__ __forceinline__ void a1_quad_item(
       // latency.  The index stops at T - 1.
       const uint32_t tmax4 = (uint32_t)(g.T - 1) * 4u;
       uint32_t off = min((uint32_t)t * 4u, tmax4);
-      float k0, k1, k2, k3;
-      if (PACKED) {
-        const float4 kq = *(const float4*)(kqp + 4u * off);
-        k0 = kq.x;
-        k1 = kq.y;
-        k2 = kq.z;
-        k3 = kq.w;
-      } else {
-        k0 = load_at(k0p, off);
-        k1 = load_at(k1p, off);
-        k2 = load_at(k2p, off);
-        k3 = load_at(k3p, off);
-      }
+      float4 kq = *(const float4*)(kqp + 4u * off);
       float ic = SAFE ? load_at(itc4t, off) : 0.f;
       // Per row: L = carry + sum of K d2^2, H = sum of K d2^2 alpha; the row's
       // voxel gets L - H (its (1 - alpha) parts plus the carry) and H is
@@ -1870,28 +1848,17 @@ __device__ __forceinline__ void a1_quad_item(
             const int rn = decide_bin(t + 1, &d2n, &aln);
             const float base = SAFE ? sq(d2 * ic) : d2 * d2;
             const float wh = base * al;
-            L.x = fmaf(k0, base, L.x);
-            L.y = fmaf(k1, base, L.y);
-            L.z = fmaf(k2, base, L.z);
-            L.w = fmaf(k3, base, L.w);
-            H.x = fmaf(k0, wh, H.x);
-            H.y = fmaf(k1, wh, H.y);
-            H.z = fmaf(k2, wh, H.z);
-            H.w = fmaf(k3, wh, H.w);
+            L.x = fmaf(kq.x, base, L.x);
+            L.y = fmaf(kq.y, base, L.y);
+            L.z = fmaf(kq.z, base, L.z);
+            L.w = fmaf(kq.w, base, L.w);
+            H.x = fmaf(kq.x, wh, H.x);
+            H.y = fmaf(kq.y, wh, H.y);
+            H.z = fmaf(kq.z, wh, H.z);
+            H.w = fmaf(kq.w, wh, H.w);
             ++t;
             off = min(off + 4u, tmax4);
-            if (PACKED) {
-              const float4 kq = *(const float4*)(kqp + 4u * off);
-              k0 = kq.x;
-              k1 = kq.y;
-              k2 = kq.z;
-              k3 = kq.w;
-            } else {
-              k0 = load_at(k0p, off);
-              k1 = load_at(k1p, off);
-              k2 = load_at(k2p, off);
-              k3 = load_at(k3p, off);
-            }
+            kq = *(const float4*)(kqp + 4u * off);
             if (SAFE) ic = load_at(itc4t, off);
             r = rn;
             al = aln;
@@ -1900,16 +1867,14 @@ __device__ __forceinline__ void a1_quad_item(
         }
         float4* row = my + rr * 64;
         float4 a = *row;
-        if (PACKED) {
-          L = xor_perm4(L, f);
-          H = xor_perm4(H, f);
-        }
+        L = xor_perm4(L, f);
+        H = xor_perm4(H, f);
         a.x += L.x - H.x;
         a.y += L.y - H.y;
         a.z += L.z - H.z;
         a.w += L.w - H.w;
         *row = a;
-        carry = PACKED ? xor_perm4(H, f) : H;  // back to the sums' order
+        carry = xor_perm4(H, f);  // back to the sums' order (the permutation is an involution)
         ++rr;
       }
       // the alpha part of the last walked row; when no row was walked (rr
@@ -1918,7 +1883,7 @@ __device__ __forceinline__ void a1_quad_item(
       if (rr <= rspan + 1) {
         float4* row = my + rr * 64;
         float4 a = *row;
-        if (PACKED) carry = xor_perm4(carry, f);
+        carry = xor_perm4(carry, f);
         a.x += carry.x;
         a.y += carry.y;
         a.z += carry.z;
@@ -1973,8 +1938,8 @@ __global__ __launch_bounds__(64) void a1_quad_kernel(
     if (lane == 0) k = atomicAdd(counter, 1);
     k = uniform_i32(__shfl(k, 0, 64));
     if (k >= n_items) break;
-    a1_quad_item<SAFE, true>(nullptr, Kq, tc2t, itc4t, mp, quads, info[1], dst, bnd, g,
-                             order[k], tiles_x, tiles_y, n_slabs, n_ranges, t_stride, acc);
+    a1_quad_item<SAFE>(Kq, tc2t, itc4t, mp, quads, info[1], dst, bnd, g, order[k], tiles_x,
+                       tiles_y, n_slabs, n_ranges, t_stride, acc);
   }
 }
 
@@ -2553,18 +2518,14 @@ A1SymPlan plan_a1_sym(int64_t X, int64_t Y, int64_t Z, int64_t N, const A1Plan& 
   return p;
 }
 
-// with_kq: room for the interleaved quad tables, which the single-request
-// xy-mirror path reads (the batched step reads the per-scan tables).
-size_t a1_workspace(int64_t X, int64_t Y, int64_t Z, int64_t N, int64_t T,
-                    bool with_kq = true) {
+size_t a1_workspace(int64_t X, int64_t Y, int64_t Z, int64_t N, int64_t T) {
   if (X <= 0 || Y <= 0 || Z < 2 || N <= 0 || T <= 0) return 0;
   // The size does not depend on the geometry, only on the safe flag's
   // table layout, which is the same in both forms.
   const A1Plan p = plan_a1(X, Y, Z, N, T, 1.f, 1.f);
   const A1SymPlan q = plan_a1_sym(X, Y, Z, N, p);
   const size_t gen = p.tables_bytes + p.partial_bytes;
-  const size_t sym = (!with_kq && q.quad.possible) ? q.kq_off : q.ws;
-  return q.possible && sym > gen ? sym : gen;
+  return q.possible && q.ws > gen ? q.ws : gen;
 }
 
 }  // namespace
