@@ -1787,6 +1787,18 @@ __device__ __forceinline__ void a1_quad_item(
         *d2o = d2;
         return r;
       };
+      // the same for the bin index held as a float (tf == (float)tt)
+      auto decide_bin_f = [&](float tf, float* d2o, float* alo) -> int {
+        float tc = tf * g.w;
+        tc = tc + g.off;
+        const float d2 = (tc * tc - dx2) - dy2;
+        bool nr;
+        int r = decide_fast(g, d2, cp, delta, half_m_delta, alo, &nr);
+        if (wave_any(nr))
+          if (nr) r = decide_exact(g, d2, wz, zshift, alo);
+        *d2o = d2;
+        return r;
+      };
       // The walk starts at the lane's first bin with d2 > 0 and row >= 0:
       // estimated from the real-valued inverse (depth of row 0's start), then
       // settled by exact decisions (backward while the previous bin also
@@ -1822,32 +1834,44 @@ __device__ __forceinline__ void a1_quad_item(
           fwd = (t <= thi) & !((d2 > 0.f) & (r >= 0));
         }
       }
-      int rr = uniform_i32(wave_min_i32(t <= thi ? r : 0x7fffffff));
+      // A lane with no bin left carries the row 0x7fffffff, so "still walking
+      // and on row rr" is the single test r <= rr.
+      if (!(t <= thi)) r = 0x7fffffff;
+      int rr = uniform_i32(wave_min_i32(r));
       float4 carry = zero4;
       // At the start of row rr every live lane's row is >= rr, so a lane
       // that is live with row <= rr is on row rr.
       // Table values of the lane's current bin, loaded one step ahead (right
       // after the previous bin used them), so a whole step covers their
       // latency.  The index stops at T - 1.
-      const uint32_t tmax4 = (uint32_t)(g.T - 1) * 4u;
-      uint32_t off = min((uint32_t)t * 4u, tmax4);
-      float4 kq = *(const float4*)(kqp + 4u * off);
-      float ic = SAFE ? load_at(itc4t, off) : 0.f;
+      // off16: byte offset of the bin in the lane's Kq row.  The bin index
+      // runs as a float (exact below 2^24), so tc = t w + off needs no
+      // conversion.
+      const uint32_t tmax16 = (uint32_t)(g.T - 1) * 16u;
+      uint32_t off16 = min((uint32_t)t * 16u, tmax16);
+      float4 kq = *(const float4*)(kqp + off16);
+      float ic = SAFE ? load_at(itc4t, off16 >> 2) : 0.f;
+      float tf = (float)t;
+      const float thif = (float)thi;
       // Per row: L = carry + sum of K d2^2, H = sum of K d2^2 alpha; the row's
       // voxel gets L - H (its (1 - alpha) parts plus the carry) and H is
       // carried into the next row.
       while (rr <= rspan) {
-        if (!wave_any(t <= thi)) break;
+        if (!wave_any(tf <= thif)) break;
         float4 L = carry, H = zero4;
         while (true) {
-          const bool act = (t <= thi) & (r <= rr);
+          const bool act = r <= rr;
           if (!wave_any(act)) break;
           if (act) {
-            // (past thi the lane stops; the decision is then unused)
-            float d2n, aln;
-            const int rn = decide_bin(t + 1, &d2n, &aln);
-            const float base = SAFE ? sq(d2 * ic) : d2 * d2;
-            const float wh = base * al;
+            // this bin's weights, then the next bin's decision in place (past
+            // thi the lane stops and the decision is unused)
+            float base = SAFE ? sq(d2 * ic) : d2 * d2;
+            float wh = base * al;
+            // keep both products ahead of the next decision, so that d2 and al
+            // are free for it (no register copies); emits no instruction
+            asm volatile("" : "+v"(base), "+v"(wh));
+            tf = tf + 1.f;
+            const int rn = decide_bin_f(tf, &d2, &al);
             L.x = fmaf(kq.x, base, L.x);
             L.y = fmaf(kq.y, base, L.y);
             L.z = fmaf(kq.z, base, L.z);
@@ -1856,13 +1880,10 @@ __device__ __forceinline__ void a1_quad_item(
             H.y = fmaf(kq.y, wh, H.y);
             H.z = fmaf(kq.z, wh, H.z);
             H.w = fmaf(kq.w, wh, H.w);
-            ++t;
-            off = min(off + 4u, tmax4);
-            kq = *(const float4*)(kqp + 4u * off);
-            if (SAFE) ic = load_at(itc4t, off);
-            r = rn;
-            al = aln;
-            d2 = d2n;
+            off16 = min(off16 + 16u, tmax16);
+            kq = *(const float4*)(kqp + off16);
+            if (SAFE) ic = load_at(itc4t, off16 >> 2);
+            r = tf <= thif ? rn : 0x7fffffff;
           }
         }
         float4* row = my + rr * 64;
