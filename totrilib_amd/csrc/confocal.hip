@@ -33,6 +33,8 @@
 #include <math.h>
 #include <string.h>
 
+#include <type_traits>
+
 #include "common.h"
 
 namespace totri_hip {
@@ -1761,8 +1763,11 @@ __device__ __forceinline__ void a1_quad_item(
     const float dxa = wx - x0, dxb = wx - x1, dya = wy - y0, dyb = wy - y1;
     float dlo, dhi;
     depth_range(g, wz, zlo, zhi, &dlo, &dhi);
-#pragma unroll 1
-    for (int f = 0; f < 4; ++f) {
+// Each family's walk is compiled separately (f a constant), so the
+    // permutations between the sums' order and the LDS rows' order cost no
+    // instructions.
+    auto walk_family = [&](auto fc) {
+      constexpr int f = decltype(fc)::value;
       const float dx2 = cvalid ? ((f & 1) ? dxb * dxb : dxa * dxa) : INFINITY;
       const float dy2 = (f & 2) ? dyb * dyb : dya * dya;
       int tlo, thi;
@@ -1771,7 +1776,7 @@ __device__ __forceinline__ void a1_quad_item(
         tlo = 0;
         thi = -1;
       }
-      if (!wave_any(tlo <= thi)) continue;
+      if (!wave_any(tlo <= thi)) return;
       // component n of the LDS rows takes scan s_(n^f); the sums' component m
       // takes s_m
       const char* kqp = (const char*)(Kq + (size_t)qi * t_stride);
@@ -1911,7 +1916,11 @@ __device__ __forceinline__ void a1_quad_item(
         a.w += carry.w;
         *row = a;
       }
-    }
+    };
+    walk_family(std::integral_constant<int, 0>{});
+    walk_family(std::integral_constant<int, 1>{});
+    walk_family(std::integral_constant<int, 2>{});
+    walk_family(std::integral_constant<int, 3>{});
   }
 
   if (cvalid) {
