@@ -1680,6 +1680,10 @@ __device__ __forceinline__ float load_at(const float* __restrict__ base, uint32_
   return *(const float*)((const char*)base + off);
 }
 
+// Row stride of the interleaved quad tables: bins 0 .. T-1 and a zero entry
+// at T, which a lane may read one step past its last bin (no clamp needed).
+__host__ __device__ __forceinline__ size_t kq_stride(int T) { return (size_t)T + 1; }
+
 // Component n of v ^ f: (v[0^f], v[1^f], v[2^f], v[3^f]) for a wave-uniform f.
 __device__ __forceinline__ float4 xor_perm4(float4 v, int f) {
   if (f & 1) v = make_float4(v.y, v.x, v.w, v.z);
@@ -1779,7 +1783,7 @@ __device__ __forceinline__ void a1_quad_item(
       if (!wave_any(tlo <= thi)) return;
       // component n of the LDS rows takes scan s_(n^f); the sums' component m
       // takes s_m
-      const char* kqp = (const char*)(Kq + (size_t)qi * t_stride);
+      const char* kqp = (const char*)(Kq + (size_t)qi * kq_stride(g.T));
       // decision of bin tt (unspecified for d2 <= 0; the walk never meets
       // such a bin).  tc^2 is recomputed with the tables' rounding.
       auto decide_bin = [&](int tt, float* d2o, float* alo) -> int {
@@ -1852,10 +1856,12 @@ __device__ __forceinline__ void a1_quad_item(
       // off16: byte offset of the bin in the lane's Kq row.  The bin index
       // runs as a float (exact below 2^24), so tc = t w + off needs no
       // conversion.
-      const uint32_t tmax16 = (uint32_t)(g.T - 1) * 16u;
-      uint32_t off16 = min((uint32_t)t * 16u, tmax16);
+      // The walk starts at t <= thi + 1 <= T and steps only from bins t <= thi,
+      // so every Kq read is at a bin <= T.  (itc4 has no entry T: clamped.)
+      const uint32_t tmax4 = (uint32_t)(g.T - 1) * 4u;
+      uint32_t off16 = (uint32_t)t * 16u;
       float4 kq = *(const float4*)(kqp + off16);
-      float ic = SAFE ? load_at(itc4t, off16 >> 2) : 0.f;
+      float ic = SAFE ? load_at(itc4t, min(off16 >> 2, tmax4)) : 0.f;
       float tf = (float)t;
       const float thif = (float)thi;
       // Per row: L = carry + sum of K d2^2, H = sum of K d2^2 alpha; the row's
@@ -1885,9 +1891,9 @@ __device__ __forceinline__ void a1_quad_item(
             H.y = fmaf(kq.y, wh, H.y);
             H.z = fmaf(kq.z, wh, H.z);
             H.w = fmaf(kq.w, wh, H.w);
-            off16 = min(off16 + 16u, tmax16);
+            off16 += 16u;
             kq = *(const float4*)(kqp + off16);
-            if (SAFE) ic = load_at(itc4t, off16 >> 2);
+            if (SAFE) ic = load_at(itc4t, min(off16 >> 2, tmax4));
             r = tf <= thif ? rn : 0x7fffffff;
           }
         }
@@ -1978,15 +1984,18 @@ __global__ __launch_bounds__(64) void a1_quad_kernel(
 // reads, interleaved.
 __global__ __launch_bounds__(256) void a1_quad_table_kernel(
     const float* __restrict__ Kf, const int* __restrict__ quads, const int* __restrict__ info,
-    int t_stride, int max_quads, float4* __restrict__ Kq) {
+    int t_stride, int T, int max_quads, float4* __restrict__ Kq) {
   if (!info[0]) return;
-  const size_t n = (size_t)min(info[1], max_quads) * t_stride;
+  const size_t ks = kq_stride(T);
+  const size_t n = (size_t)min(info[1], max_quads) * ks;
   for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
        e += (size_t)gridDim.x * blockDim.x) {
-    const size_t qi = e / t_stride, t = e - qi * t_stride;
+    const size_t qi = e / ks, t = e - qi * ks;
     const int* qs = quads + 4 * qi;
-    Kq[e] = make_float4(Kf[(size_t)qs[0] * t_stride + t], Kf[(size_t)qs[1] * t_stride + t],
-                        Kf[(size_t)qs[2] * t_stride + t], Kf[(size_t)qs[3] * t_stride + t]);
+    Kq[e] = t < (size_t)T
+                ? make_float4(Kf[(size_t)qs[0] * t_stride + t], Kf[(size_t)qs[1] * t_stride + t],
+                              Kf[(size_t)qs[2] * t_stride + t], Kf[(size_t)qs[3] * t_stride + t])
+                : make_float4(0.f, 0.f, 0.f, 0.f);
   }
 }
 
@@ -2505,7 +2514,7 @@ struct A1SymPlan {
   size_t cost_off, order_off;  // xy-mirror item costs (float) and dispatch order (int)
   size_t counter_off;          // xy-mirror work-queue head
   size_t bnd_off;              // xy-mirror boundary rows [R][X][Y][n_slabs]
-  size_t kq_off;               // xy-mirror interleaved quad tables [N/4][t_stride] float4
+  size_t kq_off;               // xy-mirror interleaved quad tables [N/4][T+1] float4
   size_t ws;
 };
 
@@ -2543,7 +2552,9 @@ A1SymPlan plan_a1_sym(int64_t X, int64_t Y, int64_t Z, int64_t N, const A1Plan& 
     p.bnd_off = p.counter_off + 256;
     p.kq_off = p.bnd_off + (((size_t)p.quad.n_ranges * X * Y * p.quad.n_slabs * sizeof(float) +
                              255) & ~(size_t)255);
-    p.ws = p.kq_off + (((size_t)(N / 4) * gen.t_stride * sizeof(float4) + 255) & ~(size_t)255);
+    // kq_stride(T) = T + 1 <= t_stride + 1
+    p.ws = p.kq_off + (((size_t)(N / 4) * (gen.t_stride + 1) * sizeof(float4) + 255) &
+                       ~(size_t)255);
   }
   return p;
 }
@@ -2635,8 +2646,8 @@ extern "C" int totri_volume_confocal_grad_volume(
       int* counter = (int*)((char*)workspace + q.counter_off);
       float4* Kq = (float4*)((char*)workspace + q.kq_off);
       hipLaunchKernelGGL(a1_quad_table_kernel,
-                         dim3(launch_grid((size_t)(N / 4) * p.t_stride)), dim3(256), 0, st, Kf,
-                         quads, info, p.t_stride, (int)(N / 4), Kq);
+                         dim3(launch_grid((size_t)(N / 4) * kq_stride((int)T))), dim3(256), 0, st, Kf,
+                         quads, info, p.t_stride, (int)T, (int)(N / 4), Kq);
       const A1QuadPlan& qd = q.quad;
       const int items = qd.tiles_x * qd.tiles_y * qd.n_slabs * qd.n_ranges;
       hipLaunchKernelGGL(a1_quad_cost_kernel, dim3(items), dim3(64), 0, st, measurement_points,
