@@ -269,7 +269,13 @@ def confocal_main(args):
         "flop_model": fm,
         "roofline": {"bound": "valu", "kernel": dom, "achieved": achieved / 1e12,
                      "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32, "traffic": traffic},
+                     "frac": achieved / PEAK_FP32, "traffic": traffic,
+                     "note": "achieved counts the reference formulation's FLOP (18 per "
+                             "active R1 sample, 17 per A1 window bin + 23 per pair); the "
+                             "mirror kernels share one decision among four samples and "
+                             "execute about a quarter of that (rocprof VALU counts in "
+                             "profiles/confocal_128_64x64_1024_valu.md), so frac can "
+                             "exceed 1 without the VALU issuing above its peak"},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))
