@@ -348,6 +348,39 @@ def test_r1_mirror_quads_and_general_path(scan):
     assert np.count_nonzero(got) > 0
 
 
+@pytest.mark.parametrize("layout", ["inside", "two_walls"])
+def test_r1_mirror_wall_inside_the_box(layout):
+    """The mirror R1 takes a shorter activity test in workgroups whose wall
+    points all lie below the box (a sample with d2 <= 0 then decides a
+    negative row).  A wall inside the box's depth range, alone or next to a
+    wall below it (so that workgroups take both tests), matches the oracle.
+    With the wall inside the box, near-wall samples make d2 a difference of
+    nearly equal squares, and the reference's own FP32 arithmetic (the
+    oracle's FP32 mode) is about 2e-5 from the FP64 oracle: the check is
+    against the FP32 oracle, and the FP64 error may not exceed 1.5x the FP32 oracle's."""
+    X, Y, Z, T = 32, 30, 40, 700
+    vol = sparse_albedo((X, Y, Z), 0.2, 33)
+    mp = wall_grid(8)
+    origin = ORIGIN
+    if layout == "inside":
+        origin = (-1.0, -1.0, -0.5)
+    else:
+        upper = wall_grid(8)
+        upper[2] = 1.2
+        mp = np.ascontiguousarray(np.concatenate([mp, upper], axis=1))
+    w = 0.0045
+    info = mirror_info(mp)
+    assert info[0] == 1 and info[1] == mp.shape[1] // 4
+    got = gpu_r1(vol, mp, T, w, 0.0, origin=origin)
+    ref = O.r1_forward(vol, mp, T, origin, SIZE, w, 0.0, mode=O.PINNED64)
+    r32 = O.r1_forward(vol, mp, T, origin, SIZE, w, 0.0, mode=O.FP32)
+    e, e32, eref = rel_l2(got, r32), rel_l2(got, ref), rel_l2(r32, ref)
+    print(f"{layout}: rel-L2 vs FP32 oracle {e:.2e}, vs FP64 {e32:.2e} (FP32 oracle {eref:.2e})")
+    assert e < TOL
+    assert e32 <= max(TOL, 1.5 * eref)
+    assert np.count_nonzero(got) > 0
+
+
 @pytest.mark.parametrize("scan", ["quads", "quads_safe", "quads_negoff", "xpairs", "odd_y",
                                   "perturbed", "random"])
 def test_a1_mirror_pairs_and_general_path(scan):

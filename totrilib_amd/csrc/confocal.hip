@@ -561,11 +561,32 @@ __global__ __launch_bounds__(1024) void scan_quads_kernel(const int* __restrict_
   scan_quads_block(partner, N, quads, pairs, info);
 }
 
+// True when the wall points of all of the workgroup's quads lie below the
+// first voxel centre (c0 = (wz - oz) s - 1/2 < 0): a sample with d2 <= 0
+// then decides a negative row.  The same value in every thread.
+__device__ __forceinline__ bool quads_below_box(const float* __restrict__ mp,
+                                                const int* __restrict__ quads, int n_quads,
+                                                int qgrp, const ConfGeom& g) {
+  bool below = true;
+#pragma unroll
+  for (int w = 0; w < kWaves; ++w) {
+    const int q = qgrp * kWaves + w;
+    if (q < n_quads) {
+      const float wz = mp[2 * g.N + quads[4 * q]];
+      float c0 = (wz - g.oz) * g.s;
+      c0 = c0 - 0.5f;
+      below = below && (c0 < 0.f);
+    }
+  }
+  return __builtin_amdgcn_readfirstlane((int)below) != 0;
+}
+
 // One workgroup item of the symmetric R1: scan quads qgrp*8 .. qgrp*8+7 (one
 // per wave), bins t_base .. t_base + 64K - 1, column-quad group grp.  The
 // chunk buffers hold, per quadrant column, its four mirror columns:
 // lds2[((j * 4) + m) * Z + z] for column quad j and mirror m.
-template <int K>
+// BELOW (quads_below_box holds): the activity test is the row range alone.
+template <int K, bool BELOW = false>
 __device__ __forceinline__ void r1_sym_block(
     const float* __restrict__ vol, const float* __restrict__ mp,
     const int* __restrict__ quads, int n_quads, float* __restrict__ dst,
@@ -578,8 +599,9 @@ __device__ __forceinline__ void r1_sym_block(
   const int HX = g.X / 2, HY = g.Y / 2;
   const int n_reps = HX * HY;
   // A chunk holds reps_per_chunk column quads as [rep][z][m] pairs with
-  // z = 0 .. Z, where row z = Z is (0, 0): a sample's four mirror pairs sit
-  // side by side, and an inactive sample reads the zero row.
+  // z = 0 .. Z, where rows Z - 1 and Z are (0, 0) (no active sample reads
+  // row Z - 1: it needs z <= Z - 2): a sample's four mirror pairs sit side
+  // by side, and an inactive sample reads a zero row.
   const int zrows = g.Z + 1;
   const int rep_pairs = 4 * zrows;
   const int chunk_vox = reps_per_chunk * rep_pairs;  // pairs
@@ -627,7 +649,7 @@ __device__ __forceinline__ void r1_sym_block(
     float2 p[4];
 #pragma unroll
     for (int m = 0; m < 4; ++m) p[m] = make_float2(0.f, 0.f);
-    if (r < n_reps && z < g.Z) {
+    if (r < n_reps && z < g.Z - 1) {
       const int rx = r / HY, ry = r - rx * HY;
 #pragma unroll
       for (int m = 0; m < 4; ++m) {
@@ -635,7 +657,7 @@ __device__ __forceinline__ void r1_sym_block(
         const int yi = (m & 2) ? g.Y - 1 - ry : ry;
         const float* col = vol + ((size_t)xi * g.Y + yi) * g.Z;
         const float v0 = col[z];
-        const float v1 = (z == g.Z - 1) ? v0 : col[z + 1];
+        const float v1 = col[z + 1];
         p[m] = make_float2(v0, v1 - v0);
       }
     }
@@ -734,10 +756,21 @@ __device__ __forceinline__ void r1_sym_block(
               if (na) za = decide_exact(g, d2ca, wz, 0, &ala);
               if (nb) zb = decide_exact(g, d2cb, wz, 0, &alb);
             }
-            const bool acta = (d2a > 0.f) & ((unsigned)za <= (unsigned)(g.Z - 2));
-            const bool actb = (d2b > 0.f) & ((unsigned)zb <= (unsigned)(g.Z - 2));
-            const float2* pa = quad + 4 * (acta ? za : g.Z);
-            const float2* pb = quad + 4 * (actb ? zb : g.Z);
+            int ra, rb;
+            if (BELOW) {
+              // rows past Z - 2 and negative rows (among them every d2 <= 0
+              // sample, or, after the exact decision, a row of weight 0)
+              // read the zero row Z - 1
+              ra = (int)min((unsigned)za, (unsigned)(g.Z - 1));
+              rb = (int)min((unsigned)zb, (unsigned)(g.Z - 1));
+            } else {
+              const bool acta = (d2a > 0.f) & ((unsigned)za <= (unsigned)(g.Z - 2));
+              const bool actb = (d2b > 0.f) & ((unsigned)zb <= (unsigned)(g.Z - 2));
+              ra = acta ? za : g.Z;
+              rb = actb ? zb : g.Z;
+            }
+            const float2* pa = quad + 4 * ra;
+            const float2* pb = quad + 4 * rb;
             const float qa = d2ca * itc4[k], qb = d2cb * itc4[k];
             const float wa = qa * qa, wb = qb * qb;
 #pragma unroll
@@ -1047,9 +1080,14 @@ void r1_sym_kernel(
   // later bin passes first: they usually reach more of the volume (the first
   // bins end short of a box placed away from the wall), so the costlier
   // workgroups start in the first round
-  r1_sym_block<K>(vol, mp, quads, n_quads, dst, g, (int)(gridDim.y - 1 - blockIdx.y) * 64 * K,
-                  (int)(blockIdx.x % n_groups), qgrp, n_groups, reps_per_chunk,
-                  chunks_per_group, lds2);
+  const int t_base = (int)(gridDim.y - 1 - blockIdx.y) * 64 * K;
+  const int grp = (int)(blockIdx.x % n_groups);
+  if (quads_below_box(mp, quads, n_quads, qgrp, g))
+    r1_sym_block<K, true>(vol, mp, quads, n_quads, dst, g, t_base, grp, qgrp, n_groups,
+                          reps_per_chunk, chunks_per_group, lds2);
+  else
+    r1_sym_block<K, false>(vol, mp, quads, n_quads, dst, g, t_base, grp, qgrp, n_groups,
+                           reps_per_chunk, chunks_per_group, lds2);
 }
 
 // Symmetric A6, all bin passes in one launch (blockIdx.y counts them from the
