@@ -348,11 +348,12 @@ def test_r1_mirror_quads_and_general_path(scan):
     assert np.count_nonzero(got) > 0
 
 
+@pytest.mark.parametrize("scan", ["mirror", "random"])
 @pytest.mark.parametrize("layout", ["inside", "two_walls"])
-def test_r1_mirror_wall_inside_the_box(layout):
-    """The mirror R1 takes a shorter activity test in workgroups whose wall
-    points all lie below the box (a sample with d2 <= 0 then decides a
-    negative row).  A wall inside the box's depth range, alone or next to a
+def test_r1_mirror_wall_inside_the_box(layout, scan):
+    """The mirror and the general R1 take a shorter activity test in
+    workgroups whose wall points all lie below the box (a sample with
+    d2 <= 0 then decides a negative row).  A wall inside the box's depth range, alone or next to a
     wall below it (so that workgroups take both tests), matches the oracle.
     With the wall inside the box, near-wall samples make d2 a difference of
     nearly equal squares, and the reference's own FP32 arithmetic (the
@@ -360,22 +361,35 @@ def test_r1_mirror_wall_inside_the_box(layout):
     against the FP32 oracle, and the FP64 error may not exceed 1.5x the FP32 oracle's."""
     X, Y, Z, T = 32, 30, 40, 700
     vol = sparse_albedo((X, Y, Z), 0.2, 33)
-    mp = wall_grid(8)
+    rng = np.random.default_rng(34)
+
+    def wall(z):
+        if scan == "mirror":
+            m = wall_grid(8)
+        else:
+            m = np.stack([rng.uniform(-1, 1, 64), rng.uniform(-1, 1, 64),
+                          np.zeros(64)]).astype(np.float32)
+        m[2] = z
+        return m
+
     origin = ORIGIN
     if layout == "inside":
         origin = (-1.0, -1.0, -0.5)
+        mp = wall(0.0)
     else:
-        upper = wall_grid(8)
-        upper[2] = 1.2
-        mp = np.ascontiguousarray(np.concatenate([mp, upper], axis=1))
+        mp = np.ascontiguousarray(np.concatenate([wall(0.0), wall(1.2)], axis=1))
     w = 0.0045
     info = mirror_info(mp)
-    assert info[0] == 1 and info[1] == mp.shape[1] // 4
+    if scan == "mirror":
+        assert info[0] == 1 and info[1] == mp.shape[1] // 4
+    else:
+        assert info[0] == 0
     got = gpu_r1(vol, mp, T, w, 0.0, origin=origin)
     ref = O.r1_forward(vol, mp, T, origin, SIZE, w, 0.0, mode=O.PINNED64)
     r32 = O.r1_forward(vol, mp, T, origin, SIZE, w, 0.0, mode=O.FP32)
     e, e32, eref = rel_l2(got, r32), rel_l2(got, ref), rel_l2(r32, ref)
-    print(f"{layout}: rel-L2 vs FP32 oracle {e:.2e}, vs FP64 {e32:.2e} (FP32 oracle {eref:.2e})")
+    print(f"{layout}/{scan}: rel-L2 vs FP32 oracle {e:.2e}, vs FP64 {e32:.2e} "
+          f"(FP32 oracle {eref:.2e})")
     assert e < TOL
     assert e32 <= max(TOL, 1.5 * eref)
     assert np.count_nonzero(got) > 0

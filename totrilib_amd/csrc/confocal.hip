@@ -202,7 +202,7 @@ __device__ __forceinline__ float lane_value(float v, int lane) {
 // workgroup calls it (it synchronises the workgroup).  LDS holds two chunk
 // buffers of (v[z], v[z+1] - v[z]) pairs, so that one 8-byte read and one
 // FMA give the interpolated albedo.
-template <int K>
+template <int K, bool BELOW = false>
 __device__ __forceinline__ void r1_block(
     const float* __restrict__ vol, const float* __restrict__ mp,
     float* __restrict__ dst,  // [n_groups][N][T] partial sums, or [N][T]
@@ -223,7 +223,8 @@ __device__ __forceinline__ void r1_block(
     const float tc = bin_distance(t, g.w, g.off);
     const float q = tc * tc;
     tc2[k] = (t < g.T) ? q : -1.f;  // -1 makes d2 negative: lane inactive
-    itc4[k] = 1.f / (q * q);
+    // BELOW: finite, so that an inactive sample (d2 clamped to 0) has weight 0
+    itc4[k] = BELOW ? (q > 0.f ? fminf(1.f / (q * q), 3.4028235e38f) : 0.f) : 1.f / (q * q);
     acc[k] = 0.f;
     a[k] = 0.f;
   }
@@ -245,13 +246,15 @@ __device__ __forceinline__ void r1_block(
   const int nthreads = kWaves * 64;
   const int vol_vox = n_cols * g.Z;
 
-  // Pair (v[z], v[z+1] - v[z]) of global voxel index ge (z = Z-1 pairs with 0).
+  // Pair (v[z], v[z+1] - v[z]) of global voxel index ge; z = Z-1 holds (0, 0):
+  // no active sample reads it (that needs z <= Z-2), and BELOW reads it for
+  // inactive samples.
   auto load_pair = [&](int ge) -> float2 {
     if (ge >= vol_vox) return make_float2(0.f, 0.f);
-    const float v0 = vol[ge];
     const bool last = (ge % g.Z) == g.Z - 1;
-    const float v1 = last ? v0 : vol[ge + 1];
-    return make_float2(v0, v1 - v0);
+    if (last) return make_float2(0.f, 0.f);
+    const float v0 = vol[ge];
+    return make_float2(v0, vol[ge + 1] - v0);
   };
 
   if (chunk_first < chunk_end) {
@@ -336,6 +339,28 @@ __device__ __forceinline__ void r1_block(
           if (!((wmask >> k) & 1u)) continue;
           const float d2a = a[k] - cdy2a;  // volume_cuda.cu:40
           const float d2b = (same_x ? a[k] : tc2[k] - cdx2b) - cdy2b;
+          if (BELOW) {
+            // The wall lies below the box: a sample with d2 <= 0 (clamped to
+            // 0) decides a negative row, or after an exact decision a row of
+            // weight 0; rows outside 0 .. Z-2 read the zero row Z-1.
+            const float d2ca = fmaxf(d2a, 0.f), d2cb = fmaxf(d2b, 0.f);
+            float ala, alb;
+            bool na, nb;
+            int za = decide_fast(g, d2ca, c0, delta, half_m_delta, &ala, &na);
+            int zb = decide_fast(g, d2cb, c0, delta, half_m_delta, &alb, &nb);
+            if (__ballot(na || nb)) {
+              if (na) za = decide_exact(g, d2ca, wz, 0, &ala);
+              if (nb) zb = decide_exact(g, d2cb, wz, 0, &alb);
+            }
+            const float2 va = col1[min((unsigned)za, (unsigned)(g.Z - 1))];
+            const float2 vb = col2[min((unsigned)zb, (unsigned)(g.Z - 1))];
+            const float fa = fmaf(ala, va.y, va.x);
+            const float fb = fmaf(alb, vb.y, vb.x);
+            const float qa = d2ca * itc4[k];
+            const float qb = d2cb * itc4[k];
+            acc[k] = fmaf(fb, qb * qb, fmaf(fa, qa * qa, acc[k]));
+            continue;
+          }
           float ala, alb;
           bool na, nb;
           int za = decide_fast(g, d2a, c0, delta, half_m_delta, &ala, &na);
@@ -382,6 +407,41 @@ __device__ __forceinline__ void r1_block(
   }
 }
 
+// True when the wall points of all of the workgroup's scan points (sgrp * 8
+// .. sgrp * 8 + 7) lie below the first voxel centre, c0 = (wz - oz) s - 1/2
+// < 0.  The same value in every thread.
+__device__ __forceinline__ bool scans_below_box(const float* __restrict__ mp, int sgrp,
+                                                const ConfGeom& g) {
+  bool below = true;
+#pragma unroll
+  for (int w = 0; w < kWaves; ++w) {
+    const int i = sgrp * kWaves + w;
+    if (i < g.N) {
+      float c0 = (mp[2 * g.N + i] - g.oz) * g.s;
+      c0 = c0 - 0.5f;
+      below = below && (c0 < 0.f);
+    }
+  }
+  return __builtin_amdgcn_readfirstlane((int)below) != 0;
+}
+
+// General R1 item; workgroups whose wall points lie below the box take the
+// select-free activity test.
+template <int K>
+__device__ __forceinline__ void r1_item(const float* __restrict__ vol,
+                                        const float* __restrict__ mp,
+                                        float* __restrict__ dst, const ConfGeom& g,
+                                        int t_base, int grp, int sgrp, int n_groups,
+                                        int chunk_cols, int chunks_per_group,
+                                        float2* __restrict__ lds2) {
+  if (scans_below_box(mp, sgrp, g))
+    r1_block<K, true>(vol, mp, dst, g, t_base, grp, sgrp, n_groups, chunk_cols,
+                      chunks_per_group, lds2);
+  else
+    r1_block<K, false>(vol, mp, dst, g, t_base, grp, sgrp, n_groups, chunk_cols,
+                       chunks_per_group, lds2);
+}
+
 // `skip` (optional): a device flag; when it is set the launch does nothing
 // (the mirror-symmetric kernel below has done the work).
 template <int K>
@@ -391,8 +451,8 @@ __global__ __launch_bounds__(kWaves * 64) void r1_forward_kernel(
     int chunks_per_group, const int* __restrict__ skip) {
   if (skip && *skip) return;
   extern __shared__ float2 lds2[];
-  r1_block<K>(vol, mp, dst, g, t_base, (int)(blockIdx.x % n_groups),
-              (int)(blockIdx.x / n_groups), n_groups, chunk_cols, chunks_per_group, lds2);
+  r1_item<K>(vol, mp, dst, g, t_base, (int)(blockIdx.x % n_groups),
+             (int)(blockIdx.x / n_groups), n_groups, chunk_cols, chunks_per_group, lds2);
 }
 
 // ---------------------------------------------------------------------------
