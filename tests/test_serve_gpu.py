@@ -18,8 +18,9 @@ def dev():
 def make_requests(seed=0):
     """Renders and backprojections of mixed sizes that exercise every plan:
     column groups (few scan points), 1/2/4/8/16 bin windows and two bin
-    passes, A1 scan-point ranges and the tiny-radius (safe) A1 form, inputs
-    on the host (pinned and pageable) and resident on the device."""
+    passes, A1 scan-point ranges and the tiny-radius (safe) A1 form, mirror
+    and general scans, inputs on the host (pinned and pageable) and resident
+    on the device."""
     from totrilib_amd.serve import BackprojectRequest, RenderRequest
     rng = np.random.default_rng(seed)
     specs = [  # kind, scan side, (X, Y, Z), T, one-way bin width, offset, where
@@ -33,10 +34,21 @@ def make_requests(seed=0):
         ("r", 6, (20, 16, 24), 100, 0.03, 0.1, "device"),
         ("r", 10, (18, 18, 30), 200, 0.02, 0.0, "pinned"),
         ("b", 20, (40, 36, 33), 1500, 0.002, 0.0, "device"),
+        # scans without mirror symmetry: the general kernels (A1 row walk;
+        # A1 window walk for a negative offset; R1 below-the-box test)
+        ("b", 12, (24, 20, 32), 400, 0.006, 0.0, "device", "perturbed"),
+        ("b", 10, (20, 20, 24), 300, 0.009, -0.01, "pinned", "perturbed"),
+        ("r", 12, (24, 20, 32), 400, 0.006, 0.0, "device", "random"),
     ]
     reqs = []
-    for k, (kind, n, res, T, w, off, where) in enumerate(specs):
-        mp = torch.from_numpy(wall_grid(n))
+    for k, (kind, n, res, T, w, off, where, *scan) in enumerate(specs):
+        mp_np = wall_grid(n)
+        if scan == ["perturbed"]:
+            mp_np[0, 5] = np.nextafter(mp_np[0, 5], np.float32(2))
+        elif scan == ["random"]:
+            mp_np = np.stack([rng.uniform(-1, 1, n * n), rng.uniform(-1, 1, n * n),
+                              np.zeros(n * n)]).astype(np.float32)
+        mp = torch.from_numpy(mp_np)
         if kind == "r":
             data = torch.from_numpy(sparse_albedo(res, 0.05, 100 + k))
         else:

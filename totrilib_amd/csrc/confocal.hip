@@ -1201,19 +1201,24 @@ __device__ __forceinline__ float a1_boundary(const float* __restrict__ bnd, size
 
 // A1 partial sums: the xy-mirror kernel ran when quad_on and info[0] (its
 // boundary rows are added too), else the x-mirror kernel when info[2], else
-// the general kernel; each has its own partial count.
+// the general kernel (its boundary rows added when bnd_gen is given, the row
+// walk); each has its own partial count.  info may be null: no mirror path.
+// Both walks' boundary rows use n_slabs = ceil(Z / kSlabQ).
 __global__ void sum_partials_a1_kernel(const float* __restrict__ part,
                                        const float* __restrict__ bnd,
                                        const int* __restrict__ info, int quad_on, int n_quad,
                                        int n_pair, int n_gen, size_t n, int Z, int n_slabs,
+                                       const float* __restrict__ bnd_gen,
                                        float* __restrict__ out) {
-  const bool quad = quad_on && info[0];
-  const int n_parts = quad ? n_quad : info[2] ? n_pair : n_gen;
+  const bool quad = info && quad_on && info[0];
+  const bool pair = info && !quad && info[2];
+  const int n_parts = quad ? n_quad : pair ? n_pair : n_gen;
+  const float* b = quad ? bnd : pair ? nullptr : bnd_gen;
   for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
        e += (size_t)gridDim.x * blockDim.x) {
     float v = part[e];
     for (int p = 1; p < n_parts; ++p) v += part[(size_t)p * n + e];
-    if (quad) v += a1_boundary(bnd, e, n, Z, n_slabs, n_parts);
+    if (b) v += a1_boundary(b, e, n, Z, n_slabs, n_parts);
     out[e] = v;
   }
 }
@@ -1852,10 +1857,8 @@ __device__ __forceinline__ void a1_quad_item(
         acc[r * 64 + lane] = zero4;
       }
     }
+    // the quad's four points share wz; s0 gives (wx, wy) of family 0
     const int s0 = uniform_i32(quads[4 * qi]);
-    const int s1 = uniform_i32(quads[4 * qi + 1]);
-    const int s2 = uniform_i32(quads[4 * qi + 2]);
-    const int s3 = uniform_i32(quads[4 * qi + 3]);
     const float wx = mp[s0], wy = mp[g.N + s0], wz = mp[2 * g.N + s0];
     float c0 = (wz - g.oz) * g.s;
     c0 = c0 - 0.5f;
@@ -2075,6 +2078,199 @@ __global__ __launch_bounds__(64) void a1_quad_kernel(
     a1_quad_item<SAFE>(Kq, tc2t, itc4t, mp, quads, info[1], dst, bnd, g, order[k], tiles_x,
                        tiles_y, n_slabs, n_ranges, t_stride, acc);
   }
+}
+
+// ---------------------------------------------------------------------------
+// General A1, row-synchronous
+// ---------------------------------------------------------------------------
+// The walk of a1_quad_item for scans without mirror symmetry: one scan point
+// at a time, one column per lane (an 8x8 tile of the whole grid), a scalar
+// table row Kf[i][t] (one 4-byte load per step; the rows have at least T + 1
+// entries, zero past T-1, so the one-step-ahead load needs no clamp), and
+// per-row sums L and H in registers.  The same conditions as the mirror walk
+// (w > 0, off >= 0, a positive z size: rows never decrease with the bin) and
+// the same boundary rows.  It replaces the window-synchronous a1_item, whose
+// lanes share one bin window and update LDS once per sample.
+template <bool SAFE>
+__device__ __forceinline__ void a1_row_item(
+    const float* __restrict__ Kf, const float* __restrict__ itc4t,
+    const float* __restrict__ mp,
+    float* __restrict__ dst,  // [R][X][Y][Z] partial volumes
+    float* __restrict__ bnd,  // [R][X][Y][n_slabs] boundary rows
+    const ConfGeom& g, int item, int tiles_x, int tiles_y, int n_slabs, int n_ranges,
+    int t_stride, float* __restrict__ acc) {
+  const int lane = threadIdx.x & 63;
+  const int tx = item % tiles_x;
+  item /= tiles_x;
+  const int ty = item % tiles_y;
+  item /= tiles_y;
+  const int slab = item % n_slabs;
+  const int range = item / n_slabs;
+
+  const int xi = tx * 8 + (lane >> 3);
+  const int yi = ty * 8 + (lane & 7);
+  const bool cvalid = xi < g.X && yi < g.Y;
+  const float x = cell_centre(xi, g.ox, g.szx, g.X);
+  const float y = cell_centre(yi, g.oy, g.szy, g.Y);
+
+  // Slab rows as in a1_quad_item: sample rows z0 .. z0 + rspan, LDS row k for
+  // voxel z0 + k, row kSlabQ for the next slab's first voxel (boundary).
+  const int z0 = slab * kSlabQ;
+  const int rspan = min(kSlabQ, (g.Z - 1) - z0) - 1;
+  const int zshift = z0;
+  const float zlo = (float)z0;
+  const float zhi = (float)min(z0 + kSlabQ, g.Z - 1);
+
+  for (int r = 0; r <= kSlabQ; ++r) acc[r * 64 + lane] = 0.f;
+  float tot[kSlabQ + 1];
+#pragma unroll
+  for (int r = 0; r <= kSlabQ; ++r) tot[r] = 0.f;
+
+  const int per = (g.N + n_ranges - 1) / n_ranges;
+  const int i_begin = range * per;
+  const int i_end = min(g.N, i_begin + per);
+  float* my = acc + lane;
+
+  for (int i = i_begin; i < i_end; ++i) {
+    if (i > i_begin && (i - i_begin) % kFlush == 0) {
+#pragma unroll
+      for (int r = 0; r <= kSlabQ; ++r) {
+        tot[r] += acc[r * 64 + lane];
+        acc[r * 64 + lane] = 0.f;
+      }
+    }
+    const float wx = mp[i], wy = mp[g.N + i], wz = mp[2 * g.N + i];
+    float c0 = (wz - g.oz) * g.s;
+    c0 = c0 - 0.5f;
+    const float cp = c0 - (float)zshift;
+    const float delta = guard_delta(g, wz, c0, cp);
+    const float half_m_delta = 0.5f - delta;
+    const float dx = wx - x, dy = wy - y;
+    const float dx2 = cvalid ? dx * dx : INFINITY, dy2 = dy * dy;
+    float dlo, dhi;
+    depth_range(g, wz, zlo, zhi, &dlo, &dhi);
+    int tlo, thi;
+    bin_range(g, dx2 + dy2, dlo, dhi, &tlo, &thi);
+    if (!cvalid || rspan < 0 || tlo > thi) {
+      tlo = 0;
+      thi = -1;
+    }
+    if (!wave_any(tlo <= thi)) continue;
+    const float* kp = Kf + (size_t)i * t_stride;
+    auto decide_bin = [&](int tt, float* d2o, float* alo) -> int {
+      const float tc = bin_distance(tt, g.w, g.off);
+      const float d2 = (tc * tc - dx2) - dy2;
+      bool nr;
+      int r = decide_fast(g, d2, cp, delta, half_m_delta, alo, &nr);
+      if (wave_any(nr))
+        if (nr) r = decide_exact(g, d2, wz, zshift, alo);
+      *d2o = d2;
+      return r;
+    };
+    auto decide_bin_f = [&](float tf, float* d2o, float* alo) -> int {
+      float tc = tf * g.w;
+      tc = tc + g.off;
+      const float d2 = (tc * tc - dx2) - dy2;
+      bool nr;
+      int r = decide_fast(g, d2, cp, delta, half_m_delta, alo, &nr);
+      if (wave_any(nr))
+        if (nr) r = decide_exact(g, d2, wz, zshift, alo);
+      *d2o = d2;
+      return r;
+    };
+    // Exact walk start (see a1_quad_item).
+    const float rho2 = dx2 + dy2;
+    const float depth0 = fmaxf(((float)zshift + 0.5f) / g.s + g.oz - wz, 0.f);
+    const float te = (sqrt_fast(rho2 + depth0 * depth0) - g.off) * __builtin_amdgcn_rcpf(g.w);
+    int t = tlo;
+    if (te > (float)tlo + 1.f) t = te < (float)thi ? (int)te - 1 : thi;
+    float d2, al;
+    int r = decide_bin(t, &d2, &al);
+    bool back = (t <= thi) & (t > tlo) & (d2 > 0.f) & (r >= 0);
+    while (wave_any(back)) {
+      if (back) {
+        float d2p, alp;
+        const int rp = decide_bin(t - 1, &d2p, &alp);
+        back = (d2p > 0.f) & (rp >= 0);
+        if (back) {
+          --t;
+          r = rp;
+          al = alp;
+          d2 = d2p;
+          back = t > tlo;
+        }
+      }
+    }
+    bool fwd = (t <= thi) & !((d2 > 0.f) & (r >= 0));
+    while (wave_any(fwd)) {
+      if (fwd) {
+        ++t;
+        r = decide_bin(t, &d2, &al);
+        fwd = (t <= thi) & !((d2 > 0.f) & (r >= 0));
+      }
+    }
+    if (!(t <= thi)) r = 0x7fffffff;
+    int rr = uniform_i32(wave_min_i32(r));
+    float carry = 0.f;
+    // t <= thi + 1 <= T < t_stride: every table read is in the row
+    uint32_t off4 = (uint32_t)t * 4u;
+    float k = load_at(kp, off4);
+    float ic = SAFE ? load_at(itc4t, off4) : 0.f;
+    float tf = (float)t;
+    const float thif = (float)thi;
+    while (rr <= rspan) {
+      if (!wave_any(tf <= thif)) break;
+      float L = carry, H = 0.f;
+      while (true) {
+        const bool act = r <= rr;
+        if (!wave_any(act)) break;
+        if (act) {
+          float base = SAFE ? sq(d2 * ic) : d2 * d2;
+          float wh = base * al;
+          asm volatile("" : "+v"(base), "+v"(wh));  // see a1_quad_item
+          tf = tf + 1.f;
+          const int rn = decide_bin_f(tf, &d2, &al);
+          L = fmaf(k, base, L);
+          H = fmaf(k, wh, H);
+          off4 += 4u;
+          k = load_at(kp, off4);
+          if (SAFE) ic = load_at(itc4t, off4);
+          r = tf <= thif ? rn : 0x7fffffff;
+        }
+      }
+      float* row = my + rr * 64;
+      *row += L - H;
+      carry = H;
+      ++rr;
+    }
+    if (rr <= rspan + 1) my[rr * 64] += carry;
+  }
+
+  if (cvalid) {
+    float* out = dst + (((size_t)range * g.X + xi) * g.Y + yi) * g.Z;
+#pragma unroll
+    for (int r = 0; r < kSlabQ; ++r) {
+      const int z = z0 + r;
+      if (z < g.Z) out[z] = tot[r] + acc[r * 64 + lane];
+    }
+    if (z0 + kSlabQ < g.Z)
+      bnd[(((size_t)range * g.X + xi) * g.Y + yi) * n_slabs + slab] =
+          tot[kSlabQ] + acc[kSlabQ * 64 + lane];
+  }
+}
+
+// One single-wave workgroup per item.  `skip` (optional): a device flag; when
+// it is set the launch does nothing (a mirror kernel has done the work).
+template <bool SAFE>
+__global__ __launch_bounds__(64) void a1_row_kernel(
+    const float* __restrict__ Kf, const float* __restrict__ itc4t,
+    const float* __restrict__ mp, float* __restrict__ dst, float* __restrict__ bnd,
+    ConfGeom g, int tiles_x, int tiles_y, int n_slabs, int n_ranges, int t_stride,
+    const int* __restrict__ skip) {
+  if (skip && *skip) return;
+  __shared__ float acc[(kSlabQ + 1) * 64];
+  a1_row_item<SAFE>(Kf, itc4t, mp, dst, bnd, g, (int)blockIdx.x, tiles_x, tiles_y, n_slabs,
+                    n_ranges, t_stride, acc);
 }
 
 // Kq[q][t] = (K[s0][t], K[s1][t], K[s2][t], K[s3][t]) for the device-found
@@ -2512,8 +2708,11 @@ namespace {
 
 struct A1Plan {
   int tiles_x, tiles_y, n_slabs, n_ranges, t_stride;
+  // general row walk (a1_row_kernel): 8x8 column tiles, kSlabQ-voxel slabs
+  int row_tiles_x, row_tiles_y, row_slabs, row_ranges;
   bool safe;
   size_t tables_bytes, partial_bytes;
+  size_t row_bnd_bytes;  // boundary rows [row_ranges][X][Y][row_slabs]
 };
 
 A1Plan plan_a1(int64_t X, int64_t Y, int64_t Z, int64_t N, int64_t T, float w,
@@ -2550,11 +2749,35 @@ A1Plan plan_a1(int64_t X, int64_t Y, int64_t Z, int64_t N, int64_t T, float w,
   // K = G / tc^8 stays finite for |G| < 3e38 * tc^8: with tc >= 1e-3 m that
   // is |G| < 3e14, far above any transient gradient.
   p.safe = !(tmin >= 1e-3f);
-  p.t_stride = (int)((T + 63) & ~(int64_t)63);
+  // at least T + 1 entries (zero past T): the row walks read one bin ahead
+  p.t_stride = (int)((T + 64) & ~(int64_t)63);
   p.tables_bytes = ((size_t)2 * p.t_stride + (size_t)N * p.t_stride) * sizeof(float);
   p.tables_bytes = (p.tables_bytes + 255) & ~(size_t)255;
-  p.partial_bytes = p.n_ranges > 1 ? (size_t)p.n_ranges * X * Y * Z * sizeof(float) : 0;
+  p.row_tiles_x = (int)((X + 7) / 8);
+  p.row_tiles_y = (int)((Y + 7) / 8);
+  p.row_slabs = (int)((Z + kSlabQ - 1) / kSlabQ);
+  {
+    const int64_t items = (int64_t)p.row_tiles_x * p.row_tiles_y * p.row_slabs;
+    const int64_t target = 64 * (int64_t)device_cu_count();
+    int64_t R = 1;
+    if (items < target) R = (target + items - 1) / items;
+    if (R > N / 4) R = N / 4;
+    if (R < 1) R = 1;
+    p.row_ranges = (int)R;
+  }
+  // the row walk always writes partial volumes (its boundary rows are added
+  // in the partial-sum pass)
+  const int64_t parts = p.n_ranges > p.row_ranges ? p.n_ranges : p.row_ranges;
+  p.partial_bytes = (size_t)parts * X * Y * Z * sizeof(float);
+  p.row_bnd_bytes = (((size_t)p.row_ranges * X * Y * p.row_slabs * sizeof(float)) + 255) &
+                    ~(size_t)255;
   return p;
+}
+
+// The row walks need rows that never decrease with the bin index.
+bool a1_row_geometry(const float* size, float w, float off) {
+  return w > 0.f && isfinite(w) && off >= 0.f && isfinite(off) && size[2] > 0.f &&
+         isfinite(size[2]);
 }
 
 }  // namespace
@@ -2633,6 +2856,7 @@ A1SymPlan plan_a1_sym(int64_t X, int64_t Y, int64_t Z, int64_t N, const A1Plan& 
   p.n_ranges = (int)R;
   p.quad = plan_a1_quad(X, Y, Z, N);
   int parts = p.n_ranges > gen.n_ranges ? p.n_ranges : gen.n_ranges;
+  if (gen.row_ranges > parts) parts = gen.row_ranges;
   if (p.quad.possible && p.quad.n_ranges > parts) parts = p.quad.n_ranges;
   // the xy-mirror path always writes partial volumes (its boundary rows are
   // added in the partial-sum pass)
@@ -2657,14 +2881,20 @@ A1SymPlan plan_a1_sym(int64_t X, int64_t Y, int64_t Z, int64_t N, const A1Plan& 
   return p;
 }
 
+// Offset of the general row walk's boundary rows: after everything else.
+size_t a1_row_bnd_off(const A1Plan& p, const A1SymPlan& q) {
+  const size_t gen = p.tables_bytes + p.partial_bytes;
+  const size_t end = q.possible && q.ws > gen ? q.ws : gen;
+  return (end + 255) & ~(size_t)255;
+}
+
 size_t a1_workspace(int64_t X, int64_t Y, int64_t Z, int64_t N, int64_t T) {
   if (X <= 0 || Y <= 0 || Z < 2 || N <= 0 || T <= 0) return 0;
   // The size does not depend on the geometry, only on the safe flag's
   // table layout, which is the same in both forms.
   const A1Plan p = plan_a1(X, Y, Z, N, T, 1.f, 1.f);
   const A1SymPlan q = plan_a1_sym(X, Y, Z, N, p);
-  const size_t gen = p.tables_bytes + p.partial_bytes;
-  return q.possible && q.ws > gen ? q.ws : gen;
+  return a1_row_bnd_off(p, q) + p.row_bnd_bytes;
 }
 
 }  // namespace
@@ -2729,7 +2959,10 @@ extern "C" int totri_volume_confocal_grad_volume(
   rc = check_launch("a1_tables_kernel");
   if (rc) return rc;
   const int* info = nullptr;
-  const bool staged = p.n_ranges > 1 || (sym && q.n_ranges > 1) || quad;
+  // the general path takes the row walk when rows are monotone
+  const bool row = a1_row_geometry(volume_size, bin_width, bin_offset);
+  float* row_bnd = (float*)((char*)workspace + a1_row_bnd_off(p, q));
+  const bool staged = p.n_ranges > 1 || (sym && q.n_ranges > 1) || quad || row;
   float* dst = staged ? partial : volume;
   if (sym) {
     rc = launch_mirror_detect(q.ml, measurement_points, (int)N, workspace, st);
@@ -2782,25 +3015,39 @@ extern "C" int totri_volume_confocal_grad_volume(
     if (rc) return rc;
   }
   const int* skip = info ? info + 2 : nullptr;
-  const int items = p.tiles_x * p.tiles_y * p.n_slabs * p.n_ranges;
-  const int blocks = (items + kA1Waves - 1) / kA1Waves;
-  if (p.safe)
-    hipLaunchKernelGGL(a1_transpose_kernel<true>, dim3(blocks), dim3(kA1Waves * 64), 0, st,
-                       Kf, tc2, itc4, measurement_points, dst, g, p.tiles_x, p.tiles_y,
-                       p.n_slabs, p.n_ranges, p.t_stride, items, skip);
-  else
-    hipLaunchKernelGGL(a1_transpose_kernel<false>, dim3(blocks), dim3(kA1Waves * 64), 0,
-                       st, Kf, tc2, itc4, measurement_points, dst, g, p.tiles_x,
-                       p.tiles_y, p.n_slabs, p.n_ranges, p.t_stride, items, skip);
-  rc = check_launch("a1_transpose_kernel");
+  if (row) {
+    const int items = p.row_tiles_x * p.row_tiles_y * p.row_slabs * p.row_ranges;
+    if (p.safe)
+      hipLaunchKernelGGL(a1_row_kernel<true>, dim3(items), dim3(64), 0, st, Kf, itc4,
+                         measurement_points, dst, row_bnd, g, p.row_tiles_x, p.row_tiles_y,
+                         p.row_slabs, p.row_ranges, p.t_stride, skip);
+    else
+      hipLaunchKernelGGL(a1_row_kernel<false>, dim3(items), dim3(64), 0, st, Kf, itc4,
+                         measurement_points, dst, row_bnd, g, p.row_tiles_x, p.row_tiles_y,
+                         p.row_slabs, p.row_ranges, p.t_stride, skip);
+    rc = check_launch("a1_row_kernel");
+  } else {
+    const int items = p.tiles_x * p.tiles_y * p.n_slabs * p.n_ranges;
+    const int blocks = (items + kA1Waves - 1) / kA1Waves;
+    if (p.safe)
+      hipLaunchKernelGGL(a1_transpose_kernel<true>, dim3(blocks), dim3(kA1Waves * 64), 0, st,
+                         Kf, tc2, itc4, measurement_points, dst, g, p.tiles_x, p.tiles_y,
+                         p.n_slabs, p.n_ranges, p.t_stride, items, skip);
+    else
+      hipLaunchKernelGGL(a1_transpose_kernel<false>, dim3(blocks), dim3(kA1Waves * 64), 0,
+                         st, Kf, tc2, itc4, measurement_points, dst, g, p.tiles_x,
+                         p.tiles_y, p.n_slabs, p.n_ranges, p.t_stride, items, skip);
+    rc = check_launch("a1_transpose_kernel");
+  }
   if (rc) return rc;
   if (!staged) return TOTRI_OK;
-  if (sym) {
-    // info[0] and info[2] say which path ran
+  if (sym || row) {
+    // info[0] and info[2] say which path ran (no info: the general one)
     hipLaunchKernelGGL(sum_partials_a1_kernel, dim3(launch_grid(nv)), dim3(256), 0, st,
-                       partial, (const float*)((char*)workspace + q.bnd_off), info, (int)quad,
-                       q.quad.n_ranges, q.n_ranges, p.n_ranges, nv, (int)Z, q.quad.n_slabs,
-                       volume);
+                       partial, sym ? (const float*)((char*)workspace + q.bnd_off) : nullptr,
+                       info, (int)quad, q.quad.n_ranges, q.n_ranges,
+                       row ? p.row_ranges : p.n_ranges, nv, (int)Z, p.row_slabs,
+                       row ? row_bnd : nullptr, volume);
     return check_launch("sum_partials_a1_kernel");
   }
   hipLaunchKernelGGL(sum_partials_kernel<float>, dim3(launch_grid(nv)), dim3(256), 0, st,
