@@ -301,6 +301,36 @@ def test_a6_mirror_quads_and_general_path(scan):
     assert e64 < TOL
 
 
+@pytest.mark.parametrize("layout", ["inside", "two_walls"])
+def test_a6_general_wall_inside_the_box(layout):
+    """The general A6 with a wall inside the box's depth range, alone or next
+    to a wall below it.  Checked against the oracle's FP32 mode: its FP64
+    mode (decisions pinned from FP32, weights in FP64) is not defined here,
+    since a near-wall sample can have d2 > 0 in FP32 and d2 <= 0 in FP64,
+    and its 1/depth term is then NaN."""
+    rng = np.random.default_rng(24)
+    X, Y, Z, T = 20, 18, 24, 700
+    vol = rng.random((X, Y, Z)).astype(np.float32)
+    mp = wall_grid(6)
+    mp[0, 7] = np.nextafter(mp[0, 7], np.float32(2))   # general path
+    origin = ORIGIN
+    if layout == "inside":
+        origin = (-1.0, -1.0, -0.5)
+    else:
+        upper = mp.copy()
+        upper[2] = 1.2
+        mp = np.ascontiguousarray(np.concatenate([mp, upper], axis=1))
+    w, off = 0.004, 0.001
+    G = rng.standard_normal((mp.shape[1], T)).astype(np.float32)
+    assert mirror_info(mp)[0] == 0
+    got = gpu_a6(vol, mp, G, w, off, origin=origin)
+    ref32 = O.a6(vol, mp, G, origin, SIZE, w, off, mode=O.FP32)
+    e = rel_l2(got, ref32)
+    print(f"{layout}: A6 rel-L2 vs FP32 oracle {e:.2e}")
+    assert np.all(np.isfinite(got))
+    assert e < TOL
+
+
 def test_autograd_volume_and_points():
     """VolumeRenderConfocal.apply: forward R1, backward A1 (albedo) and A6
     (wall points), as the reference's autograd Function."""
