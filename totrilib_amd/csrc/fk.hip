@@ -226,7 +226,7 @@ __global__ __launch_bounds__(kFkBlock) void fk_magnitude_kernel(
 // ---------------------------------------------------------------------------
 constexpr int kFkMaxPlane = 16384;    // H * W of a slice held in LDS
 constexpr int kFkMaxDepth2 = 4096;    // 2D of a column held in LDS
-constexpr int kFkColumnLds = 17408;   // float2 of columns per depth workgroup (136 KiB)
+constexpr int kFkColumnLds = 8704;    // float2 of columns per depth workgroup (68 KiB: two per CU)
 constexpr int kFkSliceThreads = 512;
 constexpr int kFkDepthThreads = 1024;
 
