@@ -595,6 +595,31 @@ def kernels_main(args):
     row("f-k Stolt interpolation", "fk_stolt_kernel",
         "one-sided spectrum 2048x64x33 -> 2048x64x64, z > 0 half written", ms, npts // 2,
         "output points", 80, 16 * Df * Hf * Wf)
+    # The large solve at one GPU's share: 256^3 volume, 2048 bins of 0.0025 m round trip,
+    # 8192 of the 256x256 scan's points (rows 0-15 and 240-255, so the share keeps its
+    # mirror quads, as a scan-point sharding would), FP32 / BF16 / FP16 transient storage
+    from totrilib_amd.totri_cpp.render import (volume_confocal_forward,
+                                               volume_confocal_grad_volume)
+    full = wall_grid(256)
+    rows_kept = list(range(16)) + list(range(240, 256))
+    idx = np.concatenate([np.arange(r * 256, (r + 1) * 256) for r in rows_kept])
+    mp4_np = np.ascontiguousarray(full[:, idx])
+    mp4 = torch.from_numpy(mp4_np).to(dev)
+    vol4 = torch.from_numpy(sparse_albedo((256, 256, 256), 0.01, 5)).to(dev)
+    T4, w4, N4 = 2048, 0.00125, mp4_np.shape[1]
+    pairs4 = 256 ** 3 * N4
+    fm4 = flop_model(mp4_np[:, ::16], 256, T4, w4)    # per-pair averages from 512 points
+    gvol = torch.empty((256, 256, 256), device=dev)
+    for dt, name in ((torch.float32, "FP32"), (torch.bfloat16, "BF16"), (torch.float16, "FP16")):
+        tr4 = torch.empty((N4, T4), dtype=dt, device=dev)
+        G4 = torch.randn((N4, T4), device=dev, generator=gen).to(dt)
+        ms_r = timeit(lambda: volume_confocal_forward(vol4, mp4, tr4, ORIGIN, SIZE, w4, 0.0))
+        ms_a = timeit(lambda: volume_confocal_grad_volume(gvol, mp4, G4, ORIGIN, SIZE, w4, 0.0))
+        esz = 4 if dt == torch.float32 else 2
+        row(f"Large solve share, R1 + A1, {name} transient", "r1_sym_kernel, a1_quad_kernel",
+            "256^3 voxels, 8192 of 256x256 scan points, 2048 bins", ms_r + ms_a, pairs4,
+            "voxel-scan pairs (render + backprojection)", fm4["f_pair"],
+            2 * (4 * 256 ** 3 + esz * N4 * T4))
     line = {"metric": "per-operation time and algorithmic rate", "value": None, "unit": "ms",
             "n_gpus": 1, "steps": reps, "warmup": 1, "higher_is_better": False,
             "dtype": "f32", "data": "synthetic (seeded)", "rows": rows,
