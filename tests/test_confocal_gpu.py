@@ -251,6 +251,36 @@ def test_degenerate_shapes():
     assert torch.count_nonzero(vol) == 0  # no scan points: zero gradient
 
 
+@pytest.mark.parametrize("shape,n_scan,scan", [
+    ((5, 7, 2), 9, "random"),      # Z = 2: one slab, no boundary row
+    ((9, 6, 3), 1, "random"),      # one scan point
+    ((7, 9, 9), 5, "random"),      # Z = 9: a one-row top slab takes the boundary row
+    ((6, 8, 17), 16, "random"),    # odd slab count
+    ((10, 8, 17), 16, "quads"),    # the xy-mirror walk with Z = 17
+    ((12, 10, 8), 64, "quads"),    # Z a multiple of the slab height: no boundary rows
+])
+def test_a1_row_walks_edge_shapes(shape, n_scan, scan):
+    """The row walks' slabs (8 voxel rows), boundary rows and scan ranges on
+    small, odd and slab-sized z extents, few scan points, against the
+    oracle's exact transpose."""
+    rng = np.random.default_rng(sum(shape) + n_scan)
+    X, Y, Z = shape
+    T, w = 300, 0.01
+    if scan == "quads":
+        mp = wall_grid(int(round(n_scan ** 0.5)))
+    else:
+        mp = np.stack([rng.uniform(-1, 1, n_scan), rng.uniform(-1, 1, n_scan),
+                       np.zeros(n_scan)]).astype(np.float32)
+    info = mirror_info(mp)
+    assert info[0] == (1 if scan == "quads" else 0)
+    G = rng.standard_normal((mp.shape[1], T)).astype(np.float32)
+    got = gpu_a1(G, mp, shape, w, 0.0)
+    ref = O.a1_transpose(G, mp, X, Y, Z, ORIGIN, SIZE, w, 0.0, mode=O.PINNED64)
+    e = rel_l2(got, ref)
+    print(f"{shape} n={n_scan} {scan}: rel-L2 vs FP64 pinned {e:.2e}")
+    assert np.all(np.isfinite(got)) and e < TOL
+
+
 def gpu_a6(vol, mp, G, w, off, origin=ORIGIN, size=SIZE):
     from totrilib_amd.totri_cpp.render import volume_confocal_grad_measurement_points
     v = torch.from_numpy(np.ascontiguousarray(vol)).to(dev())

@@ -1951,18 +1951,17 @@ __device__ __forceinline__ void a1_quad_item(
       float4 carry = zero4;
       // At the start of row rr every live lane's row is >= rr, so a lane
       // that is live with row <= rr is on row rr.
-      // Table values of the lane's current bin, loaded one step ahead (right
-      // after the previous bin used them), so a whole step covers their
-      // latency.  The index stops at T - 1.
-      // off16: byte offset of the bin in the lane's Kq row.  The bin index
-      // runs as a float (exact below 2^24), so tc = t w + off needs no
-      // conversion.
-      // The walk starts at t <= thi + 1 <= T and steps only from bins t <= thi,
-      // so every Kq read is at a bin <= T.  (itc4 has no entry T: clamped.)
-      const uint32_t tmax4 = (uint32_t)(g.T - 1) * 4u;
+      // The table values of the lane's current bin are loaded one step ahead
+      // (right after the previous bin used them), so a whole step covers
+      // their latency.  off16 is the bin's byte offset in the lane's Kq row;
+      // the bin index runs as a float (exact below 2^24), so tc = t w + off
+      // needs no conversion.  The walk starts at t <= thi + 1 <= T and steps
+      // only from bins t <= thi, so every read is at a bin <= T: the Kq rows
+      // (stride T + 1) and the itc4 table (t_stride >= T + 1 entries) hold a
+      // zero there, and no clamp is needed.
       uint32_t off16 = (uint32_t)t * 16u;
       float4 kq = *(const float4*)(kqp + off16);
-      float ic = SAFE ? load_at(itc4t, min(off16 >> 2, tmax4)) : 0.f;
+      float ic = SAFE ? load_at(itc4t, off16 >> 2) : 0.f;
       float tf = (float)t;
       const float thif = (float)thi;
       // Per row: L = carry + sum of K d2^2, H = sum of K d2^2 alpha; the row's
@@ -1994,7 +1993,7 @@ __device__ __forceinline__ void a1_quad_item(
             H.w = fmaf(kq.w, wh, H.w);
             off16 += 16u;
             kq = *(const float4*)(kqp + off16);
-            if (SAFE) ic = load_at(itc4t, min(off16 >> 2, tmax4));
+            if (SAFE) ic = load_at(itc4t, off16 >> 2);
             r = tf <= thif ? rn : 0x7fffffff;
           }
         }
