@@ -1882,146 +1882,159 @@ __device__ __forceinline__ void a1_quad_item(
         thi = -1;
       }
       if (!wave_any(tlo <= thi)) return;
-      // component n of the LDS rows takes scan s_(n^f); the sums' component m
-      // takes s_m
-      const char* kqp = (const char*)(Kq + (size_t)qi * kq_stride(g.T));
-      // decision of bin tt (unspecified for d2 <= 0; the walk never meets
-      // such a bin).  tc^2 is recomputed with the tables' rounding.
-      auto decide_bin = [&](int tt, float* d2o, float* alo) -> int {
-        const float tc = bin_distance(tt, g.w, g.off);
-        const float d2 = (tc * tc - dx2) - dy2;
-        bool nr;
-        int r = decide_fast(g, d2, cp, delta, half_m_delta, alo, &nr);
-        if (wave_any(nr))
-          if (nr) r = decide_exact(g, d2, wz, zshift, alo);
-        *d2o = d2;
-        return r;
-      };
-      // the same for the bin index held as a float (tf == (float)tt)
-      auto decide_bin_f = [&](float tf, float* d2o, float* alo) -> int {
-        float tc = tf * g.w;
-        tc = tc + g.off;
-        const float d2 = (tc * tc - dx2) - dy2;
-        bool nr;
-        int r = decide_fast(g, d2, cp, delta, half_m_delta, alo, &nr);
-        if (wave_any(nr))
-          if (nr) r = decide_exact(g, d2, wz, zshift, alo);
-        *d2o = d2;
-        return r;
-      };
-      // The walk starts at the lane's first bin with d2 > 0 and row >= 0:
-      // estimated from the real-valued inverse (depth of row 0's start), then
-      // settled by exact decisions (backward while the previous bin also
-      // qualifies, forward while this one does not).  Every later bin then
-      // has d2 > 0 and a row no lower.
-      const float rho2 = dx2 + dy2;
-      const float depth0 = fmaxf(((float)zshift + 0.5f) / g.s + g.oz - wz, 0.f);
-      const float te = (sqrt_fast(rho2 + depth0 * depth0) - g.off) * __builtin_amdgcn_rcpf(g.w);
-      int t = tlo;
-      if (te > (float)tlo + 1.f) t = te < (float)thi ? (int)te - 1 : thi;
-      float d2, al;
-      int r = decide_bin(t, &d2, &al);
-      bool back = (t <= thi) & (t > tlo) & (d2 > 0.f) & (r >= 0);
-      while (wave_any(back)) {
-        if (back) {
-          float d2p, alp;
-          const int rp = decide_bin(t - 1, &d2p, &alp);
-          back = (d2p > 0.f) & (rp >= 0);
+      // A lane's bins past thi have rows past the slab (the bin range is
+      // conservative), so a walk ends by itself there unless thi was clipped
+      // at the last bins.  When every lane ends at least two bins before
+      // the last (thi <= T - 3), the walk needs no per-step stop test and
+      // reads at most bin thi + 1 <= T - 2 of its rows; otherwise each lane
+      // stops at thi.
+      auto walk = [&](auto sc) {
+        constexpr bool STOP = decltype(sc)::value;
+        // component n of the LDS rows takes scan s_(n^f); the sums' component m
+        // takes s_m
+        const char* kqp = (const char*)(Kq + (size_t)qi * kq_stride(g.T));
+        // decision of bin tt (unspecified for d2 <= 0; the walk never meets
+        // such a bin).  tc^2 is recomputed with the tables' rounding.
+        auto decide_bin = [&](int tt, float* d2o, float* alo) -> int {
+          const float tc = bin_distance(tt, g.w, g.off);
+          const float d2 = (tc * tc - dx2) - dy2;
+          bool nr;
+          int r = decide_fast(g, d2, cp, delta, half_m_delta, alo, &nr);
+          if (wave_any(nr))
+            if (nr) r = decide_exact(g, d2, wz, zshift, alo);
+          *d2o = d2;
+          return r;
+        };
+        // the same for the bin index held as a float (tf == (float)tt)
+        auto decide_bin_f = [&](float tf, float* d2o, float* alo) -> int {
+          float tc = tf * g.w;
+          tc = tc + g.off;
+          const float d2 = (tc * tc - dx2) - dy2;
+          bool nr;
+          int r = decide_fast(g, d2, cp, delta, half_m_delta, alo, &nr);
+          if (wave_any(nr))
+            if (nr) r = decide_exact(g, d2, wz, zshift, alo);
+          *d2o = d2;
+          return r;
+        };
+        // The walk starts at the lane's first bin with d2 > 0 and row >= 0:
+        // estimated from the real-valued inverse (depth of row 0's start), then
+        // settled by exact decisions (backward while the previous bin also
+        // qualifies, forward while this one does not).  Every later bin then
+        // has d2 > 0 and a row no lower.
+        const float rho2 = dx2 + dy2;
+        const float depth0 = fmaxf(((float)zshift + 0.5f) / g.s + g.oz - wz, 0.f);
+        const float te = (sqrt_fast(rho2 + depth0 * depth0) - g.off) * __builtin_amdgcn_rcpf(g.w);
+        int t = tlo;
+        if (te > (float)tlo + 1.f) t = te < (float)thi ? (int)te - 1 : thi;
+        float d2, al;
+        int r = decide_bin(t, &d2, &al);
+        bool back = (t <= thi) & (t > tlo) & (d2 > 0.f) & (r >= 0);
+        while (wave_any(back)) {
           if (back) {
-            --t;
-            r = rp;
-            al = alp;
-            d2 = d2p;
-            back = t > tlo;
+            float d2p, alp;
+            const int rp = decide_bin(t - 1, &d2p, &alp);
+            back = (d2p > 0.f) & (rp >= 0);
+            if (back) {
+              --t;
+              r = rp;
+              al = alp;
+              d2 = d2p;
+              back = t > tlo;
+            }
           }
         }
-      }
-      bool fwd = (t <= thi) & !((d2 > 0.f) & (r >= 0));
-      while (wave_any(fwd)) {
-        if (fwd) {
-          ++t;
-          r = decide_bin(t, &d2, &al);
-          fwd = (t <= thi) & !((d2 > 0.f) & (r >= 0));
-        }
-      }
-      // A lane with no bin left carries the row 0x7fffffff, so "still walking
-      // and on row rr" is the single test r <= rr.
-      if (!(t <= thi)) r = 0x7fffffff;
-      int rr = uniform_i32(wave_min_i32(r));
-      float4 carry = zero4;
-      // At the start of row rr every live lane's row is >= rr, so a lane
-      // that is live with row <= rr is on row rr.
-      // The table values of the lane's current bin are loaded one step ahead
-      // (right after the previous bin used them), so a whole step covers
-      // their latency.  off16 is the bin's byte offset in the lane's Kq row;
-      // the bin index runs as a float (exact below 2^24), so tc = t w + off
-      // needs no conversion.  The walk starts at t <= thi + 1 <= T and steps
-      // only from bins t <= thi, so every read is at a bin <= T: the Kq rows
-      // (stride T + 1) and the itc4 table (t_stride >= T + 1 entries) hold a
-      // zero there, and no clamp is needed.
-      uint32_t off16 = (uint32_t)t * 16u;
-      float4 kq = *(const float4*)(kqp + off16);
-      float ic = SAFE ? load_at(itc4t, off16 >> 2) : 0.f;
-      float tf = (float)t;
-      const float thif = (float)thi;
-      // Per row: L = carry + sum of K d2^2, H = sum of K d2^2 alpha; the row's
-      // voxel gets L - H (its (1 - alpha) parts plus the carry) and H is
-      // carried into the next row.
-      while (rr <= rspan) {
-        if (!wave_any(tf <= thif)) break;
-        float4 L = carry, H = zero4;
-        while (true) {
-          const bool act = r <= rr;
-          if (!wave_any(act)) break;
-          if (act) {
-            // this bin's weights, then the next bin's decision in place (past
-            // thi the lane stops and the decision is unused)
-            float base = SAFE ? sq(d2 * ic) : d2 * d2;
-            float wh = base * al;
-            // keep both products ahead of the next decision, so that d2 and al
-            // are free for it (no register copies); emits no instruction
-            asm volatile("" : "+v"(base), "+v"(wh));
-            tf = tf + 1.f;
-            const int rn = decide_bin_f(tf, &d2, &al);
-            L.x = fmaf(kq.x, base, L.x);
-            L.y = fmaf(kq.y, base, L.y);
-            L.z = fmaf(kq.z, base, L.z);
-            L.w = fmaf(kq.w, base, L.w);
-            H.x = fmaf(kq.x, wh, H.x);
-            H.y = fmaf(kq.y, wh, H.y);
-            H.z = fmaf(kq.z, wh, H.z);
-            H.w = fmaf(kq.w, wh, H.w);
-            off16 += 16u;
-            kq = *(const float4*)(kqp + off16);
-            if (SAFE) ic = load_at(itc4t, off16 >> 2);
-            r = tf <= thif ? rn : 0x7fffffff;
+        bool fwd = (t <= thi) & !((d2 > 0.f) & (r >= 0));
+        while (wave_any(fwd)) {
+          if (fwd) {
+            ++t;
+            r = decide_bin(t, &d2, &al);
+            fwd = (t <= thi) & !((d2 > 0.f) & (r >= 0));
           }
         }
-        float4* row = my + rr * 64;
-        float4 a = *row;
-        L = xor_perm4(L, f);
-        H = xor_perm4(H, f);
-        a.x += L.x - H.x;
-        a.y += L.y - H.y;
-        a.z += L.z - H.z;
-        a.w += L.w - H.w;
-        *row = a;
-        carry = xor_perm4(H, f);  // back to the sums' order (the permutation is an involution)
-        ++rr;
-      }
-      // the alpha part of the last walked row; when no row was walked (rr
-      // started past the slab, or at INT_MAX with no live lane) the carry is
-      // zero and rr may lie outside the LDS block, so nothing is written
-      if (rr <= rspan + 1) {
-        float4* row = my + rr * 64;
-        float4 a = *row;
-        carry = xor_perm4(carry, f);
-        a.x += carry.x;
-        a.y += carry.y;
-        a.z += carry.z;
-        a.w += carry.w;
-        *row = a;
-      }
+        // A lane with no bin left carries the row 0x7fffffff, so "still walking
+        // and on row rr" is the single test r <= rr.
+        if (!(t <= thi)) r = 0x7fffffff;
+        int rr = uniform_i32(wave_min_i32(r));
+        float4 carry = zero4;
+        // At the start of row rr every live lane's row is >= rr, so a lane
+        // that is live with row <= rr is on row rr.
+        // The table values of the lane's current bin are loaded one step ahead
+        // (right after the previous bin used them), so a whole step covers
+        // their latency.  off16 is the bin's byte offset in the lane's Kq row;
+        // the bin index runs as a float (exact below 2^24), so tc = t w + off
+        // needs no conversion.  The walk starts at t <= thi + 1 <= T and steps
+        // only from bins t <= thi, so every read is at a bin <= T: the Kq rows
+        // (stride T + 1) and the itc4 table (t_stride >= T + 1 entries) hold a
+        // zero there, and no clamp is needed.
+        uint32_t off16 = (uint32_t)t * 16u;
+        float4 kq = *(const float4*)(kqp + off16);
+        float ic = SAFE ? load_at(itc4t, off16 >> 2) : 0.f;
+        float tf = (float)t;
+        const float thif = (float)thi;
+        // Per row: L = carry + sum of K d2^2, H = sum of K d2^2 alpha; the row's
+        // voxel gets L - H (its (1 - alpha) parts plus the carry) and H is
+        // carried into the next row.
+        while (rr <= rspan) {
+          if (!wave_any(tf <= thif)) break;
+          float4 L = carry, H = zero4;
+          while (true) {
+            const bool act = r <= rr;
+            if (!wave_any(act)) break;
+            if (act) {
+              // this bin's weights, then the next bin's decision in place (past
+              // thi the lane stops and the decision is unused)
+              float base = SAFE ? sq(d2 * ic) : d2 * d2;
+              float wh = base * al;
+              // keep both products ahead of the next decision, so that d2 and al
+              // are free for it (no register copies); emits no instruction
+              asm volatile("" : "+v"(base), "+v"(wh));
+              tf = tf + 1.f;
+              const int rn = decide_bin_f(tf, &d2, &al);
+              L.x = fmaf(kq.x, base, L.x);
+              L.y = fmaf(kq.y, base, L.y);
+              L.z = fmaf(kq.z, base, L.z);
+              L.w = fmaf(kq.w, base, L.w);
+              H.x = fmaf(kq.x, wh, H.x);
+              H.y = fmaf(kq.y, wh, H.y);
+              H.z = fmaf(kq.z, wh, H.z);
+              H.w = fmaf(kq.w, wh, H.w);
+              off16 += 16u;
+              kq = *(const float4*)(kqp + off16);
+              if (SAFE) ic = load_at(itc4t, off16 >> 2);
+              r = (!STOP || tf <= thif) ? rn : 0x7fffffff;
+            }
+          }
+          float4* row = my + rr * 64;
+          float4 a = *row;
+          L = xor_perm4(L, f);
+          H = xor_perm4(H, f);
+          a.x += L.x - H.x;
+          a.y += L.y - H.y;
+          a.z += L.z - H.z;
+          a.w += L.w - H.w;
+          *row = a;
+          carry = xor_perm4(H, f);  // back to the sums' order (the permutation is an involution)
+          ++rr;
+        }
+        // the alpha part of the last walked row; when no row was walked (rr
+        // started past the slab, or at INT_MAX with no live lane) the carry is
+        // zero and rr may lie outside the LDS block, so nothing is written
+        if (rr <= rspan + 1) {
+          float4* row = my + rr * 64;
+          float4 a = *row;
+          carry = xor_perm4(carry, f);
+          a.x += carry.x;
+          a.y += carry.y;
+          a.z += carry.z;
+          a.w += carry.w;
+          *row = a;
+        }
+      };
+      if (wave_any(thi > g.T - 3))
+        walk(std::true_type{});
+      else
+        walk(std::false_type{});
     };
     walk_family(std::integral_constant<int, 0>{});
     walk_family(std::integral_constant<int, 1>{});
@@ -2155,94 +2168,103 @@ __device__ __forceinline__ void a1_row_item(
       thi = -1;
     }
     if (!wave_any(tlo <= thi)) continue;
-    const float* kp = Kf + (size_t)i * t_stride;
-    auto decide_bin = [&](int tt, float* d2o, float* alo) -> int {
-      const float tc = bin_distance(tt, g.w, g.off);
-      const float d2 = (tc * tc - dx2) - dy2;
-      bool nr;
-      int r = decide_fast(g, d2, cp, delta, half_m_delta, alo, &nr);
-      if (wave_any(nr))
-        if (nr) r = decide_exact(g, d2, wz, zshift, alo);
-      *d2o = d2;
-      return r;
-    };
-    auto decide_bin_f = [&](float tf, float* d2o, float* alo) -> int {
-      float tc = tf * g.w;
-      tc = tc + g.off;
-      const float d2 = (tc * tc - dx2) - dy2;
-      bool nr;
-      int r = decide_fast(g, d2, cp, delta, half_m_delta, alo, &nr);
-      if (wave_any(nr))
-        if (nr) r = decide_exact(g, d2, wz, zshift, alo);
-      *d2o = d2;
-      return r;
-    };
-    // Exact walk start (see a1_quad_item).
-    const float rho2 = dx2 + dy2;
-    const float depth0 = fmaxf(((float)zshift + 0.5f) / g.s + g.oz - wz, 0.f);
-    const float te = (sqrt_fast(rho2 + depth0 * depth0) - g.off) * __builtin_amdgcn_rcpf(g.w);
-    int t = tlo;
-    if (te > (float)tlo + 1.f) t = te < (float)thi ? (int)te - 1 : thi;
-    float d2, al;
-    int r = decide_bin(t, &d2, &al);
-    bool back = (t <= thi) & (t > tlo) & (d2 > 0.f) & (r >= 0);
-    while (wave_any(back)) {
-      if (back) {
-        float d2p, alp;
-        const int rp = decide_bin(t - 1, &d2p, &alp);
-        back = (d2p > 0.f) & (rp >= 0);
+    // Per-step stop test only where some lane's range reaches the last bins
+    // (see a1_quad_item).
+    auto walk = [&](auto sc) {
+      constexpr bool STOP = decltype(sc)::value;
+      const float* kp = Kf + (size_t)i * t_stride;
+      auto decide_bin = [&](int tt, float* d2o, float* alo) -> int {
+        const float tc = bin_distance(tt, g.w, g.off);
+        const float d2 = (tc * tc - dx2) - dy2;
+        bool nr;
+        int r = decide_fast(g, d2, cp, delta, half_m_delta, alo, &nr);
+        if (wave_any(nr))
+          if (nr) r = decide_exact(g, d2, wz, zshift, alo);
+        *d2o = d2;
+        return r;
+      };
+      auto decide_bin_f = [&](float tf, float* d2o, float* alo) -> int {
+        float tc = tf * g.w;
+        tc = tc + g.off;
+        const float d2 = (tc * tc - dx2) - dy2;
+        bool nr;
+        int r = decide_fast(g, d2, cp, delta, half_m_delta, alo, &nr);
+        if (wave_any(nr))
+          if (nr) r = decide_exact(g, d2, wz, zshift, alo);
+        *d2o = d2;
+        return r;
+      };
+      // Exact walk start (see a1_quad_item).
+      const float rho2 = dx2 + dy2;
+      const float depth0 = fmaxf(((float)zshift + 0.5f) / g.s + g.oz - wz, 0.f);
+      const float te = (sqrt_fast(rho2 + depth0 * depth0) - g.off) * __builtin_amdgcn_rcpf(g.w);
+      int t = tlo;
+      if (te > (float)tlo + 1.f) t = te < (float)thi ? (int)te - 1 : thi;
+      float d2, al;
+      int r = decide_bin(t, &d2, &al);
+      bool back = (t <= thi) & (t > tlo) & (d2 > 0.f) & (r >= 0);
+      while (wave_any(back)) {
         if (back) {
-          --t;
-          r = rp;
-          al = alp;
-          d2 = d2p;
-          back = t > tlo;
+          float d2p, alp;
+          const int rp = decide_bin(t - 1, &d2p, &alp);
+          back = (d2p > 0.f) & (rp >= 0);
+          if (back) {
+            --t;
+            r = rp;
+            al = alp;
+            d2 = d2p;
+            back = t > tlo;
+          }
         }
       }
-    }
-    bool fwd = (t <= thi) & !((d2 > 0.f) & (r >= 0));
-    while (wave_any(fwd)) {
-      if (fwd) {
-        ++t;
-        r = decide_bin(t, &d2, &al);
-        fwd = (t <= thi) & !((d2 > 0.f) & (r >= 0));
-      }
-    }
-    if (!(t <= thi)) r = 0x7fffffff;
-    int rr = uniform_i32(wave_min_i32(r));
-    float carry = 0.f;
-    // t <= thi + 1 <= T < t_stride: every table read is in the row
-    uint32_t off4 = (uint32_t)t * 4u;
-    float k = load_at(kp, off4);
-    float ic = SAFE ? load_at(itc4t, off4) : 0.f;
-    float tf = (float)t;
-    const float thif = (float)thi;
-    while (rr <= rspan) {
-      if (!wave_any(tf <= thif)) break;
-      float L = carry, H = 0.f;
-      while (true) {
-        const bool act = r <= rr;
-        if (!wave_any(act)) break;
-        if (act) {
-          float base = SAFE ? sq(d2 * ic) : d2 * d2;
-          float wh = base * al;
-          asm volatile("" : "+v"(base), "+v"(wh));  // see a1_quad_item
-          tf = tf + 1.f;
-          const int rn = decide_bin_f(tf, &d2, &al);
-          L = fmaf(k, base, L);
-          H = fmaf(k, wh, H);
-          off4 += 4u;
-          k = load_at(kp, off4);
-          if (SAFE) ic = load_at(itc4t, off4);
-          r = tf <= thif ? rn : 0x7fffffff;
+      bool fwd = (t <= thi) & !((d2 > 0.f) & (r >= 0));
+      while (wave_any(fwd)) {
+        if (fwd) {
+          ++t;
+          r = decide_bin(t, &d2, &al);
+          fwd = (t <= thi) & !((d2 > 0.f) & (r >= 0));
         }
       }
-      float* row = my + rr * 64;
-      *row += L - H;
-      carry = H;
-      ++rr;
-    }
-    if (rr <= rspan + 1) my[rr * 64] += carry;
+      if (!(t <= thi)) r = 0x7fffffff;
+      int rr = uniform_i32(wave_min_i32(r));
+      float carry = 0.f;
+      // t <= thi + 1 <= T < t_stride: every table read is in the row
+      uint32_t off4 = (uint32_t)t * 4u;
+      float k = load_at(kp, off4);
+      float ic = SAFE ? load_at(itc4t, off4) : 0.f;
+      float tf = (float)t;
+      const float thif = (float)thi;
+      while (rr <= rspan) {
+        if (!wave_any(tf <= thif)) break;
+        float L = carry, H = 0.f;
+        while (true) {
+          const bool act = r <= rr;
+          if (!wave_any(act)) break;
+          if (act) {
+            float base = SAFE ? sq(d2 * ic) : d2 * d2;
+            float wh = base * al;
+            asm volatile("" : "+v"(base), "+v"(wh));  // see a1_quad_item
+            tf = tf + 1.f;
+            const int rn = decide_bin_f(tf, &d2, &al);
+            L = fmaf(k, base, L);
+            H = fmaf(k, wh, H);
+            off4 += 4u;
+            k = load_at(kp, off4);
+            if (SAFE) ic = load_at(itc4t, off4);
+            r = (!STOP || tf <= thif) ? rn : 0x7fffffff;
+          }
+        }
+        float* row = my + rr * 64;
+        *row += L - H;
+        carry = H;
+        ++rr;
+      }
+      if (rr <= rspan + 1) my[rr * 64] += carry;
+    };
+    if (wave_any(thi > g.T - 3))
+      walk(std::true_type{});
+    else
+      walk(std::false_type{});
   }
 
   if (cvalid) {
