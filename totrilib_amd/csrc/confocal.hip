@@ -742,16 +742,17 @@ __device__ __forceinline__ void r1_sym_block(
     const float2* cur = lds2 + buf * chunk_vox;
     float4* nxt4 = reinterpret_cast<float4*>(lds2 + (buf ^ 1) * chunk_vox);
     const bool has_next = c + 1 < chunk_end;
-    float4 pref_lo[kPref], pref_hi[kPref];
+    // The next chunk goes to the other buffer right away (every wave left
+    // it at the previous barrier); no registers stay held during the walk.
     if (has_next) {
 #pragma unroll
       for (int r = 0; r < kPref; ++r) {
         const int it = threadIdx.x + r * nthreads;
         if (it < chunk_items) {
-          load_row(c + 1, it, &pref_lo[r], &pref_hi[r]);
-        } else {
-          pref_lo[r] = make_float4(0.f, 0.f, 0.f, 0.f);
-          pref_hi[r] = pref_lo[r];
+          float4 lo, hi;
+          load_row(c + 1, it, &lo, &hi);
+          nxt4[2 * it] = lo;
+          nxt4[2 * it + 1] = hi;
         }
       }
     }
@@ -846,16 +847,6 @@ __device__ __forceinline__ void r1_sym_block(
       }
     }
 
-    if (has_next) {
-#pragma unroll
-      for (int r = 0; r < kPref; ++r) {
-        const int it = threadIdx.x + r * nthreads;
-        if (it < chunk_items) {
-          nxt4[2 * it] = pref_lo[r];
-          nxt4[2 * it + 1] = pref_hi[r];
-        }
-      }
-    }
     __syncthreads();
   }
 
@@ -976,16 +967,17 @@ __device__ __forceinline__ void a6_sym_block(
     const float2* cur = lds2 + buf * chunk_vox;
     float4* nxt4 = reinterpret_cast<float4*>(lds2 + (buf ^ 1) * chunk_vox);
     const bool has_next = c + 1 < chunk_end;
-    float4 pref_lo[kPref], pref_hi[kPref];
+    // The next chunk goes to the other buffer right away (every wave left
+    // it at the previous barrier); no registers stay held during the walk.
     if (has_next) {
 #pragma unroll
       for (int r = 0; r < kPref; ++r) {
         const int it = threadIdx.x + r * nthreads;
         if (it < chunk_items) {
-          load_row(c + 1, it, &pref_lo[r], &pref_hi[r]);
-        } else {
-          pref_lo[r] = make_float4(0.f, 0.f, 0.f, 0.f);
-          pref_hi[r] = pref_lo[r];
+          float4 lo, hi;
+          load_row(c + 1, it, &lo, &hi);
+          nxt4[2 * it] = lo;
+          nxt4[2 * it + 1] = hi;
         }
       }
     }
@@ -1085,16 +1077,6 @@ __device__ __forceinline__ void a6_sym_block(
       }
     }
 
-    if (has_next) {
-#pragma unroll
-      for (int r = 0; r < kPref; ++r) {
-        const int it = threadIdx.x + r * nthreads;
-        if (it < chunk_items) {
-          nxt4[2 * it] = pref_lo[r];
-          nxt4[2 * it + 1] = pref_hi[r];
-        }
-      }
-    }
     __syncthreads();
   }
 
