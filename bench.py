@@ -29,6 +29,7 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP32 = 157.3e12   # MI355X published FP32 vector peak (packed issue)
 PEAK_HBM = 8.0e12      # MI355X HBM3E bytes/s
+SIMD_CLOCK_HZ = 2.4e9  # MI355X peak engine clock (256 CUs x 4 SIMDs)
 
 WORKLOAD = dict(n_side=64, T=1024, res=128, w_round_trip=0.005, occupancy=0.03,
                 seed=1, target_seed=2)
@@ -233,6 +234,18 @@ def confocal_main(args):
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get(dom)
+    # VALU issue: the dominant kernel's wave VALU instructions per dispatch
+    # (rocprof, same workload) at 4 cycles each on 1024 SIMDs at 2.4 GHz,
+    # against the measured call duration
+    valu_issue = None
+    vpath = os.path.join(ROOT, "profiles", "valu.json")
+    if os.path.exists(vpath):
+        with open(vpath) as f:
+            instr = json.load(f).get(dom)
+        if instr:
+            issue_ms = instr * 4 / (1024 * SIMD_CLOCK_HZ) * 1e3
+            valu_issue = {"wave_instructions": instr, "issue_ms": issue_ms, "call_ms": dur,
+                          "frac": issue_ms / dur}
 
     if args.copy_probe:
         src = torch.ones(64 * 1024 * 1024, device=dev)
@@ -270,12 +283,14 @@ def confocal_main(args):
         "roofline": {"bound": "valu", "kernel": dom, "achieved": achieved / 1e12,
                      "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32, "traffic": traffic,
+                     "valu_issue": valu_issue,
                      "note": "achieved counts the reference formulation's FLOP (18 per "
                              "active R1 sample, 17 per A1 window bin + 23 per pair); the "
                              "mirror kernels share one decision among four samples and "
                              "execute about a quarter of that (rocprof VALU counts in "
                              "profiles/confocal_128_64x64_1024_valu.md), so frac can "
-                             "exceed 1 without the VALU issuing above its peak"},
+                             "exceed 1 without the VALU issuing above its peak; valu_issue "
+                             "gives the executed instructions' share of the call"},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))
