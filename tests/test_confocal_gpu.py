@@ -281,6 +281,35 @@ def test_a1_row_walks_edge_shapes(shape, n_scan, scan):
     assert np.all(np.isfinite(got)) and e < TOL
 
 
+@pytest.mark.parametrize("scan", ["quads", "random"])
+@pytest.mark.parametrize("dt", [-6, -2, 0, 2, 6])
+def test_a1_row_walks_near_last_bin(scan, dt):
+    """Bin counts around the one where the farthest voxel centre's radius
+    meets the last bin: waves whose bin ranges end at least two bins before
+    the last walk without a per-step stop test, the others stop at their
+    range end; both against the oracle's exact transpose."""
+    X, Y, Z = 16, 16, 24
+    w = 0.01
+    rng = np.random.default_rng(40 + dt + (0 if scan == "quads" else 100))
+    if scan == "quads":
+        mp = wall_grid(8)
+    else:
+        mp = np.stack([rng.uniform(-1, 1, 40), rng.uniform(-1, 1, 40),
+                       np.zeros(40)]).astype(np.float32)
+    cx = ORIGIN[0] + (np.arange(X) + 0.5) * SIZE[0] / X
+    cy = ORIGIN[1] + (np.arange(Y) + 0.5) * SIZE[1] / Y
+    cz = ORIGIN[2] + (np.arange(Z) + 0.5) * SIZE[2] / Z
+    far = max(np.sqrt((np.abs(mp[0, i] - cx).max()) ** 2 + (np.abs(mp[1, i] - cy).max()) ** 2 +
+                      (cz.max() - mp[2, i]) ** 2) for i in range(mp.shape[1]))
+    T = int(np.ceil(far / w)) + dt
+    G = rng.standard_normal((mp.shape[1], T)).astype(np.float32)
+    got = gpu_a1(G, mp, (X, Y, Z), w, 0.0)
+    ref = O.a1_transpose(G, mp, X, Y, Z, ORIGIN, SIZE, w, 0.0, mode=O.PINNED64)
+    e = rel_l2(got, ref)
+    print(f"{scan} T={T}: rel-L2 vs FP64 pinned {e:.2e}")
+    assert np.all(np.isfinite(got)) and e < TOL
+
+
 def gpu_a6(vol, mp, G, w, off, origin=ORIGIN, size=SIZE):
     from totrilib_amd.totri_cpp.render import volume_confocal_grad_measurement_points
     v = torch.from_numpy(np.ascontiguousarray(vol)).to(dev())
