@@ -258,6 +258,8 @@ def test_degenerate_shapes():
     ((6, 8, 17), 16, "random"),    # odd slab count
     ((10, 8, 17), 16, "quads"),    # the xy-mirror walk with Z = 17
     ((12, 10, 8), 64, "quads"),    # Z a multiple of the slab height: no boundary rows
+    ((10, 9, 17), 16, "quads"),    # odd Y: the x-pair walk with Z = 17
+    ((6, 7, 8), 64, "quads"),      # the x-pair walk, one tile row, no boundary rows
 ])
 def test_a1_row_walks_edge_shapes(shape, n_scan, scan):
     """The row walks' slabs (8 voxel rows), boundary rows and scan ranges on
@@ -281,17 +283,17 @@ def test_a1_row_walks_edge_shapes(shape, n_scan, scan):
     assert np.all(np.isfinite(got)) and e < TOL
 
 
-@pytest.mark.parametrize("scan", ["quads", "random"])
+@pytest.mark.parametrize("scan", ["quads", "pairs", "random"])
 @pytest.mark.parametrize("dt", [-6, -2, 0, 2, 6])
 def test_a1_row_walks_near_last_bin(scan, dt):
     """Bin counts around the one where the farthest voxel centre's radius
     meets the last bin: waves whose bin ranges end at least two bins before
     the last walk without a per-step stop test, the others stop at their
     range end; both against the oracle's exact transpose."""
-    X, Y, Z = 16, 16, 24
+    X, Y, Z = 16, 15 if scan == "pairs" else 16, 24  # odd Y: x-pairs only
     w = 0.01
     rng = np.random.default_rng(40 + dt + (0 if scan == "quads" else 100))
-    if scan == "quads":
+    if scan in ("quads", "pairs"):
         mp = wall_grid(8)
     else:
         mp = np.stack([rng.uniform(-1, 1, 40), rng.uniform(-1, 1, 40),
@@ -485,7 +487,7 @@ def test_r1_mirror_wall_inside_the_box(layout, scan):
 
 
 @pytest.mark.parametrize("scan", ["quads", "quads_safe", "quads_negoff", "xpairs", "odd_y",
-                                  "perturbed", "random"])
+                                  "odd_y_safe", "odd_y_negoff", "perturbed", "random"])
 def test_a1_mirror_pairs_and_general_path(scan):
     """Path selection of A1, each checked against the oracle's exact adjoint.
 
@@ -493,9 +495,12 @@ def test_a1_mirror_pairs_and_general_path(scan):
       and offset >= 0, take the xy-mirror walk (one decision per four
       samples): "quads", and "quads_safe" whose offset 0.5 mm makes 1/tc^8
       overflow-prone, so the table takes its safe form.
-    - A negative offset (tc < 0 for the first bins) fails the walk's
-      monotonicity condition, and x-pairs without y-mirrors or an odd Y allow
-      only the x-mirror kernel (one decision per two samples).
+    - x-pairs without y-mirrors or an odd Y allow only the x-mirror paths
+      (one decision per two samples): the x-pair row walk ("xpairs",
+      "odd_y", and "odd_y_safe" with the safe table form), or, where a
+      negative offset (tc < 0 for the first bins) fails the row walks'
+      monotonicity condition, the x-mirror window walk ("odd_y_negoff").
+      "quads_negoff" takes the window walk too.
     - A broken pair or a random scan takes the general kernel.
     """
     rng = np.random.default_rng(33)
@@ -511,8 +516,9 @@ def test_a1_mirror_pairs_and_general_path(scan):
         k = 9
         j = int(np.flatnonzero((mp[0] == -mp[0, k]) & (mp[1] == mp[1, k]))[0])
         mp[1, [k, j]] = np.nextafter(mp[1, k], np.float32(2))
-    elif scan == "odd_y":
+    elif scan in ("odd_y", "odd_y_safe", "odd_y_negoff"):
         Y = 25
+        off = {"odd_y": 0.0, "odd_y_safe": 0.0005, "odd_y_negoff": -0.05}[scan]
     elif scan == "perturbed":
         mp[1, 9] = np.nextafter(mp[1, 9], np.float32(2))   # breaks one x-pair
         mp[0, 9] = np.nextafter(mp[0, 9], np.float32(2))
@@ -523,6 +529,7 @@ def test_a1_mirror_pairs_and_general_path(scan):
     info = mirror_info(mp)
     expect = {"quads": [1, 16, 1, 32], "quads_safe": [1, 16, 1, 32],
               "quads_negoff": [1, 16, 1, 32], "odd_y": [1, 16, 1, 32],
+              "odd_y_safe": [1, 16, 1, 32], "odd_y_negoff": [1, 16, 1, 32],
               "xpairs": [0, info[1], 1, 32]}.get(scan, [0, info[1], 0, info[3]])
     assert info == expect
     got = gpu_a1(G, mp, (X, Y, Z), w, off)

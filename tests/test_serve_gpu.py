@@ -39,6 +39,10 @@ def make_requests(seed=0):
         ("b", 12, (24, 20, 32), 400, 0.006, 0.0, "device", "perturbed"),
         ("b", 10, (20, 20, 24), 300, 0.009, -0.01, "pinned", "perturbed"),
         ("r", 12, (24, 20, 32), 400, 0.006, 0.0, "device", "random"),
+        # odd Y: x-pairs only (the x-pair row walk; the x-mirror window walk
+        # for a negative offset)
+        ("b", 10, (20, 15, 24), 300, 0.009, 0.0, "device"),
+        ("b", 10, (20, 15, 24), 300, 0.009, -0.01, "pinned"),
     ]
     reqs = []
     for k, (kind, n, res, T, w, off, where, *scan) in enumerate(specs):

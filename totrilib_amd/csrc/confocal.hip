@@ -1182,20 +1182,22 @@ __device__ __forceinline__ float a1_boundary(const float* __restrict__ bnd, size
 }
 
 // A1 partial sums: the xy-mirror kernel ran when quad_on and info[0] (its
-// boundary rows are added too), else the x-mirror kernel when info[2], else
-// the general kernel (its boundary rows added when bnd_gen is given, the row
-// walk); each has its own partial count.  info may be null: no mirror path.
-// Both walks' boundary rows use n_slabs = ceil(Z / kSlabQ).
+// boundary rows are added too), else the x-mirror kernel when info[2] (the
+// pair row walk with its boundary rows bnd_pair when given, else the window
+// walk), else the general kernel (its boundary rows added when bnd_gen is
+// given, the row walk); each has its own partial count.  info may be null:
+// no mirror path.  The row walks' boundary rows use n_slabs = ceil(Z / kSlabQ).
 __global__ void sum_partials_a1_kernel(const float* __restrict__ part,
                                        const float* __restrict__ bnd,
                                        const int* __restrict__ info, int quad_on, int n_quad,
                                        int n_pair, int n_gen, size_t n, int Z, int n_slabs,
                                        const float* __restrict__ bnd_gen,
+                                       const float* __restrict__ bnd_pair,
                                        float* __restrict__ out) {
   const bool quad = info && quad_on && info[0];
   const bool pair = info && !quad && info[2];
   const int n_parts = quad ? n_quad : pair ? n_pair : n_gen;
-  const float* b = quad ? bnd : pair ? nullptr : bnd_gen;
+  const float* b = quad ? bnd : pair ? bnd_pair : bnd_gen;
   for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
        e += (size_t)gridDim.x * blockDim.x) {
     float v = part[e];
@@ -1776,18 +1778,76 @@ __device__ __forceinline__ float4 xor_perm4(float4 v, int f) {
   return v;
 }
 
-// The quad's four table rows are read interleaved (Kq[q][t], one 16-byte
-// load per step) and summed in table order (component m for scan s_m); the
-// sums are permuted into the family's order when a row is flushed.
-template <bool SAFE>
+// Per-lane accumulator vectors of the mirror walks: component n for the
+// lane's column n (M = 4: float4, M = 2: float2).
+template <int M> struct MirrorVecT;
+template <> struct MirrorVecT<4> { using type = float4; };
+template <> struct MirrorVecT<2> { using type = float2; };
+template <int M> using MirrorVec = typename MirrorVecT<M>::type;
+
+template <int M>
+__device__ __forceinline__ MirrorVec<M> mv_zero() {
+  if constexpr (M == 4) return make_float4(0.f, 0.f, 0.f, 0.f);
+  else return make_float2(0.f, 0.f);
+}
+__device__ __forceinline__ void mv_add(float4& a, float4 b) {
+  a.x += b.x;
+  a.y += b.y;
+  a.z += b.z;
+  a.w += b.w;
+}
+__device__ __forceinline__ void mv_add(float2& a, float2 b) {
+  a.x += b.x;
+  a.y += b.y;
+}
+// a += L - H per component
+__device__ __forceinline__ void mv_add_diff(float4& a, float4 L, float4 H) {
+  a.x += L.x - H.x;
+  a.y += L.y - H.y;
+  a.z += L.z - H.z;
+  a.w += L.w - H.w;
+}
+__device__ __forceinline__ void mv_add_diff(float2& a, float2 L, float2 H) {
+  a.x += L.x - H.x;
+  a.y += L.y - H.y;
+}
+// a += k * s per component
+__device__ __forceinline__ void mv_fma(float4& a, float4 k, float s) {
+  a.x = fmaf(k.x, s, a.x);
+  a.y = fmaf(k.y, s, a.y);
+  a.z = fmaf(k.z, s, a.z);
+  a.w = fmaf(k.w, s, a.w);
+}
+__device__ __forceinline__ void mv_fma(float2& a, float2 k, float s) {
+  a.x = fmaf(k.x, s, a.x);
+  a.y = fmaf(k.y, s, a.y);
+}
+__device__ __forceinline__ float4 mv_perm(float4 v, int f) { return xor_perm4(v, f); }
+__device__ __forceinline__ float2 mv_perm(float2 v, int f) {
+  return (f & 1) ? make_float2(v.y, v.x) : v;
+}
+__device__ __forceinline__ float mv_get(float4 v, int n) {
+  return n == 0 ? v.x : n == 1 ? v.y : n == 2 ? v.z : v.w;
+}
+__device__ __forceinline__ float mv_get(float2 v, int n) { return n == 0 ? v.x : v.y; }
+
+// The walk for M = 4 (xy-mirror quads of scan points, a lane's column and
+// its three mirror columns) and M = 2 (x-mirror pairs, a lane's column and
+// its x-mirror).  The group's M table rows are read interleaved (Kq[q][t],
+// one 16- or 8-byte load per step) and summed in table order (component m
+// for scan s_m); the sums are permuted into the family's order when a row is
+// flushed.
+template <bool SAFE, int M = 4>
 __device__ __forceinline__ void a1_quad_item(
-    const float4* __restrict__ Kq, const float* __restrict__ tc2t,
+    const MirrorVec<M>* __restrict__ Kq, const float* __restrict__ tc2t,
     const float* __restrict__ itc4t, const float* __restrict__ mp,
     const int* __restrict__ quads, int n_quads,
     float* __restrict__ dst,  // [R][X][Y][Z] partial volumes
     float* __restrict__ bnd,  // [R][X][Y][n_slabs] boundary rows
     const ConfGeom& g, int item, int tiles_x, int tiles_y, int n_slabs, int n_ranges,
-    int t_stride, float4* __restrict__ acc) {
+    int t_stride, MirrorVec<M>* __restrict__ acc) {
+  using V = MirrorVec<M>;
+  constexpr uint32_t kVB = sizeof(V);  // table bytes per bin
   const int lane = threadIdx.x & 63;
   const int tx = item % tiles_x;
   item /= tiles_x;
@@ -1798,7 +1858,7 @@ __device__ __forceinline__ void a1_quad_item(
 
   const int xi = tx * 8 + (lane >> 3);
   const int yi = ty * 8 + (lane & 7);
-  const bool cvalid = xi < g.X / 2 && yi < g.Y / 2;
+  const bool cvalid = xi < g.X / 2 && yi < (M == 4 ? g.Y / 2 : g.Y);
   const int xm = g.X - 1 - xi, ym = g.Y - 1 - yi;
   const float x0 = cell_centre(xi, g.ox, g.szx, g.X);
   const float x1 = cell_centre(xm, g.ox, g.szx, g.X);
@@ -1816,31 +1876,27 @@ __device__ __forceinline__ void a1_quad_item(
   const float zlo = (float)z0;
   const float zhi = (float)min(z0 + kSlabQ, g.Z - 1);
 
-  const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
+  const V zero4 = mv_zero<M>();
   for (int r = 0; r <= kSlabQ; ++r) acc[r * 64 + lane] = zero4;
-  float4 tot[kSlabQ + 1];
+  V tot[kSlabQ + 1];
 #pragma unroll
   for (int r = 0; r <= kSlabQ; ++r) tot[r] = zero4;
 
   const int per = (n_quads + n_ranges - 1) / n_ranges;
   const int q_begin = range * per;
   const int q_end = min(n_quads, q_begin + per);
-  float4* my = acc + lane;
+  V* my = acc + lane;
 
   for (int qi = q_begin; qi < q_end; ++qi) {
     if (qi > q_begin && (qi - q_begin) % (kFlush / 4) == 0) {
 #pragma unroll
       for (int r = 0; r <= kSlabQ; ++r) {
-        const float4 a = acc[r * 64 + lane];
-        tot[r].x += a.x;
-        tot[r].y += a.y;
-        tot[r].z += a.z;
-        tot[r].w += a.w;
+        mv_add(tot[r], acc[r * 64 + lane]);
         acc[r * 64 + lane] = zero4;
       }
     }
-    // the quad's four points share wz; s0 gives (wx, wy) of family 0
-    const int s0 = uniform_i32(quads[4 * qi]);
+    // the group's points share wz; s0 gives (wx, wy) of family 0
+    const int s0 = uniform_i32(quads[M * qi]);
     const float wx = mp[s0], wy = mp[g.N + s0], wz = mp[2 * g.N + s0];
     float c0 = (wz - g.oz) * g.s;
     c0 = c0 - 0.5f;
@@ -1856,7 +1912,7 @@ __device__ __forceinline__ void a1_quad_item(
     auto walk_family = [&](auto fc) {
       constexpr int f = decltype(fc)::value;
       const float dx2 = cvalid ? ((f & 1) ? dxb * dxb : dxa * dxa) : INFINITY;
-      const float dy2 = (f & 2) ? dyb * dyb : dya * dya;
+      const float dy2 = (M == 4 && (f & 2)) ? dyb * dyb : dya * dya;
       int tlo, thi;
       bin_range(g, dx2 + dy2, dlo, dhi, &tlo, &thi);
       if (!cvalid || rspan < 0 || tlo > thi) {
@@ -1938,7 +1994,7 @@ __device__ __forceinline__ void a1_quad_item(
         // and on row rr" is the single test r <= rr.
         if (!(t <= thi)) r = 0x7fffffff;
         int rr = uniform_i32(wave_min_i32(r));
-        float4 carry = zero4;
+        V carry = zero4;
         // At the start of row rr every live lane's row is >= rr, so a lane
         // that is live with row <= rr is on row rr.
         // The table values of the lane's current bin are loaded one step ahead
@@ -1949,9 +2005,9 @@ __device__ __forceinline__ void a1_quad_item(
         // only from bins t <= thi, so every read is at a bin <= T: the Kq rows
         // (stride T + 1) and the itc4 table (t_stride >= T + 1 entries) hold a
         // zero there, and no clamp is needed.
-        uint32_t off16 = (uint32_t)t * 16u;
-        float4 kq = *(const float4*)(kqp + off16);
-        float ic = SAFE ? load_at(itc4t, off16 >> 2) : 0.f;
+        uint32_t off16 = (uint32_t)t * kVB;
+        V kq = *(const V*)(kqp + off16);
+        float ic = SAFE ? load_at(itc4t, off16 / (kVB / 4)) : 0.f;
         float tf = (float)t;
         const float thif = (float)thi;
         // Per row: L = carry + sum of K d2^2, H = sum of K d2^2 alpha; the row's
@@ -1959,7 +2015,7 @@ __device__ __forceinline__ void a1_quad_item(
         // carried into the next row.
         while (rr <= rspan) {
           if (!wave_any(tf <= thif)) break;
-          float4 L = carry, H = zero4;
+          V L = carry, H = zero4;
           while (true) {
             const bool act = r <= rr;
             if (!wave_any(act)) break;
@@ -1973,43 +2029,31 @@ __device__ __forceinline__ void a1_quad_item(
               asm volatile("" : "+v"(base), "+v"(wh));
               tf = tf + 1.f;
               const int rn = decide_bin_f(tf, &d2, &al);
-              L.x = fmaf(kq.x, base, L.x);
-              L.y = fmaf(kq.y, base, L.y);
-              L.z = fmaf(kq.z, base, L.z);
-              L.w = fmaf(kq.w, base, L.w);
-              H.x = fmaf(kq.x, wh, H.x);
-              H.y = fmaf(kq.y, wh, H.y);
-              H.z = fmaf(kq.z, wh, H.z);
-              H.w = fmaf(kq.w, wh, H.w);
-              off16 += 16u;
-              kq = *(const float4*)(kqp + off16);
-              if (SAFE) ic = load_at(itc4t, off16 >> 2);
+              mv_fma(L, kq, base);
+              mv_fma(H, kq, wh);
+              off16 += kVB;
+              kq = *(const V*)(kqp + off16);
+              if (SAFE) ic = load_at(itc4t, off16 / (kVB / 4));
               r = (!STOP || tf <= thif) ? rn : 0x7fffffff;
             }
           }
-          float4* row = my + rr * 64;
-          float4 a = *row;
-          L = xor_perm4(L, f);
-          H = xor_perm4(H, f);
-          a.x += L.x - H.x;
-          a.y += L.y - H.y;
-          a.z += L.z - H.z;
-          a.w += L.w - H.w;
+          V* row = my + rr * 64;
+          V a = *row;
+          L = mv_perm(L, f);
+          H = mv_perm(H, f);
+          mv_add_diff(a, L, H);
           *row = a;
-          carry = xor_perm4(H, f);  // back to the sums' order (the permutation is an involution)
+          carry = mv_perm(H, f);  // back to the sums' order (the permutation is an involution)
           ++rr;
         }
         // the alpha part of the last walked row; when no row was walked (rr
         // started past the slab, or at INT_MAX with no live lane) the carry is
         // zero and rr may lie outside the LDS block, so nothing is written
         if (rr <= rspan + 1) {
-          float4* row = my + rr * 64;
-          float4 a = *row;
-          carry = xor_perm4(carry, f);
-          a.x += carry.x;
-          a.y += carry.y;
-          a.z += carry.z;
-          a.w += carry.w;
+          V* row = my + rr * 64;
+          V a = *row;
+          carry = mv_perm(carry, f);
+          mv_add(a, carry);
           *row = a;
         }
       };
@@ -2020,33 +2064,35 @@ __device__ __forceinline__ void a1_quad_item(
     };
     walk_family(std::integral_constant<int, 0>{});
     walk_family(std::integral_constant<int, 1>{});
-    walk_family(std::integral_constant<int, 2>{});
-    walk_family(std::integral_constant<int, 3>{});
+    if constexpr (M == 4) {
+      walk_family(std::integral_constant<int, 2>{});
+      walk_family(std::integral_constant<int, 3>{});
+    }
   }
 
   if (cvalid) {
-    float* out00 = dst + (((size_t)range * g.X + xi) * g.Y + yi) * g.Z;
-    float* out10 = dst + (((size_t)range * g.X + xm) * g.Y + yi) * g.Z;
-    float* out01 = dst + (((size_t)range * g.X + xi) * g.Y + ym) * g.Z;
-    float* out11 = dst + (((size_t)range * g.X + xm) * g.Y + ym) * g.Z;
+    // column n of the group: x mirrored when n & 1, y mirrored when n & 2
+    size_t col[M];
+#pragma unroll
+    for (int n = 0; n < M; ++n)
+      col[n] = (size_t)((n & 1) ? xm : xi) * g.Y + ((n & 2) ? ym : yi);
 #pragma unroll
     for (int r = 0; r < kSlabQ; ++r) {
       const int z = z0 + r;
       if (z < g.Z) {
-        const float4 a = acc[r * 64 + lane];
-        out00[z] = tot[r].x + a.x;
-        out10[z] = tot[r].y + a.y;
-        out01[z] = tot[r].z + a.z;
-        out11[z] = tot[r].w + a.w;
+        V a = acc[r * 64 + lane];
+        mv_add(a, tot[r]);
+#pragma unroll
+        for (int n = 0; n < M; ++n)
+          dst[((size_t)range * g.X * g.Y + col[n]) * g.Z + z] = mv_get(a, n);
       }
     }
     if (z0 + kSlabQ < g.Z) {
-      const float4 a = acc[kSlabQ * 64 + lane];
+      V a = acc[kSlabQ * 64 + lane];
+      mv_add(a, tot[kSlabQ]);
       float* b = bnd + (size_t)range * g.X * g.Y * n_slabs + slab;
-      b[((size_t)xi * g.Y + yi) * n_slabs] = tot[kSlabQ].x + a.x;
-      b[((size_t)xm * g.Y + yi) * n_slabs] = tot[kSlabQ].y + a.y;
-      b[((size_t)xi * g.Y + ym) * n_slabs] = tot[kSlabQ].z + a.z;
-      b[((size_t)xm * g.Y + ym) * n_slabs] = tot[kSlabQ].w + a.w;
+#pragma unroll
+      for (int n = 0; n < M; ++n) b[col[n] * n_slabs] = mv_get(a, n);
     }
   }
 }
@@ -2294,6 +2340,40 @@ __global__ __launch_bounds__(256) void a1_quad_table_kernel(
                               Kf[(size_t)qs[2] * t_stride + t], Kf[(size_t)qs[3] * t_stride + t])
                 : make_float4(0.f, 0.f, 0.f, 0.f);
   }
+}
+
+// Kp[p][t] = (K[s0][t], K[s1][t]) for the device-found x-pairs p < info[3]
+// (s_m = pairs[2p + m]), built only when the x-pair walk runs (pairs found,
+// and not the xy-mirror walk).
+__global__ __launch_bounds__(256) void a1_pair_table_kernel(
+    const float* __restrict__ Kf, const int* __restrict__ pairs, const int* __restrict__ info,
+    int quad_on, int t_stride, int T, int max_pairs, float2* __restrict__ Kp) {
+  if (!info[2] || (quad_on && info[0])) return;
+  const size_t ks = kq_stride(T);
+  const size_t n = (size_t)min(info[3], max_pairs) * ks;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (size_t)gridDim.x * blockDim.x) {
+    const size_t pi = e / ks, t = e - pi * ks;
+    const int* ps = pairs + 2 * pi;
+    Kp[e] = t < (size_t)T ? make_float2(Kf[(size_t)ps[0] * t_stride + t],
+                                        Kf[(size_t)ps[1] * t_stride + t])
+                          : make_float2(0.f, 0.f);
+  }
+}
+
+// x-pair A1 row walk: one single-wave workgroup per item (a tile of 8x8
+// columns of the half grid x < 0, one 8-row slab, one pair range).
+template <bool SAFE>
+__global__ __launch_bounds__(64) void a1_pair_kernel(
+    const float2* __restrict__ Kp, const float* __restrict__ tc2t,
+    const float* __restrict__ itc4t, const float* __restrict__ mp,
+    const int* __restrict__ pairs, const int* __restrict__ info, float* __restrict__ dst,
+    float* __restrict__ bnd, ConfGeom g, int tiles_x, int tiles_y, int n_slabs, int n_ranges,
+    int t_stride, int quad_on) {
+  if (!info[2] || (quad_on && info[0])) return;
+  __shared__ float2 acc[(kSlabQ + 1) * 64];
+  a1_quad_item<SAFE, 2>(Kp, tc2t, itc4t, mp, pairs, info[3], dst, bnd, g, (int)blockIdx.x,
+                        tiles_x, tiles_y, n_slabs, n_ranges, t_stride, acc);
 }
 
 // Dispatch order of the xy-mirror A1 items.  Item costs differ by a factor of
@@ -2822,6 +2902,26 @@ A1QuadPlan plan_a1_quad(int64_t X, int64_t Y, int64_t Z, int64_t N) {
   return p;
 }
 
+// Plan of the x-pair row walk: tiles of 8x8 columns of the half grid
+// x < 0, 8-row slabs, pair ranges as in plan_a1_quad (at least 4 pairs each).
+A1QuadPlan plan_a1_pair_rows(int64_t X, int64_t Y, int64_t Z, int64_t N) {
+  A1QuadPlan p;
+  memset(&p, 0, sizeof(p));
+  p.possible = X >= 2 && !(X & 1) && N >= 2 && !(N & 1) && N <= (1 << 24);
+  if (!p.possible) return p;
+  p.tiles_x = (int)((X / 2 + 7) / 8);
+  p.tiles_y = (int)((Y + 7) / 8);
+  p.n_slabs = (int)((Z + kSlabQ - 1) / kSlabQ);
+  const int64_t items = (int64_t)p.tiles_x * p.tiles_y * p.n_slabs;
+  const int64_t target = 64 * (int64_t)device_cu_count();
+  int64_t R = 1;
+  if (items < target) R = (target + items - 1) / items;
+  if (R > N / 8) R = N / 8;
+  if (R < 1) R = 1;
+  p.n_ranges = (int)R;
+  return p;
+}
+
 // The xy-mirror walk needs each lane's row to grow with the bin index:
 // tc = t w + off >= 0 for every bin and a positive z scale.
 bool a1_quad_geometry(const float* origin, const float* size, int64_t Y, float w, float off) {
@@ -2839,6 +2939,9 @@ struct A1SymPlan {
   size_t counter_off;          // xy-mirror work-queue head
   size_t bnd_off;              // xy-mirror boundary rows [R][X][Y][n_slabs]
   size_t kq_off;               // xy-mirror interleaved quad tables [N/4][T+1] float4
+  A1QuadPlan prow;             // x-pair row walk
+  size_t pbnd_off;             // x-pair row walk boundary rows [R][X][Y][n_slabs]
+  size_t kp_off;               // x-pair row walk interleaved pair tables [N/2][T+1] float2
   size_t ws;
 };
 
@@ -2858,13 +2961,16 @@ A1SymPlan plan_a1_sym(int64_t X, int64_t Y, int64_t Z, int64_t N, const A1Plan& 
   if (R < 1) R = 1;
   p.n_ranges = (int)R;
   p.quad = plan_a1_quad(X, Y, Z, N);
+  p.prow = plan_a1_pair_rows(X, Y, Z, N);
   int parts = p.n_ranges > gen.n_ranges ? p.n_ranges : gen.n_ranges;
   if (gen.row_ranges > parts) parts = gen.row_ranges;
   if (p.quad.possible && p.quad.n_ranges > parts) parts = p.quad.n_ranges;
-  // the xy-mirror path always writes partial volumes (its boundary rows are
+  if (p.prow.possible && p.prow.n_ranges > parts) parts = p.prow.n_ranges;
+  // the row walks always write partial volumes (their boundary rows are
   // added in the partial-sum pass)
-  p.partial_bytes =
-      (parts > 1 || p.quad.possible) ? (size_t)parts * X * Y * Z * sizeof(float) : 0;
+  p.partial_bytes = (parts > 1 || p.quad.possible || p.prow.possible)
+                        ? (size_t)parts * X * Y * Z * sizeof(float)
+                        : 0;
   p.ml = mirror_layout(N, gen.tables_bytes + p.partial_bytes);
   p.ws = p.ml.end;
   if (p.quad.possible) {
@@ -2879,6 +2985,14 @@ A1SymPlan plan_a1_sym(int64_t X, int64_t Y, int64_t Z, int64_t N, const A1Plan& 
                              255) & ~(size_t)255);
     // kq_stride(T) = T + 1 <= t_stride + 1
     p.ws = p.kq_off + (((size_t)(N / 4) * (gen.t_stride + 1) * sizeof(float4) + 255) &
+                       ~(size_t)255);
+  }
+  if (p.prow.possible) {
+    p.pbnd_off = p.ws;
+    p.kp_off = p.pbnd_off + (((size_t)p.prow.n_ranges * X * Y * p.prow.n_slabs * sizeof(float) +
+                              255) & ~(size_t)255);
+    // kq_stride(T) = T + 1 <= t_stride + 1
+    p.ws = p.kp_off + (((size_t)(N / 2) * (gen.t_stride + 1) * sizeof(float2) + 255) &
                        ~(size_t)255);
   }
   return p;
@@ -2965,7 +3079,9 @@ extern "C" int totri_volume_confocal_grad_volume(
   // the general path takes the row walk when rows are monotone
   const bool row = a1_row_geometry(volume_size, bin_width, bin_offset);
   float* row_bnd = (float*)((char*)workspace + a1_row_bnd_off(p, q));
-  const bool staged = p.n_ranges > 1 || (sym && q.n_ranges > 1) || quad || row;
+  // the x-mirror path takes the pair row walk under the same conditions
+  const bool prow = sym && q.prow.possible && row;
+  const bool staged = p.n_ranges > 1 || (sym && q.n_ranges > 1) || quad || row || prow;
   float* dst = staged ? partial : volume;
   if (sym) {
     rc = launch_mirror_detect(q.ml, measurement_points, (int)N, workspace, st);
@@ -3004,18 +3120,38 @@ extern "C" int totri_volume_confocal_grad_volume(
       rc = check_launch("a1_quad_kernel");
       if (rc) return rc;
     }
-    const int items = q.tiles_x * q.tiles_y * q.n_slabs * q.n_ranges;
-    const int blocks = (items + kA1SymWaves - 1) / kA1SymWaves;
-    if (p.safe)
-      hipLaunchKernelGGL(a1_sym_kernel<true>, dim3(blocks), dim3(kA1SymWaves * 64), 0, st, Kf,
-                         tc2, itc4, measurement_points, pairs, info, dst, g, q.tiles_x,
-                         q.tiles_y, q.n_slabs, q.n_ranges, p.t_stride, items, (int)quad);
-    else
-      hipLaunchKernelGGL(a1_sym_kernel<false>, dim3(blocks), dim3(kA1SymWaves * 64), 0, st, Kf,
-                         tc2, itc4, measurement_points, pairs, info, dst, g, q.tiles_x,
-                         q.tiles_y, q.n_slabs, q.n_ranges, p.t_stride, items, (int)quad);
-    rc = check_launch("a1_sym_kernel");
-    if (rc) return rc;
+    if (prow) {
+      const A1QuadPlan& pr = q.prow;
+      float2* Kp = (float2*)((char*)workspace + q.kp_off);
+      float* pbnd = (float*)((char*)workspace + q.pbnd_off);
+      hipLaunchKernelGGL(a1_pair_table_kernel,
+                         dim3(launch_grid((size_t)(N / 2) * kq_stride((int)T))), dim3(256), 0, st, Kf,
+                         pairs, info, (int)quad, p.t_stride, (int)T, (int)(N / 2), Kp);
+      const int items = pr.tiles_x * pr.tiles_y * pr.n_slabs * pr.n_ranges;
+      if (p.safe)
+        hipLaunchKernelGGL(a1_pair_kernel<true>, dim3(items), dim3(64), 0, st, Kp, tc2, itc4,
+                           measurement_points, pairs, info, dst, pbnd, g, pr.tiles_x,
+                           pr.tiles_y, pr.n_slabs, pr.n_ranges, p.t_stride, (int)quad);
+      else
+        hipLaunchKernelGGL(a1_pair_kernel<false>, dim3(items), dim3(64), 0, st, Kp, tc2, itc4,
+                           measurement_points, pairs, info, dst, pbnd, g, pr.tiles_x,
+                           pr.tiles_y, pr.n_slabs, pr.n_ranges, p.t_stride, (int)quad);
+      rc = check_launch("a1_pair_kernel");
+      if (rc) return rc;
+    } else {
+      const int items = q.tiles_x * q.tiles_y * q.n_slabs * q.n_ranges;
+      const int blocks = (items + kA1SymWaves - 1) / kA1SymWaves;
+      if (p.safe)
+        hipLaunchKernelGGL(a1_sym_kernel<true>, dim3(blocks), dim3(kA1SymWaves * 64), 0, st,
+                           Kf, tc2, itc4, measurement_points, pairs, info, dst, g, q.tiles_x,
+                           q.tiles_y, q.n_slabs, q.n_ranges, p.t_stride, items, (int)quad);
+      else
+        hipLaunchKernelGGL(a1_sym_kernel<false>, dim3(blocks), dim3(kA1SymWaves * 64), 0, st,
+                           Kf, tc2, itc4, measurement_points, pairs, info, dst, g, q.tiles_x,
+                           q.tiles_y, q.n_slabs, q.n_ranges, p.t_stride, items, (int)quad);
+      rc = check_launch("a1_sym_kernel");
+      if (rc) return rc;
+    }
   }
   const int* skip = info ? info + 2 : nullptr;
   if (row) {
@@ -3048,9 +3184,10 @@ extern "C" int totri_volume_confocal_grad_volume(
     // info[0] and info[2] say which path ran (no info: the general one)
     hipLaunchKernelGGL(sum_partials_a1_kernel, dim3(launch_grid(nv)), dim3(256), 0, st,
                        partial, sym ? (const float*)((char*)workspace + q.bnd_off) : nullptr,
-                       info, (int)quad, q.quad.n_ranges, q.n_ranges,
+                       info, (int)quad, q.quad.n_ranges, prow ? q.prow.n_ranges : q.n_ranges,
                        row ? p.row_ranges : p.n_ranges, nv, (int)Z, p.row_slabs,
-                       row ? row_bnd : nullptr, volume);
+                       row ? row_bnd : nullptr,
+                       prow ? (const float*)((char*)workspace + q.pbnd_off) : nullptr, volume);
     return check_launch("sum_partials_a1_kernel");
   }
   hipLaunchKernelGGL(sum_partials_kernel<float>, dim3(launch_grid(nv)), dim3(256), 0, st,
