@@ -587,6 +587,38 @@ def kernels_main(args):
     fm = flop_model(mp.detach().cpu().numpy(), 128, 1024, w)
     row("A6 scan-point gradient", "a6_sym_kernel", "128^3 voxels, 64x64 scan, 1024 bins", ms,
         fm["a_fwd"] * 128 ** 3 * 4096, "active samples", 30, 4 * (2 * 4096 * 1024 + 128 ** 3))
+    # The other A1 / R1 / A6 paths on the same workload: scans with x-pairs
+    # only (the grid moved 0.01 m in y, so no y-mirrors) and without mirror
+    # pairs (one point moved by an ulp)
+    from totrilib_amd.totri_cpp.render import (volume_confocal_forward,
+                                               volume_confocal_grad_measurement_points,
+                                               volume_confocal_grad_volume)
+    g1 = torch.empty(128, 128, 128, device=dev)
+    tr1 = torch.empty(4096, 1024, device=dev)
+    gm1 = torch.empty(3, 4096, device=dev)
+    Gd = Gc.detach().contiguous()
+    mp_x = wall_grid(64)
+    mp_x[1] += np.float32(0.01)
+    mp_g = wall_grid(64)
+    mp_g[0, 5] = np.nextafter(mp_g[0, 5], np.float32(2))
+    pairs = 128.0 ** 3 * 4096
+    for name, kern, mpn in (("A1 albedo gradient, x-pair scan", "a1_pair_kernel", mp_x),
+                            ("A1 albedo gradient, general scan", "a1_row_kernel", mp_g)):
+        m1 = torch.from_numpy(mpn).to(dev)
+        ms = timeit(lambda: volume_confocal_grad_volume(g1, m1, Gd, ORIGIN, SIZE, w, 0.0))
+        fm1 = flop_model(mpn, 128, 1024, w)
+        row(name, kern, "128^3 voxels, 64x64 scan, 1024 bins", ms, pairs,
+            "voxel-scan pairs", fm1["f_a1"], 4 * (4096 * 1024 + 128 ** 3))
+    m1 = torch.from_numpy(mp_g).to(dev)
+    fm1 = flop_model(mp_g, 128, 1024, w)
+    ms = timeit(lambda: volume_confocal_forward(alb, m1, tr1, ORIGIN, SIZE, w, 0.0))
+    row("R1 render, general scan", "r1_forward_kernel", "128^3 voxels, 64x64 scan, 1024 bins",
+        ms, pairs, "voxel-scan pairs", fm1["f_r1"], 4 * (4096 * 1024 + 128 ** 3))
+    ms = timeit(lambda: volume_confocal_grad_measurement_points(gm1, alb, m1, Gd, ORIGIN, SIZE,
+                                                                 w, 0.0))
+    row("A6 scan-point gradient, general scan", "a6_kernel", "128^3 voxels, 64x64 scan, 1024 bins",
+        ms, fm1["a_fwd"] * 128 ** 3 * 4096, "active samples", 30,
+        4 * (2 * 4096 * 1024 + 128 ** 3))
     # f-k migration: 1024 bins on a 64 x 64 wall; the spectrum is [2048, 64, 64] complex
     from totrilib_amd import _lib
     from totrilib_amd.reconstruct import FkMigration
