@@ -107,25 +107,48 @@ def mirror_info(torch, mp):
     return info.cpu().tolist()
 
 
+def cpu_model() -> str:
+    """The host CPU's model name (lscpu's "Model name", from /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_threads() -> int:
+    """Threads for the CPU leg: the host's cores, capped at the 16-core
+    share that one GPU's process gets on a shared host (OMP_NUM_THREADS, when
+    set, lowers it)."""
+    n = min(os.cpu_count() or 1, 16)
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return n
+
+
 def cpu_baseline(mp, vol, T, w):
     """Oracle FP32 restatement (same formulas, brute force as the reference
     kernels) on host cores: R1 on a seeded subset of scan points, A1 on a
     seeded subset of columns; work is linear in both, so the full-size time
-    is extrapolated from the subsets."""
+    is extrapolated from the subsets (about 10 s of CPU work)."""
     from oracle import oracle as O
-    threads = min(os.cpu_count() or 1, 16)
+    threads = cpu_threads()
     O.set_threads(threads)
     X, Y, Z = vol.shape
     N = mp.shape[1]
     rng = np.random.default_rng(0)
     # R1: rows
-    rows = min(N, 256)
+    rows = min(N, 1024)
     i0 = int(rng.integers(0, N - rows + 1))
     t0 = time.perf_counter()
     O.r1_forward(vol, mp, T, ORIGIN, SIZE, w, 0.0, mode=O.FP32, i0=i0, i1=i0 + rows)
     t_r1 = (time.perf_counter() - t0) * N / rows
     # A1: columns
-    ncols = min(X * Y, 256)
+    ncols = min(X * Y, 2048)
     cols = np.stack([rng.integers(0, X, ncols), rng.integers(0, Y, ncols)], axis=1)
     G = rng.standard_normal((N, T)).astype(np.float32)
     t0 = time.perf_counter()
@@ -133,9 +156,14 @@ def cpu_baseline(mp, vol, T, w):
     t_a1 = (time.perf_counter() - t0) * (X * Y) / ncols
     pairs = float(X * Y * Z) * N
     return dict(value=pairs / (t_r1 + t_a1), unit="pairs/s", cores=threads, kind="port",
-                sample=(f"oracle FP32 restatement, {threads} OpenMP threads: R1 on {rows} of "
-                        f"{N} scan points, A1 on {ncols} of {X * Y} columns, extrapolated "
-                        f"linearly (R1 {t_r1:.1f} s + A1 {t_a1:.1f} s per full step)"))
+                threads=threads, host_cores=os.cpu_count(), cpu_model=cpu_model(),
+                implementation="C/OpenMP port of the reference formulas (the oracle's "
+                               "FP32 mode), not PyTorch",
+                sample=(f"oracle FP32 restatement, {threads} OpenMP threads on "
+                        f"{os.cpu_count()} host cores ({cpu_model()}; one GPU's 16-core "
+                        f"share): R1 on {rows} of {N} scan points, A1 on {ncols} of "
+                        f"{X * Y} columns, extrapolated linearly (R1 {t_r1:.1f} s + A1 "
+                        f"{t_a1:.1f} s per full step)"))
 
 
 def main():
@@ -166,6 +194,30 @@ def main():
         confocal_main(args)
 
 
+def general_scan(mp_np: np.ndarray) -> np.ndarray:
+    """The scan with one point moved by one ulp in x: no mirror quads or
+    pairs remain, so every kernel takes its general form (the geometry of a
+    real capture, an off-centre box or a jittered scan)."""
+    mp = mp_np.copy()
+    mp[0, 5] = np.nextafter(mp[0, 5], np.float32(2))
+    return mp
+
+
+def kernel_names(info):
+    r1 = "r1_sym_kernel" if info[0] else "r1_forward_kernel"
+    a1 = ("a1_quad_kernel" if info[0] else
+          "a1_pair_kernel" if info[2] else "a1_row_kernel")
+    return r1, a1
+
+
+def load_profile(name):
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        return json.load(f)
+
+
 def confocal_main(args):
     import torch
     from totrilib_amd.totri_cpp.render import (volume_confocal_forward,
@@ -176,76 +228,114 @@ def confocal_main(args):
     T, res = WORKLOAD["T"], WORKLOAD["res"]
     w = WORKLOAD["w_round_trip"] / 2  # one-way radius per bin
     mp_np, vol_np, tgt_np = make_inputs(torch, dev)
-    mp = torch.from_numpy(mp_np).to(dev)
     vol = torch.from_numpy(vol_np).to(dev)
-    N = mp.shape[1]
-    target = torch.empty((N, T), device=dev)
-    volume_confocal_forward(torch.from_numpy(tgt_np).to(dev), mp, target, ORIGIN, SIZE, w, 0.0)
+    tgt = torch.from_numpy(tgt_np).to(dev)
+    N = mp_np.shape[1]
     transient = torch.empty((N, T), device=dev)
     G = torch.empty((N, T), device=dev)
     grad = torch.empty((res, res, res), device=dev)
 
-    def step(ev=None):
-        if ev is not None:
-            ev[0].record()
-        volume_confocal_forward(vol, mp, transient, ORIGIN, SIZE, w, 0.0)
-        if ev is not None:
-            ev[1].record()
-        torch.sub(transient, target, out=G)
-        G.mul_(2.0)
-        if ev is not None:
-            ev[2].record()
-        volume_confocal_grad_volume(grad, mp, G, ORIGIN, SIZE, w, 0.0)
-        if ev is not None:
-            ev[3].record()
+    def run_leg(mp_leg, steps, warmup):
+        """Times `steps` steps (after `warmup`) on the scan mp_leg; returns the
+        wall time and per-step event times of the step, R1 and A1."""
+        mp = torch.from_numpy(mp_leg).to(dev)
+        target = torch.empty((N, T), device=dev)
+        volume_confocal_forward(tgt, mp, target, ORIGIN, SIZE, w, 0.0)
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        step(evs[k])
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    step_ms = np.array([e[0].elapsed_time(e[3]) for e in evs])
-    r1_ms = np.array([e[0].elapsed_time(e[1]) for e in evs])
-    a1_ms = np.array([e[2].elapsed_time(e[3]) for e in evs])
+        def step(ev=None):
+            if ev is not None:
+                ev[0].record()
+            volume_confocal_forward(vol, mp, transient, ORIGIN, SIZE, w, 0.0)
+            if ev is not None:
+                ev[1].record()
+            torch.sub(transient, target, out=G)
+            G.mul_(2.0)
+            if ev is not None:
+                ev[2].record()
+            volume_confocal_grad_volume(grad, mp, G, ORIGIN, SIZE, w, 0.0)
+            if ev is not None:
+                ev[3].record()
+
+        for _ in range(warmup):
+            step()
+        torch.cuda.synchronize()
+        evs = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(steps)]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(steps):
+            step(evs[k])
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        step_ms = np.array([e[0].elapsed_time(e[3]) for e in evs])
+        r1_ms = np.array([e[0].elapsed_time(e[1]) for e in evs])
+        a1_ms = np.array([e[2].elapsed_time(e[3]) for e in evs])
+        return wall, step_ms, r1_ms, a1_ms, mirror_info(torch, mp)
 
     pairs = float(res ** 3) * N
-    fm = flop_model(mp_np, res, T, w)
-    value = pairs * args.steps / wall
-    r1_avg, a1_avg = float(r1_ms.mean()), float(a1_ms.mean())
-    # which kernels ran: the mirror-symmetric ones when the scan points form
-    # mirror quads / pairs (the box is centred at x = y = 0; with w > 0 and
-    # offset 0 the A1 quad walk applies)
-    info = mirror_info(torch, mp)
-    r1_name = "r1_sym_kernel" if info[0] else "r1_forward_kernel"
-    a1_name = ("a1_quad_kernel" if info[0] else
-               "a1_sym_kernel" if info[2] else "a1_transpose_kernel")
-    if r1_avg >= a1_avg:
-        dom, flops, dur = r1_name, fm["f_r1"] * pairs, r1_avg
-    else:
-        dom, flops, dur = a1_name, fm["f_a1"] * pairs, a1_avg
-    achieved = flops / (dur * 1e-3)
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "traffic.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            traffic = json.load(f).get(dom)
-    # VALU issue: the dominant kernel's wave VALU instructions per dispatch
-    # (rocprof, same workload) at 4 cycles each on 1024 SIMDs at 2.4 GHz,
-    # against the measured call duration
-    valu_issue = None
-    vpath = os.path.join(ROOT, "profiles", "valu.json")
-    if os.path.exists(vpath):
-        with open(vpath) as f:
-            instr = json.load(f).get(dom)
+    executed = load_profile("valu_flops.json")
+    traffic_p = load_profile("traffic.json")
+    issue_p = load_profile("valu.json")
+
+    def roofline(mp_leg, info, r1_avg, a1_avg):
+        """Roofline of the leg's dominant kernel: executed FP32 FLOP per
+        dispatch (rocprof VALU counters of the same workload) over the
+        event-timed call, and the reference formulation's algorithmic FLOP
+        over the same time."""
+        fm = flop_model(mp_leg, res, T, w)
+        r1_name, a1_name = kernel_names(info)
+        if r1_avg >= a1_avg:
+            dom, alg, dur = r1_name, fm["f_r1"] * pairs, r1_avg
+        else:
+            dom, alg, dur = a1_name, fm["f_a1"] * pairs, a1_avg
+        alg_rate = alg / (dur * 1e-3)
+        ex = executed.get(dom)
+        ex_rate = ex / (dur * 1e-3) if ex else None
+        issue = None
+        instr = issue_p.get(dom)
         if instr:
             issue_ms = instr * 4 / (1024 * SIMD_CLOCK_HZ) * 1e3
-            valu_issue = {"wave_instructions": instr, "issue_ms": issue_ms, "call_ms": dur,
-                          "frac": issue_ms / dur}
+            issue = {"wave_instructions": instr, "issue_ms": issue_ms, "call_ms": dur,
+                     "frac": issue_ms / dur}
+        return fm, {
+            "bound": "valu", "kernel": dom,
+            "achieved": ex_rate / 1e12 if ex_rate else None,
+            "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
+            "frac": ex_rate / PEAK_FP32 if ex_rate else None,
+            "traffic": traffic_p.get(dom),
+            "executed_flop_per_launch": ex,
+            "algorithmic_flop_per_launch": alg,
+            "algorithmic_achieved": alg_rate / 1e12,
+            "algorithmic_frac": alg_rate / PEAK_FP32,
+            "valu_issue": issue,
+            "note": "achieved/frac: FP32 FLOP the kernel executes per launch (rocprof "
+                    "SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F32, 64 lanes x (ADD + MUL + 2 FMA + "
+                    "TRANS), profiles/valu_flops.json) over the event-timed call; "
+                    "algorithmic_*: the reference formulation's FLOP (18 per active R1 "
+                    "sample, 17 per A1 window bin + 23 per pair) over the same time, which "
+                    "the mirror kernels exceed by sharing one decision among four samples; "
+                    "valu_issue: wave VALU instructions x 4 cycles over the call"}
+
+    wall, step_ms, r1_ms, a1_ms, info = run_leg(mp_np, args.steps, args.warmup)
+    value = pairs * args.steps / wall
+    r1_avg, a1_avg = float(r1_ms.mean()), float(a1_ms.mean())
+    fm, roof = roofline(mp_np, info, r1_avg, a1_avg)
+    r1_name, a1_name = kernel_names(info)
+
+    # the same step on a scan without mirror symmetry (general kernels)
+    mp_g = general_scan(mp_np)
+    g_steps = max(1, min(args.steps, 10))
+    g_wall, g_step, g_r1, g_a1, g_info = run_leg(mp_g, g_steps, 1)
+    _, g_roof = roofline(mp_g, g_info, float(g_r1.mean()), float(g_a1.mean()))
+    g_r1n, g_a1n = kernel_names(g_info)
+    general = {"value": pairs * g_steps / g_wall, "unit": "pairs/s", "steps": g_steps,
+               "ms_per_step": g_wall * 1e3 / g_steps,
+               "p50_ms": float(np.percentile(g_step, 50)),
+               "p99_ms": float(np.percentile(g_step, 99)),
+               "kernels_ms": {"r1_forward_avg": float(g_r1.mean()),
+                              "a1_backward_avg": float(g_a1.mean()),
+                              "r1_kernel": g_r1n, "a1_kernel": g_a1n},
+               "roofline": g_roof,
+               "scan": "64x64 wall grid with point 5 moved one ulp in x (no mirror pairs)"}
 
     if args.copy_probe:
         src = torch.ones(64 * 1024 * 1024, device=dev)
@@ -280,17 +370,8 @@ def confocal_main(args):
         "kernels_ms": {"r1_forward_avg": r1_avg, "a1_backward_avg": a1_avg,
                        "r1_kernel": r1_name, "a1_kernel": a1_name},
         "flop_model": fm,
-        "roofline": {"bound": "valu", "kernel": dom, "achieved": achieved / 1e12,
-                     "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32, "traffic": traffic,
-                     "valu_issue": valu_issue,
-                     "note": "achieved counts the reference formulation's FLOP (18 per "
-                             "active R1 sample, 17 per A1 window bin + 23 per pair); the "
-                             "mirror kernels share one decision among four samples and "
-                             "execute about a quarter of that (rocprof VALU counts in "
-                             "profiles/confocal_128_64x64_1024_valu.md), so frac can "
-                             "exceed 1 without the VALU issuing above its peak; valu_issue "
-                             "gives the executed instructions' share of the call"},
+        "roofline": roof,
+        "general_geometry": general,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -387,6 +387,21 @@ int totri_debug_mirror_info(const float* measurement_points, int64_t N,
                             void* workspace, size_t workspace_bytes,
                             int32_t* info_out, void* stream);
 
+/*
+ * Testing aid with no counterpart in the reference: the A1 walks that the
+ * planner of totri_volume_confocal_grad_volume (and of the batched step)
+ * allows for a geometry, computed on the host without device work.  Bits:
+ *   1  x-mirror paths (X even, N even, grid antisymmetric in x)
+ *   2  xy-mirror row walk (also antisymmetric in y, rows monotone in the bin)
+ *   4  general row walk (bin_width > 0, bin_offset >= 0, z size > 0 and
+ *      T < 2^24 - 1: the walks count bins in a float)
+ *   8  x-pair row walk (bits 1 and 4)
+ * Returns a negative status code for invalid sizes.
+ */
+int totri_debug_a1_walks(int64_t X, int64_t Y, int64_t Z, int64_t N, int64_t T,
+                         const float volume_origin[3], const float volume_size[3],
+                         float bin_width, float bin_offset);
+
 #ifdef __cplusplus
 }
 #endif

@@ -134,3 +134,25 @@ def test_install_as_totri_aliases_reference_modules():
     from totri_cpp.render import volume_confocal_forward  # noqa: F401
     from totri_cpp.util import rbf2volume_gaussian_grad  # noqa: F401
     assert sys.modules["totri_cpp.reconstruct"].backprojection_confocal_velten_forward
+
+
+def test_a1_row_walks_are_gated_by_the_float_bin_counter():
+    """The A1 row walks count bins in a float, exact below 2^24; longer
+    transients must take the window walks (host-side plan, no GPU)."""
+    from totrilib_amd import _lib
+    lib = _lib.load()
+    o, sz = _lib.f3((-1, -1, 0.5)), _lib.f3((2, 2, 2))
+
+    def walks(T, N=4, w=0.0025, off=0.0, X=8, Y=8, origin=o):
+        return lib.totri_debug_a1_walks(X, Y, 8, N, T, origin, sz, w, off)
+
+    assert walks(1024) == 1 | 2 | 4 | 8
+    big = (1 << 24) + 16
+    assert walks(big) == 1                 # x-mirror window walk only
+    assert walks((1 << 24) - 2) == 15
+    assert walks((1 << 24) - 1) == 1
+    assert walks(1024, off=-0.1) == 1      # rows not monotone
+    assert walks(1024, N=3) == 4           # no pairs: general row walk
+    assert walks(big, N=3) == 0            # general window walk
+    assert walks(1024, origin=_lib.f3((-1, -0.9, 0.5))) == 1 | 4 | 8  # y not antisymmetric
+    assert walks(1024, X=7) == 4

@@ -61,6 +61,64 @@ def test_velten_exhaustive_matches_oracle():
     assert np.array_equal(got.astype(np.float64), ref), rel_l2(got, ref)
 
 
+def test_velten_exhaustive_laser_origin_matches_oracle():
+    """Both origin terms of A3 (backprojection_cuda.cu:268-279): a per-point
+    scan origin and a per-point laser origin, then each alone."""
+    from totrilib_amd.reconstruct import BackprojectionExhaustiveVelten
+    from totrilib_amd.types import BinDef, VolumeDef
+    rng = np.random.default_rng(56)
+    B, T, S, L = 2, 400, 5, 3
+    res = (18, 16, 20)
+    tr = rng.random((B, T, S, L)).astype(np.float32)
+    scan = np.concatenate([rng.uniform(-1, 1, (B, S, 2)), np.zeros((B, S, 1))], 2).astype(np.float32)
+    laser = np.concatenate([rng.uniform(-1, 1, (B, L, 2)), np.zeros((B, L, 1))], 2).astype(np.float32)
+    so = (scan + rng.uniform(-0.3, 0.3, scan.shape) - [0, 0, 1.0]).astype(np.float32)
+    lo = (laser + rng.uniform(-0.3, 0.3, laser.shape) - [0, 0, 0.6]).astype(np.float32)
+    vd, bd = VolumeDef(VS, VE, res), BinDef(0.015, 0.2, T)
+    for sorg, lorg in ((so, lo), (None, lo), (so, None)):
+        got = BackprojectionExhaustiveVelten.apply(
+            t(tr), t(scan), t(laser), vd, bd, None if sorg is None else t(sorg),
+            None if lorg is None else t(lorg)).cpu().numpy()
+        ref = O.velten_exhaustive(tr, scan, laser, sorg, lorg, res, VS, VE, 0.015, 0.2,
+                                  mode=O.FP32)
+        assert np.count_nonzero(ref) > 1000
+        assert np.array_equal(got.astype(np.float64), ref), rel_l2(got, ref)
+
+
+def test_arellano_with_origins_matches_oracle():
+    """Arellano with scan != laser and both origins (tc -= d_so + d_lo,
+    backprojection_cuda.cu:38-54,79-80), then each origin alone and an origin
+    given per batch ([B, 3]).  Hit counts bit for bit with a transient of
+    ones; sums to rounding."""
+    from totrilib_amd.reconstruct import BackprojectionExhaustiveArellano
+    from totrilib_amd.types import BinDef, VolumeDef
+    rng = np.random.default_rng(57)
+    B, T, S, L = 2, 256, 4, 3
+    res = (40, 18, 20)
+    tr = (rng.random((B, T, S, L)) * (rng.random((B, T, S, L)) < 0.3)).astype(np.float32)
+    scan = np.concatenate([rng.uniform(-1, 1, (B, 2, S)), np.zeros((B, 1, S))], 1).astype(np.float32)
+    laser = np.concatenate([rng.uniform(-1, 1, (B, 2, L)), np.zeros((B, 1, L))], 1).astype(np.float32)
+    so = (scan + rng.uniform(-0.2, 0.2, scan.shape) - np.array([0, 0, 0.4])[None, :, None]).astype(np.float32)
+    lo = (laser + rng.uniform(-0.2, 0.2, laser.shape) - np.array([0, 0, 0.3])[None, :, None]).astype(np.float32)
+    so_b = np.array([[0.0, 0.0, -0.5], [0.1, -0.1, -0.45]], np.float32)
+    vd, bd = VolumeDef(VS, VE, res), BinDef(0.025, 0.8, T)
+    ones = np.ones_like(tr)
+    cases = ((so, lo, so, lo), (None, lo, None, lo), (so, None, so, None),
+             (so_b, lo, np.broadcast_to(so_b[:, :, None], scan.shape), lo))
+    for sorg, lorg, sref, lref in cases:
+        args = (None if sorg is None else t(sorg), None if lorg is None else t(lorg))
+        got1 = BackprojectionExhaustiveArellano.apply(t(ones), t(scan), t(laser), vd, bd,
+                                                      *args).cpu().numpy()
+        ref1 = O.arellano(ones, scan, laser, sref, lref, res, VS, VE, 0.025, 0.8, mode=O.FP32)
+        assert np.count_nonzero(ref1) > 100
+        assert np.array_equal(got1.astype(np.float64), ref1)
+        got = BackprojectionExhaustiveArellano.apply(t(tr), t(scan), t(laser), vd, bd,
+                                                     *args).cpu().numpy()
+        ref = O.arellano(tr, scan, laser, sref, lref, res, VS, VE, 0.025, 0.8, mode=O.FP32)
+        assert np.array_equal(got != 0, ref != 0)
+        assert rel_l2(got, ref) < 1e-6
+
+
 @pytest.mark.parametrize("B,res,S,L", [(1, (40, 20, 24), 4, 3), (2, (300, 4, 6), 5, 2)])
 def test_arellano_matches_oracle(B, res, S, L):
     """Voxel decisions bit for bit: with a transient of ones every voxel holds

@@ -538,3 +538,38 @@ def test_a1_mirror_pairs_and_general_path(scan):
     print(f"{scan}: mirror info {info}, rel-L2 vs FP64 pinned {e:.2e}")
     assert e < TOL
     assert np.count_nonzero(got) > 0
+
+
+@pytest.mark.parametrize("scan", ["quads", "random"])
+def test_a1_transient_longer_than_float_bin_counter(scan):
+    """T = 2^24 + 64 bins: the row walks count bins in a float, which stops
+    advancing at 2^24, so such transients take the window walks (x-mirror for
+    the quad scan, general for the random one).  The gradient is nonzero only
+    in bins near the top of the range, where the geometry puts samples in the
+    volume (bin width 2.6 m / 2^24), and matches the oracle's exact
+    transpose."""
+    from totrilib_amd import _lib
+    X, Y, Z = 4, 4, 8
+    T = (1 << 24) + 64
+    w = 2.6 / (1 << 24)
+    rng = np.random.default_rng(77)
+    if scan == "quads":
+        mp = wall_grid(2) * np.float32(0.5)
+    else:
+        mp = np.stack([rng.uniform(-0.5, 0.5, 3), rng.uniform(-0.5, 0.5, 3),
+                       np.zeros(3)]).astype(np.float32)
+    lib = _lib.load()
+    walks = lib.totri_debug_a1_walks(X, Y, Z, mp.shape[1], T, _lib.f3(ORIGIN), _lib.f3(SIZE),
+                                     w, 0.0)
+    assert walks == (1 if scan == "quads" else 0)
+    G = np.zeros((mp.shape[1], T), np.float32)
+    for i in range(mp.shape[1]):
+        hot = rng.integers(T - 3_000_000, T, 3000)
+        G[i, hot] = rng.random(3000).astype(np.float32)
+    print(f"{scan}: N={mp.shape[1]} T={T}", flush=True)
+    got = gpu_a1(torch.from_numpy(G), mp, (X, Y, Z), w, 0.0)
+    ref = O.a1_transpose(G, mp, X, Y, Z, ORIGIN, SIZE, w, 0.0, mode=O.PINNED64)
+    e = rel_l2(got, ref)
+    print(f"{scan}: rel-L2 vs FP64 pinned {e:.2e}, nonzero {np.count_nonzero(ref)}")
+    assert np.count_nonzero(ref) > 20
+    assert e < TOL

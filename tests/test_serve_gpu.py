@@ -182,3 +182,62 @@ def test_shortest_job_first_order():
             assert ts.t_done < tb.t_done
         else:
             assert eng.jobs_per_step == [2]
+
+
+def test_resident_inputs_wait_for_the_submitting_stream():
+    """A resident input produced by asynchronous work on the caller's current
+    stream right before submit() (and a non-contiguous resident view) is read
+    only after that work: the lane's stream waits on an event recorded at
+    submit.  The long matrix chain keeps the producing kernel queued while
+    the step would otherwise start."""
+    from totrilib_amd.serve import BackprojectRequest, RenderRequest, ServingEngine
+    eng = ServingEngine(device_budget=1 << 30, host_budget=256 << 20, lanes=2)
+    rng = np.random.default_rng(8)
+    res, n, T, w = (48, 40, 44), 16, 600, 0.004
+    vol_np = sparse_albedo(res, 0.05, 81)
+    G_np = rng.standard_normal((n * n, T)).astype(np.float32)
+    mp_np = wall_grid(n)
+    reqs_ref = [RenderRequest(torch.from_numpy(vol_np), torch.from_numpy(mp_np), ORIGIN, SIZE,
+                              T, w, 0.0),
+                BackprojectRequest(torch.from_numpy(G_np), torch.from_numpy(mp_np), ORIGIN,
+                                   SIZE, res, w, 0.0)]
+    want = [single_request(r) for r in reqs_ref]
+    for trial in range(3):
+        a = torch.randn(3072, 3072, device=dev())
+        for _ in range(12):
+            a = torch.tanh(a @ a * (1.0 / 3072))
+        z = a[0, 0] * 0.0
+        vol = torch.from_numpy(vol_np).to(dev()) + z      # ready only after the chain
+        G = torch.from_numpy(G_np).to(dev()) + z
+        mp_t = (torch.from_numpy(np.ascontiguousarray(mp_np.T)).to(dev()) + z).T  # [3, N] view
+        assert not mp_t.is_contiguous()
+        t1 = eng.submit(RenderRequest(vol, mp_t, ORIGIN, SIZE, T, w, 0.0))
+        t2 = eng.submit(BackprojectRequest(G, mp_t, ORIGIN, SIZE, res, w, 0.0))
+        del vol, G, mp_t, a
+        eng.drain()
+        got = collect([t1, t2])
+        for k in range(2):
+            assert torch.equal(got[k], want[k]), (trial, k)
+
+
+def test_drain_raises_when_unreleased_results_fill_the_result_arena():
+    """Results held by the caller fill the pinned arena: the next request can
+    never start, so drain() raises instead of spinning; after release() the
+    queue drains."""
+    from totrilib_amd.serve import RenderRequest, ServingEngine
+    n, T = 16, 1000
+    mp = torch.from_numpy(wall_grid(n))
+    vol = torch.from_numpy(sparse_albedo((24, 24, 24), 0.05, 82))
+    out_bytes = n * n * T * 4
+    eng = ServingEngine(device_budget=512 << 20, host_budget=out_bytes + 4096, lanes=2)
+    t1 = eng.submit(RenderRequest(vol, mp, ORIGIN, SIZE, T, 0.003, 0.0))
+    eng.drain()
+    assert t1.done and t1.error is None
+    t2 = eng.submit(RenderRequest(vol, mp, ORIGIN, SIZE, T, 0.003, 0.0))
+    with pytest.raises(RuntimeError, match="unreleased results"):
+        eng.drain()
+    assert eng.stalled() and not t2.done
+    first = t1.result().clone()
+    t1.release()
+    eng.drain()
+    assert torch.equal(collect([t2])[0], first)

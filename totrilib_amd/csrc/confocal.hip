@@ -2857,10 +2857,13 @@ A1Plan plan_a1(int64_t X, int64_t Y, int64_t Z, int64_t N, int64_t T, float w,
   return p;
 }
 
-// The row walks need rows that never decrease with the bin index.
-bool a1_row_geometry(const float* size, float w, float off) {
-  return w > 0.f && isfinite(w) && off >= 0.f && isfinite(off) && size[2] > 0.f &&
-         isfinite(size[2]);
+// The row walks need rows that never decrease with the bin index.  They
+// count bins in a float (tc = t w + off without a conversion), which is exact
+// only below 2^24, so longer transients take the window walks.
+constexpr int64_t kRowWalkMaxBins = (1 << 24) - 2;
+bool a1_row_geometry(const float* size, int64_t T, float w, float off) {
+  return T <= kRowWalkMaxBins && w > 0.f && isfinite(w) && off >= 0.f && isfinite(off) &&
+         size[2] > 0.f && isfinite(size[2]);
 }
 
 }  // namespace
@@ -2924,9 +2927,9 @@ A1QuadPlan plan_a1_pair_rows(int64_t X, int64_t Y, int64_t Z, int64_t N) {
 
 // The xy-mirror walk needs each lane's row to grow with the bin index:
 // tc = t w + off >= 0 for every bin and a positive z scale.
-bool a1_quad_geometry(const float* origin, const float* size, int64_t Y, float w, float off) {
-  return w > 0.f && isfinite(w) && off >= 0.f && isfinite(off) && size[2] > 0.f &&
-         isfinite(size[2]) && axis_antisymmetric(origin[1], size[1], Y);
+bool a1_quad_geometry(const float* origin, const float* size, int64_t Y, int64_t T, float w,
+                      float off) {
+  return a1_row_geometry(size, T, w, off) && axis_antisymmetric(origin[1], size[1], Y);
 }
 
 struct A1SymPlan {
@@ -2998,6 +3001,25 @@ A1SymPlan plan_a1_sym(int64_t X, int64_t Y, int64_t Z, int64_t N, const A1Plan& 
   return p;
 }
 
+// Which A1 walks a geometry allows; the device's mirror detection then picks
+// among the mirror ones.
+struct A1Walks {
+  bool sym;   // x-mirror paths: sizes fit and the grid is antisymmetric in x
+  bool quad;  // xy-mirror row walk: also antisymmetric in y, monotone rows
+  bool row;   // general row walk: monotone rows, T below the float-counter limit
+  bool prow;  // x-pair row walk: x-mirror sizes and the row walk's conditions
+};
+
+A1Walks choose_a1_walks(const A1SymPlan& q, int64_t X, int64_t Y, int64_t T,
+                        const float* origin, const float* size, float w, float off) {
+  A1Walks k;
+  k.sym = q.possible && axis_antisymmetric(origin[0], size[0], X);
+  k.quad = k.sym && q.quad.possible && a1_quad_geometry(origin, size, Y, T, w, off);
+  k.row = a1_row_geometry(size, T, w, off);
+  k.prow = k.sym && q.prow.possible && k.row;
+  return k;
+}
+
 // Offset of the general row walk's boundary rows: after everything else.
 size_t a1_row_bnd_off(const A1Plan& p, const A1SymPlan& q) {
   const size_t gen = p.tables_bytes + p.partial_bytes;
@@ -3040,12 +3062,9 @@ extern "C" int totri_volume_confocal_grad_volume(
                   "null transient or measurement points");
   const A1Plan p = plan_a1(X, Y, Z, N, T, bin_width, bin_offset);
   const A1SymPlan q = plan_a1_sym(X, Y, Z, N, p);
-  // The x-mirror path needs the voxel grid antisymmetric in x; the scan
-  // pairs are then looked for on the device.
-  const bool sym = q.possible && axis_antisymmetric(volume_origin[0], volume_size[0], X);
-  // The xy-mirror path also needs the y antisymmetry and a monotone walk.
-  const bool quad = sym && q.quad.possible &&
-                    a1_quad_geometry(volume_origin, volume_size, Y, bin_width, bin_offset);
+  const A1Walks wk = choose_a1_walks(q, X, Y, T, volume_origin, volume_size, bin_width,
+                                     bin_offset);
+  const bool sym = wk.sym, quad = wk.quad;
   const size_t need = a1_workspace(X, Y, Z, N, T);
   if (workspace_bytes < need)
     return fail(TOTRI_EWORKSPACE, "workspace too small: need %zu bytes, got %zu",
@@ -3076,11 +3095,9 @@ extern "C" int totri_volume_confocal_grad_volume(
   rc = check_launch("a1_tables_kernel");
   if (rc) return rc;
   const int* info = nullptr;
-  // the general path takes the row walk when rows are monotone
-  const bool row = a1_row_geometry(volume_size, bin_width, bin_offset);
+  const bool row = wk.row;
   float* row_bnd = (float*)((char*)workspace + a1_row_bnd_off(p, q));
-  // the x-mirror path takes the pair row walk under the same conditions
-  const bool prow = sym && q.prow.possible && row;
+  const bool prow = wk.prow;
   const bool staged = p.n_ranges > 1 || (sym && q.n_ranges > 1) || quad || row || prow;
   float* dst = staged ? partial : volume;
   if (sym) {
@@ -3439,6 +3456,22 @@ extern "C" int totri_volume_confocal_grad_measurement_points(
   hipLaunchKernelGGL(sum_partials_kernel<float>, dim3(launch_grid(n)), dim3(256), 0, st, part,
                      p.passes * p.r.n_groups, n, measurement_points_grad);
   return check_launch("sum_partials_kernel");
+}
+
+// Host-side: the A1 walks the planner allows for a geometry, as bits
+// (1 x-mirror paths, 2 xy-mirror row walk, 4 general row walk, 8 x-pair row
+// walk); no device work.
+extern "C" int totri_debug_a1_walks(int64_t X, int64_t Y, int64_t Z, int64_t N, int64_t T,
+                                    const float volume_origin[3], const float volume_size[3],
+                                    float bin_width, float bin_offset) {
+  int rc = check_common(X, Y, Z, N, T, volume_origin, volume_size, bin_width, bin_offset,
+                        TOTRI_F32);
+  if (rc) return -rc;
+  const A1Plan p = plan_a1(X, Y, Z, N, T, bin_width, bin_offset);
+  const A1SymPlan q = plan_a1_sym(X, Y, Z, N, p);
+  const A1Walks k = choose_a1_walks(q, X, Y, T, volume_origin, volume_size, bin_width,
+                                    bin_offset);
+  return (k.sym ? 1 : 0) | (k.quad ? 2 : 0) | (k.row ? 4 : 0) | (k.prow ? 8 : 0);
 }
 
 extern "C" size_t totri_debug_mirror_workspace(int64_t N) {

@@ -128,6 +128,10 @@ class Ticket:
     cost: float = 0.0
     _host_out: Optional[torch.Tensor] = None
     _release: Optional[object] = None
+    # device-resident inputs made contiguous at submit, and an event recorded
+    # on the submitting stream after them: the step's stream waits on it
+    _resident: Optional[dict] = None
+    _ready: Optional[torch.cuda.Event] = None
 
     @property
     def done(self) -> bool:
@@ -220,9 +224,22 @@ class ServingEngine:
 
     # -- submission -------------------------------------------------------
     def submit(self, request, t_submit: Optional[float] = None) -> Ticket:
+        """Queue a request.  Inputs already on the device are used in place;
+        work that produces them on the caller's current stream (of the
+        engine's device) before submit() is complete before the request's
+        step reads them."""
         self._validate(request)
         tk = Ticket(request, self._next_id, time.perf_counter() if t_submit is None else t_submit)
         tk.cost = request_cost(request)
+        data = request.volume if isinstance(request, RenderRequest) else request.transient
+        resident = {}
+        for name, t in (("measurement_points", request.measurement_points), ("data", data)):
+            if t.is_cuda:
+                resident[name] = t.contiguous()
+        if resident:
+            tk._resident = resident
+            tk._ready = torch.cuda.Event()
+            tk._ready.record(torch.cuda.current_stream(self.device))
         self._next_id += 1
         self.queue.append(tk)
         return tk
@@ -305,9 +322,9 @@ class ServingEngine:
         data = req.volume if isinstance(req, RenderRequest) else req.transient
         k = 0
         placed = []
-        for t in (req.measurement_points, data):
+        for name, t in (("measurement_points", req.measurement_points), ("data", data)):
             if t.is_cuda:
-                placed.append(t.contiguous())
+                placed.append(tk._resident[name])
             else:
                 dst = self.dev.view(blocks[k], t.shape)
                 job.copies_in.append((dst, t))
@@ -360,6 +377,9 @@ class ServingEngine:
         # upload; the lane is idle here, so that upload has completed.
         with torch.cuda.stream(st):
             for j in jobs:
+                if j.ticket._ready is not None:
+                    st.wait_event(j.ticket._ready)
+            for j in jobs:
                 for dst, src in j.copies_in:
                     dst.copy_(src, non_blocking=True)
             if self.use_graphs:
@@ -395,6 +415,8 @@ class ServingEngine:
             hb = j.host_block
             tk._release = (lambda hb=hb: self.host.arena.free(hb))
             tk.t_done = now
+            tk._resident = None
+            tk._ready = None
         lane.jobs = []
         lane.event = None
 
@@ -417,8 +439,26 @@ class ServingEngine:
         running = sum(len(l.jobs) for l in self.lanes)
         return len(self.queue) + len(self.admitted) + running
 
+    def stalled(self) -> bool:
+        """True when queued requests can never start without the caller:
+        nothing runs and nothing is admitted, so the device arena is empty,
+        yet admission took nothing -- the pinned result arena is held by
+        results the caller has not released."""
+        return (bool(self.queue) and not self.admitted
+                and all(l.event is None for l in self.lanes))
+
+    def _raise_stalled(self) -> None:
+        raise RuntimeError(
+            f"{len(self.queue)} queued request(s) cannot start: the pinned result arena "
+            f"({self.host.arena.used} of {self.host.arena.capacity} bytes in use) is held by "
+            "unreleased results; release() finished tickets or raise host_budget")
+
     def drain(self, sleep: float = 1e-4) -> None:
+        """Serve until nothing is queued or running.  Raises RuntimeError when
+        the queue is stalled on unreleased results (see stalled())."""
         while self.poll():
+            if self.stalled():
+                self._raise_stalled()
             time.sleep(sleep)
 
     def run(self, requests: Sequence, arrivals: Sequence[float], sleep: float = 1e-4,
@@ -445,5 +485,7 @@ class ServingEngine:
                     if tk is not None and tk.done and tk._release is not None:
                         on_done(tk)
                         tk.release()
+            if self.stalled():
+                self._raise_stalled()
             time.sleep(sleep)
         return tickets  # type: ignore[return-value]
