@@ -29,13 +29,18 @@ import pytest
 import torch
 
 from oracle import oracle as O
-from tests.cases import ORIGIN, SIZE, rel_l2, wall_grid
+from tests.cases import ORIGIN, SIZE, wall_grid
 
 pytestmark = pytest.mark.gpu
 
 RES = (128, 128, 128)
 T, W = 1024, 0.0025
 WIN = 2e-3
+# bound on |alpha_gpu - alpha_oracle|: outside the guard the kernels take
+# alpha from the fast z_hat, which the guard's proven bound delta keeps within
+# 2^-22 * B of the exact one (about 3.4e-4 at this geometry); typical
+# differences are an ulp or two of z_hat
+ALPHA_TOL = 4e-4
 
 
 def dev():
@@ -149,7 +154,9 @@ def test_a1_walk_decisions_one_hot(name):
         flips += int(outside.sum()) + int(missing.sum())
         assert not outside.any(), np.argwhere(outside)[:5]
         assert not missing.any(), np.argwhere(missing)[:5]
-        assert rel_l2(got, want) < 1e-5
+        # one sample per voxel: the weights carry that sample's alpha, whose
+        # fast-path rounding is not averaged over scan points as in a sum
+        assert np.abs(got - want).max() <= ALPHA_TOL * want.max()
         # the near-boundary column itself: exactly the oracle's pair
         nz = np.flatnonzero(got[xi, yi] != 0)
         assert len(nz) >= 1 and set(nz) <= {zi0, zi0 + 1}, (nz, zi0)
@@ -176,4 +183,6 @@ def test_r1_walk_decisions_one_hot(name):
         assert np.count_nonzero(ref) > 100
         diff = (got != 0) != (ref != 0)
         assert not diff.any(), np.argwhere(diff)[:5]
-        assert rel_l2(got, ref) < 1e-5
+        # one occupied voxel: each bin is one or a few samples' weights (see
+        # the A1 test for the alpha rounding)
+        assert np.abs(got - ref).max() <= ALPHA_TOL * np.abs(ref).max()
