@@ -135,6 +135,26 @@ int totri_backprojection_confocal_velten_forward(
     float bin_offset, void* workspace, size_t workspace_bytes, void* stream);
 
 /*
+ * A2 without allocation -- no counterpart in the reference.  For callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc): the same computation and
+ * kernels as totri_backprojection_confocal_velten_forward, with the scratch
+ * (the transposed transient, B*T*S floats) in a file-scope device array of
+ * totri_backprojection_confocal_velten_noalloc_max_bytes() bytes (64 MiB)
+ * that each GPU the library runs on holds until the process ends.  Larger
+ * transients are rejected with TOTRI_EINVAL.  The call allocates nothing; the
+ * caller passes inputs and output as for the entry point above.  All calls on
+ * one GPU share that GPU's array: calls that overlap on one GPU (different
+ * streams, or host threads) race and must be serialised by the caller.
+ */
+size_t totri_backprojection_confocal_velten_noalloc_max_bytes(void);
+int totri_backprojection_confocal_velten_forward_noalloc(
+    const float* transient, int64_t B, int64_t T, int64_t S, const float* scan_points,
+    const float* scan_origin, int64_t So, float* volume, int64_t D, int64_t H, int64_t W,
+    const float volume_start[3], const float volume_end[3], float bin_width,
+    float bin_offset, void* stream);
+
+/*
  * A3 -- exhaustive Velten backprojection.
  * Replaces totri_cpp.reconstruct.backprojection_exhaustive_velten_forward
  * (src/reconstruct/backprojection.h:30-39, backprojection.cpp:36-47,

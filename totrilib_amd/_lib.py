@@ -57,6 +57,11 @@ SIGNATURES: dict[str, tuple] = {
         _c_int,
         [_c_p, _c_i64, _c_i64, _c_i64, _c_p, _c_p, _c_i64, _c_p, _c_i64, _c_i64, _c_i64,
          _F3, _F3, _c_f, _c_f, _c_p, _c_sz, _c_p]),
+    "totri_backprojection_confocal_velten_noalloc_max_bytes": (_c_sz, []),
+    "totri_backprojection_confocal_velten_forward_noalloc": (
+        _c_int,
+        [_c_p, _c_i64, _c_i64, _c_i64, _c_p, _c_p, _c_i64, _c_p, _c_i64, _c_i64, _c_i64,
+         _F3, _F3, _c_f, _c_f, _c_p]),
     "totri_backprojection_exhaustive_velten_workspace": (_c_sz, [_c_i64] * 4),
     "totri_backprojection_exhaustive_velten_forward": (
         _c_int,

@@ -24,6 +24,8 @@
 // volumes that are summed in chunk order.
 #include <math.h>
 
+#include <mutex>
+
 #include "common.h"
 #include "compact.h"
 
@@ -394,11 +396,17 @@ extern "C" size_t totri_backprojection_confocal_velten_workspace(int64_t B, int6
   return (size_t)B * T * S * sizeof(float);
 }
 
-extern "C" int totri_backprojection_confocal_velten_forward(
-    const float* transient, int64_t B, int64_t T, int64_t S, const float* scan_points,
-    const float* scan_origin, int64_t So, float* volume, int64_t D, int64_t H, int64_t W,
-    const float volume_start[3], const float volume_end[3], float bin_width,
-    float bin_offset, void* workspace, size_t workspace_bytes, void* stream) {
+namespace totri_hip {
+namespace {
+
+// A2 with its scratch (the transposed transient, B*T*S floats) supplied by
+// the caller: the workspace of the entry point below, or the library's own
+// array for the allocation-free variant.
+int velten_confocal(const float* transient, int64_t B, int64_t T, int64_t S,
+                    const float* scan_points, const float* scan_origin, int64_t So,
+                    float* volume, int64_t D, int64_t H, int64_t W,
+                    const float volume_start[3], const float volume_end[3], float bin_width,
+                    float bin_offset, void* workspace, size_t workspace_bytes, void* stream) {
   int rc = check_dims({B, T, S, So, D, H, W}, B * T * S, B * D * H * W);
   if (rc) return rc;
   TOTRI_CHECK_ARG(So == 0 || So == S, "scan_origin must have 0 or S points");
@@ -431,6 +439,77 @@ extern "C" int totri_backprojection_confocal_velten_forward(
                      (size_t)rows * T * sizeof(float), st, trT, scan_points, scan_origin,
                      volume, g, (int)S, (int)So, rows);
   return check_launch("velten_confocal_kernel");
+}
+
+// Scratch of the allocation-free A2: the transposed transient [B][S][T].
+// Every device that runs the library holds its own copy of this array, at
+// its own address, until the process ends.  Host code never passes the
+// array's name to a kernel (on the host it only stands for the symbol): it
+// uses the device address that hipGetSymbolAddress returns, kept per device.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+__device__ __align__(256) float scratch[kNoAllocScratchBytes / sizeof(float)];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices];  // per device, resolved on its first call
+
+// The current device's address of `scratch`, resolved once per device with
+// that device current.
+int scratch_address(void** out) {
+  int dev = 0;
+  TOTRI_CHECK_HIP(hipGetDevice(&dev));
+  TOTRI_CHECK_ARG(dev >= 0 && dev < kMaxDevices, "device %d beyond the %d supported", dev,
+                  kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr)
+      return fail(TOTRI_EHIP, "hipGetSymbolAddress(scratch) on device %d failed: %s", dev,
+                  hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return TOTRI_OK;
+}
+
+}  // namespace
+}  // namespace totri_hip
+
+extern "C" int totri_backprojection_confocal_velten_forward(
+    const float* transient, int64_t B, int64_t T, int64_t S, const float* scan_points,
+    const float* scan_origin, int64_t So, float* volume, int64_t D, int64_t H, int64_t W,
+    const float volume_start[3], const float volume_end[3], float bin_width,
+    float bin_offset, void* workspace, size_t workspace_bytes, void* stream) {
+  return totri_hip::velten_confocal(transient, B, T, S, scan_points, scan_origin, So, volume, D,
+                                    H, W, volume_start, volume_end, bin_width, bin_offset,
+                                    workspace, workspace_bytes, stream);
+}
+
+extern "C" size_t totri_backprojection_confocal_velten_noalloc_max_bytes(void) {
+  return totri_hip::kNoAllocScratchBytes;
+}
+
+extern "C" int totri_backprojection_confocal_velten_forward_noalloc(
+    const float* transient, int64_t B, int64_t T, int64_t S, const float* scan_points,
+    const float* scan_origin, int64_t So, float* volume, int64_t D, int64_t H, int64_t W,
+    const float volume_start[3], const float volume_end[3], float bin_width,
+    float bin_offset, void* stream) {
+  using namespace totri_hip;
+  TOTRI_CHECK_ARG(B >= 0 && T >= 0 && S >= 0, "negative dimension");
+  const size_t need = totri_backprojection_confocal_velten_workspace(B, T, S);
+  if (need > kNoAllocScratchBytes)
+    return fail(TOTRI_EINVAL,
+                "transient of %zu bytes (B=%lld T=%lld S=%lld) exceeds the %zu-byte scratch of "
+                "the allocation-free backprojection; use "
+                "totri_backprojection_confocal_velten_forward with a workspace",
+                need, (long long)B, (long long)T, (long long)S, kNoAllocScratchBytes);
+  void* ws = nullptr;
+  int rc = scratch_address(&ws);
+  if (rc) return rc;
+  return velten_confocal(transient, B, T, S, scan_points, scan_origin, So, volume, D, H, W,
+                         volume_start, volume_end, bin_width, bin_offset, ws,
+                         kNoAllocScratchBytes, stream);
 }
 
 extern "C" size_t totri_backprojection_exhaustive_velten_workspace(int64_t B, int64_t T,
