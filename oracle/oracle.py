@@ -254,3 +254,16 @@ def splat_grad(rbf, vol, vgrad, vs, ve, trunc=0.0, mode=FP32):
     lib().oracle_splat_grad(_p(r), B, N, A, _p(v), _p(g), D, H, W, _f3(vs), _f3(ve),
                             ctypes.c_float(trunc), mode, _p(out))
     return out
+
+
+def r1_contraction_flips(mp, T, X, Y, Z, origin, size, w, off, i0=0, i1=None):
+    """Compare R1's decisions in the uncontracted FP32 reading with those under
+    nvcc's default FMA contraction, over scan rows [i0, i1), every bin and
+    column: (active samples, activity flips, floor-index flips)."""
+    m = _f32(mp)
+    N = m.shape[1]
+    i1 = N if i1 is None else i1
+    out = np.zeros(3, dtype=np.int64)
+    lib().oracle_r1_contraction_flips(_p(m), N, T, X, Y, Z, _f3(origin), _f3(size),
+                                      ctypes.c_float(w), ctypes.c_float(off), i0, i1, _p(out))
+    return tuple(int(v) for v in out)

@@ -1150,3 +1150,68 @@ void oracle_splat_grad(const float* rbf, int B, int N, int A,
     }
   }
 }
+
+/* ------------------------------------------------------------------------ */
+/* R1 decisions under nvcc's default contraction (--fmad=true), against the */
+/* uncontracted reading of volume_cuda.cu.  nvcc fuses a product into the   */
+/* following sum where the source writes one:                                */
+/*   :23  tc = ti * bin_width + bin_offset   -> fmaf(ti, w, off)             */
+/*   :46  z_hat = (z - o) * s - 0.5          -> fmaf(z - o, s, -0.5)         */
+/* (depth^2 is built from powf calls, and x from a product then a quotient, */
+/* so neither has a product feeding a sum in the source.)                    */
+/* Counts, over rows [i0, i1) of the scan, every bin and every column:       */
+/*   out[0] samples active in the uncontracted reading                        */
+/*   out[1] samples whose activity differs between the readings               */
+/*   out[2] samples active in both whose floor index differs                  */
+/* ------------------------------------------------------------------------ */
+static inline int r1_decide(float tc, float wx, float wy, float wz, float x, float y,
+                            float oz, float s, int Z, int fused, int* zi_out) {
+  float t2 = sq32(tc);
+  float dx = wx - x, dy = wy - y;
+  float d2 = t2 - sq32(dx);
+  d2 = d2 - sq32(dy);
+  if (!(d2 > 0.f)) return 0;
+  float depth = sqrtf(d2);
+  float z = wz + depth;
+  float zh;
+  if (fused) {
+    zh = fmaf(z - oz, s, -0.5f);
+  } else {
+    zh = (z - oz) * s;
+    zh = zh - 0.5f;
+  }
+  const float fl = floorf(zh);
+  if (!(fl >= 0.f && fl + 1.f < (float)Z)) return 0;
+  *zi_out = (int)fl;
+  return 1;
+}
+
+void oracle_r1_contraction_flips(const float* mp, int N, int T, int X, int Y, int Z,
+                                 const float* origin, const float* size, float w,
+                                 float off, int i0, int i1, long long* out) {
+  const float s = (float)Z / size[2];
+  long long act = 0, aflip = 0, zflip = 0;
+#pragma omp parallel for collapse(2) schedule(dynamic, 4) reduction(+ : act, aflip, zflip)
+  for (int i = i0; i < i1; ++i) {
+    for (int ti = 0; ti < T; ++ti) {
+      const float wx = mp[i], wy = mp[N + i], wz = mp[2 * N + i];
+      const float tc = r1_tc32(ti, w, off);
+      const float tcf = fmaf((float)ti, w, off);
+      for (int xi = 0; xi < X; ++xi) {
+        const float x = r1_x32(xi, origin[0], size[0], X);
+        for (int yi = 0; yi < Y; ++yi) {
+          const float y = r1_x32(yi, origin[1], size[1], Y);
+          int za = 0, zb = 0;
+          const int a = r1_decide(tc, wx, wy, wz, x, y, origin[2], s, Z, 0, &za);
+          const int b = r1_decide(tcf, wx, wy, wz, x, y, origin[2], s, Z, 1, &zb);
+          act += a;
+          aflip += (a != b);
+          zflip += (a && b && za != zb);
+        }
+      }
+    }
+  }
+  out[0] = act;
+  out[1] = aflip;
+  out[2] = zflip;
+}

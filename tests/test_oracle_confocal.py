@@ -290,3 +290,28 @@ def test_column_restricted_render_equals_full_render():
         rows = O.r1_forward(vol, mp, T, ORIGIN, SIZE, 0.03, 0.01, mode=mode, i0=3, i1=7)
         assert np.count_nonzero(full) > 0
         assert np.array_equal(full, sub) and np.array_equal(full[3:7], rows)
+
+
+def test_decisions_under_nvcc_default_contraction():
+    """The reference's .cu files build with nvcc's default --fmad=true, which
+    fuses tc = t*w + off (volume_cuda.cu:23) and z_hat = (z - o)*s - 0.5 (:46)
+    into FMAs.  This build pins the uncontracted reading.  With off = 0 the
+    fused tc equals the unfused one, and z_hat's single rounding differs from
+    fl(fl((z-o)s) - 0.5) only where the product sits at a binade edge, so the
+    decisions agree: every sample of 64^3 / 32^2 / 512 (1.1e9 active), 64 scan
+    rows of 128^3 / 64^2 / 1024 and 16 rows of 192^3 / 128^2 / 2048 (s = 96,
+    not a power of two).  A nonzero offset does flip a few floors (about 1e-6
+    of the samples), which shows the count is live."""
+    from oracle import oracle as O
+    from tests.cases import ORIGIN, SIZE, wall_grid
+    cases = [(32, 512, 64, 0.005, 0.0, None), (64, 1024, 128, 0.0025, 0.0, 64),
+             (128, 2048, 192, 2.56 / 2048, 0.0, 16)]
+    for n, T, R, w, off, rows in cases:
+        act, aflip, zflip = O.r1_contraction_flips(wall_grid(n), T, R, R, R, ORIGIN, SIZE, w,
+                                                   off, 0, rows)
+        print(f"{R}^3/{n}^2/{T}: {act} active samples, {aflip} activity and {zflip} floor flips")
+        assert act > 4e8 and aflip == 0 and zflip == 0
+    act, aflip, zflip = O.r1_contraction_flips(wall_grid(64), 1024, 128, 128, 128, ORIGIN, SIZE,
+                                               0.0025, 0.013, 0, 64)
+    print(f"offset 0.013: {act} active samples, {aflip} activity and {zflip} floor flips")
+    assert 0 < zflip < 1e-5 * act
