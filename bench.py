@@ -658,6 +658,16 @@ def kernels_main(args):
     ms = timeit(lambda: torch.autograd.grad(V, r, GV, retain_graph=True))
     row("S2 RBF splat gradient", "splat_grad_kernel", "20,000 RBFs -> 128^3, 3 sigma", ms,
         in_support, "voxel-RBF pairs within 3 sigma", 25, 4 * (4 * RR ** 3 + 10 * M))
+    # the reference's untruncated splat (truncation 0): every RBF at every voxel
+    ms = timeit(lambda: Rbf2VolumeGaussian.apply(rbf, vd, 0.0))
+    row("S1 RBF splat, untruncated", "splat_forward_kernel", "20,000 RBFs -> 128^3, no truncation",
+        ms, float(M) * RR ** 3, "voxel-RBF pairs", 14, 4 * (2 * RR ** 3 + 5 * M))
+    r0 = rbf.clone().requires_grad_(True)
+    V0 = Rbf2VolumeGaussian.apply(r0, vd, 0.0)
+    ms = timeit(lambda: torch.autograd.grad(V0, r0, GV, retain_graph=True))
+    row("S2 RBF splat gradient, untruncated", "splat_grad_kernel",
+        "20,000 RBFs -> 128^3, no truncation", ms, float(M) * RR ** 3, "voxel-RBF pairs", 25,
+        4 * (4 * RR ** 3 + 10 * M))
     # A6: scan-point gradient, 128^3 / 64x64 / 1024 bins
     mp = torch.from_numpy(wall_grid(64)).to(dev).requires_grad_(True)
     alb = torch.from_numpy(sparse_albedo((128, 128, 128), 0.03, 1)).to(dev)

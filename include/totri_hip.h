@@ -97,6 +97,22 @@ int totri_volume_confocal_grad_volume(float* volume, int64_t X, int64_t Y,
                                       void* stream);
 
 /*
+ * A1 in the reference's own window form, for callers of apply_transposed
+ * that need the reference's result (src/render/volume_cuda.cu:95-160, the
+ * kernel totri_cpp.render.volume_confocal_grad_volume runs there).  Each voxel
+ * sums, over scan points, the bins whose radius lies between its
+ * neighbours' centres, weighted (1 - alpha) from its own centre's bin on and
+ * alpha below it.  It differs from totri_volume_confocal_grad_volume on the
+ * top slice z = Z-1 (samples the render never reads) and by rounding at the
+ * window edges.  Same arguments; no workspace.
+ */
+int totri_volume_confocal_grad_volume_window(
+    float* volume, int64_t X, int64_t Y, int64_t Z, const float* measurement_points,
+    int64_t N, const void* transient, int64_t T, int transient_dtype,
+    const float volume_origin[3], const float volume_size[3], float bin_width,
+    float bin_offset, void* stream);
+
+/*
  * A6 -- gradient of sum(G * R1(volume; w)) with respect to the wall points w.
  * Replaces totri_cpp.render.volume_confocal_grad_measurement_points
  * (src/render/volume.h:26-33, src/render/volume.cpp:43-81,
@@ -238,11 +254,17 @@ int totri_volume_render_exhaustive_grad_volume(
  *           Gaussians, channels 1..A the weighted attribute means
  *   truncation  > 0 skips RBFs farther than truncation * sigma from a voxel;
  *               <= 0 sums all RBFs (the reference's behaviour)
+ *   workspace   truncated splats list each brick's RBFs first; the size
+ *               comes from the query (0 without truncation)
  */
+size_t totri_rbf2volume_gaussian_forward_workspace(int64_t B, int64_t N, int64_t A,
+                                                   int64_t D, int64_t H, int64_t W,
+                                                   float truncation);
 int totri_rbf2volume_gaussian_forward(const float* rbf, int64_t B, int64_t N, int64_t A,
                                       float* volume, int64_t D, int64_t H, int64_t W,
                                       const float volume_start[3], const float volume_end[3],
-                                      float truncation, void* stream);
+                                      float truncation, void* workspace,
+                                      size_t workspace_bytes, void* stream);
 
 /*
  * S2 -- gradient of the splat with respect to the RBF parameters.

@@ -3,6 +3,10 @@ import sys
 
 import pytest
 
+# Measured errors and figures that tests report; printed in the terminal
+# summary so that they appear in a quiet (-q) run's log as well.
+_REPORTED = []
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
@@ -25,3 +29,19 @@ def pytest_collection_modifyitems(config, items):
     for item in items:
         if "gpu" in item.keywords:
             item.add_marker(skip)
+
+
+@pytest.fixture
+def report():
+    """report(line): add a measured figure to the run's terminal summary."""
+    def add(line: str) -> None:
+        print(line)
+        _REPORTED.append(line)
+    return add
+
+
+def pytest_terminal_summary(terminalreporter):
+    if _REPORTED:
+        terminalreporter.section("measured errors and figures")
+        for line in _REPORTED:
+            terminalreporter.write_line(line)

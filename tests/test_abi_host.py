@@ -40,7 +40,8 @@ def test_error_reporting_without_gpu():
     assert "negative" in _lib.last_error()
     with pytest.raises(ValueError):
         _lib.check(rc, "probe")
-    rc = lib.totri_rbf2volume_gaussian_forward(None, 1, 1, 9, None, 1, 1, 1, o, o, 0.0, None)
+    rc = lib.totri_rbf2volume_gaussian_forward(None, 1, 1, 9, None, 1, 1, 1, o, o, 0.0, None, 0,
+                                               None)
     assert rc == _lib.TOTRI_EINVAL and "attributes" in _lib.last_error()
     # empty problems succeed without touching the device
     rc = lib.totri_volume_confocal_forward(None, 4, 4, 4, None, 0, None, 8, 0, o, o, 1.0, 0.0,

@@ -50,7 +50,7 @@ def render(vol, mp, T, w, dtype=None):
     return out
 
 
-def test_64_cubed_render_noise_backproject():
+def test_64_cubed_render_noise_backproject(report):
     from totrilib_amd.reconstruct import BackprojectionConfocalVelten
     from totrilib_amd.render import VolumeRenderConfocal
     from totrilib_amd.types import BinDef, VolumeDef
@@ -76,14 +76,14 @@ def test_64_cubed_render_noise_backproject():
                              mode=O.FP32)
     v64 = O.velten_confocal(trT, scan, None, (R, R, R), vd.start, vd.end, 2 * w, 0.0,
                             mode=O.PINNED64)
-    print(f"\n64^3 render rel-L2 vs FP64 {e_r:.2e}; A1 backprojection {e_a:.2e}; "
+    report(f"64^3 render rel-L2 vs FP64 {e_r:.2e}; A1 backprojection {e_a:.2e}; "
           f"Velten bit-exact vs FP32 oracle: {np.array_equal(vel, vref)}, "
           f"rel-L2 vs FP64 {rel_l2(vel, v64):.2e}")
     assert e_r < TOL and e_a < TOL
     assert np.array_equal(vel.astype(np.float64), vref)
 
 
-def test_128_cubed_albedo_gradient_matches_central_differences():
+def test_128_cubed_albedo_gradient_matches_central_differences(report):
     """L(v) = ||R1 v - R1 v_target||^2.  R1 is linear, so with r = R1 v - y
     and a_j = R1 e_j (the FP64 oracle's render of voxel j alone),
     L(v +- h e_j) = ||r +- h a_j||^2 exactly; the central difference of
@@ -112,7 +112,7 @@ def test_128_cubed_albedo_gradient_matches_central_differences():
         gg[k] = g[xi, yi, zi]
     scale = np.abs(fd).max()
     err = np.abs(gg - fd).max() / scale
-    print(f"\n128^3 gradient vs FP64 central differences on 64 voxels: max error "
+    report(f"128^3 gradient vs FP64 central differences on 64 voxels: max error "
           f"{err:.2e} of max |dL/dv| = {scale:.3e} ({np.count_nonzero(fd)} nonzero)")
     assert np.count_nonzero(fd) > 32
     assert err < 1e-5
@@ -128,7 +128,7 @@ def cfg4_inputs():
     return mp_np, vol_np
 
 
-def test_256_cubed_render_error_per_storage_dtype(cfg4_inputs):
+def test_256_cubed_render_error_per_storage_dtype(cfg4_inputs, report):
     mp_np, vol_np = cfg4_inputs
     T, w = CFG4["T"], CFG4["w"]
     rows = np.sort(np.random.default_rng(51).choice(mp_np.shape[1], 24, replace=False))
@@ -141,12 +141,12 @@ def test_256_cubed_render_error_per_storage_dtype(cfg4_inputs):
         out = render(vol, mp, T, w, dt)
         errs[name] = rel_l2(out[torch.from_numpy(rows).to(dev())].float().cpu().numpy(), ref)
         del out
-    print("\n256^3 render rel-L2 vs FP64 (24 scan rows): " +
+    report("256^3 render rel-L2 vs FP64 (24 scan rows): " +
           ", ".join(f"{k} {v:.2e}" for k, v in errs.items()))
     assert errs["fp32"] < TOL and errs["fp16"] < 1e-3 and errs["bf16"] < 8e-3
 
 
-def test_256_cubed_backprojection_error_per_storage_dtype(cfg4_inputs):
+def test_256_cubed_backprojection_error_per_storage_dtype(cfg4_inputs, report):
     from totrilib_amd.render import VolumeRenderConfocal
     mp_np, vol_np = cfg4_inputs
     T, w, R = CFG4["T"], CFG4["w"], CFG4["R"]
@@ -166,12 +166,12 @@ def test_256_cubed_backprojection_error_per_storage_dtype(cfg4_inputs):
         ci = torch.from_numpy(cols).to(dev())
         errs[name] = rel_l2(vol[ci[:, 0], ci[:, 1]].cpu().numpy(), ref)
         del vol
-    print("\n256^3 backprojection rel-L2 vs FP64 (16 columns, 4096 voxels): " +
+    report("256^3 backprojection rel-L2 vs FP64 (16 columns, 4096 voxels): " +
           ", ".join(f"{k} {v:.2e}" for k, v in errs.items()))
     assert errs["fp32"] < TOL and errs["fp16"] < 1e-3 and errs["bf16"] < 8e-3
 
 
-def test_20000_rbf_splat_render_gradient():
+def test_20000_rbf_splat_render_gradient(report):
     """Splat (3 sigma) -> albedo V0 * V1 -> render -> L2 loss.  The splat is
     checked on 4096 voxels against the FP64 oracle; the RBF gradient against
     the oracle's FP64 analytic gradient of the same truncated splat, given
@@ -207,8 +207,8 @@ def test_20000_rbf_splat_render_gradient():
     got = r.grad.cpu().numpy()
     e_g = rel_l2(got, gref)
     per = [rel_l2(got[..., c], gref[..., c]) for c in range(5)]
-    print(f"\n20,000 RBFs: splat rel-L2 vs FP64 (4096 voxels) {e_s:.2e}; gradient rel-L2 "
+    report(f"20,000 RBFs: splat rel-L2 vs FP64 (4096 voxels) {e_s:.2e}; gradient rel-L2 "
           f"{e_g:.2e} (x {per[0]:.1e}, y {per[1]:.1e}, z {per[2]:.1e}, sigma {per[3]:.1e}, "
           f"weight {per[4]:.1e})")
     assert e_s < TOL
-    assert e_g < 1e-4
+    assert e_g < 1e-5

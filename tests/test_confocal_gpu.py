@@ -573,3 +573,32 @@ def test_a1_transient_longer_than_float_bin_counter(scan):
     print(f"{scan}: rel-L2 vs FP64 pinned {e:.2e}, nonzero {np.count_nonzero(ref)}")
     assert np.count_nonzero(ref) > 20
     assert e < TOL
+
+
+@pytest.mark.parametrize("shape,n,off", [((16, 14, 24), 6, 0.0), ((10, 12, 100), 5, 0.03)])
+def test_a1_reference_window_form_opt_in(shape, n, off):
+    """apply_transposed(..., reference_window=True) computes the reference's
+    own window form (volume_cuda.cu:95-160): bit for bit the FP32 oracle's
+    restatement (correctly rounded roots and quotients in the same order),
+    including the top slice, where it differs from the exact transpose.
+    Z = 100 puts a column's voxels in two waves."""
+    from totrilib_amd.render import VolumeRenderConfocal
+    X, Y, Z = shape
+    T, w = 500, 0.006
+    rng = np.random.default_rng(sum(shape))
+    mp = wall_grid(n)
+    mp[0] += rng.uniform(-0.01, 0.01, mp.shape[1]).astype(np.float32)
+    G = rng.standard_normal((mp.shape[1], T)).astype(np.float32)
+    m = torch.from_numpy(mp).to(dev())
+    g = torch.from_numpy(G).to(dev())
+    got = VolumeRenderConfocal.apply_transposed(g, m, ORIGIN, SIZE, shape, w, off,
+                                                reference_window=True).cpu().numpy()
+    ref32 = O.a1_reference(G, mp, X, Y, Z, ORIGIN, SIZE, w, off, mode=O.FP32)
+    assert np.array_equal(got.astype(np.float64), ref32)
+    ref64 = O.a1_reference(G, mp, X, Y, Z, ORIGIN, SIZE, w, off, mode=O.PINNED64)
+    assert rel_l2(got, ref64) < TOL
+    exact = VolumeRenderConfocal.apply_transposed(g, m, ORIGIN, SIZE, shape, w, off).cpu().numpy()
+    top = rel_l2(got[:, :, -1], exact[:, :, -1])
+    rest = rel_l2(got[:, :, :-1], exact[:, :, :-1])
+    print(f"{shape}: window form vs exact transpose: top slice {top:.2e}, rest {rest:.2e}")
+    assert top > 1e-2 and rest < 1e-3

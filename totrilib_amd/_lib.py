@@ -47,6 +47,10 @@ SIGNATURES: dict[str, tuple] = {
         _c_int,
         [_c_p, _c_i64, _c_i64, _c_i64, _c_p, _c_i64, _c_p, _c_i64, _c_int,
          _F3, _F3, _c_f, _c_f, _c_p, _c_sz, _c_p]),
+    "totri_volume_confocal_grad_volume_window": (
+        _c_int,
+        [_c_p, _c_i64, _c_i64, _c_i64, _c_p, _c_i64, _c_p, _c_i64, _c_int,
+         _F3, _F3, _c_f, _c_f, _c_p]),
     "totri_volume_confocal_grad_measurement_points_workspace": (_c_sz, [_c_i64] * 5),
     "totri_volume_confocal_grad_measurement_points": (
         _c_int,
@@ -82,9 +86,12 @@ SIGNATURES: dict[str, tuple] = {
         _c_int,
         [_c_p, _c_i64, _c_i64, _c_i64, _c_i64, _c_p, _c_p, _c_p, _c_i64, _c_i64, _c_i64,
          _F3, _F3, _c_f, _c_f, _c_p, _c_sz, _c_p]),
+    "totri_rbf2volume_gaussian_forward_workspace": (
+        _c_sz, [_c_i64, _c_i64, _c_i64, _c_i64, _c_i64, _c_i64, _c_f]),
     "totri_rbf2volume_gaussian_forward": (
         _c_int,
-        [_c_p, _c_i64, _c_i64, _c_i64, _c_p, _c_i64, _c_i64, _c_i64, _F3, _F3, _c_f, _c_p]),
+        [_c_p, _c_i64, _c_i64, _c_i64, _c_p, _c_i64, _c_i64, _c_i64, _F3, _F3, _c_f, _c_p,
+         _c_sz, _c_p]),
     "totri_rbf2volume_gaussian_grad": (
         _c_int,
         [_c_p, _c_i64, _c_i64, _c_i64, _c_p, _c_p, _c_p, _c_i64, _c_i64, _c_i64, _F3, _F3,

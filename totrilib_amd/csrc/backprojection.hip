@@ -103,11 +103,19 @@ __global__ __launch_bounds__(256) void velten_confocal_kernel(
   const float* ob = orig + (size_t)b * So * 3;
   const float* tb = trT + (size_t)b * S * g.T;
   float value = 0.f;
-  for (int s0 = 0; s0 < S; s0 += rows_per_stage) {
-    const int ns = min(rows_per_stage, S - s0);
-    __syncthreads();
-    for (int e = threadIdx.x; e < ns * g.T; e += blockDim.x) rows[e] = tb[(size_t)s0 * g.T + e];
-    __syncthreads();
+  // rows_per_stage == 0 (rows longer than the LDS stage): read the rows from
+  // global memory; the choice is uniform over the launch.
+  const bool staged = rows_per_stage > 0;
+  const int step = staged ? rows_per_stage : 1;
+  for (int s0 = 0; s0 < S; s0 += step) {
+    const int ns = min(step, S - s0);
+    const float* src = tb + (size_t)s0 * g.T;
+    if (staged) {
+      __syncthreads();
+      for (int e = threadIdx.x; e < ns * g.T; e += blockDim.x) rows[e] = src[e];
+      __syncthreads();
+      src = rows;
+    }
     for (int k = 0; k < ns; ++k) {
       const int s = s0 + k;
       const float px = sb[3 * s], py = sb[3 * s + 1], pz = sb[3 * s + 2];
@@ -116,7 +124,7 @@ __global__ __launch_bounds__(256) void velten_confocal_kernel(
                                : 0.f;                                        // :178-183
       const float dist = 2.f * dso + 2.f * dsv;                              // :185
       const int ti = nearest_bin(dist, g.w, g.off);                          // :187
-      if (ti >= 0 && ti < g.T) value += rows[k * g.T + ti];                  // :188-190
+      if (ti >= 0 && ti < g.T) value += src[k * g.T + ti];                   // :188-190
     }
   }
   if (ok) vol[(size_t)b * nvox + v] = value;                                 // :192
@@ -154,11 +162,18 @@ __global__ __launch_bounds__(256) void velten_exhaustive_kernel(
   const float* tb = trT + (size_t)b * S * L * g.T;
   const int nrows = S * L;
   float value = 0.f;
-  for (int r0 = 0; r0 < nrows; r0 += rows_per_stage) {
-    const int nr = min(rows_per_stage, nrows - r0);
-    __syncthreads();
-    for (int e = threadIdx.x; e < nr * g.T; e += blockDim.x) rows[e] = tb[(size_t)r0 * g.T + e];
-    __syncthreads();
+  // rows_per_stage == 0: rows from global memory (as in A2)
+  const bool staged = rows_per_stage > 0;
+  const int step = staged ? rows_per_stage : 1;
+  for (int r0 = 0; r0 < nrows; r0 += step) {
+    const int nr = min(step, nrows - r0);
+    const float* src = tb + (size_t)r0 * g.T;
+    if (staged) {
+      __syncthreads();
+      for (int e = threadIdx.x; e < nr * g.T; e += blockDim.x) rows[e] = src[e];
+      __syncthreads();
+      src = rows;
+    }
     for (int k = 0; k < nr; ++k) {
       const int r = r0 + k;
       const int s = r / L, l = r - s * L;
@@ -174,7 +189,7 @@ __global__ __launch_bounds__(256) void velten_exhaustive_kernel(
       dist = dist + dlv;
       dist = dist + dlo;
       const int ti = nearest_bin(dist, g.w, g.off);                          // :279
-      if (ti >= 0 && ti < g.T) value += rows[k * g.T + ti];                  // :280-282
+      if (ti >= 0 && ti < g.T) value += src[k * g.T + ti];                   // :280-282
     }
   }
   if (ok) vol[(size_t)b * nvox + v] = value;                                 // :285
@@ -429,12 +444,11 @@ int velten_confocal(const float* transient, int64_t B, int64_t T, int64_t S,
     return fail(TOTRI_EWORKSPACE, "workspace too small: need %zu bytes, got %zu", need,
                 workspace_bytes);
   TOTRI_CHECK_ARG(workspace != nullptr, "null workspace");
-  TOTRI_CHECK_ARG(T <= kStageFloats, "T=%lld bins exceed the staging limit %d",
-                  (long long)T, kStageFloats);
   float* trT = (float*)workspace;
   rc = launch_transpose(transient, trT, (int)B, (int)T, (int)S, st);
   if (rc) return rc;
-  const int rows = (int)(kStageFloats / T);
+  // rows longer than the LDS stage are read from global memory (rows = 0)
+  const int rows = T <= kStageFloats ? (int)(kStageFloats / T) : 0;
   hipLaunchKernelGGL(velten_confocal_kernel, dim3(grid1d(nvox), (unsigned)B), dim3(256),
                      (size_t)rows * T * sizeof(float), st, trT, scan_points, scan_origin,
                      volume, g, (int)S, (int)So, rows);
@@ -546,12 +560,10 @@ extern "C" int totri_backprojection_exhaustive_velten_forward(
     return fail(TOTRI_EWORKSPACE, "workspace too small: need %zu bytes, got %zu", need,
                 workspace_bytes);
   TOTRI_CHECK_ARG(workspace != nullptr, "null workspace");
-  TOTRI_CHECK_ARG(T <= kStageFloats, "T=%lld bins exceed the staging limit %d",
-                  (long long)T, kStageFloats);
   float* trT = (float*)workspace;
   rc = launch_transpose(transient, trT, (int)B, (int)T, (int)(S * L), st);
   if (rc) return rc;
-  const int rows = (int)(kStageFloats / T);
+  const int rows = T <= kStageFloats ? (int)(kStageFloats / T) : 0;
   hipLaunchKernelGGL(velten_exhaustive_kernel, dim3(grid1d(nvox), (unsigned)B), dim3(256),
                      (size_t)rows * T * sizeof(float), st, trT, scan_points, laser_points,
                      scan_origin, laser_origin, volume, g, (int)S, (int)L, (int)So, (int)Lo,

@@ -174,18 +174,26 @@ __global__ __launch_bounds__(256) void r2_adjoint_kernel(
   const float* gb = gT + (size_t)b * S * L * g.T;
   const int nrows = S * L;
   float acc = 0.f;
-  for (int r0 = 0; r0 < nrows; r0 += rows_per_stage) {
-    const int nr = min(rows_per_stage, nrows - r0);
-    __syncthreads();
-    for (int e = threadIdx.x; e < nr * g.T; e += blockDim.x) rows[e] = gb[(size_t)r0 * g.T + e];
-    __syncthreads();
+  // rows_per_stage == 0 (rows longer than the LDS stage): read the rows from
+  // global memory; the choice is uniform over the launch.
+  const bool staged = rows_per_stage > 0;
+  const int step = staged ? rows_per_stage : 1;
+  for (int r0 = 0; r0 < nrows; r0 += step) {
+    const int nr = min(step, nrows - r0);
+    const float* src = gb + (size_t)r0 * g.T;
+    if (staged) {
+      __syncthreads();
+      for (int e = threadIdx.x; e < nr * g.T; e += blockDim.x) rows[e] = src[e];
+      __syncthreads();
+      src = rows;
+    }
     for (int k = 0; k < nr; ++k) {
       const int r = r0 + k;
       const int s = r / L, l = r - s * L;
       const Span sp = voxel_span(g, 1.f, xf, yf, zf0, zf1, sb[s], sb[S + s], sb[2 * S + s],
                                  lb[l], lb[L + l], lb[2 * L + l]);
       if (sp.first > sp.last) continue;
-      const float* row = rows + k * g.T;
+      const float* row = src + k * g.T;
       if (sp.first == sp.last) {
         acc += sp.wf * row[sp.first];
       } else {
@@ -304,7 +312,6 @@ extern "C" int totri_volume_render_exhaustive_grad_volume(
     return check_launch("r2_adjoint_kernel");
   }
   TOTRI_CHECK_ARG(transient_grad && scan_points && laser_points, "null input");
-  TOTRI_CHECK_ARG(T <= kStage, "T=%lld bins exceed the staging limit %d", (long long)T, kStage);
   const size_t need = totri_volume_render_exhaustive_grad_volume_workspace(B, T, S, L);
   if (workspace_bytes < need)
     return fail(TOTRI_EWORKSPACE, "workspace too small: need %zu bytes, got %zu", need,
@@ -314,7 +321,8 @@ extern "C" int totri_volume_render_exhaustive_grad_volume(
   // [B][T][S*L] -> [B][S*L][T]
   rc = transpose_batched(transient_grad, gT, (int)B, (int)T, (int)(S * L), st);
   if (rc) return rc;
-  const int rows = (int)(kStage / T);
+  // rows longer than the LDS stage are read from global memory (rows = 0)
+  const int rows = T <= kStage ? (int)(kStage / T) : 0;
   hipLaunchKernelGGL(r2_adjoint_kernel, grid, dim3(256), (size_t)rows * T * sizeof(float), st,
                      gT, scan_points, laser_points, volume_grad, g, (int)S, (int)L, rows);
   return check_launch("r2_adjoint_kernel");

@@ -20,6 +20,7 @@
 // fixed order; the reference instead launches one block per (16 columns x
 // 4 slices, RBF) and adds per-block partial sums with float atomics.
 #include <math.h>
+#include <string.h>
 
 #include "common.h"
 
@@ -67,12 +68,144 @@ __device__ __forceinline__ bool within(float r2, float sg, float trunc) {
   return !(r2 > lim * lim);
 }
 
+// Binned lists (truncated splats): per (batch, brick) the RBFs whose support
+// may reach the brick, found with the brick-cull test of the full scan, so a
+// brick sums the same RBFs in the same (index) order either way.
+struct SplatBins {
+  int* count;     // [B][bricks]
+  int* offset;    // [B][bricks] exclusive scan of count
+  int* cursor;    // [B][bricks] fill positions
+  int* list;      // [capacity] RBF indices
+  long long capacity;
+};
+
+constexpr int kListMax = 2048;  // longest list a brick sorts in LDS; longer lists scan
+
+// Conservative brick cull (the full scan's test): the squared distance from
+// the RBF centre to the brick's sample box against (trunc sigma)^2.
+__device__ __forceinline__ bool brick_may_hold(float xr, float yr, float zr, float sg,
+                                               float trunc, float lox, float hix, float loy,
+                                               float hiy, float loz, float hiz) {
+  const float dx = fmaxf(fmaxf(lox - xr, xr - hix), 0.f);
+  const float dy = fmaxf(fmaxf(loy - yr, yr - hiy), 0.f);
+  const float dz = fmaxf(fmaxf(loz - zr, zr - hiz), 0.f);
+  const float lim = trunc * fabsf(sg);
+  return !((dx * dx + dy * dy + dz * dz) > lim * lim * 1.0001f + 1e-12f);
+}
+
+// The sample box of brick (bx, by, bz): the voxels it really holds.
+__device__ __forceinline__ void brick_box(const SplatGeom& g, int bx, int by, int bz,
+                                          float* lox, float* hix, float* loy, float* hiy,
+                                          float* loz, float* hiz) {
+  const int x0 = bx * kBX, x1 = min(bx * kBX + kBX, g.W) - 1;
+  const int y0 = by * kBY, y1 = min(by * kBY + kBY, g.H) - 1;
+  const int z0 = bz * kBZ, z1 = min(bz * kBZ + kBZ, g.D) - 1;
+  const float bxa = splat_coord(x0, g.W, g.sx, g.ex), bxb = splat_coord(x1, g.W, g.sx, g.ex);
+  const float bya = splat_coord(y0, g.H, g.sy, g.ey), byb = splat_coord(y1, g.H, g.sy, g.ey);
+  const float bza = splat_coord(z0, g.D, g.sz, g.ez), bzb = splat_coord(z1, g.D, g.sz, g.ez);
+  *lox = fminf(bxa, bxb);
+  *hix = fmaxf(bxa, bxb);
+  *loy = fminf(bya, byb);
+  *hiy = fmaxf(bya, byb);
+  *loz = fminf(bza, bzb);
+  *hiz = fmaxf(bza, bzb);
+}
+
+// Brick index range [lo, hi] along an axis whose bricks hold `bs` samples
+// each, covering the samples within [c - r, c + r] (one sample of margin; the
+// exact test is brick_may_hold).
+__device__ __forceinline__ void brick_range(float c, float r, int n, int bs, float a, float b,
+                                            int* lo, int* hi) {
+  const int nb = (n + bs - 1) / bs;
+  if (n == 1) {
+    *lo = 0;
+    *hi = nb - 1;
+    return;
+  }
+  const float step = (b - a) / (float)(n - 1);
+  float u0 = (c - r - a) / step, u1 = (c + r - a) / step;
+  if (u0 > u1) {
+    const float t = u0;
+    u0 = u1;
+    u1 = t;
+  }
+  u0 = fminf(fmaxf(u0 - 1.f, -1.f), (float)n);
+  u1 = fminf(fmaxf(u1 + 1.f, -1.f), (float)n);
+  *lo = max((int)floorf(u0), 0) / bs;
+  *hi = min((int)ceilf(u1), n - 1) / bs;
+  if (!(r < INFINITY)) {
+    *lo = 0;
+    *hi = nb - 1;
+  }
+}
+
+// Pass 1 (FILL = false) counts, per brick, the RBFs that may reach it; pass 2
+// (FILL = true) writes their indices at offset + a fill position.  One thread
+// per RBF.
+template <bool FILL>
+__global__ __launch_bounds__(256) void splat_bin_kernel(const float* __restrict__ rbf,
+                                                        SplatGeom g, int bricks_x,
+                                                        int bricks_y, int bricks_z,
+                                                        SplatBins bins) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (i >= g.N) return;
+  const int C = 4 + g.A;
+  const float* p = rbf + ((size_t)b * g.N + i) * C;
+  const float xr = p[0], yr = p[1], zr = p[2], sg = p[3];
+  const float r = g.trunc * fabsf(sg) * 1.0001f + 1e-6f;
+  int xa, xb, ya, yb, za, zb;
+  brick_range(xr, r, g.W, kBX, g.sx, g.ex, &xa, &xb);
+  brick_range(yr, r, g.H, kBY, g.sy, g.ey, &ya, &yb);
+  brick_range(zr, r, g.D, kBZ, g.sz, g.ez, &za, &zb);
+  const size_t nb = (size_t)bricks_x * bricks_y * bricks_z;
+  for (int bz = za; bz <= zb; ++bz)
+    for (int by = ya; by <= yb; ++by)
+      for (int bx = xa; bx <= xb; ++bx) {
+        float lox, hix, loy, hiy, loz, hiz;
+        brick_box(g, bx, by, bz, &lox, &hix, &loy, &hiy, &loz, &hiz);
+        if (!brick_may_hold(xr, yr, zr, sg, g.trunc, lox, hix, loy, hiy, loz, hiz)) continue;
+        const size_t k = b * nb + ((size_t)bz * bricks_y + by) * bricks_x + bx;
+        if (!FILL) {
+          atomicAdd(bins.count + k, 1);
+        } else {
+          const long long at = (long long)bins.offset[k] + atomicAdd(bins.cursor + k, 1);
+          if (at < bins.capacity) bins.list[at] = i;
+        }
+      }
+}
+
+// Exclusive scan of count[0 .. n) into offset (one workgroup, in order), and
+// cursor = 0.  Offsets past the list's capacity make their bricks scan.
+__global__ __launch_bounds__(1024) void splat_scan_kernel(SplatBins bins, int n) {
+  __shared__ long long part[1024];
+  const int tid = threadIdx.x;
+  const int per = (n + 1023) / 1024;
+  const int a = min(tid * per, n), e = min(a + per, n);
+  long long sum = 0;
+  for (int k = a; k < e; ++k) sum += bins.count[k];
+  part[tid] = sum;
+  __syncthreads();
+  for (int d = 1; d < 1024; d <<= 1) {
+    const long long v = tid >= d ? part[tid - d] : 0;
+    __syncthreads();
+    part[tid] += v;
+    __syncthreads();
+  }
+  long long run = tid > 0 ? part[tid - 1] : 0;
+  for (int k = a; k < e; ++k) {
+    bins.offset[k] = (int)(run < 0x7fffffff ? run : 0x7fffffff);
+    bins.cursor[k] = 0;
+    run += bins.count[k];
+  }
+}
+
 __global__ __launch_bounds__(256) void splat_forward_kernel(
     const float* __restrict__ rbf,  // [B][N][4+A]
     float* __restrict__ vol,        // [B][1+A][D][H][W]
-    SplatGeom g, int bricks_x, int bricks_y) {
+    SplatGeom g, int bricks_x, int bricks_y, SplatBins bins, int binned) {
   __shared__ float prm[kChunk * (4 + kMaxAttr)];
-  __shared__ int keep[kChunk];
+  __shared__ int keep[kListMax];
   __shared__ int wave_cnt[4];
   const int C = 4 + g.A;
   const int b = blockIdx.y;
@@ -91,15 +224,8 @@ __global__ __launch_bounds__(256) void splat_forward_kernel(
   const float y = splat_coord(yi, g.H, g.sy, g.ey);
   const float z = splat_coord(zi, g.D, g.sz, g.ez);
   // Brick bounding box over the voxels it really holds.
-  const int x0 = bx * kBX, x1 = min(bx * kBX + kBX, g.W) - 1;
-  const int y0 = by * kBY, y1 = min(by * kBY + kBY, g.H) - 1;
-  const int z0 = bz * kBZ, z1 = min(bz * kBZ + kBZ, g.D) - 1;
-  const float bxa = splat_coord(x0, g.W, g.sx, g.ex), bxb = splat_coord(x1, g.W, g.sx, g.ex);
-  const float bya = splat_coord(y0, g.H, g.sy, g.ey), byb = splat_coord(y1, g.H, g.sy, g.ey);
-  const float bza = splat_coord(z0, g.D, g.sz, g.ez), bzb = splat_coord(z1, g.D, g.sz, g.ez);
-  const float lox = fminf(bxa, bxb), hix = fmaxf(bxa, bxb);
-  const float loy = fminf(bya, byb), hiy = fmaxf(bya, byb);
-  const float loz = fminf(bza, bzb), hiz = fmaxf(bza, bzb);
+  float lox, hix, loy, hiy, loz, hiz;
+  brick_box(g, bx, by, bz, &lox, &hix, &loy, &hiy, &loz, &hiz);
 
   float sum = 0.f;
   float attr[kMaxAttr];
@@ -107,7 +233,62 @@ __global__ __launch_bounds__(256) void splat_forward_kernel(
   for (int a = 0; a < kMaxAttr; ++a) attr[a] = 0.f;
   const float* rb = rbf + (size_t)b * g.N * C;
 
-  for (int c0 = 0; c0 < g.N; c0 += kChunk) {
+  // Binned path: this brick's list, sorted into index order in LDS, then
+  // evaluated in chunks.  Lists past the capacity or kListMax scan instead.
+  int n_list = -1;
+  if (binned) {
+    const size_t k = (size_t)b * gridDim.x + blockIdx.x;
+    const int cnt = bins.count[k];
+    const long long off = bins.offset[k];
+    if (cnt <= kListMax && off + cnt <= bins.capacity) n_list = cnt;
+    if (n_list >= 0) {
+      int np = 1;
+      while (np < n_list) np <<= 1;
+      __syncthreads();
+      for (int e = tid; e < np; e += blockDim.x)
+        keep[e] = e < n_list ? bins.list[off + e] : 0x7fffffff;
+      __syncthreads();
+      // bitonic sort of keep[0 .. np) (ascending)
+      for (int len = 2; len <= np; len <<= 1)
+        for (int st = len >> 1; st > 0; st >>= 1) {
+          for (int e = tid; e < np; e += blockDim.x) {
+            const int o = e ^ st;
+            if (o > e) {
+              const int va = keep[e], vb = keep[o];
+              const bool up = (e & len) == 0;
+              if ((va > vb) == up) {
+                keep[e] = vb;
+                keep[o] = va;
+              }
+            }
+          }
+          __syncthreads();
+        }
+    }
+  }
+  if (n_list >= 0) {
+    for (int c0 = 0; c0 < n_list; c0 += kChunk) {
+      const int nc = min(kChunk, n_list - c0);
+      __syncthreads();
+      for (int e = tid; e < nc * C; e += blockDim.x) {
+        const int r = e / C;
+        prm[e] = rb[(size_t)keep[c0 + r] * C + (e - r * C)];
+      }
+      __syncthreads();
+      for (int k = 0; k < nc; ++k) {
+        const float* p = prm + k * C;
+        const float r2 = rbf_r2(x, y, z, p[0], p[1], p[2]);
+        if (!within(r2, p[3], g.trunc)) continue;
+        const float wgt = gaussian(r2, p[3]);                // :77
+#pragma unroll
+        for (int a = 0; a < kMaxAttr; ++a)
+          if (a < g.A) attr[a] += wgt * p[4 + a];            // :78-80
+        sum += wgt;                                          // :81
+      }
+    }
+  }
+
+  for (int c0 = 0; n_list < 0 && c0 < g.N; c0 += kChunk) {
     const int nc = min(kChunk, g.N - c0);
     __syncthreads();
     for (int e = tid; e < nc * C; e += blockDim.x) prm[e] = rb[(size_t)c0 * C + e];
@@ -117,13 +298,8 @@ __global__ __launch_bounds__(256) void splat_forward_kernel(
     bool pass = false;
     if (tid < nc) {
       if (g.trunc > 0.f) {
-        const float xr = prm[tid * C], yr = prm[tid * C + 1], zr = prm[tid * C + 2];
-        const float sg = prm[tid * C + 3];
-        const float dx = fmaxf(fmaxf(lox - xr, xr - hix), 0.f);
-        const float dy = fmaxf(fmaxf(loy - yr, yr - hiy), 0.f);
-        const float dz = fmaxf(fmaxf(loz - zr, zr - hiz), 0.f);
-        const float lim = g.trunc * fabsf(sg);
-        pass = !((dx * dx + dy * dy + dz * dz) > lim * lim * 1.0001f + 1e-12f);
+        pass = brick_may_hold(prm[tid * C], prm[tid * C + 1], prm[tid * C + 2],
+                              prm[tid * C + 3], g.trunc, lox, hix, loy, hiy, loz, hiz);
       } else {
         pass = true;
       }
@@ -261,6 +437,12 @@ __global__ __launch_bounds__(256) void splat_grad_kernel(
   }
 }
 
+__global__ void fill_zero_i32_kernel(int* __restrict__ p, size_t n) {
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (size_t)gridDim.x * blockDim.x)
+    p[e] = 0;
+}
+
 SplatGeom splat_geom(int64_t B, int64_t N, int64_t A, int64_t D, int64_t H, int64_t W,
                      const float* vs, const float* ve, float trunc) {
   SplatGeom g;
@@ -297,22 +479,101 @@ int check_splat(int64_t B, int64_t N, int64_t A, int64_t D, int64_t H, int64_t W
 
 using namespace totri_hip;
 
+namespace totri_hip {
+namespace {
+
+struct BinLayout {
+  size_t nb;                    // bricks per batch entry
+  size_t count, offset, cursor; // byte offsets of the [B][nb] int arrays
+  size_t list;                  // byte offset of the list
+  long long capacity;           // list entries
+  size_t end;
+};
+
+// Workspace of the binned splat (truncated splats only): three [B][bricks]
+// int arrays and a list of 64 entries per RBF (about 20 at the usual
+// sigma / voxel ratios); bricks whose lists do not fit scan all RBFs.
+BinLayout bin_layout(int64_t B, int64_t N, int64_t D, int64_t H, int64_t W) {
+  BinLayout l;
+  const size_t bx = (size_t)((W + kBX - 1) / kBX), by = (size_t)((H + kBY - 1) / kBY),
+               bz = (size_t)((D + kBZ - 1) / kBZ);
+  l.nb = bx * by * bz;
+  const size_t arr = (((size_t)B * l.nb * sizeof(int)) + 255) & ~(size_t)255;
+  l.count = 0;
+  l.offset = arr;
+  l.cursor = 2 * arr;
+  l.list = 3 * arr;
+  // offsets are ints: at most 2^31 - 1 entries
+  const long long want = 64ll * B * N;
+  l.capacity = want < 0x7fffffffll ? want : 0x7fffffffll;
+  l.end = l.list + (((size_t)l.capacity * sizeof(int) + 255) & ~(size_t)255);
+  return l;
+}
+
+}  // namespace
+}  // namespace totri_hip
+
+extern "C" size_t totri_rbf2volume_gaussian_forward_workspace(int64_t B, int64_t N, int64_t A,
+                                                              int64_t D, int64_t H, int64_t W,
+                                                              float truncation) {
+  (void)A;
+  if (!(truncation > 0.f) || B <= 0 || N <= 0 || D <= 0 || H <= 0 || W <= 0) return 0;
+  return bin_layout(B, N, D, H, W).end;
+}
+
 extern "C" int totri_rbf2volume_gaussian_forward(const float* rbf, int64_t B, int64_t N,
                                                  int64_t A, float* volume, int64_t D,
                                                  int64_t H, int64_t W,
                                                  const float volume_start[3],
                                                  const float volume_end[3],
-                                                 float truncation, void* stream) {
+                                                 float truncation, void* workspace,
+                                                 size_t workspace_bytes, void* stream) {
   int rc = check_splat(B, N, A, D, H, W, volume_start, volume_end);
   if (rc) return rc;
   const size_t V = (size_t)D * H * W;
   if (B == 0 || V == 0) return TOTRI_OK;
   TOTRI_CHECK_ARG(volume != nullptr && (N == 0 || rbf != nullptr), "null pointer");
+  hipStream_t st = (hipStream_t)stream;
   const SplatGeom g = splat_geom(B, N, A, D, H, W, volume_start, volume_end, truncation);
   const int bx = (int)((W + kBX - 1) / kBX), by = (int)((H + kBY - 1) / kBY),
             bz = (int)((D + kBZ - 1) / kBZ);
+  SplatBins bins;
+  memset(&bins, 0, sizeof(bins));
+  const size_t need = totri_rbf2volume_gaussian_forward_workspace(B, N, A, D, H, W, truncation);
+  const int binned = need > 0 ? 1 : 0;
+  if (binned) {
+    if (workspace_bytes < need)
+      return fail(TOTRI_EWORKSPACE, "workspace too small: need %zu bytes, got %zu", need,
+                  workspace_bytes);
+    TOTRI_CHECK_ARG(workspace != nullptr, "null workspace");
+    const BinLayout l = bin_layout(B, N, D, H, W);
+    char* w = (char*)workspace;
+    bins.count = (int*)(w + l.count);
+    bins.offset = (int*)(w + l.offset);
+    bins.cursor = (int*)(w + l.cursor);
+    bins.list = (int*)(w + l.list);
+    bins.capacity = l.capacity;
+    const size_t n = (size_t)B * l.nb;
+    TOTRI_CHECK_ARG(n < (1u << 31), "too many bricks");
+    hipLaunchKernelGGL(fill_zero_i32_kernel, dim3((unsigned)((n + 255) / 256 < 65536 ? (n + 255) / 256 : 65536)),
+                       dim3(256), 0, st, bins.count, n);
+    rc = check_launch("fill_zero_i32_kernel");
+    if (rc) return rc;
+    const dim3 grid((unsigned)((N + 255) / 256), (unsigned)B);
+    hipLaunchKernelGGL(splat_bin_kernel<false>, grid, dim3(256), 0, st, rbf, g, bx, by, bz,
+                       bins);
+    rc = check_launch("splat_bin_kernel");
+    if (rc) return rc;
+    hipLaunchKernelGGL(splat_scan_kernel, dim3(1), dim3(1024), 0, st, bins, (int)n);
+    rc = check_launch("splat_scan_kernel");
+    if (rc) return rc;
+    hipLaunchKernelGGL(splat_bin_kernel<true>, grid, dim3(256), 0, st, rbf, g, bx, by, bz,
+                       bins);
+    rc = check_launch("splat_bin_kernel");
+    if (rc) return rc;
+  }
   hipLaunchKernelGGL(splat_forward_kernel, dim3((unsigned)(bx * by * bz), (unsigned)B),
-                     dim3(256), 0, (hipStream_t)stream, rbf, volume, g, bx, by);
+                     dim3(256), 0, st, rbf, volume, g, bx, by, bins, binned);
   return check_launch("splat_forward_kernel");
 }
 

@@ -33,14 +33,17 @@ class VolumeRenderConfocal(torch.autograd.Function):
     def apply_transposed(transient: torch.Tensor, measurement_points: torch.Tensor,
                          volume_bbox_origin: typing.Tuple, volume_bbox_size: typing.Tuple,
                          volume_resolution: typing.Tuple, bin_width: float,
-                         bin_offset: float) -> torch.Tensor:
+                         bin_offset: float, reference_window: bool = False) -> torch.Tensor:
         """Backprojection R^T applied to `transient` [N, T] -> volume [X, Y, Z]
-        (float32), reference totri/render/volume.py:14-23."""
+        (float32), reference totri/render/volume.py:14-23.  The default is the
+        exact transpose of the render; reference_window=True gives the
+        reference's own window form (volume_cuda.cu:95-160), which also fills
+        the top slice z = Z-1."""
         volume = torch.empty(tuple(volume_resolution), device=transient.device,
                              dtype=torch.float32)
         volume_confocal_grad_volume(volume, measurement_points, transient,
                                     volume_bbox_origin, volume_bbox_size,
-                                    bin_width, bin_offset)
+                                    bin_width, bin_offset, reference_window=reference_window)
         return volume
 
     @staticmethod

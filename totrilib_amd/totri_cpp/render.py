@@ -57,8 +57,12 @@ def volume_confocal_forward(volume: torch.Tensor, measurement_points: torch.Tens
 def volume_confocal_grad_volume(volume: torch.Tensor, measurement_points: torch.Tensor,
                                 transient: torch.Tensor, volume_origin: Vec3,
                                 volume_size: Vec3, bin_width: float,
-                                bin_offset: float) -> None:
+                                bin_offset: float, reference_window: bool = False) -> None:
     """A1: volume[X, Y, Z] = R1^T(transient[N, T]); the exact transpose of R1.
+
+    reference_window=True computes the reference's own window form instead
+    (volume_cuda.cu:95-160): it adds, on the top slice z = Z-1, samples the
+    render never reads, and decides each voxel's bins by its window.
 
     Reference: src/render/volume.cpp:25-41, src/render/volume_cuda.cu:95-186.
     """
@@ -80,6 +84,15 @@ def volume_confocal_grad_volume(volume: torch.Tensor, measurement_points: torch.
     mp = measurement_points.contiguous()
     tr = transient.contiguous()
     out = OutputSlot(volume)
+    if reference_window:
+        with torch.cuda.device(dev):
+            rc = lib.totri_volume_confocal_grad_volume_window(
+                out.ptr(), X, Y, Z, mp.data_ptr(), N, tr.data_ptr(), T, code,
+                _lib.f3(volume_origin), _lib.f3(volume_size), float(bin_width),
+                float(bin_offset), stream_ptr(dev))
+            _lib.check(rc, "volume_confocal_grad_volume_window")
+            out.finish()
+        return
     with torch.cuda.device(dev):
         ws, wsp = workspace(lib.totri_volume_confocal_grad_volume_workspace(X, Y, Z, N, T), dev)
         rc = lib.totri_volume_confocal_grad_volume(

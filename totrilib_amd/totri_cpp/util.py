@@ -11,7 +11,8 @@ import typing
 import torch
 
 from totrilib_amd import _lib
-from totrilib_amd.totri_cpp._common import OutputSlot, require_f32, require_gpu, stream_ptr
+from totrilib_amd.totri_cpp._common import (OutputSlot, require_f32, require_gpu,
+                                            stream_ptr, workspace)
 
 Vec3 = typing.Sequence[float]
 
@@ -40,9 +41,12 @@ def rbf2volume_gaussian_forward(rbf: torch.Tensor, volume: torch.Tensor, volume_
     lib = _lib.load()
     out = OutputSlot(volume)
     with torch.cuda.device(dev):
+        ws, wsp = workspace(lib.totri_rbf2volume_gaussian_forward_workspace(
+            B, N, A, D, H, W, float(truncation)), dev)
         rc = lib.totri_rbf2volume_gaussian_forward(
             r.data_ptr(), B, N, A, out.ptr(), D, H, W, _lib.f3(volume_start),
-            _lib.f3(volume_end), float(truncation), stream_ptr(dev))
+            _lib.f3(volume_end), float(truncation), wsp, ws.numel() if ws is not None else 0,
+            stream_ptr(dev))
         _lib.check(rc, "rbf2volume_gaussian_forward")
         out.finish()
 
