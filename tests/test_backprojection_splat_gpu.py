@@ -167,15 +167,19 @@ def test_exhaustive_render_and_adjoint_match_oracle():
     assert rel_l2(v.grad.cpu().numpy(), adj) < 1e-6
 
 
-@pytest.mark.parametrize("trunc", [0.0, 3.0])
-def test_splat_forward_and_grad_match_oracle(trunc, report):
+@pytest.mark.parametrize("trunc,A", [(0.0, 1), (3.0, 1), (0.0, 0), (0.0, 3), (3.0, 3)])
+def test_splat_forward_and_grad_match_oracle(trunc, A, report):
+    """Forward and RBF gradient against the oracle with 0, 1 and 3 attributes
+    per RBF, untruncated (shared voxel tiles) and truncated at 3 sigma (one
+    wave per RBF over its support box)."""
     from totrilib_amd.types import VolumeDef
     from totrilib_amd.util import Rbf2VolumeGaussian
-    rng = np.random.default_rng(56)
+    rng = np.random.default_rng(56 + A)
     N = 300
     res = (40, 36, 32)
     rbf = np.concatenate([rng.uniform(-1, 1, (1, N, 2)), rng.uniform(0.5, 2.5, (1, N, 1)),
-                          rng.uniform(0.02, 0.1, (1, N, 1)), rng.random((1, N, 1))], 2).astype(np.float32)
+                          rng.uniform(0.02, 0.1, (1, N, 1)), rng.random((1, N, A))],
+                         2).astype(np.float32)
     r = t(rbf).requires_grad_(True)
     vol = Rbf2VolumeGaussian.apply(r, VolumeDef(VS, VE, res), trunc)
     got = vol.detach().cpu().numpy()
@@ -185,7 +189,7 @@ def test_splat_forward_and_grad_match_oracle(trunc, report):
     (vol * t(G)).sum().backward()
     gref = O.splat_grad(rbf, ref, G, VS, VE, trunc=trunc, mode=O.PINNED64)
     e = rel_l2(r.grad.cpu().numpy(), gref)
-    report(f"splat trunc={trunc}: splat rel-L2 vs FP32 oracle {rel_l2(got, ref):.2e}, "
+    report(f"splat trunc={trunc}, {A} attributes: splat rel-L2 vs FP32 oracle {rel_l2(got, ref):.2e}, "
            f"RBF gradient rel-L2 vs FP64 {e:.2e}")
     assert e < 1e-5
 

@@ -44,7 +44,11 @@ constexpr int kWaves = 8;            // scan points per R1 workgroup
 constexpr int kChunkVoxels = 4096;   // voxels per R1 LDS chunk buffer (32 KiB of pairs)
 constexpr int kSlab = 32;            // A1 z-slab depth
 constexpr int kFlush = 32;           // A1 scan points per first-level sum
-constexpr int kSlabQ = 8;            // xy-mirror A1 z-slab depth
+constexpr int kSlabQ = 8;            // xy-mirror and x-pair A1 row-walk z-slab depth
+// General A1 row-walk z-slab depth.  Deeper slabs amortise a walk's start
+// over more rows but measured slower at 128^3 / 64^2 / 1024 (16 rows 27.2 ms,
+// 32 rows 27.7 ms with four times the scan ranges, against 26.2 ms).
+constexpr int kSlabG = 8;
 
 struct ConfGeom {
   int X, Y, Z, N, T;
@@ -1167,15 +1171,17 @@ __global__ void sum_partials_flag_kernel(const float* __restrict__ part,
   }
 }
 
-// Boundary rows of the xy-mirror A1 for element e of an [X][Y][Z] volume of
-// n voxels: voxel z = k * kSlabQ (k >= 1) receives the alpha part of slab
-// k - 1's top sample row, summed over the n_parts scan ranges in order.
+// Boundary rows of a row walk for element e of an [X][Y][Z] volume of n
+// voxels: with slabs of slab_h rows, voxel z = k * slab_h (k >= 1) receives
+// the alpha part of slab k - 1's top sample row, summed over the n_parts
+// scan ranges in order.
 __device__ __forceinline__ float a1_boundary(const float* __restrict__ bnd, size_t e,
-                                             size_t n, int Z, int n_slabs, int n_parts) {
+                                             size_t n, int Z, int n_slabs, int n_parts,
+                                             int slab_h) {
   const int z = (int)(e % (size_t)Z);
-  if (z == 0 || z % kSlabQ) return 0.f;
+  if (z == 0 || z % slab_h) return 0.f;
   const size_t ncol = n / (size_t)Z, col = e / (size_t)Z;
-  const int sb = z / kSlabQ - 1;
+  const int sb = z / slab_h - 1;
   float b = bnd[col * n_slabs + sb];
   for (int p = 1; p < n_parts; ++p) b += bnd[((size_t)p * ncol + col) * n_slabs + sb];
   return b;
@@ -1186,23 +1192,26 @@ __device__ __forceinline__ float a1_boundary(const float* __restrict__ bnd, size
 // pair row walk with its boundary rows bnd_pair when given, else the window
 // walk), else the general kernel (its boundary rows added when bnd_gen is
 // given, the row walk); each has its own partial count.  info may be null:
-// no mirror path.  The row walks' boundary rows use n_slabs = ceil(Z / kSlabQ).
+// no mirror path.  The mirror row walks' boundary rows use n_slabs =
+// ceil(Z / kSlabQ), the general row walk's n_slabs_gen = ceil(Z / kSlabG).
 __global__ void sum_partials_a1_kernel(const float* __restrict__ part,
                                        const float* __restrict__ bnd,
                                        const int* __restrict__ info, int quad_on, int n_quad,
                                        int n_pair, int n_gen, size_t n, int Z, int n_slabs,
-                                       const float* __restrict__ bnd_gen,
+                                       int n_slabs_gen, const float* __restrict__ bnd_gen,
                                        const float* __restrict__ bnd_pair,
                                        float* __restrict__ out) {
   const bool quad = info && quad_on && info[0];
   const bool pair = info && !quad && info[2];
   const int n_parts = quad ? n_quad : pair ? n_pair : n_gen;
   const float* b = quad ? bnd : pair ? bnd_pair : bnd_gen;
+  const int slab_h = (quad || pair) ? kSlabQ : kSlabG;
+  const int slabs = (quad || pair) ? n_slabs : n_slabs_gen;
   for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
        e += (size_t)gridDim.x * blockDim.x) {
     float v = part[e];
     for (int p = 1; p < n_parts; ++p) v += part[(size_t)p * n + e];
-    if (b) v += a1_boundary(b, e, n, Z, n_slabs, n_parts);
+    if (b) v += a1_boundary(b, e, n, Z, slabs, n_parts, slab_h);
     out[e] = v;
   }
 }
@@ -2154,17 +2163,17 @@ __device__ __forceinline__ void a1_row_item(
   const float y = cell_centre(yi, g.oy, g.szy, g.Y);
 
   // Slab rows as in a1_quad_item: sample rows z0 .. z0 + rspan, LDS row k for
-  // voxel z0 + k, row kSlabQ for the next slab's first voxel (boundary).
-  const int z0 = slab * kSlabQ;
-  const int rspan = min(kSlabQ, (g.Z - 1) - z0) - 1;
+  // voxel z0 + k, row kSlabG for the next slab's first voxel (boundary).
+  const int z0 = slab * kSlabG;
+  const int rspan = min(kSlabG, (g.Z - 1) - z0) - 1;
   const int zshift = z0;
   const float zlo = (float)z0;
-  const float zhi = (float)min(z0 + kSlabQ, g.Z - 1);
+  const float zhi = (float)min(z0 + kSlabG, g.Z - 1);
 
-  for (int r = 0; r <= kSlabQ; ++r) acc[r * 64 + lane] = 0.f;
-  float tot[kSlabQ + 1];
+  for (int r = 0; r <= kSlabG; ++r) acc[r * 64 + lane] = 0.f;
+  float tot[kSlabG + 1];
 #pragma unroll
-  for (int r = 0; r <= kSlabQ; ++r) tot[r] = 0.f;
+  for (int r = 0; r <= kSlabG; ++r) tot[r] = 0.f;
 
   const int per = (g.N + n_ranges - 1) / n_ranges;
   const int i_begin = range * per;
@@ -2174,7 +2183,7 @@ __device__ __forceinline__ void a1_row_item(
   for (int i = i_begin; i < i_end; ++i) {
     if (i > i_begin && (i - i_begin) % kFlush == 0) {
 #pragma unroll
-      for (int r = 0; r <= kSlabQ; ++r) {
+      for (int r = 0; r <= kSlabG; ++r) {
         tot[r] += acc[r * 64 + lane];
         acc[r * 64 + lane] = 0.f;
       }
@@ -2198,8 +2207,11 @@ __device__ __forceinline__ void a1_row_item(
     if (!wave_any(tlo <= thi)) continue;
     // Per-step stop test only where some lane's range reaches the last bins
     // (see a1_quad_item).
-    auto walk = [&](auto sc) {
+    // OFF0: a zero bin offset, where t w + 0 rounds to t w, so the step's
+    // radius needs no addition.
+    auto walk = [&](auto sc, auto oc) {
       constexpr bool STOP = decltype(sc)::value;
+      constexpr bool OFF0 = decltype(oc)::value;
       const float* kp = Kf + (size_t)i * t_stride;
       auto decide_bin = [&](int tt, float* d2o, float* alo) -> int {
         const float tc = bin_distance(tt, g.w, g.off);
@@ -2213,7 +2225,7 @@ __device__ __forceinline__ void a1_row_item(
       };
       auto decide_bin_f = [&](float tf, float* d2o, float* alo) -> int {
         float tc = tf * g.w;
-        tc = tc + g.off;
+        if (!OFF0) tc = tc + g.off;
         const float d2 = (tc * tc - dx2) - dy2;
         bool nr;
         int r = decide_fast(g, d2, cp, delta, half_m_delta, alo, &nr);
@@ -2289,22 +2301,30 @@ __device__ __forceinline__ void a1_row_item(
       }
       if (rr <= rspan + 1) my[rr * 64] += carry;
     };
-    if (wave_any(thi > g.T - 3))
-      walk(std::true_type{});
-    else
-      walk(std::false_type{});
+    const bool stop = wave_any(thi > g.T - 3);
+    if (g.off == 0.f) {
+      if (stop)
+        walk(std::true_type{}, std::true_type{});
+      else
+        walk(std::false_type{}, std::true_type{});
+    } else {
+      if (stop)
+        walk(std::true_type{}, std::false_type{});
+      else
+        walk(std::false_type{}, std::false_type{});
+    }
   }
 
   if (cvalid) {
     float* out = dst + (((size_t)range * g.X + xi) * g.Y + yi) * g.Z;
 #pragma unroll
-    for (int r = 0; r < kSlabQ; ++r) {
+    for (int r = 0; r < kSlabG; ++r) {
       const int z = z0 + r;
       if (z < g.Z) out[z] = tot[r] + acc[r * 64 + lane];
     }
-    if (z0 + kSlabQ < g.Z)
+    if (z0 + kSlabG < g.Z)
       bnd[(((size_t)range * g.X + xi) * g.Y + yi) * n_slabs + slab] =
-          tot[kSlabQ] + acc[kSlabQ * 64 + lane];
+          tot[kSlabG] + acc[kSlabG * 64 + lane];
   }
 }
 
@@ -2317,7 +2337,7 @@ __global__ __launch_bounds__(64) void a1_row_kernel(
     ConfGeom g, int tiles_x, int tiles_y, int n_slabs, int n_ranges, int t_stride,
     const int* __restrict__ skip) {
   if (skip && *skip) return;
-  __shared__ float acc[(kSlabQ + 1) * 64];
+  __shared__ float acc[(kSlabG + 1) * 64];
   a1_row_item<SAFE>(Kf, itc4t, mp, dst, bnd, g, (int)blockIdx.x, tiles_x, tiles_y, n_slabs,
                     n_ranges, t_stride, acc);
 }
@@ -2791,7 +2811,7 @@ namespace {
 
 struct A1Plan {
   int tiles_x, tiles_y, n_slabs, n_ranges, t_stride;
-  // general row walk (a1_row_kernel): 8x8 column tiles, kSlabQ-voxel slabs
+  // general row walk (a1_row_kernel): 8x8 column tiles, kSlabG-voxel slabs
   int row_tiles_x, row_tiles_y, row_slabs, row_ranges;
   bool safe;
   size_t tables_bytes, partial_bytes;
@@ -2838,7 +2858,7 @@ A1Plan plan_a1(int64_t X, int64_t Y, int64_t Z, int64_t N, int64_t T, float w,
   p.tables_bytes = (p.tables_bytes + 255) & ~(size_t)255;
   p.row_tiles_x = (int)((X + 7) / 8);
   p.row_tiles_y = (int)((Y + 7) / 8);
-  p.row_slabs = (int)((Z + kSlabQ - 1) / kSlabQ);
+  p.row_slabs = (int)((Z + kSlabG - 1) / kSlabG);
   {
     const int64_t items = (int64_t)p.row_tiles_x * p.row_tiles_y * p.row_slabs;
     const int64_t target = 64 * (int64_t)device_cu_count();
@@ -3202,7 +3222,8 @@ extern "C" int totri_volume_confocal_grad_volume(
     hipLaunchKernelGGL(sum_partials_a1_kernel, dim3(launch_grid(nv)), dim3(256), 0, st,
                        partial, sym ? (const float*)((char*)workspace + q.bnd_off) : nullptr,
                        info, (int)quad, q.quad.n_ranges, prow ? q.prow.n_ranges : q.n_ranges,
-                       row ? p.row_ranges : p.n_ranges, nv, (int)Z, p.row_slabs,
+                       row ? p.row_ranges : p.n_ranges, nv, (int)Z,
+                       (int)((Z + kSlabQ - 1) / kSlabQ), p.row_slabs,
                        row ? row_bnd : nullptr,
                        prow ? (const float*)((char*)workspace + q.pbnd_off) : nullptr, volume);
     return check_launch("sum_partials_a1_kernel");

@@ -437,6 +437,147 @@ __global__ __launch_bounds__(256) void splat_grad_kernel(
   }
 }
 
+// S2 without truncation: every RBF sees every voxel, so a wave per RBF would
+// read the whole volume once per RBF.  Here a workgroup's waves own
+// kGradRpw RBFs each and share voxel tiles staged in LDS: per voxel the tile
+// holds its coordinates, dL/dV0, 1/V0 (guarded as in the forward), the sum
+// over attributes of dL/dVa * Va and dL/dVa, structure of arrays.  Each lane
+// sums its voxels in tile order, then the wave reduces in a fixed order.
+constexpr int kGradWaves = 8;        // waves per workgroup
+constexpr int kGradTileFloats = 14336;
+
+// AT: the attribute count when known at compile time (0 or 1), or -1 (any,
+// up to kMaxAttr, read from g.A).  RPW: RBFs per wave.
+template <int AT, int RPW>
+__global__ __launch_bounds__(kGradWaves * 64) void splat_grad_all_kernel(
+    const float* __restrict__ rbf, const float* __restrict__ vol,
+    const float* __restrict__ vgrad, float* __restrict__ rbf_grad, SplatGeom g, int tv) {
+  constexpr int AN = AT >= 0 ? AT : kMaxAttr;  // attribute slots
+  extern __shared__ float tile[];  // [6 + A][tv]
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int b = blockIdx.y;
+  const int A = AT >= 0 ? AT : g.A;
+  const int C = 4 + A;
+  const int i0 = (blockIdx.x * kGradWaves + wave) * RPW;
+  // this wave's RBFs (indices past N carry sigma 1 and are not written)
+  float xr[RPW], yr[RPW], zr[RPW], si2[RPW], si3[RPW], nh[RPW], av[RPW][AN > 0 ? AN : 1];
+#pragma unroll
+  for (int r = 0; r < RPW; ++r) {
+    const int i = i0 + r;
+    const float* p = rbf + ((size_t)b * g.N + (i < g.N ? i : 0)) * C;
+    xr[r] = p[0];
+    yr[r] = p[1];
+    zr[r] = p[2];
+    const float sg = i < g.N ? p[3] : 1.f;
+    const float si = 1.f / sg;                                 // :38
+    si2[r] = si * si;
+    si3[r] = si2[r] * si;
+    nh[r] = -0.5f * si2[r];
+#pragma unroll
+    for (int a = 0; a < AN; ++a) av[r][a] = a < A ? p[4 + a] : 0.f;
+  }
+  float acc[RPW][4 + AN];
+#pragma unroll
+  for (int r = 0; r < RPW; ++r)
+#pragma unroll
+    for (int c = 0; c < 4 + AN; ++c) acc[r][c] = 0.f;
+
+  const int V = g.D * g.H * g.W;
+  const float* v0b = vol + (size_t)b * (1 + A) * V;
+  const float* g0b = vgrad + (size_t)b * (1 + A) * V;
+  float* tx = tile;
+  float* ty = tile + tv;
+  float* tz = tile + 2 * tv;
+  float* tg = tile + 3 * tv;
+  float* ti = tile + 4 * tv;
+  float* tp = tile + 5 * tv;
+  float* ta = tile + 6 * tv;  // [A][tv]
+  for (int v0 = 0; v0 < V; v0 += tv) {
+    const int nt = min(tv, V - v0);
+    __syncthreads();
+    for (int e = threadIdx.x; e < nt; e += blockDim.x) {
+      const int v = v0 + e;
+      const int xi = v % g.W, yi = (v / g.W) % g.H, zi = v / (g.W * g.H);
+      tx[e] = splat_coord(xi, g.W, g.sx, g.ex);
+      ty[e] = splat_coord(yi, g.H, g.sy, g.ey);
+      tz[e] = splat_coord(zi, g.D, g.sz, g.ez);
+      tg[e] = g0b[v];
+      if (A > 0) {
+        float w0 = v0b[v];
+        if (w0 < 1.e-8f) w0 = 1.f;                             // :167-169
+        ti[e] = 1.f / w0;                                      // :170
+        float pa = 0.f;
+        for (int a = 0; a < A; ++a) {
+          const float dla = g0b[(size_t)(1 + a) * V + v];
+          pa += dla * v0b[(size_t)(1 + a) * V + v];
+          ta[a * tv + e] = dla;
+        }
+        tp[e] = pa;
+      }
+    }
+    __syncthreads();
+    for (int e = lane; e < nt; e += 64) {
+      const float x = tx[e], y = ty[e], z = tz[e], dl0 = tg[e];
+      float inv = 0.f, pa = 0.f, dla[AN > 0 ? AN : 1];
+      if (AN > 0) {
+        inv = ti[e];
+        pa = tp[e];
+#pragma unroll
+        for (int a = 0; a < AN; ++a) dla[a] = (AT >= 0 || a < A) ? ta[a * tv + e] : 0.f;
+      }
+#pragma unroll
+      for (int r = 0; r < RPW; ++r) {
+        const float dx = x - xr[r], dy = y - yr[r], dz = z - zr[r];
+        float r2 = dx * dx;
+        r2 = r2 + dy * dy;
+        r2 = r2 + dz * dz;
+        const float wgt = expf(nh[r] * r2);                    // :15-29, :150
+        const float ws = si2[r] * wgt;
+        const float dxr = dx * ws, dyr = dy * ws, dzr = dz * ws; // :39-41
+        const float dsr = (r2 * si3[r]) * wgt;                   // :42-46
+        float t = dl0;
+        if (AN > 0) {
+          float sa = 0.f;
+#pragma unroll
+          for (int a = 0; a < AN; ++a) {
+            sa += dla[a] * av[r][a];
+            acc[r][4 + a] += (dla[a] * wgt) * inv;             // :178
+          }
+          t = dl0 + (sa - pa) * inv;                           // :176-180
+        }
+        acc[r][0] += dxr * t;                                  // :181-184
+        acc[r][1] += dyr * t;
+        acc[r][2] += dzr * t;
+        acc[r][3] += dsr * t;
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < RPW; ++r) {
+    const int i = i0 + r;
+    float* out = rbf_grad + ((size_t)b * g.N + (i < g.N ? i : 0)) * C;
+#pragma unroll
+    for (int c = 0; c < 4 + AN; ++c) {
+      if (c < C) {
+        const float sum = wave_sum_f32(acc[r][c]);
+        if (lane == 0 && i < g.N) out[c] = sum;
+      }
+    }
+  }
+}
+
+template <int AT, int RPW>
+int launch_grad_all(const float* rbf, const float* volume, const float* volume_grad,
+                    float* rbf_grad, const SplatGeom& g, hipStream_t st) {
+  const int per = kGradWaves * RPW;
+  const int tv = (kGradTileFloats / (6 + g.A)) & ~63;
+  hipLaunchKernelGGL((splat_grad_all_kernel<AT, RPW>),
+                     dim3((unsigned)((g.N + per - 1) / per), (unsigned)g.B),
+                     dim3(kGradWaves * 64), (size_t)(6 + g.A) * tv * sizeof(float), st, rbf,
+                     volume, volume_grad, rbf_grad, g, tv);
+  return check_launch("splat_grad_all_kernel");
+}
+
 __global__ void fill_zero_i32_kernel(int* __restrict__ p, size_t n) {
   for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
        e += (size_t)gridDim.x * blockDim.x)
@@ -589,6 +730,13 @@ extern "C" int totri_rbf2volume_gaussian_grad(const float* rbf, int64_t B, int64
   if (B == 0 || N == 0) return TOTRI_OK;
   TOTRI_CHECK_ARG(rbf && volume && volume_grad && rbf_grad, "null pointer");
   const SplatGeom g = splat_geom(B, N, A, D, H, W, volume_start, volume_end, truncation);
+  if (!(truncation > 0.f)) {
+    // every RBF reaches every voxel: shared voxel tiles
+    hipStream_t st = (hipStream_t)stream;
+    if (A == 0) return launch_grad_all<0, 8>(rbf, volume, volume_grad, rbf_grad, g, st);
+    if (A == 1) return launch_grad_all<1, 8>(rbf, volume, volume_grad, rbf_grad, g, st);
+    return launch_grad_all<-1, 4>(rbf, volume, volume_grad, rbf_grad, g, st);
+  }
   hipLaunchKernelGGL(splat_grad_kernel, dim3((unsigned)((N + 3) / 4), (unsigned)B), dim3(256),
                      0, (hipStream_t)stream, rbf, volume, volume_grad, rbf_grad, g);
   return check_launch("splat_grad_kernel");
