@@ -113,6 +113,31 @@ int totri_volume_confocal_grad_volume_window(
     float bin_offset, void* stream);
 
 /*
+ * Double precision: R1, A1 and A2 for float64 tensors, as the reference
+ * dispatches float and double (AT_DISPATCH_FLOATING_TYPES,
+ * src/render/volume_cuda.cu:85,175, src/reconstruct/backprojection_cuda.cu:215).
+ * They follow the reference's double semantics: float geometry, the bin
+ * radius t * w + off in float, floorf/ceilf narrowing to float, and for A2
+ * float points and distances with double bin arithmetic and sums.
+ * window_form = 1 selects the reference's window form of A1, 0 the exact
+ * transpose of totri_volume_confocal_forward_f64.  No workspace.
+ */
+int totri_volume_confocal_forward_f64(
+    const double* volume, int64_t X, int64_t Y, int64_t Z, const double* measurement_points,
+    int64_t N, double* transient, int64_t T, const float volume_origin[3],
+    const float volume_size[3], float bin_width, float bin_offset, void* stream);
+int totri_volume_confocal_grad_volume_f64(
+    double* volume, int64_t X, int64_t Y, int64_t Z, const double* measurement_points,
+    int64_t N, const double* transient, int64_t T, const float volume_origin[3],
+    const float volume_size[3], float bin_width, float bin_offset, int window_form,
+    void* stream);
+int totri_backprojection_confocal_velten_forward_f64(
+    const double* transient, int64_t B, int64_t T, int64_t S, const double* scan_points,
+    const double* scan_origin, int64_t So, double* volume, int64_t D, int64_t H, int64_t W,
+    const float volume_start[3], const float volume_end[3], float bin_width, float bin_offset,
+    void* stream);
+
+/*
  * A6 -- gradient of sum(G * R1(volume; w)) with respect to the wall points w.
  * Replaces totri_cpp.render.volume_confocal_grad_measurement_points
  * (src/render/volume.h:26-33, src/render/volume.cpp:43-81,

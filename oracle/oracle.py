@@ -267,3 +267,62 @@ def r1_contraction_flips(mp, T, X, Y, Z, origin, size, w, off, i0=0, i1=None):
     lib().oracle_r1_contraction_flips(_p(m), N, T, X, Y, Z, _f3(origin), _f3(size),
                                       ctypes.c_float(w), ctypes.c_float(off), i0, i1, _p(out))
     return tuple(int(v) for v in out)
+
+
+# -- the reference's double dispatch (see totri_oracle.c) ---------------------
+def r1_forward_f64(volume, mp, T, origin, size, w, off, i0=0, i1=None):
+    """R1 with scalar_t = double: transient rows [i1-i0, T]."""
+    vol = _f64(volume)
+    m = _f64(mp)
+    X, Y, Z = vol.shape
+    N = m.shape[1]
+    i1 = N if i1 is None else i1
+    out = np.zeros((max(i1 - i0, 0), T), dtype=np.float64)
+    lib().oracle_r1_forward_f64(_p(vol), X, Y, Z, _p(m), N, T, _f3(origin), _f3(size),
+                                ctypes.c_float(w), ctypes.c_float(off), _p(out), i0, i1)
+    return out
+
+
+def _cols(cols):
+    if cols is None:
+        return None, None, 0
+    c = np.ascontiguousarray(np.asarray(cols, dtype=np.int32))
+    return c, _p(c), c.shape[0]
+
+
+def a1_transpose_f64(G, mp, X, Y, Z, origin, size, w, off, cols=None):
+    """Exact transpose of r1_forward_f64 applied to G [N, T] -> [X, Y, Z]."""
+    g = _f64(G)
+    m = _f64(mp)
+    N, T = g.shape
+    out = np.zeros((X, Y, Z), dtype=np.float64)
+    keep, cp, nc = _cols(cols)
+    lib().oracle_a1_transpose_f64(_p(g), _p(m), N, T, X, Y, Z, _f3(origin), _f3(size),
+                                  ctypes.c_float(w), ctypes.c_float(off), cp, nc, _p(out))
+    return out
+
+
+def a1_reference_f64(G, mp, X, Y, Z, origin, size, w, off, cols=None):
+    """The reference's window-form A1 with scalar_t = double."""
+    g = _f64(G)
+    m = _f64(mp)
+    N, T = g.shape
+    out = np.zeros((X, Y, Z), dtype=np.float64)
+    keep, cp, nc = _cols(cols)
+    lib().oracle_a1_reference_f64(_p(g), _p(m), N, T, X, Y, Z, _f3(origin), _f3(size),
+                                  ctypes.c_float(w), ctypes.c_float(off), cp, nc, _p(out))
+    return out
+
+
+def velten_confocal_f64(tr, scan, sorigin, res, vs, ve, w, off):
+    """A2 with scalar_t = double: tr [B,T,S], scan [B,S,3], sorigin [B,So,3]."""
+    t = _f64(tr)
+    s = _f64(scan)
+    B, T, S = t.shape
+    so = _f64(sorigin) if sorigin is not None else np.zeros((B, 0, 3))
+    D, H, W = res
+    out = np.zeros((B, D, H, W))
+    lib().oracle_velten_confocal_f64(_p(t), B, T, S, _p(s), _p(so), so.shape[1], D, H, W,
+                                     _f3(vs), _f3(ve), ctypes.c_float(w), ctypes.c_float(off),
+                                     _p(out))
+    return out

@@ -1215,3 +1215,205 @@ void oracle_r1_contraction_flips(const float* mp, int N, int T, int X, int Y, in
   out[1] = aflip;
   out[2] = zflip;
 }
+
+/* ------------------------------------------------------------------------ */
+/* The reference's double dispatch (AT_DISPATCH_FLOATING_TYPES with          */
+/* scalar_t = double: volume_cuda.cu:85,175; backprojection_cuda.cu:215).    */
+/* Geometry arguments stay float, and some steps keep float arithmetic:      */
+/*   tc = ti * bin_width + bin_offset is a float expression, widened         */
+/*        (volume_cuda.cu:23,137)                                             */
+/*   floorf(z_hat) narrows z_hat to float before the floor (:47,:147)        */
+/*   ceilf(tf) narrows tf to float (:132-134)                                 */
+/*   A2 takes its points as float3 and measures float distances              */
+/*        (backprojection_cuda.cu:167-183); only the bin arithmetic and the   */
+/*        sum are double (:185-190), and floorf narrows again.                */
+/* pow(x, 2) and pow(x, 4) are restated as products (CUDA's pow: unpinned).   */
+/* ------------------------------------------------------------------------ */
+static inline double r1_x64d(int xi, float o, float sz, int n) {
+  /* volume_cuda.cu:35  scalar_t(o) + (xi + scalar_t(0.5)) * scalar_t(sz) / scalar_t(n) */
+  double a = ((double)xi + 0.5) * (double)sz;
+  a = a / (double)n;
+  return (double)o + a;
+}
+
+typedef struct {
+  int active;
+  int zi;
+  double alpha;
+  double f2;
+} r1_sample_d;
+
+static inline r1_sample_d r1_sample_f64(double tc, double wx, double wy, double wz,
+                                        double x, double y, double o, double s, int Z) {
+  r1_sample_d r = {0, 0, 0.0, 0.0};
+  const double t2 = tc * tc;                /* :40 left to right */
+  const double dx = wx - x, dy = wy - y;
+  double d2 = t2 - dx * dx;
+  d2 = d2 - dy * dy;
+  if (!(d2 > 0.0)) return r;                /* :41 */
+  const double depth = sqrt(d2);
+  const double z = wz + depth;              /* :45 */
+  double zh = (z - o) * s;
+  zh = zh - 0.5;                            /* :47 */
+  const int zi = (int)floorf((float)zh);    /* :48 floorf narrows to float */
+  if (!(zi >= 0 && zi + 1 < Z)) return r;   /* :50 */
+  r.active = 1;
+  r.zi = zi;
+  r.alpha = zh - (double)zi;                /* :52 */
+  const double q = -depth / t2;
+  const double q2 = q * q;
+  r.f2 = q2 * q2;                           /* :58 */
+  return r;
+}
+
+/* R1 in double, transient rows [i0, i1) of [N][T]. */
+void oracle_r1_forward_f64(const double* vol, int X, int Y, int Z, const double* mp, int N,
+                           int T, const float* origin, const float* size, float w, float off,
+                           double* out, int i0, int i1) {
+  const double o = (double)origin[2];
+  const double s = (double)Z / (double)size[2];   /* :29 */
+#pragma omp parallel for collapse(2) schedule(dynamic, 16)
+  for (int i = i0; i < i1; ++i) {
+    for (int ti = 0; ti < T; ++ti) {
+      const double wx = mp[i], wy = mp[N + i], wz = mp[2 * N + i];
+      const double tc = (double)r1_tc32(ti, w, off);   /* :23 float, widened */
+      double value = 0.0;
+      for (int xi = 0; xi < X; ++xi) {
+        const double x = r1_x64d(xi, origin[0], size[0], X);
+        for (int yi = 0; yi < Y; ++yi) {
+          const double y = r1_x64d(yi, origin[1], size[1], Y);
+          const r1_sample_d sm = r1_sample_f64(tc, wx, wy, wz, x, y, o, s, Z);
+          if (!sm.active) continue;
+          const double* col = vol + ((size_t)xi * Y + yi) * Z;
+          const double f1 = (1.0 - sm.alpha) * col[sm.zi] + sm.alpha * col[sm.zi + 1]; /* :55 */
+          value = value + f1 * sm.f2;                                                  /* :60 */
+        }
+      }
+      out[(size_t)(i - i0) * T + ti] = value;
+    }
+  }
+}
+
+/* A1 in double as the exact transpose of oracle_r1_forward_f64: each sample
+ * adds (1 - alpha) f2 G into zi and alpha f2 G into zi + 1, scan points and
+ * bins in order.  Columns listed in `cols` (pairs xi, yi), all when NULL. */
+void oracle_a1_transpose_f64(const double* G, const double* mp, int N, int T, int X, int Y,
+                             int Z, const float* origin, const float* size, float w, float off,
+                             const int* cols, int ncols, double* out) {
+  const double o = (double)origin[2];
+  const double s = (double)Z / (double)size[2];
+  const int nc = cols ? ncols : X * Y;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int k = 0; k < nc; ++k) {
+    const int xi = cols ? cols[2 * k] : k / Y;
+    const int yi = cols ? cols[2 * k + 1] : k % Y;
+    double* col = out + ((size_t)xi * Y + yi) * Z;
+    for (int z = 0; z < Z; ++z) col[z] = 0.0;
+    const double x = r1_x64d(xi, origin[0], size[0], X);
+    const double y = r1_x64d(yi, origin[1], size[1], Y);
+    for (int i = 0; i < N; ++i) {
+      const double wx = mp[i], wy = mp[N + i], wz = mp[2 * N + i];
+      for (int ti = 0; ti < T; ++ti) {
+        const double tc = (double)r1_tc32(ti, w, off);
+        const r1_sample_d sm = r1_sample_f64(tc, wx, wy, wz, x, y, o, s, Z);
+        if (!sm.active) continue;
+        const double wg = sm.f2 * G[(size_t)i * T + ti];
+        col[sm.zi] += (1.0 - sm.alpha) * wg;
+        col[sm.zi + 1] += sm.alpha * wg;
+      }
+    }
+  }
+}
+
+/* The reference's window-form A1 in double (volume_cuda.cu:95-160). */
+void oracle_a1_reference_f64(const double* G, const double* mp, int N, int T, int X, int Y,
+                             int Z, const float* origin, const float* size, float w, float off,
+                             const int* cols, int ncols, double* out) {
+  const double o = (double)origin[2];
+  const double s = (double)Z / (double)size[2];   /* :112 */
+  const double dz = (double)size[2] / (double)Z;  /* :114 */
+  const int nc = cols ? ncols : X * Y;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int k = 0; k < nc; ++k) {
+    const int xi = cols ? cols[2 * k] : k / Y;
+    const int yi = cols ? cols[2 * k + 1] : k % Y;
+    const double x = r1_x64d(xi, origin[0], size[0], X);   /* :107 */
+    const double y = r1_x64d(yi, origin[1], size[1], Y);   /* :108 */
+    for (int zi = 0; zi < Z; ++zi) {
+      const double z = r1_x64d(zi, origin[2], size[2], Z); /* :109 */
+      double value = 0.0;
+      for (int i = 0; i < N; ++i) {
+        const double wx = mp[i], wy = mp[N + i], wz = mp[2 * N + i];
+        const double zz = zi > 0 ? dz : 0.0;                         /* :123 */
+        const double ax = x - wx, ay = y - wy;
+        const double dps = ax * ax + ay * ay;                        /* :124 */
+        const double a = (z - zz) - wz, b = z - wz, c = (z + dz) - wz;
+        const double ds = sqrt(dps + a * a);                         /* :125 */
+        const double dm = sqrt(dps + b * b);                         /* :126 */
+        const double de = sqrt(dps + c * c);                         /* :127 */
+        const double tfs = (ds - (double)off) / (double)w;           /* :129-131 */
+        const double tfm = (dm - (double)off) / (double)w;
+        const double tfe = (de - (double)off) / (double)w;
+        int ts = (int)ceilf((float)tfs), tm = (int)ceilf((float)tfm),
+            te = (int)ceilf((float)tfe);                             /* ceilf narrows */
+        ts = ts < 0 ? 0 : (ts > T ? T : ts);                         /* :132-134 */
+        tm = tm < 0 ? 0 : (tm > T ? T : tm);
+        te = te < 0 ? 0 : (te > T ? T : te);
+        for (int ti = ts; ti < te; ++ti) {                           /* :135 */
+          const double tc = (double)r1_tc32(ti, w, off);             /* :137 */
+          const double t2 = tc * tc;
+          const double d2 = t2 - dps;                                /* :139 */
+          if (!(d2 > 0.0)) continue;
+          const double depth = sqrt(d2);
+          const double zc = wz + depth;                              /* :143 */
+          double zh = (zc - o) * s;
+          zh = zh - 0.5;                                             /* :145 */
+          const int zf = (int)floorf((float)zh);                     /* :146 floorf narrows */
+          const double alpha = zh - (double)zf;                      /* :148 */
+          const double df1 = (ti >= tm) ? (1.0 - alpha) : alpha;     /* :150 */
+          const double q = -depth / t2;
+          const double q2 = q * q;
+          const double f2 = q2 * q2;                                 /* :152 */
+          value = value + (df1 * f2) * G[(size_t)i * T + ti];        /* :154 */
+        }
+      }
+      out[((size_t)xi * Y + yi) * Z + zi] = value;
+    }
+  }
+}
+
+/* A2 in double (backprojection_cuda.cu:152-194 with scalar_t = double):
+ * float points and distances, double bin arithmetic and sum. */
+void oracle_velten_confocal_f64(const double* tr, int B, int T, int S, const double* scan,
+                                const double* sorigin, int So, int D, int H, int W,
+                                const float* vs, const float* ve, float w, float off,
+                                double* out) {
+#pragma omp parallel for collapse(2) schedule(dynamic, 4)
+  for (int b = 0; b < B; ++b) {
+    for (int z = 0; z < D; ++z) {
+      for (int y = 0; y < H; ++y) {
+        for (int x = 0; x < W; ++x) {
+          const float qx = inclusive32(x, W, vs[0], ve[0]);  /* :167-170 */
+          const float qy = inclusive32(y, H, vs[1], ve[1]);
+          const float qz = inclusive32(z, D, vs[2], ve[2]);
+          double v = 0.0;
+          for (int s = 0; s < S; ++s) {
+            const double* p = scan + ((size_t)b * S + s) * 3;
+            const float px = (float)p[0], py = (float)p[1], pz = (float)p[2]; /* :173-176 */
+            const double dsv = (double)dist32(px, py, pz, qx, qy, qz);       /* :177 */
+            double dso = 0.0;
+            if (So > 0) {
+              const double* q = sorigin + ((size_t)b * So + s) * 3;
+              dso = (double)dist32(px, py, pz, (float)q[0], (float)q[1], (float)q[2]); /* :178-183 */
+            }
+            const double dist = 2.0 * dso + 2.0 * dsv;                        /* :185 */
+            const double tf = (dist - (double)off) / (double)w + 0.5;
+            const int ti = (int)floorf((float)tf);                            /* :187 */
+            if (ti >= 0 && ti < T) v += tr[((size_t)b * T + ti) * S + s];    /* :188-190 */
+          }
+          out[(((size_t)b * D + z) * H + y) * W + x] = v;
+        }
+      }
+    }
+  }
+}

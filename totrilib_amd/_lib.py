@@ -51,6 +51,17 @@ SIGNATURES: dict[str, tuple] = {
         _c_int,
         [_c_p, _c_i64, _c_i64, _c_i64, _c_p, _c_i64, _c_p, _c_i64, _c_int,
          _F3, _F3, _c_f, _c_f, _c_p]),
+    "totri_volume_confocal_forward_f64": (
+        _c_int,
+        [_c_p, _c_i64, _c_i64, _c_i64, _c_p, _c_i64, _c_p, _c_i64, _F3, _F3, _c_f, _c_f, _c_p]),
+    "totri_volume_confocal_grad_volume_f64": (
+        _c_int,
+        [_c_p, _c_i64, _c_i64, _c_i64, _c_p, _c_i64, _c_p, _c_i64, _F3, _F3, _c_f, _c_f, _c_int,
+         _c_p]),
+    "totri_backprojection_confocal_velten_forward_f64": (
+        _c_int,
+        [_c_p, _c_i64, _c_i64, _c_i64, _c_p, _c_p, _c_i64, _c_p, _c_i64, _c_i64, _c_i64,
+         _F3, _F3, _c_f, _c_f, _c_p]),
     "totri_volume_confocal_grad_measurement_points_workspace": (_c_sz, [_c_i64] * 5),
     "totri_volume_confocal_grad_measurement_points": (
         _c_int,

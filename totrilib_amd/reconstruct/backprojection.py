@@ -90,7 +90,12 @@ class BackprojectionConfocalVelten(torch.autograd.Function):
             raise RuntimeError('BackprojectionConfocalVelten needs a GPU, '
                                'but no devices are available.')
         dev = _gpu_device(transient)
-        volume = torch.zeros((transient.shape[0],) + tuple(volume_def.resolution), device=dev)
+        # float32 as in the reference; a float64 transient gets a float64
+        # volume (the reference's kernel dispatches double, but its wrapper
+        # allocates float32 and so cannot run it)
+        dtype = torch.float64 if transient.dtype == torch.float64 else torch.float32
+        volume = torch.zeros((transient.shape[0],) + tuple(volume_def.resolution), device=dev,
+                             dtype=dtype)
         scan_origin = origin_of_wallpoints(scan_origin, scan_points, 'scan_origin')
         backprojection_confocal_velten_forward(
             transient.to(dev), scan_points.to(dev), scan_origin.to(dev), volume,

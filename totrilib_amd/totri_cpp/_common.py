@@ -34,6 +34,11 @@ def require_f32(t: torch.Tensor, name: str) -> None:
         raise TypeError(f"{name} must be float32, got {t.dtype}")
 
 
+def require_f64(t: torch.Tensor, name: str) -> None:
+    if t.dtype != torch.float64:
+        raise TypeError(f"{name} must be float64 like the volume, got {t.dtype}")
+
+
 def storage_code(t: torch.Tensor, name: str) -> int:
     try:
         return _DTYPE_CODE[t.dtype]

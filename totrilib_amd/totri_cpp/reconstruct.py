@@ -12,7 +12,7 @@ import torch
 
 from totrilib_amd import _lib
 from totrilib_amd.totri_cpp._common import (
-    OutputSlot, require_f32, require_gpu, stream_ptr, workspace)
+    OutputSlot, require_f32, require_f64, require_gpu, stream_ptr, workspace)
 
 Vec3 = typing.Sequence[float]
 
@@ -45,6 +45,23 @@ def backprojection_confocal_velten_forward(transient: torch.Tensor,
         raise ValueError(f"scan_origin must be [{B},0 or {S},3], got {tuple(scan_origin.shape)}")
     if volume.shape[0] != B:
         raise ValueError("volume batch size does not match the transient")
+    if transient.dtype == torch.float64:
+        # the reference's double dispatch (backprojection_cuda.cu:215)
+        for t, n in ((scan_points, "scan_points"), (scan_origin, "scan_origin"),
+                     (volume, "volume")):
+            require_f64(t, n)
+        D, H, W = (int(s) for s in volume.shape[1:])
+        tr, sp, so = transient.contiguous(), scan_points.contiguous(), scan_origin.contiguous()
+        lib = _lib.load()
+        out = OutputSlot(volume)
+        with torch.cuda.device(dev):
+            rc = lib.totri_backprojection_confocal_velten_forward_f64(
+                tr.data_ptr(), B, T, S, sp.data_ptr(), so.data_ptr(), So, out.ptr(), D, H, W,
+                _lib.f3(volume_start), _lib.f3(volume_end), float(bin_width), float(bin_offset),
+                stream_ptr(dev))
+            _lib.check(rc, "backprojection_confocal_velten_forward")
+            out.finish()
+        return
     require_f32(volume, "volume")
     tr, sp, so = _prep(transient, "transient"), _prep(scan_points, "scan_points"), \
         _prep(scan_origin, "scan_origin")

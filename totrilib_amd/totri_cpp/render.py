@@ -12,7 +12,7 @@ import torch
 
 from totrilib_amd import _lib
 from totrilib_amd.totri_cpp._common import (
-    OutputSlot, require_f32, require_gpu, storage_code, stream_ptr, workspace)
+    OutputSlot, require_f32, require_f64, require_gpu, storage_code, stream_ptr, workspace)
 
 Vec3 = typing.Sequence[float]
 
@@ -34,6 +34,22 @@ def volume_confocal_forward(volume: torch.Tensor, measurement_points: torch.Tens
         raise ValueError("Measurement and transient dimensions do not match.")
     if measurement_points.shape[0] != 3:
         raise ValueError("Measurement points must have shape 3xN.")
+    if volume.dtype == torch.float64:
+        # the reference's double dispatch (volume_cuda.cu:85)
+        require_f64(measurement_points, "measurement_points")
+        require_f64(transient, "transient")
+        lib = _lib.load()
+        X, Y, Z = (int(s) for s in volume.shape)
+        N, T = (int(s) for s in transient.shape)
+        vol, mp = volume.contiguous(), measurement_points.contiguous()
+        out = OutputSlot(transient)
+        with torch.cuda.device(dev):
+            rc = lib.totri_volume_confocal_forward_f64(
+                vol.data_ptr(), X, Y, Z, mp.data_ptr(), N, out.ptr(), T, _lib.f3(volume_origin),
+                _lib.f3(volume_size), float(bin_width), float(bin_offset), stream_ptr(dev))
+            _lib.check(rc, "volume_confocal_forward")
+            out.finish()
+        return
     require_f32(volume, "volume")
     require_f32(measurement_points, "measurement_points")
     code = storage_code(transient, "transient")
@@ -75,6 +91,23 @@ def volume_confocal_grad_volume(volume: torch.Tensor, measurement_points: torch.
         raise ValueError("Measurement and transient dimensions do not match.")
     if measurement_points.shape[0] != 3:
         raise ValueError("Measurement points must have shape 3xN.")
+    if volume.dtype == torch.float64:
+        # the reference's double dispatch (volume_cuda.cu:175)
+        require_f64(measurement_points, "measurement_points")
+        require_f64(transient, "transient")
+        lib = _lib.load()
+        X, Y, Z = (int(s) for s in volume.shape)
+        N, T = (int(s) for s in transient.shape)
+        mp, tr = measurement_points.contiguous(), transient.contiguous()
+        out = OutputSlot(volume)
+        with torch.cuda.device(dev):
+            rc = lib.totri_volume_confocal_grad_volume_f64(
+                out.ptr(), X, Y, Z, mp.data_ptr(), N, tr.data_ptr(), T, _lib.f3(volume_origin),
+                _lib.f3(volume_size), float(bin_width), float(bin_offset),
+                1 if reference_window else 0, stream_ptr(dev))
+            _lib.check(rc, "volume_confocal_grad_volume")
+            out.finish()
+        return
     require_f32(volume, "volume")
     require_f32(measurement_points, "measurement_points")
     code = storage_code(transient, "transient")
@@ -114,6 +147,10 @@ def volume_confocal_grad_measurement_points(measurement_points_grad: torch.Tenso
 
     Reference: src/render/volume.cpp:43-81, src/render/volume_cuda.cu:188-295.
     """
+    if volume.dtype == torch.float64 or measurement_points.dtype == torch.float64:
+        raise TypeError("the scan-point gradient is provided in float32 only; the float64 "
+                        "forms cover the render, its albedo adjoint and the Velten "
+                        "backprojection")
     dev = require_gpu(measurement_points_grad, volume, measurement_points, transient_grad,
                       names=("measurement_points_grad", "volume", "measurement_points",
                              "transient_grad"))
