@@ -140,16 +140,18 @@ __device__ __forceinline__ void brick_range(float c, float r, int n, int bs, flo
 }
 
 // Pass 1 (FILL = false) counts, per brick, the RBFs that may reach it; pass 2
-// (FILL = true) writes their indices at offset + a fill position.  One thread
-// per RBF.
+// (FILL = true) writes their indices at offset + a fill position.  One wave
+// per RBF, its lanes over the bricks of the RBF's support box.
+constexpr int kBinWaves = 4;
+
 template <bool FILL>
-__global__ __launch_bounds__(256) void splat_bin_kernel(const float* __restrict__ rbf,
-                                                        SplatGeom g, int bricks_x,
-                                                        int bricks_y, int bricks_z,
-                                                        SplatBins bins) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(kBinWaves * 64) void splat_bin_kernel(
+    const float* __restrict__ rbf, SplatGeom g, int bricks_x, int bricks_y, int bricks_z,
+    SplatBins bins) {
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * kBinWaves + (threadIdx.x >> 6);
   const int b = blockIdx.y;
-  if (i >= g.N) return;
+  if (i >= g.N) return;  // whole wave; no barrier follows
   const int C = 4 + g.A;
   const float* p = rbf + ((size_t)b * g.N + i) * C;
   const float xr = p[0], yr = p[1], zr = p[2], sg = p[3];
@@ -158,21 +160,25 @@ __global__ __launch_bounds__(256) void splat_bin_kernel(const float* __restrict_
   brick_range(xr, r, g.W, kBX, g.sx, g.ex, &xa, &xb);
   brick_range(yr, r, g.H, kBY, g.sy, g.ey, &ya, &yb);
   brick_range(zr, r, g.D, kBZ, g.sz, g.ez, &za, &zb);
+  const int nx = xb - xa + 1, ny = yb - ya + 1, nz = zb - za + 1;
+  if (nx <= 0 || ny <= 0 || nz <= 0) return;
+  const int total = nx * ny * nz;
   const size_t nb = (size_t)bricks_x * bricks_y * bricks_z;
-  for (int bz = za; bz <= zb; ++bz)
-    for (int by = ya; by <= yb; ++by)
-      for (int bx = xa; bx <= xb; ++bx) {
-        float lox, hix, loy, hiy, loz, hiz;
-        brick_box(g, bx, by, bz, &lox, &hix, &loy, &hiy, &loz, &hiz);
-        if (!brick_may_hold(xr, yr, zr, sg, g.trunc, lox, hix, loy, hiy, loz, hiz)) continue;
-        const size_t k = b * nb + ((size_t)bz * bricks_y + by) * bricks_x + bx;
-        if (!FILL) {
-          atomicAdd(bins.count + k, 1);
-        } else {
-          const long long at = (long long)bins.offset[k] + atomicAdd(bins.cursor + k, 1);
-          if (at < bins.capacity) bins.list[at] = i;
-        }
-      }
+  for (int e = lane; e < total; e += 64) {
+    const int bx = xa + e % nx;
+    const int by = ya + (e / nx) % ny;
+    const int bz = za + e / (nx * ny);
+    float lox, hix, loy, hiy, loz, hiz;
+    brick_box(g, bx, by, bz, &lox, &hix, &loy, &hiy, &loz, &hiz);
+    if (!brick_may_hold(xr, yr, zr, sg, g.trunc, lox, hix, loy, hiy, loz, hiz)) continue;
+    const size_t k = b * nb + ((size_t)bz * bricks_y + by) * bricks_x + bx;
+    if (!FILL) {
+      atomicAdd(bins.count + k, 1);
+    } else {
+      const long long at = (long long)bins.offset[k] + atomicAdd(bins.cursor + k, 1);
+      if (at < bins.capacity) bins.list[at] = i;
+    }
+  }
 }
 
 // Exclusive scan of count[0 .. n) into offset (one workgroup, in order), and
@@ -700,16 +706,16 @@ extern "C" int totri_rbf2volume_gaussian_forward(const float* rbf, int64_t B, in
                        dim3(256), 0, st, bins.count, n);
     rc = check_launch("fill_zero_i32_kernel");
     if (rc) return rc;
-    const dim3 grid((unsigned)((N + 255) / 256), (unsigned)B);
-    hipLaunchKernelGGL(splat_bin_kernel<false>, grid, dim3(256), 0, st, rbf, g, bx, by, bz,
-                       bins);
+    const dim3 grid((unsigned)((N + kBinWaves - 1) / kBinWaves), (unsigned)B);
+    hipLaunchKernelGGL(splat_bin_kernel<false>, grid, dim3(kBinWaves * 64), 0, st, rbf, g, bx,
+                       by, bz, bins);
     rc = check_launch("splat_bin_kernel");
     if (rc) return rc;
     hipLaunchKernelGGL(splat_scan_kernel, dim3(1), dim3(1024), 0, st, bins, (int)n);
     rc = check_launch("splat_scan_kernel");
     if (rc) return rc;
-    hipLaunchKernelGGL(splat_bin_kernel<true>, grid, dim3(256), 0, st, rbf, g, bx, by, bz,
-                       bins);
+    hipLaunchKernelGGL(splat_bin_kernel<true>, grid, dim3(kBinWaves * 64), 0, st, rbf, g, bx,
+                       by, bz, bins);
     rc = check_launch("splat_bin_kernel");
     if (rc) return rc;
   }
