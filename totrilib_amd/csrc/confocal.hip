@@ -219,7 +219,11 @@ __device__ __forceinline__ void r1_block(
   const int n_cols = g.X * g.Y;
   const int chunk_vox = chunk_cols * g.Z;
 
-  // Per-lane bin constants for each window.
+  // Per-lane bin constants for each window.  A sample adds v * d2^2 to its
+  // bin's sum, which is scaled by 1/tc^4 twice at the end: the weight
+  // (d2 / tc^4)^2 = (depth / tc^2)^4 of volume_cuda.cu:49-51 with the common
+  // factor taken out of the sum (1/tc^4 is kept finite, so that a bin with
+  // no active sample, tc = 0 among them, ends at 0).
   float tc2[K], itc4[K], a[K], acc[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
@@ -227,8 +231,7 @@ __device__ __forceinline__ void r1_block(
     const float tc = bin_distance(t, g.w, g.off);
     const float q = tc * tc;
     tc2[k] = (t < g.T) ? q : -1.f;  // -1 makes d2 negative: lane inactive
-    // BELOW: finite, so that an inactive sample (d2 clamped to 0) has weight 0
-    itc4[k] = BELOW ? (q > 0.f ? fminf(1.f / (q * q), 3.4028235e38f) : 0.f) : 1.f / (q * q);
+    itc4[k] = q > 0.f ? fminf(1.f / (q * q), 3.4028235e38f) : 0.f;
     acc[k] = 0.f;
     a[k] = 0.f;
   }
@@ -304,21 +307,28 @@ __device__ __forceinline__ void r1_block(
       const unsigned packed =
           any ? (((2u << khi) - 1u) & ~((1u << klo) - 1u)) : 0u;
       unsigned long long mask = __ballot(any);
-      // Two columns per pass: their samples are independent, which gives the
-      // scheduler two dependency chains to interleave.  Windows are the union
-      // of both columns' windows; a sample outside its column's range simply
-      // fails the exact activity test.
+      // Two columns of the same x per pass: their samples are independent,
+      // which gives the scheduler two dependency chains to interleave, and
+      // they share tc^2 - dx^2.  Windows are the union of both columns'
+      // windows; a sample outside its column's range simply fails the exact
+      // activity test.  A column whose next one has another x goes alone.
       while (mask) {
         const int j1 = __builtin_ctzll(mask);
         mask &= mask - 1;
-        const int j2 = mask ? __builtin_ctzll(mask) : j1;
-        const bool two = mask != 0;
-        if (two) mask &= mask - 1;
+        const int xi1 = __builtin_amdgcn_readlane(xi, j1);
+        int j2 = j1;
+        bool two = false;
+        if (mask) {
+          const int jn = __builtin_ctzll(mask);
+          if (__builtin_amdgcn_readlane(xi, jn) == xi1) {
+            j2 = jn;
+            two = true;
+            mask &= mask - 1;
+          }
+        }
         const unsigned wm1 = (unsigned)__builtin_amdgcn_readlane((int)packed, j1);
         const unsigned wm2 = two ? (unsigned)__builtin_amdgcn_readlane((int)packed, j2) : 0u;
         const unsigned wmask = wm1 | wm2;
-        const int xi1 = __builtin_amdgcn_readlane(xi, j1);
-        const int xi2 = __builtin_amdgcn_readlane(xi, j2);
         if (xi1 != cached_xi) {
           // tc^2 - dx^2 (volume_cuda.cu:40, first subtraction) depends on the
           // column's x only; recompute it when x changes.
@@ -328,10 +338,6 @@ __device__ __forceinline__ void r1_block(
           for (int k = 0; k < K; ++k) a[k] = tc2[k] - cdx2;
           cached_xi = xi1;
         }
-        // Second column's tc^2 - dx^2 (equal to a[] when it shares x).
-        const float cdx2b = __builtin_bit_cast(
-            float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, dx2), j2));
-        const bool same_x = xi2 == cached_xi;
         const float cdy2a = __builtin_bit_cast(
             float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, dy2), j1));
         const float cdy2b = two ? __builtin_bit_cast(
@@ -342,7 +348,7 @@ __device__ __forceinline__ void r1_block(
         for (int k = 0; k < K; ++k) {
           if (!((wmask >> k) & 1u)) continue;
           const float d2a = a[k] - cdy2a;  // volume_cuda.cu:40
-          const float d2b = (same_x ? a[k] : tc2[k] - cdx2b) - cdy2b;
+          const float d2b = a[k] - cdy2b;
           if (BELOW) {
             // The wall lies below the box: a sample with d2 <= 0 (clamped to
             // 0) decides a negative row, or after an exact decision a row of
@@ -360,9 +366,7 @@ __device__ __forceinline__ void r1_block(
             const float2 vb = col2[min((unsigned)zb, (unsigned)(g.Z - 1))];
             const float fa = fmaf(ala, va.y, va.x);
             const float fb = fmaf(alb, vb.y, vb.x);
-            const float qa = d2ca * itc4[k];
-            const float qb = d2cb * itc4[k];
-            acc[k] = fmaf(fb, qb * qb, fmaf(fa, qa * qa, acc[k]));
+            acc[k] = fmaf(fb, d2cb * d2cb, fmaf(fa, d2ca * d2ca, acc[k]));
             continue;
           }
           float ala, alb;
@@ -382,10 +386,8 @@ __device__ __forceinline__ void r1_block(
           const float2 vb = col2[actb ? zb : 0];
           const float fa = fmaf(ala, va.y, va.x);
           const float fb = fmaf(alb, vb.y, vb.x);
-          const float qa = d2a * itc4[k];
-          const float qb = d2b * itc4[k];
-          const float ca = fmaf(fa, qa * qa, acc[k]);
-          const float cb = fmaf(fb, qb * qb, acta ? ca : acc[k]);
+          const float ca = fmaf(fa, d2a * d2a, acc[k]);
+          const float cb = fmaf(fb, d2b * d2b, acta ? ca : acc[k]);
           acc[k] = actb ? cb : (acta ? ca : acc[k]);
         }
       }
@@ -406,7 +408,7 @@ __device__ __forceinline__ void r1_block(
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int t = t_base + 64 * k + lane;
-      if (t < g.T) row[t] = acc[k];
+      if (t < g.T) row[t] = (acc[k] * itc4[k]) * itc4[k];
     }
   }
 }
@@ -678,7 +680,8 @@ __device__ __forceinline__ void r1_sym_block(
     const float tc = bin_distance(t, g.w, g.off);
     const float qq = tc * tc;
     tc2[k] = (t < g.T) ? qq : -1.f;
-    // finite, so that an inactive sample (d2 clamped to 0) has weight 0
+    // finite, so that a bin with no active sample (tc = 0 among them) ends at 0;
+    // the sums take v * d2^2 and are scaled by 1/tc^4 twice at the end
     itc4[k] = qq > 0.f ? fminf(1.f / (qq * qq), 3.4028235e38f) : 0.f;
 #pragma unroll
     for (int m = 0; m < 4; ++m) acc[m][k] = 0.f;
@@ -798,17 +801,21 @@ __device__ __forceinline__ void r1_sym_block(
         const float2* quad = cur + j * rep_pairs;
         // Families 2h and 2h+1 share dy^2; both are evaluated for a window
         // where either is active (two independent decisions per pass; a
-        // sample outside its family's range fails the activity test).
+        // sample outside its family's range fails the activity test).  The
+        // first subtraction tc^2 - dx^2 is shared by h = 0 and h = 1.
+        const unsigned wmh0 = (wm | (wm >> 8)) & 0xffu;
+        const unsigned wmh1 = ((wm >> 16) | (wm >> 24)) & 0xffu;
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          const int fa = 2 * h, fb = 2 * h + 1;
-          const unsigned wmask = ((wm >> (8 * fa)) | (wm >> (8 * fb))) & 0xffu;
-          const float dy2 = h ? cD : cC;
+        for (int k = 0; k < K; ++k) {
+          if (!(((wmh0 | wmh1) >> k) & 1u)) continue;
+          const float pA = tc2[k] - cA, pB = tc2[k] - cB;  // volume_cuda.cu:40
 #pragma unroll
-          for (int k = 0; k < K; ++k) {
-            if (!((wmask >> k) & 1u)) continue;
-            const float d2a = (tc2[k] - cA) - dy2;  // volume_cuda.cu:40
-            const float d2b = (tc2[k] - cB) - dy2;
+          for (int h = 0; h < 2; ++h) {
+            const int fa = 2 * h, fb = 2 * h + 1;
+            if (!(((h ? wmh1 : wmh0) >> k) & 1u)) continue;
+            const float dy2 = h ? cD : cC;
+            const float d2a = pA - dy2;
+            const float d2b = pB - dy2;
             // Decisions and weights use d2 clamped at 0: identical for an
             // active sample (d2 > 0), finite for an inactive one, which then
             // reads the zero row: its term is 0 * 0 + 0 times a finite weight.
@@ -836,8 +843,8 @@ __device__ __forceinline__ void r1_sym_block(
             }
             const float2* pa = quad + 4 * ra;
             const float2* pb = quad + 4 * rb;
-            const float qa = d2ca * itc4[k], qb = d2cb * itc4[k];
-            const float wa = qa * qa, wb = qb * qb;
+            // the common factor 1/tc^8 is applied to the sums at the end
+            const float wa = d2ca * d2ca, wb = d2cb * d2cb;
 #pragma unroll
             for (int m = 0; m < 4; ++m) {
               const float2 va = pa[m ^ fa];
@@ -861,7 +868,7 @@ __device__ __forceinline__ void r1_sym_block(
 #pragma unroll
       for (int k = 0; k < K; ++k) {
         const int t = t_base + 64 * k + lane;
-        if (t < g.T) row[t] = acc[m][k];
+        if (t < g.T) row[t] = (acc[m][k] * itc4[k]) * itc4[k];
       }
     }
   }
@@ -1780,6 +1787,32 @@ __device__ __forceinline__ float load_at(const float* __restrict__ base, uint32_
 // at T, which a lane may read one step past its last bin (no clamp needed).
 __host__ __device__ __forceinline__ size_t kq_stride(int T) { return (size_t)T + 1; }
 
+// The row walks' tc^2 table in LDS: tc^2 = (t w + off)^2 of bins 0 .. T-1 at
+// the walks' rounding, shared by a workgroup's waves, for T up to
+// kQuadTabMaxT (xy-mirror walk) or kRowTabMaxT (general walk).  A step then
+// takes tc^2 with one LDS read instead of three VALU operations.  The table
+// sits at the start of the kernel's one static LDS array (so its address
+// folds into the read's offset) and the accumulator rows follow it.  A step
+// reads the entry of the bin after the next one, up to bin T + 1: past T - 1
+// that is an unwritten entry or an accumulator row of the same array, and
+// feeds only a decision that a stopped lane discards.
+// At 128^3 / 64^2 / 1024 the table took the xy-mirror A1 from 8.91 to 8.60 ms
+// and the general A1 from 27.6 to 27.0 ms (2-wave workgroups; 8-wave ones,
+// whose LDS stays held until their slowest item ends, measured 28.3 ms).  The
+// x-pair walk measured 1 % slower with it (more VGPRs) and computes tc^2.
+constexpr int kQuadWaves = 4;
+constexpr int kRowWaves = 2;
+constexpr int kQuadTabMaxT = 1024;
+constexpr int kRowTabMaxT = 1024;
+
+__device__ __forceinline__ void fill_tc2_table(float* __restrict__ tab, const ConfGeom& g) {
+  for (int e = threadIdx.x; e < g.T; e += blockDim.x) {
+    const float tc = bin_distance(e, g.w, g.off);
+    tab[e] = tc * tc;
+  }
+  __syncthreads();
+}
+
 // Component n of v ^ f: (v[0^f], v[1^f], v[2^f], v[3^f]) for a wave-uniform f.
 __device__ __forceinline__ float4 xor_perm4(float4 v, int f) {
   if (f & 1) v = make_float4(v.y, v.x, v.w, v.z);
@@ -1846,7 +1879,7 @@ __device__ __forceinline__ float mv_get(float2 v, int n) { return n == 0 ? v.x :
 // one 16- or 8-byte load per step) and summed in table order (component m
 // for scan s_m); the sums are permuted into the family's order when a row is
 // flushed.
-template <bool SAFE, int M = 4>
+template <bool SAFE, int M = 4, bool TAB = false>
 __device__ __forceinline__ void a1_quad_item(
     const MirrorVec<M>* __restrict__ Kq, const float* __restrict__ tc2t,
     const float* __restrict__ itc4t, const float* __restrict__ mp,
@@ -1854,7 +1887,7 @@ __device__ __forceinline__ void a1_quad_item(
     float* __restrict__ dst,  // [R][X][Y][Z] partial volumes
     float* __restrict__ bnd,  // [R][X][Y][n_slabs] boundary rows
     const ConfGeom& g, int item, int tiles_x, int tiles_y, int n_slabs, int n_ranges,
-    int t_stride, MirrorVec<M>* __restrict__ acc) {
+    int t_stride, MirrorVec<M>* __restrict__ acc, const float* __restrict__ tab = nullptr) {
   using V = MirrorVec<M>;
   constexpr uint32_t kVB = sizeof(V);  // table bytes per bin
   const int lane = threadIdx.x & 63;
@@ -1964,6 +1997,16 @@ __device__ __forceinline__ void a1_quad_item(
           *d2o = d2;
           return r;
         };
+        // the same from d2 = (tc^2 - dx^2) - dy^2 with tc^2 from the LDS
+        // table (TAB)
+        auto decide_d2 = [&](float d2, float* d2o, float* alo) -> int {
+          bool nr;
+          int r = decide_fast(g, d2, cp, delta, half_m_delta, alo, &nr);
+          if (wave_any(nr))
+            if (nr) r = decide_exact(g, d2, wz, zshift, alo);
+          *d2o = d2;
+          return r;
+        };
         // The walk starts at the lane's first bin with d2 > 0 and row >= 0:
         // estimated from the real-valued inverse (depth of row 0's start), then
         // settled by exact decisions (backward while the previous bin also
@@ -2019,11 +2062,17 @@ __device__ __forceinline__ void a1_quad_item(
         float ic = SAFE ? load_at(itc4t, off16 / (kVB / 4)) : 0.f;
         float tf = (float)t;
         const float thif = (float)thi;
+        // TAB: tc^2 of the next bin, read from the LDS table one step ahead
+        // (entries past T - 1 are read only for decisions a stopped lane
+        // discards), and the stop test on byte offsets (thi >= -1).
+        float tcn = TAB ? tab[t + 1] : 0.f;
+        const int thib = thi * (int)kVB;
+        auto live = [&]() -> bool { return TAB ? (int)off16 <= thib : tf <= thif; };
         // Per row: L = carry + sum of K d2^2, H = sum of K d2^2 alpha; the row's
         // voxel gets L - H (its (1 - alpha) parts plus the carry) and H is
         // carried into the next row.
         while (rr <= rspan) {
-          if (!wave_any(tf <= thif)) break;
+          if (!wave_any(live())) break;
           V L = carry, H = zero4;
           while (true) {
             const bool act = r <= rr;
@@ -2036,14 +2085,23 @@ __device__ __forceinline__ void a1_quad_item(
               // keep both products ahead of the next decision, so that d2 and al
               // are free for it (no register copies); emits no instruction
               asm volatile("" : "+v"(base), "+v"(wh));
-              tf = tf + 1.f;
-              const int rn = decide_bin_f(tf, &d2, &al);
+              int rn;
+              if (TAB) {
+                // the next bin's decision from tcn, which then takes tc^2 of
+                // the bin after it (a whole step ahead of its use)
+                const float d2n = (tcn - dx2) - dy2;
+                tcn = load_at(tab, off16 / (kVB / 4) + 8u);
+                rn = decide_d2(d2n, &d2, &al);
+              } else {
+                tf = tf + 1.f;
+                rn = decide_bin_f(tf, &d2, &al);
+              }
               mv_fma(L, kq, base);
               mv_fma(H, kq, wh);
               off16 += kVB;
               kq = *(const V*)(kqp + off16);
               if (SAFE) ic = load_at(itc4t, off16 / (kVB / 4));
-              r = (!STOP || tf <= thif) ? rn : 0x7fffffff;
+              r = (!STOP || live()) ? rn : 0x7fffffff;
             }
           }
           V* row = my + rr * 64;
@@ -2109,23 +2167,26 @@ __device__ __forceinline__ void a1_quad_item(
 // Persistent: one single-wave workgroup per wave slot takes items from
 // `counter` (zeroed by a1_quad_order_kernel) in the cost order, so no slot
 // waits for the dispatcher between items.
-template <bool SAFE>
-__global__ __launch_bounds__(64) void a1_quad_kernel(
+template <bool SAFE, bool TAB>
+__global__ __launch_bounds__(kQuadWaves * 64) void a1_quad_kernel(
     const float4* __restrict__ Kq, const float* __restrict__ tc2t,
     const float* __restrict__ itc4t, const float* __restrict__ mp,
     const int* __restrict__ quads, const int* __restrict__ info, float* __restrict__ dst,
     float* __restrict__ bnd, ConfGeom g, int tiles_x, int tiles_y, int n_slabs, int n_ranges,
     int t_stride, int n_items, const int* __restrict__ order, int* __restrict__ counter) {
   if (!info[0]) return;
-  __shared__ float4 acc[(kSlabQ + 1) * 64];
-  const int lane = threadIdx.x;
+  __shared__ float4 lds4[kQuadTabMaxT / 4 + kQuadWaves * (kSlabQ + 1) * 64];
+  const int lane = threadIdx.x & 63;
+  float* tab = (float*)lds4;
+  float4* acc = lds4 + kQuadTabMaxT / 4 + (threadIdx.x >> 6) * (kSlabQ + 1) * 64;
+  if (TAB) fill_tc2_table(tab, g);
   for (;;) {
     int k = 0;
     if (lane == 0) k = atomicAdd(counter, 1);
     k = uniform_i32(__shfl(k, 0, 64));
     if (k >= n_items) break;
-    a1_quad_item<SAFE>(Kq, tc2t, itc4t, mp, quads, info[1], dst, bnd, g, order[k], tiles_x,
-                       tiles_y, n_slabs, n_ranges, t_stride, acc);
+    a1_quad_item<SAFE, 4, TAB>(Kq, tc2t, itc4t, mp, quads, info[1], dst, bnd, g, order[k],
+                               tiles_x, tiles_y, n_slabs, n_ranges, t_stride, acc, tab);
   }
 }
 
@@ -2140,14 +2201,14 @@ __global__ __launch_bounds__(64) void a1_quad_kernel(
 // (w > 0, off >= 0, a positive z size: rows never decrease with the bin) and
 // the same boundary rows.  It replaces the window-synchronous a1_item, whose
 // lanes share one bin window and update LDS once per sample.
-template <bool SAFE>
+template <bool SAFE, bool TAB = false>
 __device__ __forceinline__ void a1_row_item(
     const float* __restrict__ Kf, const float* __restrict__ itc4t,
     const float* __restrict__ mp,
     float* __restrict__ dst,  // [R][X][Y][Z] partial volumes
     float* __restrict__ bnd,  // [R][X][Y][n_slabs] boundary rows
     const ConfGeom& g, int item, int tiles_x, int tiles_y, int n_slabs, int n_ranges,
-    int t_stride, float* __restrict__ acc) {
+    int t_stride, float* __restrict__ acc, const float* __restrict__ tab = nullptr) {
   const int lane = threadIdx.x & 63;
   const int tx = item % tiles_x;
   item /= tiles_x;
@@ -2234,6 +2295,14 @@ __device__ __forceinline__ void a1_row_item(
         *d2o = d2;
         return r;
       };
+      auto decide_d2 = [&](float d2, float* d2o, float* alo) -> int {
+        bool nr;
+        int r = decide_fast(g, d2, cp, delta, half_m_delta, alo, &nr);
+        if (wave_any(nr))
+          if (nr) r = decide_exact(g, d2, wz, zshift, alo);
+        *d2o = d2;
+        return r;
+      };
       // Exact walk start (see a1_quad_item).
       const float rho2 = dx2 + dy2;
       const float depth0 = fmaxf(((float)zshift + 0.5f) / g.s + g.oz - wz, 0.f);
@@ -2274,8 +2343,13 @@ __device__ __forceinline__ void a1_row_item(
       float ic = SAFE ? load_at(itc4t, off4) : 0.f;
       float tf = (float)t;
       const float thif = (float)thi;
+      // TAB: the next bin's tc^2 from the LDS table, one step ahead (see
+      // a1_quad_item); the table and the K row share the byte offset.
+      float tcn = TAB ? tab[t + 1] : 0.f;
+      const int thib = thi * 4;
+      auto live = [&]() -> bool { return TAB ? (int)off4 <= thib : tf <= thif; };
       while (rr <= rspan) {
-        if (!wave_any(tf <= thif)) break;
+        if (!wave_any(live())) break;
         float L = carry, H = 0.f;
         while (true) {
           const bool act = r <= rr;
@@ -2284,14 +2358,21 @@ __device__ __forceinline__ void a1_row_item(
             float base = SAFE ? sq(d2 * ic) : d2 * d2;
             float wh = base * al;
             asm volatile("" : "+v"(base), "+v"(wh));  // see a1_quad_item
-            tf = tf + 1.f;
-            const int rn = decide_bin_f(tf, &d2, &al);
+            int rn;
+            if (TAB) {  // as in a1_quad_item
+              const float d2n = (tcn - dx2) - dy2;
+              tcn = load_at(tab, off4 + 8u);
+              rn = decide_d2(d2n, &d2, &al);
+            } else {
+              tf = tf + 1.f;
+              rn = decide_bin_f(tf, &d2, &al);
+            }
             L = fmaf(k, base, L);
             H = fmaf(k, wh, H);
             off4 += 4u;
             k = load_at(kp, off4);
             if (SAFE) ic = load_at(itc4t, off4);
-            r = (!STOP || tf <= thif) ? rn : 0x7fffffff;
+            r = (!STOP || live()) ? rn : 0x7fffffff;
           }
         }
         float* row = my + rr * 64;
@@ -2302,7 +2383,12 @@ __device__ __forceinline__ void a1_row_item(
       if (rr <= rspan + 1) my[rr * 64] += carry;
     };
     const bool stop = wave_any(thi > g.T - 3);
-    if (g.off == 0.f) {
+    if (TAB) {  // the table holds tc^2 for either offset
+      if (stop)
+        walk(std::true_type{}, std::false_type{});
+      else
+        walk(std::false_type{}, std::false_type{});
+    } else if (g.off == 0.f) {
       if (stop)
         walk(std::true_type{}, std::true_type{});
       else
@@ -2328,18 +2414,25 @@ __device__ __forceinline__ void a1_row_item(
   }
 }
 
-// One single-wave workgroup per item.  `skip` (optional): a device flag; when
-// it is set the launch does nothing (a mirror kernel has done the work).
-template <bool SAFE>
-__global__ __launch_bounds__(64) void a1_row_kernel(
+// One item per wave, kRowWaves waves per workgroup (they share the tc^2
+// table when TAB).  `skip` (optional): a device flag; when it is set the
+// launch does nothing (a mirror kernel has done the work).
+template <bool SAFE, bool TAB>
+__global__ __launch_bounds__(kRowWaves * 64) __attribute__((amdgpu_waves_per_eu(8)))
+void a1_row_kernel(
     const float* __restrict__ Kf, const float* __restrict__ itc4t,
     const float* __restrict__ mp, float* __restrict__ dst, float* __restrict__ bnd,
     ConfGeom g, int tiles_x, int tiles_y, int n_slabs, int n_ranges, int t_stride,
-    const int* __restrict__ skip) {
+    int n_items, const int* __restrict__ skip) {
   if (skip && *skip) return;
-  __shared__ float acc[(kSlabG + 1) * 64];
-  a1_row_item<SAFE>(Kf, itc4t, mp, dst, bnd, g, (int)blockIdx.x, tiles_x, tiles_y, n_slabs,
-                    n_ranges, t_stride, acc);
+  __shared__ float lds1[(TAB ? kRowTabMaxT : 0) + kRowWaves * (kSlabG + 1) * 64];
+  float* tab = lds1;
+  float* acc = lds1 + (TAB ? kRowTabMaxT : 0) + (threadIdx.x >> 6) * (kSlabG + 1) * 64;
+  if (TAB) fill_tc2_table(tab, g);
+  const int item = (int)blockIdx.x * kRowWaves + (int)(threadIdx.x >> 6);
+  if (item >= n_items) return;  // whole wave; no barrier follows
+  a1_row_item<SAFE, TAB>(Kf, itc4t, mp, dst, bnd, g, item, tiles_x, tiles_y, n_slabs,
+                         n_ranges, t_stride, acc, tab);
 }
 
 // Kq[q][t] = (K[s0][t], K[s1][t], K[s2][t], K[s3][t]) for the device-found
@@ -3142,18 +3235,17 @@ extern "C" int totri_volume_confocal_grad_volume(
                          cost);
       hipLaunchKernelGGL(a1_quad_order_kernel, dim3(1), dim3(1024), 0, st, cost, info, items,
                          order, counter);
-      // one single-wave workgroup per wave slot (16 per CU with 9 KiB of LDS each)
+      // one persistent wave per wave slot (16 per CU: four workgroups of
+      // kQuadWaves waves, each with 40 KiB of LDS)
       const int waves = items < 16 * device_cu_count() ? items : 16 * device_cu_count();
-      if (p.safe)
-        hipLaunchKernelGGL(a1_quad_kernel<true>, dim3(waves), dim3(64), 0, st, Kq, tc2, itc4,
-                           measurement_points, quads, info, dst, bnd, g, qd.tiles_x,
-                           qd.tiles_y, qd.n_slabs, qd.n_ranges, p.t_stride, items, order,
-                           counter);
-      else
-        hipLaunchKernelGGL(a1_quad_kernel<false>, dim3(waves), dim3(64), 0, st, Kq, tc2, itc4,
-                           measurement_points, quads, info, dst, bnd, g, qd.tiles_x,
-                           qd.tiles_y, qd.n_slabs, qd.n_ranges, p.t_stride, items, order,
-                           counter);
+      const int blocks = (waves + kQuadWaves - 1) / kQuadWaves;
+      const bool tab = T <= kQuadTabMaxT;
+      auto quad_kernel = p.safe ? (tab ? a1_quad_kernel<true, true> : a1_quad_kernel<true, false>)
+                                : (tab ? a1_quad_kernel<false, true> : a1_quad_kernel<false, false>);
+      hipLaunchKernelGGL(quad_kernel, dim3(blocks), dim3(kQuadWaves * 64), 0, st, Kq, tc2,
+                         itc4, measurement_points, quads, info, dst, bnd, g, qd.tiles_x,
+                         qd.tiles_y, qd.n_slabs, qd.n_ranges, p.t_stride, items, order,
+                         counter);
       rc = check_launch("a1_quad_kernel");
       if (rc) return rc;
     }
@@ -3193,14 +3285,13 @@ extern "C" int totri_volume_confocal_grad_volume(
   const int* skip = info ? info + 2 : nullptr;
   if (row) {
     const int items = p.row_tiles_x * p.row_tiles_y * p.row_slabs * p.row_ranges;
-    if (p.safe)
-      hipLaunchKernelGGL(a1_row_kernel<true>, dim3(items), dim3(64), 0, st, Kf, itc4,
-                         measurement_points, dst, row_bnd, g, p.row_tiles_x, p.row_tiles_y,
-                         p.row_slabs, p.row_ranges, p.t_stride, skip);
-    else
-      hipLaunchKernelGGL(a1_row_kernel<false>, dim3(items), dim3(64), 0, st, Kf, itc4,
-                         measurement_points, dst, row_bnd, g, p.row_tiles_x, p.row_tiles_y,
-                         p.row_slabs, p.row_ranges, p.t_stride, skip);
+    const int blocks = (items + kRowWaves - 1) / kRowWaves;
+    const bool tab = T <= kRowTabMaxT;
+    auto row_kernel = p.safe ? (tab ? a1_row_kernel<true, true> : a1_row_kernel<true, false>)
+                             : (tab ? a1_row_kernel<false, true> : a1_row_kernel<false, false>);
+    hipLaunchKernelGGL(row_kernel, dim3(blocks), dim3(kRowWaves * 64), 0, st, Kf, itc4,
+                       measurement_points, dst, row_bnd, g, p.row_tiles_x, p.row_tiles_y,
+                       p.row_slabs, p.row_ranges, p.t_stride, items, skip);
     rc = check_launch("a1_row_kernel");
   } else {
     const int items = p.tiles_x * p.tiles_y * p.n_slabs * p.n_ranges;
