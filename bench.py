@@ -29,7 +29,9 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP32 = 157.3e12   # MI355X published FP32 vector peak (packed issue)
 PEAK_HBM = 8.0e12      # MI355X HBM3E bytes/s
-SIMD_CLOCK_HZ = 2.4e9  # MI355X peak engine clock (256 CUs x 4 SIMDs)
+# Engine clock under these kernels' load: GRBM_GUI_ACTIVE over the dispatch time reads
+# 2.26-2.35 GHz on MI355X (peak 2.4 GHz); 1024 SIMDs (256 CUs x 4)
+SIMD_CLOCK_HZ = 2.3e9
 
 WORKLOAD = dict(n_side=64, T=1024, res=128, w_round_trip=0.005, occupancy=0.03,
                 seed=1, target_seed=2)
@@ -313,7 +315,8 @@ def confocal_main(args):
                     "algorithmic_*: the reference formulation's FLOP (18 per active R1 "
                     "sample, 17 per A1 window bin + 23 per pair) over the same time, which "
                     "the mirror kernels exceed by sharing one decision among four samples; "
-                    "valu_issue: wave VALU instructions x 4 cycles over the call"}
+                    "valu_issue: wave VALU instructions x 4 cycles at 2.3 GHz (the measured clock) "
+                    "over 1024 SIMDs, over the call"}
 
     wall, step_ms, r1_ms, a1_ms, info = run_leg(mp_np, args.steps, args.warmup)
     value = pairs * args.steps / wall
@@ -650,7 +653,8 @@ def kernels_main(args):
     sig = rbf[0, :, 3].double()
     in_support = float(torch.sum(4.0 / 3.0 * np.pi * (3 * sig) ** 3) / dv ** 3)
     ms = timeit(lambda: Rbf2VolumeGaussian.apply(rbf, vd, 3.0))
-    row("S1 RBF splat", "splat_forward_kernel", "20,000 RBFs -> 128^3, 3 sigma", ms, in_support,
+    row("S1 RBF splat", "splat_bin_kernel, splat_scan_kernel, splat_forward_kernel",
+        "20,000 RBFs -> 128^3, 3 sigma", ms, in_support,
         "voxel-RBF pairs within 3 sigma", 14, 4 * (2 * RR ** 3 + 5 * M))
     r = rbf.clone().requires_grad_(True)
     V = Rbf2VolumeGaussian.apply(r, vd, 3.0)
@@ -665,7 +669,7 @@ def kernels_main(args):
     r0 = rbf.clone().requires_grad_(True)
     V0 = Rbf2VolumeGaussian.apply(r0, vd, 0.0)
     ms = timeit(lambda: torch.autograd.grad(V0, r0, GV, retain_graph=True))
-    row("S2 RBF splat gradient, untruncated", "splat_grad_kernel",
+    row("S2 RBF splat gradient, untruncated", "splat_grad_all_kernel",
         "20,000 RBFs -> 128^3, no truncation", ms, float(M) * RR ** 3, "voxel-RBF pairs", 25,
         4 * (4 * RR ** 3 + 10 * M))
     # A6: scan-point gradient, 128^3 / 64x64 / 1024 bins
