@@ -287,7 +287,8 @@ def test_exhaustive_render_repeats_bit_for_bit(report):
 def test_long_transients_beyond_the_lds_stage():
     """Rows longer than the 8192-float LDS stage (T = 9000) are read from
     global memory: A2 and A3 bit for bit against the FP32 oracle, A5 against
-    the FP64-pinned adjoint."""
+    the FP64-pinned adjoint.  R2's histograms cover 4096 bins, so T = 9000
+    takes three bin tiles (4096, 4096, 808)."""
     from totrilib_amd.reconstruct import (BackprojectionConfocalVelten,
                                           BackprojectionExhaustiveVelten)
     from totrilib_amd.types import BinDef, VolumeDef
@@ -317,6 +318,14 @@ def test_long_transients_beyond_the_lds_stage():
     assert np.count_nonzero(adj) > 100
     # each voxel sums spans of dozens of 0.56 mm bins over 10 point pairs in FP32
     assert rel_l2(grad.cpu().numpy(), adj) < 1e-5
+    from totrilib_amd.render.volume import VolumeRenderExhaustive
+    vol = sparse_albedo(res, 0.3, 61)[None]
+    tr = VolumeRenderExhaustive.apply(t(vol), t(sc3), t(la3), vd, bd).cpu().numpy()
+    ref = O.r2_forward(vol, sc3, la3, T, VS, VE, w, 0.0, mode=O.PINNED64)
+    # nonzero bins in every tile, and the same support as the oracle's
+    assert all(np.count_nonzero(ref[:, a:b]) > 20 for a, b in ((0, 4096), (4096, 8192), (8192, T)))
+    assert np.array_equal(tr != 0, ref != 0)
+    assert rel_l2(tr, ref) < 1e-6
 
 
 @pytest.mark.parametrize("case", ["binned", "overflow"])

@@ -28,6 +28,10 @@ namespace totri_hip {
 namespace {
 
 constexpr int kR2Waves = 4;
+// Bins per R2 histogram tile: the waves' private histograms fill 64 KiB of
+// LDS.  Longer transients take several tiles (grid z), each pass recomputing
+// the voxel spans and keeping the bins of its tile.
+constexpr int kR2Tile = 64 * 1024 / (kR2Waves * (int)sizeof(float));
 constexpr int kStage = 8192;  // floats of staged gradient rows (32 KiB)
 
 struct ExGeom {
@@ -107,15 +111,17 @@ __global__ __launch_bounds__(kR2Waves * 64) void r2_forward_kernel(
     const int* __restrict__ totals, const float* __restrict__ scan,  // [B][3][S]
     const float* __restrict__ laser,                                  // [B][3][L]
     float* __restrict__ transient,                                    // [B][T][S][L]
-    ExGeom g, int S, int L) {
-  extern __shared__ float hist[];  // [kR2Waves][T]
+    ExGeom g, int S, int L, int tile) {
+  extern __shared__ float hist[];  // [kR2Waves][tile]
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int b = blockIdx.y;
   const int pair = blockIdx.x;
   const int s = pair / L, l = pair - s * L;
   const int nvox = g.D * g.H * g.W;
-  float* h = hist + wave * g.T;
-  for (int t = lane; t < g.T; t += 64) h[t] = 0.f;
+  const int t0 = (int)blockIdx.z * tile;  // this block's bins t0 .. t0 + tn - 1
+  const int tn = min(tile, g.T - t0);
+  float* h = hist + wave * tile;
+  for (int t = lane; t < tn; t += 64) h[t] = 0.f;
   const float spx = scan[(size_t)b * 3 * S + s], spy = scan[(size_t)b * 3 * S + S + s],
               spz = scan[(size_t)b * 3 * S + 2 * S + s];
   const float lpx = laser[(size_t)b * 3 * L + l], lpy = laser[(size_t)b * 3 * L + L + l],
@@ -131,26 +137,24 @@ __global__ __launch_bounds__(kR2Waves * 64) void r2_forward_kernel(
       float xf, yf, zf0, zf1;
       voxel_faces(g, xi, yi, zi, &xf, &yf, &zf0, &zf1);
       const Span sp = voxel_span(g, vb[v], xf, yf, zf0, zf1, spx, spy, spz, lpx, lpy, lpz);
-      if (sp.first <= sp.last) {
-        // Lanes of a wave may hit the same bins: LDS atomics on the wave's
-        // private histogram.
-        if (sp.first == sp.last) {
-          atomicAdd(h + sp.first, sp.wf);
-        } else {
-          atomicAdd(h + sp.first, sp.wf);
-          for (int bn = sp.first + 1; bn < sp.last; ++bn) atomicAdd(h + bn, sp.wm);
-          atomicAdd(h + sp.last, sp.wl);
-        }
+      // The span's bins inside this tile, in increasing order: wf at the
+      // first bin, wl at the last (when it differs), wm between.  Lanes of a
+      // wave may hit the same bins: LDS atomics on the wave's private
+      // histogram.
+      const int lo = max(sp.first, t0), hi = min(sp.last, t0 + tn - 1);
+      for (int bn = lo; bn <= hi; ++bn) {
+        const float wgt = bn == sp.first ? sp.wf : (bn == sp.last ? sp.wl : sp.wm);
+        atomicAdd(h + (bn - t0), wgt);
       }
     }
   }
   __syncthreads();
   // Fixed-order reduction of the wave histograms; one write per bin.
   float* out = transient + (size_t)b * g.T * S * L;
-  for (int t = threadIdx.x; t < g.T; t += blockDim.x) {
+  for (int t = threadIdx.x; t < tn; t += blockDim.x) {
     float v = hist[t];
-    for (int w = 1; w < kR2Waves; ++w) v += hist[w * g.T + t];
-    out[((size_t)t * S + s) * L + l] = v;
+    for (int w = 1; w < kR2Waves; ++w) v += hist[w * tile + t];
+    out[((size_t)(t0 + t) * S + s) * L + l] = v;
   }
 }
 
@@ -256,9 +260,6 @@ extern "C" int totri_volume_render_exhaustive_forward(
   hipStream_t st = (hipStream_t)stream;
   if (B == 0 || T == 0 || S == 0 || L == 0) return TOTRI_OK;
   TOTRI_CHECK_ARG(transient && scan_points && laser_points, "null input");
-  TOTRI_CHECK_ARG((size_t)T * kR2Waves * sizeof(float) <= 64 * 1024,
-                  "T=%lld bins exceed the LDS histogram limit %d", (long long)T,
-                  (int)(64 * 1024 / (kR2Waves * sizeof(float))));
   const size_t nvox = (size_t)D * H * W;
   const ExGeom g = ex_geom(B, D, H, W, T, volume_start, volume_end, bin_width, bin_offset);
   const size_t need = totri_volume_render_exhaustive_forward_workspace(B, D, H, W);
@@ -280,9 +281,15 @@ extern "C" int totri_volume_render_exhaustive_forward(
     rc = check_launch("compact_scan_kernel");
     if (rc) return rc;
   }
-  hipLaunchKernelGGL(r2_forward_kernel, dim3((unsigned)(S * L), (unsigned)B),
-                     dim3(kR2Waves * 64), (size_t)kR2Waves * T * sizeof(float), st, volume,
-                     list, totals, scan_points, laser_points, transient, g, (int)S, (int)L);
+  const int tile = T < kR2Tile ? (int)T : kR2Tile;
+  const int64_t tiles = (T + tile - 1) / tile;
+  TOTRI_CHECK_ARG(S * L < (1ll << 31) && B <= 65535 && tiles <= 65535,
+                  "launch grid too large (S*L=%lld, B=%lld, %lld bin tiles)",
+                  (long long)(S * L), (long long)B, (long long)tiles);
+  hipLaunchKernelGGL(r2_forward_kernel, dim3((unsigned)(S * L), (unsigned)B, (unsigned)tiles),
+                     dim3(kR2Waves * 64), (size_t)kR2Waves * tile * sizeof(float), st, volume,
+                     list, totals, scan_points, laser_points, transient, g, (int)S, (int)L,
+                     tile);
   return check_launch("r2_forward_kernel");
 }
 
