@@ -206,7 +206,8 @@ def general_scan(mp_np: np.ndarray) -> np.ndarray:
 
 
 def kernel_names(info):
-    r1 = "r1_sym_kernel" if info[0] else "r1_forward_kernel"
+    r1 = ("r1_sym_kernel" if info[0] else
+          "r1_pair_kernel" if info[2] else "r1_forward_kernel")
     a1 = ("a1_quad_kernel" if info[0] else
           "a1_pair_kernel" if info[2] else "a1_row_kernel")
     return r1, a1
@@ -704,6 +705,11 @@ def kernels_main(args):
         fm1 = flop_model(mpn, 128, 1024, w)
         row(name, kern, "128^3 voxels, 64x64 scan, 1024 bins", ms, pairs,
             "voxel-scan pairs", fm1["f_a1"], 4 * (4096 * 1024 + 128 ** 3))
+    mx = torch.from_numpy(mp_x).to(dev)
+    fmx = flop_model(mp_x, 128, 1024, w)
+    ms = timeit(lambda: volume_confocal_forward(alb, mx, tr1, ORIGIN, SIZE, w, 0.0))
+    row("R1 render, x-pair scan", "r1_pair_kernel", "128^3 voxels, 64x64 scan, 1024 bins",
+        ms, pairs, "voxel-scan pairs", fmx["f_r1"], 4 * (4096 * 1024 + 128 ** 3))
     m1 = torch.from_numpy(mp_g).to(dev)
     fm1 = flop_model(mp_g, 128, 1024, w)
     ms = timeit(lambda: volume_confocal_forward(alb, m1, tr1, ORIGIN, SIZE, w, 0.0))

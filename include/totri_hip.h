@@ -469,6 +469,19 @@ int totri_debug_a1_walks(int64_t X, int64_t Y, int64_t Z, int64_t N, int64_t T,
                          const float volume_origin[3], const float volume_size[3],
                          float bin_width, float bin_offset);
 
+/*
+ * Testing aid with no counterpart in the reference: the mirror R1 kernels
+ * that totri_volume_confocal_forward (and the batched step) launch for a
+ * geometry, computed on the host without device work; the device's mirror
+ * detection then picks among them.  Bits:
+ *   1  xy-mirror R1 (X, Y even, N a multiple of 4, grid antisymmetric in x
+ *      and y)
+ *   2  x-pair R1 (X even, N even, grid antisymmetric in x)
+ * Returns a negative status code for invalid sizes.
+ */
+int totri_debug_r1_paths(int64_t X, int64_t Y, int64_t Z, int64_t N, int64_t T,
+                         const float volume_origin[3], const float volume_size[3]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -137,6 +137,27 @@ def test_install_as_totri_aliases_reference_modules():
     assert sys.modules["totri_cpp.reconstruct"].backprojection_confocal_velten_forward
 
 
+def test_r1_mirror_paths_on_the_host():
+    """The mirror R1 kernels a geometry allows (host-side plan, no GPU): the
+    xy-mirror kernel needs a grid antisymmetric in x and y and N a multiple
+    of 4; the x-pair kernel only x antisymmetry and an even N."""
+    from totrilib_amd import _lib
+    lib = _lib.load()
+    o, sz = _lib.f3((-1, -1, 0.5)), _lib.f3((2, 2, 2))
+
+    def paths(X=8, Y=8, N=16, origin=o):
+        return lib.totri_debug_r1_paths(X, Y, 8, N, 64, origin, sz)
+
+    assert paths() == 1 | 2
+    assert paths(Y=7) == 2                                # odd Y: x-pairs only
+    assert paths(origin=_lib.f3((-1, -0.9, 0.5))) == 2    # box off-centre in y
+    assert paths(origin=_lib.f3((-0.9, -1, 0.5))) == 0    # off-centre in x
+    assert paths(N=6) == 2                                # pairs, no quads
+    assert paths(N=5) == 0
+    assert paths(X=7) == 0
+    assert paths(X=-1) < 0
+
+
 def test_a1_row_walks_are_gated_by_the_float_bin_counter():
     """The A1 row walks count bins in a float, exact below 2^24; longer
     transients must take the window walks (host-side plan, no GPU)."""

@@ -414,13 +414,25 @@ def test_autograd_volume_and_points():
     assert ev < TOL and em < TOL, (ev, em)
 
 
-@pytest.mark.parametrize("scan", ["mirror", "perturbed", "random"])
+def r1_paths(shape, n, T, origin=ORIGIN):
+    from totrilib_amd import _lib
+    lib = _lib.load()
+    return lib.totri_debug_r1_paths(*shape, n, T, _lib.f3(origin), _lib.f3(SIZE))
+
+
+@pytest.mark.parametrize("scan", ["mirror", "xpairs", "odd_y", "xpairs_only", "perturbed",
+                                  "random"])
 def test_r1_mirror_quads_and_general_path(scan):
     """A scan of mirror quads on a box centred at x = y = 0 takes the
-    symmetric kernel (one decision per four samples); one moved point or a
-    random scan takes the general kernel.  All match the oracle."""
+    symmetric kernel (one decision per four samples).  The x-pair kernel (one
+    decision per two samples) takes a box off-centre in y ("xpairs"), an odd
+    Y ("odd_y") or a scan of x-pairs without y-mirrors ("xpairs_only"); one
+    moved point or a random scan takes the general kernel.  All match the
+    oracle."""
     rng = np.random.default_rng(31)
-    X, Y, Z, T = 32, 30, 40, 700
+    # (Y = 32: cell centres of [-1, 1] antisymmetric in float; Y = 30's are not)
+    X, Y, Z, T = 32, 31 if scan == "odd_y" else 32, 40, 700
+    origin = (ORIGIN[0], ORIGIN[1] + 0.1, ORIGIN[2]) if scan == "xpairs" else ORIGIN
     vol = sparse_albedo((X, Y, Z), 0.2, 32)
     mp = wall_grid(8)
     if scan == "perturbed":
@@ -428,18 +440,27 @@ def test_r1_mirror_quads_and_general_path(scan):
     elif scan == "random":
         mp = np.stack([rng.uniform(-1, 1, 64), rng.uniform(-1, 1, 64),
                        np.zeros(64)]).astype(np.float32)
+    elif scan == "xpairs_only":
+        # x-pairs at random y: no point has a y-mirror
+        h = np.stack([rng.uniform(0.05, 1, 32), rng.uniform(-1, 1, 32)]).astype(np.float32)
+        mp = np.concatenate([np.concatenate([h, h * [[-1], [1]]], 1), np.zeros((1, 64))],
+                            0).astype(np.float32)
     w = 0.0045
     info = mirror_info(mp)
-    assert info[:2] == ([1, 16] if scan == "mirror" else [0, info[1]])
-    got = gpu_r1(vol, mp, T, w, 0.0)
-    ref = O.r1_forward(vol, mp, T, ORIGIN, SIZE, w, 0.0, mode=O.PINNED64)
+    want = {"mirror": [1, 16, 1, 32], "xpairs": [1, 16, 1, 32], "odd_y": [1, 16, 1, 32],
+            "xpairs_only": [0, info[1], 1, 32]}.get(scan, [0, info[1], 0, info[3]])
+    assert info == want, info
+    paths = r1_paths((X, Y, Z), mp.shape[1], T, origin)
+    assert paths == (3 if scan in ("mirror", "xpairs_only", "perturbed", "random") else 2)
+    got = gpu_r1(vol, mp, T, w, 0.0, origin=origin)
+    ref = O.r1_forward(vol, mp, T, origin, SIZE, w, 0.0, mode=O.PINNED64)
     e = rel_l2(got, ref)
-    print(f"{scan}: mirror info {info}, rel-L2 vs FP64 pinned {e:.2e}")
+    print(f"{scan}: mirror info {info}, R1 paths {paths}, rel-L2 vs FP64 pinned {e:.2e}")
     assert e < TOL
     assert np.count_nonzero(got) > 0
 
 
-@pytest.mark.parametrize("scan", ["mirror", "random"])
+@pytest.mark.parametrize("scan", ["mirror", "xpairs", "random"])
 @pytest.mark.parametrize("layout", ["inside", "two_walls"])
 def test_r1_mirror_wall_inside_the_box(layout, scan):
     """The mirror and the general R1 take a shorter activity test in
@@ -450,12 +471,12 @@ def test_r1_mirror_wall_inside_the_box(layout, scan):
     nearly equal squares, and the reference's own FP32 arithmetic (the
     oracle's FP32 mode) is about 2e-5 from the FP64 oracle: the check is
     against the FP32 oracle, and the FP64 error may not exceed 1.5x the FP32 oracle's."""
-    X, Y, Z, T = 32, 30, 40, 700
+    X, Y, Z, T = 32, 32, 40, 700
     vol = sparse_albedo((X, Y, Z), 0.2, 33)
     rng = np.random.default_rng(34)
 
     def wall(z):
-        if scan == "mirror":
+        if scan in ("mirror", "xpairs"):
             m = wall_grid(8)
         else:
             m = np.stack([rng.uniform(-1, 1, 64), rng.uniform(-1, 1, 64),
@@ -469,10 +490,13 @@ def test_r1_mirror_wall_inside_the_box(layout, scan):
         mp = wall(0.0)
     else:
         mp = np.ascontiguousarray(np.concatenate([wall(0.0), wall(1.2)], axis=1))
+    if scan == "xpairs":  # the box off-centre in y: the x-pair kernel
+        origin = (origin[0], origin[1] + 0.1, origin[2])
     w = 0.0045
     info = mirror_info(mp)
-    if scan == "mirror":
+    if scan != "random":
         assert info[0] == 1 and info[1] == mp.shape[1] // 4
+        assert r1_paths((X, Y, Z), mp.shape[1], T, origin) == (2 if scan == "xpairs" else 3)
     else:
         assert info[0] == 0
     got = gpu_r1(vol, mp, T, w, 0.0, origin=origin)

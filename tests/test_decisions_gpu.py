@@ -16,7 +16,8 @@ change of support:
 
 They run the bench workload's shape (128^3 voxels, 64x64 scan, 1024 bins of
 2.5 mm one-way) through the entry points, so the mirror-quad walks (R1
-r1_sym_kernel, A1 a1_quad_kernel), the x-pair walk (a box shifted in y), the
+r1_sym_kernel, A1 a1_quad_kernel), the x-pair walks (a box shifted in y: R1
+r1_pair_kernel, A1 a1_pair_kernel), the
 general walks (one scan point moved by an ulp) and the window walks (a
 negative bin offset) each take their own decisions (fast path behind the
 guard, exact path within it).  Near-boundary samples are picked from the
@@ -163,7 +164,7 @@ def test_a1_walk_decisions_one_hot(name):
     assert flips == 0
 
 
-@pytest.mark.parametrize("name", ["quads", "general", "window"])
+@pytest.mark.parametrize("name", ["quads", "xpairs", "general", "window"])
 def test_r1_walk_decisions_one_hot(name):
     from totrilib_amd.totri_cpp.render import volume_confocal_forward
     mp, origin, off = scenario(name)

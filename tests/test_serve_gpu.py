@@ -40,9 +40,11 @@ def make_requests(seed=0):
         ("b", 10, (20, 20, 24), 300, 0.009, -0.01, "pinned", "perturbed"),
         ("r", 12, (24, 20, 32), 400, 0.006, 0.0, "device", "random"),
         # odd Y: x-pairs only (the x-pair row walk; the x-mirror window walk
-        # for a negative offset)
+        # for a negative offset; the x-pair R1)
         ("b", 10, (20, 15, 24), 300, 0.009, 0.0, "device"),
         ("b", 10, (20, 15, 24), 300, 0.009, -0.01, "pinned"),
+        ("r", 10, (20, 15, 24), 300, 0.009, 0.0, "device"),
+        ("r", 10, (20, 15, 24), 300, 0.009, -0.01, "pageable"),
     ]
     reqs = []
     for k, (kind, n, res, T, w, off, where, *scan) in enumerate(specs):

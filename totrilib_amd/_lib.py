@@ -126,6 +126,7 @@ SIGNATURES: dict[str, tuple] = {
     "totri_debug_mirror_info": (_c_int, [_c_p, _c_i64, _c_p, _c_sz, _c_p, _c_p]),
     "totri_debug_a1_walks": (
         _c_int, [_c_i64, _c_i64, _c_i64, _c_i64, _c_i64, _F3, _F3, _c_f, _c_f]),
+    "totri_debug_r1_paths": (_c_int, [_c_i64, _c_i64, _c_i64, _c_i64, _c_i64, _F3, _F3]),
     "totri_debug_confocal_samples": (
         _c_int,
         [_c_p, _c_i64, _c_p, _c_i64, _c_i64, _c_i64, _c_i64, _c_i64, _F3, _F3,

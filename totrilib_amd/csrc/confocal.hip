@@ -627,9 +627,11 @@ __global__ __launch_bounds__(1024) void scan_quads_kernel(const int* __restrict_
   scan_quads_block(partner, N, quads, pairs, info);
 }
 
-// True when the wall points of all of the workgroup's quads lie below the
-// first voxel centre (c0 = (wz - oz) s - 1/2 < 0): a sample with d2 <= 0
-// then decides a negative row.  The same value in every thread.
+// True when the wall points of all of the workgroup's quads (M = 4) or
+// x-pairs (M = 2) lie below the first voxel centre (c0 = (wz - oz) s - 1/2
+// < 0): a sample with d2 <= 0 then decides a negative row.  The same value in
+// every thread.
+template <int M = 4>
 __device__ __forceinline__ bool quads_below_box(const float* __restrict__ mp,
                                                 const int* __restrict__ quads, int n_quads,
                                                 int qgrp, const ConfGeom& g) {
@@ -638,7 +640,7 @@ __device__ __forceinline__ bool quads_below_box(const float* __restrict__ mp,
   for (int w = 0; w < kWaves; ++w) {
     const int q = qgrp * kWaves + w;
     if (q < n_quads) {
-      const float wz = mp[2 * g.N + quads[4 * q]];
+      const float wz = mp[2 * g.N + quads[M * q]];
       float c0 = (wz - g.oz) * g.s;
       c0 = c0 - 0.5f;
       below = below && (c0 < 0.f);
@@ -652,7 +654,11 @@ __device__ __forceinline__ bool quads_below_box(const float* __restrict__ mp,
 // chunk buffers hold, per quadrant column, its four mirror columns:
 // lds2[((j * 4) + m) * Z + z] for column quad j and mirror m.
 // BELOW (quads_below_box holds): the activity test is the row range alone.
-template <int K, bool BELOW = false>
+// M = 2: the same walk for x-pairs of scan points (s0, s1 = x-mirror of s0)
+// over the half grid x < X/2 and every y: a column and its x-mirror make a
+// column pair, families f = 0, 1 share dy^2, and one decision serves two
+// samples (scan s_m at column c_(m^f)).
+template <int K, bool BELOW = false, int M = 4>
 __device__ __forceinline__ void r1_sym_block(
     const float* __restrict__ vol, const float* __restrict__ mp,
     const int* __restrict__ quads, int n_quads, float* __restrict__ dst,
@@ -660,20 +666,21 @@ __device__ __forceinline__ void r1_sym_block(
     int chunks_per_group, float2* __restrict__ lds2) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
+  static_assert(M == 4 || M == 2, "quads or x-pairs");
   const int q = qgrp * kWaves + wave;
   const bool quad_ok = q < n_quads;
-  const int HX = g.X / 2, HY = g.Y / 2;
+  const int HX = g.X / 2, HY = M == 4 ? g.Y / 2 : g.Y;
   const int n_reps = HX * HY;
-  // A chunk holds reps_per_chunk column quads as [rep][z][m] pairs with
-  // z = 0 .. Z, where rows Z - 1 and Z are (0, 0) (no active sample reads
-  // row Z - 1: it needs z <= Z - 2): a sample's four mirror pairs sit side
+  // A chunk holds reps_per_chunk column quads (pairs) as [rep][z][m] pairs
+  // with z = 0 .. Z, where rows Z - 1 and Z are (0, 0) (no active sample
+  // reads row Z - 1: it needs z <= Z - 2): a sample's M mirror pairs sit side
   // by side, and an inactive sample reads a zero row.
   const int zrows = g.Z + 1;
-  const int rep_pairs = 4 * zrows;
+  const int rep_pairs = M * zrows;
   const int chunk_vox = reps_per_chunk * rep_pairs;  // pairs
-  const int chunk_items = reps_per_chunk * zrows;    // (rep, z) rows of 4 pairs
+  const int chunk_items = reps_per_chunk * zrows;    // (rep, z) rows of M pairs
 
-  float tc2[K], itc4[K], acc[4][K];
+  float tc2[K], itc4[K], acc[M][K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int t = t_base + 64 * k + lane;
@@ -684,11 +691,11 @@ __device__ __forceinline__ void r1_sym_block(
     // the sums take v * d2^2 and are scaled by 1/tc^4 twice at the end
     itc4[k] = qq > 0.f ? fminf(1.f / (qq * qq), 3.4028235e38f) : 0.f;
 #pragma unroll
-    for (int m = 0; m < 4; ++m) acc[m][k] = 0.f;
+    for (int m = 0; m < M; ++m) acc[m][k] = 0.f;
   }
-  int sidx[4];
+  int sidx[M];
 #pragma unroll
-  for (int m = 0; m < 4; ++m) sidx[m] = quad_ok ? quads[4 * q + m] : 0;
+  for (int m = 0; m < M; ++m) sidx[m] = quad_ok ? quads[M * q + m] : 0;
   const int s0 = uniform_i32(sidx[0]);
   const float wx = uniform_f32(mp[s0]);
   const float wy = uniform_f32(mp[g.N + s0]);
@@ -705,10 +712,10 @@ __device__ __forceinline__ void r1_sym_block(
   const int chunk_end = min(chunk_first + chunks_per_group, total_chunks);
   const int nthreads = kWaves * 64;
 
-  // Row `it` of chunk c: the pairs (v[z], v[z+1] - v[z]) of the four mirror
-  // columns m = 0..3 of one column quad at one z, as two float4 (m = 0, 1 and
-  // m = 2, 3).  Consecutive rows are consecutive z of the same columns, so a
-  // wave's loads run along four columns.
+  // Row `it` of chunk c: the pairs (v[z], v[z+1] - v[z]) of the M mirror
+  // columns m = 0..M-1 of one column quad (pair) at one z, as float4 (m = 0, 1
+  // and m = 2, 3).  Consecutive rows are consecutive z of the same columns,
+  // so a wave's loads run along M columns.
   auto load_row = [&](int c, int it, float4* lo, float4* hi) {
     const int jr = it / zrows;
     const int z = it - jr * zrows;
@@ -719,7 +726,7 @@ __device__ __forceinline__ void r1_sym_block(
     if (r < n_reps && z < g.Z - 1) {
       const int rx = r / HY, ry = r - rx * HY;
 #pragma unroll
-      for (int m = 0; m < 4; ++m) {
+      for (int m = 0; m < M; ++m) {
         const int xi = (m & 1) ? g.X - 1 - rx : rx;
         const int yi = (m & 2) ? g.Y - 1 - ry : ry;
         const float* col = vol + ((size_t)xi * g.Y + yi) * g.Z;
@@ -732,18 +739,25 @@ __device__ __forceinline__ void r1_sym_block(
     *hi = make_float4(p[2].x, p[2].y, p[3].x, p[3].y);
   };
 
+  auto store_row = [&](float4* base, int it, float4 lo, float4 hi) {
+    if (M == 4) {
+      base[2 * it] = lo;
+      base[2 * it + 1] = hi;
+    } else {
+      base[it] = lo;
+    }
+  };
   if (chunk_first < chunk_end) {
     float4* dst4 = reinterpret_cast<float4*>(lds2);
     for (int it = threadIdx.x; it < chunk_items; it += nthreads) {
       float4 lo, hi;
       load_row(chunk_first, it, &lo, &hi);
-      dst4[2 * it] = lo;
-      dst4[2 * it + 1] = hi;
+      store_row(dst4, it, lo, hi);
     }
   }
   __syncthreads();
 
-  constexpr int kPref = kChunkVoxels / (4 * kWaves * 64);  // rows per thread
+  constexpr int kPref = kChunkVoxels / (M * kWaves * 64);  // rows per thread
   for (int c = chunk_first; c < chunk_end; ++c) {
     const int buf = (c - chunk_first) & 1;
     const float2* cur = lds2 + buf * chunk_vox;
@@ -758,8 +772,7 @@ __device__ __forceinline__ void r1_sym_block(
         if (it < chunk_items) {
           float4 lo, hi;
           load_row(c + 1, it, &lo, &hi);
-          nxt4[2 * it] = lo;
-          nxt4[2 * it + 1] = hi;
+          store_row(nxt4, it, lo, hi);
         }
       }
     }
@@ -779,7 +792,7 @@ __device__ __forceinline__ void r1_sym_block(
       unsigned packed = 0u;
       bool any = false;
 #pragma unroll
-      for (int f = 0; f < 4; ++f) {
+      for (int f = 0; f < M; ++f) {
         const float dx2 = (f & 1) ? B : A;
         const float dy2 = (f & 2) ? D : C;
         int tlo, thi;
@@ -797,20 +810,20 @@ __device__ __forceinline__ void r1_sym_block(
         mask &= mask - 1;
         const unsigned wm = (unsigned)__builtin_amdgcn_readlane((int)packed, j);
         const float cA = lane_value(A, j), cB = lane_value(B, j);
-        const float cC = lane_value(C, j), cD = lane_value(D, j);
+        const float cC = lane_value(C, j), cD = M == 4 ? lane_value(D, j) : 0.f;
         const float2* quad = cur + j * rep_pairs;
         // Families 2h and 2h+1 share dy^2; both are evaluated for a window
         // where either is active (two independent decisions per pass; a
         // sample outside its family's range fails the activity test).  The
         // first subtraction tc^2 - dx^2 is shared by h = 0 and h = 1.
         const unsigned wmh0 = (wm | (wm >> 8)) & 0xffu;
-        const unsigned wmh1 = ((wm >> 16) | (wm >> 24)) & 0xffu;
+        const unsigned wmh1 = M == 4 ? ((wm >> 16) | (wm >> 24)) & 0xffu : 0u;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
           if (!(((wmh0 | wmh1) >> k) & 1u)) continue;
           const float pA = tc2[k] - cA, pB = tc2[k] - cB;  // volume_cuda.cu:40
 #pragma unroll
-          for (int h = 0; h < 2; ++h) {
+          for (int h = 0; h < M / 2; ++h) {
             const int fa = 2 * h, fb = 2 * h + 1;
             if (!(((h ? wmh1 : wmh0) >> k) & 1u)) continue;
             const float dy2 = h ? cD : cC;
@@ -841,12 +854,12 @@ __device__ __forceinline__ void r1_sym_block(
               ra = acta ? za : g.Z;
               rb = actb ? zb : g.Z;
             }
-            const float2* pa = quad + 4 * ra;
-            const float2* pb = quad + 4 * rb;
+            const float2* pa = quad + M * ra;
+            const float2* pb = quad + M * rb;
             // the common factor 1/tc^8 is applied to the sums at the end
             const float wa = d2ca * d2ca, wb = d2cb * d2cb;
 #pragma unroll
-            for (int m = 0; m < 4; ++m) {
+            for (int m = 0; m < M; ++m) {
               const float2 va = pa[m ^ fa];
               const float2 vb = pb[m ^ fb];
               const float fva = fmaf(ala, va.y, va.x);
@@ -863,7 +876,7 @@ __device__ __forceinline__ void r1_sym_block(
 
   if (quad_ok) {
 #pragma unroll
-    for (int m = 0; m < 4; ++m) {
+    for (int m = 0; m < M; ++m) {
       float* row = dst + ((size_t)grp * g.N + sidx[m]) * g.T;
 #pragma unroll
       for (int k = 0; k < K; ++k) {
@@ -1143,6 +1156,30 @@ void r1_sym_kernel(
                            reps_per_chunk, chunks_per_group, lds2);
 }
 
+// x-pair R1: the mirror R1 over x-pairs of scan points (info[3] of them,
+// found on the device), for scans whose points all have x-mirrors (info[2])
+// and that the xy-mirror kernel does not take (quad_on and info[0]).
+template <int K>
+__global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(4)))
+void r1_pair_kernel(
+    const float* __restrict__ vol, const float* __restrict__ mp,
+    const int* __restrict__ pairs, const int* __restrict__ info, float* __restrict__ dst,
+    ConfGeom g, int n_groups, int reps_per_chunk, int chunks_per_group, int quad_on) {
+  if (!info[2] || (quad_on && info[0])) return;
+  extern __shared__ float2 lds2[];
+  const int n_pairs = info[3];
+  const int qgrp = (int)(blockIdx.x / n_groups);
+  if (qgrp * kWaves >= n_pairs) return;  // whole workgroup idle
+  const int t_base = (int)(gridDim.y - 1 - blockIdx.y) * 64 * K;  // later passes first
+  const int grp = (int)(blockIdx.x % n_groups);
+  if (quads_below_box<2>(mp, pairs, n_pairs, qgrp, g))
+    r1_sym_block<K, true, 2>(vol, mp, pairs, n_pairs, dst, g, t_base, grp, qgrp, n_groups,
+                             reps_per_chunk, chunks_per_group, lds2);
+  else
+    r1_sym_block<K, false, 2>(vol, mp, pairs, n_pairs, dst, g, t_base, grp, qgrp, n_groups,
+                              reps_per_chunk, chunks_per_group, lds2);
+}
+
 // Symmetric A6, all bin passes in one launch (blockIdx.y counts them from the
 // last one, as in r1_sym_kernel).
 template <int K, typename GT>
@@ -1164,12 +1201,14 @@ void a6_sym_kernel(
 }
 
 // Sum of partials in a fixed order where the number of parts depends on
-// which path ran (info[0] set: the symmetric one).
+// which path ran: the xy-mirror one (info[0] set and n_sym given), else the
+// x-pair one (info[2] set and n_pair given), else the general one.
 template <typename OutT>
 __global__ void sum_partials_flag_kernel(const float* __restrict__ part,
-                                         const int* __restrict__ info, int n_sym, int n_gen,
-                                         size_t n, OutT* __restrict__ out) {
-  const int n_parts = info[0] ? n_sym : n_gen;
+                                         const int* __restrict__ info, int n_sym, int n_pair,
+                                         int n_gen, size_t n, OutT* __restrict__ out) {
+  // n_sym (n_pair) is 0 when the xy-mirror (x-pair) kernel was not launched
+  const int n_parts = (n_sym && info[0]) ? n_sym : (n_pair && info[2]) ? n_pair : n_gen;
   for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
        e += (size_t)gridDim.x * blockDim.x) {
     float v = part[e];
@@ -2737,11 +2776,46 @@ R1SymPlan plan_r1_sym(int64_t X, int64_t Y, int64_t Z, int64_t N, int64_t T,
   return p;
 }
 
+// Plan of the x-pair R1 (used when the device finds x-pairs of scan points and
+// no quads the xy-mirror R1 takes): the mirror plan over column pairs of the
+// half grid x < X/2.
+R1SymPlan plan_r1_pair(int64_t X, int64_t Y, int64_t Z, int64_t N, int64_t T,
+                       const R1Plan& gen) {
+  R1SymPlan p;
+  memset(&p, 0, sizeof(p));
+  p.possible = X >= 2 && !(X & 1) && Y >= 1 && N >= 2 && !(N & 1) && Z >= 2 &&
+               2 * (Z + 1) <= kChunkVoxels && N <= (1 << 24);
+  if (!p.possible) return p;
+  p.K = gen.K < 8 ? gen.K : 8;
+  p.passes = (int)((T + 64 * p.K - 1) / (64 * p.K));
+  const int64_t n_reps = (X / 2) * Y;
+  int64_t rpc = kChunkVoxels / (2 * (Z + 1));  // column pairs plus a zero row each
+  if (rpc > 64) rpc = 64;
+  p.reps_per_chunk = (int)rpc;
+  const int64_t total_chunks = (n_reps + rpc - 1) / rpc;
+  const int64_t pair_blocks = (N / 2 + kWaves - 1) / kWaves;
+  const int64_t target = 4 * (int64_t)device_cu_count();
+  int64_t G = 1;
+  if (pair_blocks < target) G = (target + pair_blocks - 1) / pair_blocks;
+  if (G > total_chunks) G = total_chunks;
+  p.chunks_per_group = (int)((total_chunks + G - 1) / G);
+  p.n_groups = (int)((total_chunks + p.chunks_per_group - 1) / p.chunks_per_group);
+  p.ml = mirror_layout(N, 0);
+  p.staging_off = p.ml.end;
+  const int parts = p.n_groups > gen.n_groups ? p.n_groups : gen.n_groups;
+  p.ws = p.staging_off + (size_t)parts * N * T * sizeof(float);
+  return p;
+}
+
 size_t r1_workspace(int64_t X, int64_t Y, int64_t Z, int64_t N, int64_t T) {
   if (X <= 0 || Y <= 0 || Z < 2 || N <= 0 || T <= 0) return 0;
   const R1Plan p = plan_r1(X, Y, Z, N, T);
   const R1SymPlan q = plan_r1_sym(X, Y, Z, N, T, p);
-  return q.possible && q.ws > p.ws ? q.ws : p.ws;
+  const R1SymPlan pr = plan_r1_pair(X, Y, Z, N, T, p);
+  size_t ws = p.ws;
+  if (q.possible && q.ws > ws) ws = q.ws;
+  if (pr.possible && pr.ws > ws) ws = pr.ws;
+  return ws;
 }
 
 template <int K>
@@ -2753,6 +2827,18 @@ void launch_r1(const R1Plan& p, const float* vol, const float* mp, float* dst,
   hipLaunchKernelGGL(r1_forward_kernel<K>, dim3(blocks), dim3(kWaves * 64), lds, st,
                      vol, mp, dst, g, t_base, p.n_groups, p.chunk_cols,
                      p.chunks_per_group, skip);
+}
+
+template <int K>
+void launch_r1_pair(const R1SymPlan& p, const float* vol, const float* mp, const int* pairs,
+                    const int* info, float* dst, const ConfGeom& g, int quad_on,
+                    hipStream_t st) {
+  const int pair_blocks = (g.N / 2 + kWaves - 1) / kWaves;
+  const int blocks = pair_blocks * p.n_groups;
+  const size_t lds = 2 * (size_t)p.reps_per_chunk * 2 * (g.Z + 1) * sizeof(float2);
+  hipLaunchKernelGGL(r1_pair_kernel<K>, dim3(blocks, p.passes), dim3(kWaves * 64), lds, st,
+                     vol, mp, pairs, info, dst, g, p.n_groups, p.reps_per_chunk,
+                     p.chunks_per_group, quad_on);
 }
 
 template <int K>
@@ -2831,13 +2917,16 @@ extern "C" int totri_volume_confocal_forward(
                   kChunkVoxels);
   const R1Plan p = plan_r1(X, Y, Z, N, T);
   const R1SymPlan q = plan_r1_sym(X, Y, Z, N, T, p);
+  const R1SymPlan pr = plan_r1_pair(X, Y, Z, N, T, p);
   const size_t need = r1_workspace(X, Y, Z, N, T);
-  // The symmetric path needs the volume grid antisymmetric in x and y; the
-  // scan-point quads are then looked for on the device.
-  const bool sym = q.possible &&
-                   axis_antisymmetric(volume_origin[0], volume_size[0], X) &&
+  // The xy-mirror path needs the volume grid antisymmetric in x and y, the
+  // x-pair path in x; the scan-point quads and x-pairs are then looked for
+  // on the device.  (Every grid that allows the first allows the second.)
+  const bool xanti = axis_antisymmetric(volume_origin[0], volume_size[0], X);
+  const bool sym = q.possible && xanti &&
                    axis_antisymmetric(volume_origin[1], volume_size[1], Y);
-  const bool direct = !sym && p.n_groups == 1 && transient_dtype == TOTRI_F32;
+  const bool xsym = pr.possible && xanti;
+  const bool direct = !sym && !xsym && p.n_groups == 1 && transient_dtype == TOTRI_F32;
   if (!direct && workspace_bytes < need)
     return fail(TOTRI_EWORKSPACE, "workspace too small: need %zu bytes, got %zu",
                 need, workspace_bytes);
@@ -2845,13 +2934,19 @@ extern "C" int totri_volume_confocal_forward(
   const ConfGeom g = make_geom(X, Y, Z, N, T, volume_origin, volume_size,
                                bin_width, bin_offset);
   const int* info = nullptr;
+  const int* skip = nullptr;
   float* dst = direct ? (float*)transient : (float*)workspace;
-  if (sym) {
-    rc = launch_mirror_detect(q.ml, measurement_points, (int)N, workspace, st);
+  if (xsym) {
+    rc = launch_mirror_detect(pr.ml, measurement_points, (int)N, workspace, st);
     if (rc) return rc;
-    info = (const int*)((char*)workspace + q.ml.info_off);
+    info = (const int*)((char*)workspace + pr.ml.info_off);
+    // the general kernel exits when every point has an x-mirror (info[2]):
+    // one of the mirror kernels below does the work (quads imply x-pairs)
+    skip = info + 2;
+    dst = (float*)((char*)workspace + pr.staging_off);
+  }
+  if (sym) {
     const int* quads = (const int*)((char*)workspace + q.ml.quads_off);
-    dst = (float*)((char*)workspace + q.staging_off);
     switch (q.K) {
       case 1: launch_r1_sym<1>(q, volume, measurement_points, quads, info, dst, g, st); break;
       case 2: launch_r1_sym<2>(q, volume, measurement_points, quads, info, dst, g, st); break;
@@ -2861,33 +2956,46 @@ extern "C" int totri_volume_confocal_forward(
     rc = check_launch("r1_sym_kernel");
     if (rc) return rc;
   }
+  if (xsym) {
+    const int* pairs = (const int*)((char*)workspace + pr.ml.pairs_off);
+    const int qo = sym ? 1 : 0;
+    switch (pr.K) {
+      case 1: launch_r1_pair<1>(pr, volume, measurement_points, pairs, info, dst, g, qo, st); break;
+      case 2: launch_r1_pair<2>(pr, volume, measurement_points, pairs, info, dst, g, qo, st); break;
+      case 4: launch_r1_pair<4>(pr, volume, measurement_points, pairs, info, dst, g, qo, st); break;
+      default: launch_r1_pair<8>(pr, volume, measurement_points, pairs, info, dst, g, qo, st); break;
+    }
+    rc = check_launch("r1_pair_kernel");
+    if (rc) return rc;
+  }
   for (int pass = 0; pass < p.passes; ++pass) {
     const int t_base = pass * 64 * p.K;
     switch (p.K) {
-      case 1: launch_r1<1>(p, volume, measurement_points, dst, g, t_base, st, info); break;
-      case 2: launch_r1<2>(p, volume, measurement_points, dst, g, t_base, st, info); break;
-      case 4: launch_r1<4>(p, volume, measurement_points, dst, g, t_base, st, info); break;
-      case 8: launch_r1<8>(p, volume, measurement_points, dst, g, t_base, st, info); break;
-      default: launch_r1<16>(p, volume, measurement_points, dst, g, t_base, st, info); break;
+      case 1: launch_r1<1>(p, volume, measurement_points, dst, g, t_base, st, skip); break;
+      case 2: launch_r1<2>(p, volume, measurement_points, dst, g, t_base, st, skip); break;
+      case 4: launch_r1<4>(p, volume, measurement_points, dst, g, t_base, st, skip); break;
+      case 8: launch_r1<8>(p, volume, measurement_points, dst, g, t_base, st, skip); break;
+      default: launch_r1<16>(p, volume, measurement_points, dst, g, t_base, st, skip); break;
     }
     rc = check_launch("r1_forward_kernel");
     if (rc) return rc;
   }
   if (direct) return TOTRI_OK;
-  if (sym) {
+  if (xsym) {
     const dim3 grid(launch_grid(n)), block(256);
+    const int nq = sym ? q.n_groups : 0;
     switch (transient_dtype) {
       case TOTRI_F32:
         hipLaunchKernelGGL(sum_partials_flag_kernel<float>, grid, block, 0, st, dst, info,
-                           q.n_groups, p.n_groups, n, (float*)transient);
+                           nq, pr.n_groups, p.n_groups, n, (float*)transient);
         break;
       case TOTRI_BF16:
         hipLaunchKernelGGL(sum_partials_flag_kernel<__hip_bfloat16>, grid, block, 0, st, dst,
-                           info, q.n_groups, p.n_groups, n, (__hip_bfloat16*)transient);
+                           info, nq, pr.n_groups, p.n_groups, n, (__hip_bfloat16*)transient);
         break;
       default:
         hipLaunchKernelGGL(sum_partials_flag_kernel<__half>, grid, block, 0, st, dst, info,
-                           q.n_groups, p.n_groups, n, (__half*)transient);
+                           nq, pr.n_groups, p.n_groups, n, (__half*)transient);
         break;
     }
     return check_launch("sum_partials_flag_kernel");
@@ -3561,7 +3669,7 @@ extern "C" int totri_volume_confocal_grad_measurement_points(
   if (sym) {
     // info[0] says which path ran
     hipLaunchKernelGGL(sum_partials_flag_kernel<float>, dim3(launch_grid(n)), dim3(256), 0, st,
-                       part, info, q.passes * q.n_groups, p.passes * p.r.n_groups, n,
+                       part, info, q.passes * q.n_groups, 0, p.passes * p.r.n_groups, n,
                        measurement_points_grad);
     return check_launch("sum_partials_flag_kernel");
   }
@@ -3584,6 +3692,20 @@ extern "C" int totri_debug_a1_walks(int64_t X, int64_t Y, int64_t Z, int64_t N, 
   const A1Walks k = choose_a1_walks(q, X, Y, T, volume_origin, volume_size, bin_width,
                                     bin_offset);
   return (k.sym ? 1 : 0) | (k.quad ? 2 : 0) | (k.row ? 4 : 0) | (k.prow ? 8 : 0);
+}
+
+extern "C" int totri_debug_r1_paths(int64_t X, int64_t Y, int64_t Z, int64_t N, int64_t T,
+                                    const float volume_origin[3], const float volume_size[3]) {
+  int rc = check_common(X, Y, Z, N, T, volume_origin, volume_size, 1.f, 0.f, TOTRI_F32);
+  if (rc) return -rc;
+  if (X <= 0 || Y <= 0 || Z < 2 || N <= 0 || T <= 0) return 0;
+  const R1Plan p = plan_r1(X, Y, Z, N, T);
+  const R1SymPlan q = plan_r1_sym(X, Y, Z, N, T, p);
+  const R1SymPlan pr = plan_r1_pair(X, Y, Z, N, T, p);
+  const bool xanti = axis_antisymmetric(volume_origin[0], volume_size[0], X);
+  const bool sym = q.possible && xanti && axis_antisymmetric(volume_origin[1], volume_size[1], Y);
+  const bool xsym = pr.possible && xanti;
+  return (sym ? 1 : 0) | (xsym ? 2 : 0);
 }
 
 extern "C" size_t totri_debug_mirror_workspace(int64_t N) {
