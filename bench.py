@@ -720,6 +720,11 @@ def kernels_main(args):
     row("A6 scan-point gradient, general scan", "a6_kernel", "128^3 voxels, 64x64 scan, 1024 bins",
         ms, fm1["a_fwd"] * 128 ** 3 * 4096, "active samples", 30,
         4 * (2 * 4096 * 1024 + 128 ** 3))
+    ms = timeit(lambda: volume_confocal_grad_measurement_points(gm1, alb, mx, Gd, ORIGIN, SIZE,
+                                                                 w, 0.0))
+    row("A6 scan-point gradient, x-pair scan", "a6_pair_kernel",
+        "128^3 voxels, 64x64 scan, 1024 bins", ms, fmx["a_fwd"] * 128 ** 3 * 4096,
+        "active samples", 30, 4 * (2 * 4096 * 1024 + 128 ** 3))
     # f-k migration: 1024 bins on a 64 x 64 wall; the spectrum is [2048, 64, 64] complex
     from totrilib_amd import _lib
     from totrilib_amd.reconstruct import FkMigration

@@ -339,14 +339,17 @@ def test_a6_matches_oracle():
     assert e64 < TOL
 
 
-@pytest.mark.parametrize("scan", ["mirror", "perturbed"])
+@pytest.mark.parametrize("scan", ["mirror", "xpairs", "odd_y", "perturbed"])
 def test_a6_mirror_quads_and_general_path(scan):
     """Mirror quads of scan points on a box centred at x = y = 0 take the
     symmetric A6 (one decision per four scan points, gradients with the
-    mirror signs); one point moved by an ulp takes the general kernel.  Both
-    match the oracle."""
+    mirror signs); a box off-centre in y ("xpairs") or an odd Y ("odd_y")
+    the x-pair A6 (one decision per two); one point moved by an ulp the
+    general kernel.  All match the oracle.  (16 cells over [-1, 1] have
+    antisymmetric centres in float; 18 or 20 do not.)"""
     rng = np.random.default_rng(23)
-    X, Y, Z, T = 20, 18, 24, 700
+    X, Y, Z, T = 16, 15 if scan == "odd_y" else 16, 24, 700
+    origin = (ORIGIN[0], ORIGIN[1] + 0.1, ORIGIN[2]) if scan == "xpairs" else ORIGIN
     vol = rng.random((X, Y, Z)).astype(np.float32)
     mp = wall_grid(6)
     if scan == "perturbed":
@@ -354,9 +357,11 @@ def test_a6_mirror_quads_and_general_path(scan):
     w, off = 0.004, 0.001
     G = rng.standard_normal((mp.shape[1], T)).astype(np.float32)
     info = mirror_info(mp)
-    assert info[:2] == ([1, 9] if scan == "mirror" else [0, info[1]])
-    got = gpu_a6(vol, mp, G, w, off)
-    ref64 = O.a6(vol, mp, G, ORIGIN, SIZE, w, off, mode=O.PINNED64)
+    assert info[:2] == ([0, info[1]] if scan == "perturbed" else [1, 9])
+    assert r1_paths((X, Y, Z), mp.shape[1], T, origin) == (2 if scan in ("xpairs", "odd_y")
+                                                           else 3)
+    got = gpu_a6(vol, mp, G, w, off, origin=origin)
+    ref64 = O.a6(vol, mp, G, origin, SIZE, w, off, mode=O.PINNED64)
     e64 = rel_l2(got, ref64)
     print(f"{scan}: A6 rel-L2 vs FP64 pinned {e64:.2e}")
     assert e64 < TOL

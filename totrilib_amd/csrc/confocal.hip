@@ -895,8 +895,9 @@ __device__ __forceinline__ void r1_sym_block(
 // (sigma = -1 on a mirrored axis), so the shared decision serves four scan
 // points whose gradients differ only in those signs and in the gathered
 // pair.  The wave then contracts its bins with G[s_m][t] and writes one
-// (gx, gy, gz) per scan point to part[pass][grp][3][N].
-template <int K, typename GT>
+// (gx, gy, gz) per scan point to part[pass][grp][3][N].  M = 2: the same walk
+// over x-pairs of scan points (as r1_sym_block<K, BELOW, 2>).
+template <int K, typename GT, int M = 4>
 __device__ __forceinline__ void a6_sym_block(
     const float* __restrict__ vol, const float* __restrict__ mp,
     const int* __restrict__ quads, int n_quads, const GT* __restrict__ G,
@@ -904,19 +905,20 @@ __device__ __forceinline__ void a6_sym_block(
     int n_groups, int reps_per_chunk, int chunks_per_group, float2* __restrict__ lds2) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
+  static_assert(M == 4 || M == 2, "quads or x-pairs");
   const int q = qgrp * kWaves + wave;
   const bool quad_ok = q < n_quads;
-  const int HX = g.X / 2, HY = g.Y / 2;
+  const int HX = g.X / 2, HY = M == 4 ? g.Y / 2 : g.Y;
   const int n_reps = HX * HY;
-  // A chunk holds reps_per_chunk column quads as [rep][z][m] pairs with
-  // z = 0 .. Z, where row z = Z is (0, 0): a sample's four mirror pairs sit
+  // A chunk holds reps_per_chunk column quads (pairs) as [rep][z][m] pairs
+  // with z = 0 .. Z, where row z = Z is (0, 0): a sample's M mirror pairs sit
   // side by side, and an inactive sample reads the zero row.
   const int zrows = g.Z + 1;
-  const int rep_pairs = 4 * zrows;
+  const int rep_pairs = M * zrows;
   const int chunk_vox = reps_per_chunk * rep_pairs;  // pairs
-  const int chunk_items = reps_per_chunk * zrows;    // (rep, z) rows of 4 pairs
+  const int chunk_items = reps_per_chunk * zrows;    // (rep, z) rows of M pairs
 
-  float tc2[K], itc4[K], gx[4][K], gy[4][K], gz[4][K];
+  float tc2[K], itc4[K], gx[M][K], gy[M][K], gz[M][K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int t = t_base + 64 * k + lane;
@@ -926,11 +928,11 @@ __device__ __forceinline__ void a6_sym_block(
     // finite, so that an inactive sample (d2 clamped to 0) has weight 0
     itc4[k] = qq > 0.f ? fminf(1.f / (qq * qq), 3.4028235e38f) : 0.f;
 #pragma unroll
-    for (int m = 0; m < 4; ++m) gx[m][k] = gy[m][k] = gz[m][k] = 0.f;
+    for (int m = 0; m < M; ++m) gx[m][k] = gy[m][k] = gz[m][k] = 0.f;
   }
-  int sidx[4];
+  int sidx[M];
 #pragma unroll
-  for (int m = 0; m < 4; ++m) sidx[m] = quad_ok ? quads[4 * q + m] : 0;
+  for (int m = 0; m < M; ++m) sidx[m] = quad_ok ? quads[M * q + m] : 0;
   const int s0 = uniform_i32(sidx[0]);
   const float wx = uniform_f32(mp[s0]);
   const float wy = uniform_f32(mp[g.N + s0]);
@@ -961,7 +963,7 @@ __device__ __forceinline__ void a6_sym_block(
     if (r < n_reps && z < g.Z) {
       const int rx = r / HY, ry = r - rx * HY;
 #pragma unroll
-      for (int m = 0; m < 4; ++m) {
+      for (int m = 0; m < M; ++m) {
         const int xi = (m & 1) ? g.X - 1 - rx : rx;
         const int yi = (m & 2) ? g.Y - 1 - ry : ry;
         const float* col = vol + ((size_t)xi * g.Y + yi) * g.Z;
@@ -974,18 +976,25 @@ __device__ __forceinline__ void a6_sym_block(
     *hi = make_float4(p[2].x, p[2].y, p[3].x, p[3].y);
   };
 
+  auto store_row = [&](float4* base, int it, float4 lo, float4 hi) {
+    if (M == 4) {
+      base[2 * it] = lo;
+      base[2 * it + 1] = hi;
+    } else {
+      base[it] = lo;
+    }
+  };
   if (chunk_first < chunk_end) {
     float4* dst4 = reinterpret_cast<float4*>(lds2);
     for (int it = threadIdx.x; it < chunk_items; it += nthreads) {
       float4 lo, hi;
       load_row(chunk_first, it, &lo, &hi);
-      dst4[2 * it] = lo;
-      dst4[2 * it + 1] = hi;
+      store_row(dst4, it, lo, hi);
     }
   }
   __syncthreads();
 
-  constexpr int kPref = kChunkVoxels / (4 * kWaves * 64);  // rows per thread
+  constexpr int kPref = kChunkVoxels / (M * kWaves * 64);  // rows per thread
   for (int c = chunk_first; c < chunk_end; ++c) {
     const int buf = (c - chunk_first) & 1;
     const float2* cur = lds2 + buf * chunk_vox;
@@ -1000,8 +1009,7 @@ __device__ __forceinline__ void a6_sym_block(
         if (it < chunk_items) {
           float4 lo, hi;
           load_row(c + 1, it, &lo, &hi);
-          nxt4[2 * it] = lo;
-          nxt4[2 * it + 1] = hi;
+          store_row(nxt4, it, lo, hi);
         }
       }
     }
@@ -1021,7 +1029,7 @@ __device__ __forceinline__ void a6_sym_block(
       unsigned packed = 0u;
       bool any = false;
 #pragma unroll
-      for (int f = 0; f < 4; ++f) {
+      for (int f = 0; f < M; ++f) {
         const float dx2 = (f & 1) ? B : A;
         const float dy2 = (f & 2) ? D : C;
         int tlo, thi;
@@ -1039,15 +1047,15 @@ __device__ __forceinline__ void a6_sym_block(
         mask &= mask - 1;
         const unsigned wm = (unsigned)__builtin_amdgcn_readlane((int)packed, j);
         const float cA = lane_value(A, j), cB = lane_value(B, j);
-        const float cC = lane_value(C, j), cD = lane_value(D, j);
+        const float cC = lane_value(C, j), cD = M == 4 ? lane_value(D, j) : 0.f;
         const float cxa = lane_value(dxa, j), cxb = lane_value(dxb, j);
-        const float cya = lane_value(dya, j), cyb = lane_value(dyb, j);
+        const float cya = lane_value(dya, j), cyb = M == 4 ? lane_value(dyb, j) : 0.f;
         const float2* quad = cur + j * rep_pairs;
         // Families 2h and 2h+1 share dy^2; both are evaluated for a window
         // where either is active (two independent decisions per pass; a
         // sample outside its family's range fails the activity test).
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
+        for (int h = 0; h < M / 2; ++h) {
           const int fa = 2 * h, fb = 2 * h + 1;
           const unsigned wmask = ((wm >> (8 * fa)) | (wm >> (8 * fb))) & 0xffu;
           const float dy2 = h ? cD : cC;
@@ -1071,8 +1079,8 @@ __device__ __forceinline__ void a6_sym_block(
             }
             const bool acta = (d2a > 0.f) & ((unsigned)za <= (unsigned)(g.Z - 2));
             const bool actb = (d2b > 0.f) & ((unsigned)zb <= (unsigned)(g.Z - 2));
-            const float2* pa = quad + 4 * (acta ? za : g.Z);
-            const float2* pb = quad + 4 * (actb ? zb : g.Z);
+            const float2* pa = quad + M * (acta ? za : g.Z);
+            const float2* pb = quad + M * (actb ? zb : g.Z);
             // Shared per decision, with u = d2 / tc^8: f2 = (depth / tc^2)^4
             // = u d2, the first term's factor 4 u, and f2 / depth (an
             // inactive sample has d2 = 0 and a finite 1 / max(depth, tiny)).
@@ -1083,7 +1091,7 @@ __device__ __forceinline__ void a6_sym_block(
             const float Ba = f2a * __builtin_amdgcn_rcpf(fmaxf(sqrt_fast(d2ca), 1e-30f));
             const float Bb = f2b * __builtin_amdgcn_rcpf(fmaxf(sqrt_fast(d2cb), 1e-30f));
 #pragma unroll
-            for (int m = 0; m < 4; ++m) {
+            for (int m = 0; m < M; ++m) {
               // d/dw of scan s_m (a6_kernel): -dx (4 f1 u + s dv f2 / depth),
               // likewise y, and s dv f2 for z; sample offsets carry m's signs
               const float sx = (m & 1) ? 1.f : -1.f, sy = (m & 2) ? 1.f : -1.f;
@@ -1107,7 +1115,7 @@ __device__ __forceinline__ void a6_sym_block(
   if (quad_ok) {
     float* o = part + ((size_t)pass * n_groups + grp) * 3 * g.N;
 #pragma unroll
-    for (int m = 0; m < 4; ++m) {
+    for (int m = 0; m < M; ++m) {
       float sxs = 0.f, sys = 0.f, szs = 0.f;
 #pragma unroll
       for (int k = 0; k < K; ++k) {
@@ -1198,6 +1206,25 @@ void a6_sym_kernel(
   a6_sym_block<K, GT>(vol, mp, quads, n_quads, G, part, g, pass * 64 * K, pass,
                       (int)(blockIdx.x % n_groups), qgrp, n_groups, reps_per_chunk,
                       chunks_per_group, lds2);
+}
+
+// x-pair A6, all bin passes in one launch, for the scans r1_pair_kernel takes.
+template <int K, typename GT>
+__global__ __launch_bounds__(kWaves * 64) __attribute__((amdgpu_waves_per_eu(4)))
+void a6_pair_kernel(
+    const float* __restrict__ vol, const float* __restrict__ mp,
+    const int* __restrict__ pairs, const int* __restrict__ info, const GT* __restrict__ G,
+    float* __restrict__ part, ConfGeom g, int n_groups, int reps_per_chunk,
+    int chunks_per_group, int quad_on) {
+  if (!info[2] || (quad_on && info[0])) return;
+  extern __shared__ float2 lds2[];
+  const int n_pairs = info[3];
+  const int qgrp = (int)(blockIdx.x / n_groups);
+  if (qgrp * kWaves >= n_pairs) return;  // whole workgroup idle
+  const int pass = (int)(gridDim.y - 1 - blockIdx.y);
+  a6_sym_block<K, GT, 2>(vol, mp, pairs, n_pairs, G, part, g, pass * 64 * K, pass,
+                         (int)(blockIdx.x % n_groups), qgrp, n_groups, reps_per_chunk,
+                         chunks_per_group, lds2);
 }
 
 // Sum of partials in a fixed order where the number of parts depends on
@@ -3554,10 +3581,35 @@ A6SymPlan plan_a6_sym(int64_t X, int64_t Y, int64_t Z, int64_t N, int64_t T,
   return p;
 }
 
+// Plan of the x-pair A6: the x-pair R1's grid with K <= 4 windows.
+A6SymPlan plan_a6_pair(int64_t X, int64_t Y, int64_t Z, int64_t N, int64_t T,
+                       const A6Plan& gen) {
+  A6SymPlan p;
+  memset(&p, 0, sizeof(p));
+  const R1SymPlan r = plan_r1_pair(X, Y, Z, N, T, gen.r);
+  p.possible = r.possible;
+  if (!p.possible) return p;
+  p.K = r.K < 4 ? r.K : 4;
+  p.passes = (int)((T + 64 * p.K - 1) / (64 * p.K));
+  p.n_groups = r.n_groups;
+  p.reps_per_chunk = r.reps_per_chunk;
+  p.chunks_per_group = r.chunks_per_group;
+  p.ml = mirror_layout(N, 0);
+  p.part_off = (p.ml.end + 255) & ~(size_t)255;
+  const size_t parts_s = (size_t)p.passes * p.n_groups;
+  const size_t parts_g = (size_t)gen.passes * gen.r.n_groups;
+  p.ws = p.part_off + (parts_s > parts_g ? parts_s : parts_g) * 3 * N * sizeof(float);
+  return p;
+}
+
 size_t a6_workspace(int64_t X, int64_t Y, int64_t Z, int64_t N, int64_t T) {
   const A6Plan p = plan_a6(X, Y, Z, N, T);
   const A6SymPlan q = plan_a6_sym(X, Y, Z, N, T, p);
-  return q.possible && q.ws > p.ws ? q.ws : p.ws;
+  const A6SymPlan pr = plan_a6_pair(X, Y, Z, N, T, p);
+  size_t ws = p.ws;
+  if (q.possible && q.ws > ws) ws = q.ws;
+  if (pr.possible && pr.ws > ws) ws = pr.ws;
+  return ws;
 }
 
 template <typename GT>
@@ -3606,6 +3658,34 @@ int run_a6_sym(const A6SymPlan& q, const float* vol, const float* mp, const int*
   return check_launch("a6_sym_kernel");
 }
 
+template <typename GT>
+int run_a6_pair(const A6SymPlan& q, const float* vol, const float* mp, const int* pairs,
+                const int* info, const void* Gv, float* part, const ConfGeom& g, int quad_on,
+                hipStream_t st) {
+  const GT* G = (const GT*)Gv;
+  const int pair_blocks = (g.N / 2 + kWaves - 1) / kWaves;
+  const dim3 grid(pair_blocks * q.n_groups, q.passes);
+  const size_t lds = 2 * (size_t)q.reps_per_chunk * 2 * (g.Z + 1) * sizeof(float2);
+  switch (q.K) {
+    case 1:
+      hipLaunchKernelGGL((a6_pair_kernel<1, GT>), grid, dim3(kWaves * 64), lds, st, vol, mp,
+                         pairs, info, G, part, g, q.n_groups, q.reps_per_chunk,
+                         q.chunks_per_group, quad_on);
+      break;
+    case 2:
+      hipLaunchKernelGGL((a6_pair_kernel<2, GT>), grid, dim3(kWaves * 64), lds, st, vol, mp,
+                         pairs, info, G, part, g, q.n_groups, q.reps_per_chunk,
+                         q.chunks_per_group, quad_on);
+      break;
+    default:
+      hipLaunchKernelGGL((a6_pair_kernel<4, GT>), grid, dim3(kWaves * 64), lds, st, vol, mp,
+                         pairs, info, G, part, g, q.n_groups, q.reps_per_chunk,
+                         q.chunks_per_group, quad_on);
+      break;
+  }
+  return check_launch("a6_pair_kernel");
+}
+
 }  // namespace
 }  // namespace totri_hip
 
@@ -3634,10 +3714,14 @@ extern "C" int totri_volume_confocal_grad_measurement_points(
                   kChunkVoxels);
   const A6Plan p = plan_a6(X, Y, Z, N, T);
   const A6SymPlan q = plan_a6_sym(X, Y, Z, N, T, p);
-  // The mirror path needs the voxel grid antisymmetric in x and y; the scan
-  // quads are then looked for on the device.
-  const bool sym = q.possible && axis_antisymmetric(volume_origin[0], volume_size[0], X) &&
+  const A6SymPlan pr = plan_a6_pair(X, Y, Z, N, T, p);
+  // As for R1: the xy-mirror path needs the voxel grid antisymmetric in x
+  // and y, the x-pair path in x; the scan quads and x-pairs are then looked
+  // for on the device.
+  const bool xanti = axis_antisymmetric(volume_origin[0], volume_size[0], X);
+  const bool sym = q.possible && xanti &&
                    axis_antisymmetric(volume_origin[1], volume_size[1], Y);
+  const bool xsym = pr.possible && xanti;
   const size_t need = a6_workspace(X, Y, Z, N, T);
   if (workspace_bytes < need)
     return fail(TOTRI_EWORKSPACE, "workspace too small: need %zu bytes, got %zu", need,
@@ -3645,12 +3729,24 @@ extern "C" int totri_volume_confocal_grad_measurement_points(
   TOTRI_CHECK_ARG(workspace != nullptr, "null workspace");
   const ConfGeom g = make_geom(X, Y, Z, N, T, volume_origin, volume_size, bin_width,
                                bin_offset);
-  float* part = (float*)((char*)workspace + (sym ? q.part_off : 0));
+  float* part = (float*)((char*)workspace + (xsym ? pr.part_off : 0));
   const int* info = nullptr;
-  if (sym) {
-    rc = launch_mirror_detect(q.ml, measurement_points, (int)N, workspace, st);
+  const int* skip = nullptr;
+  if (xsym) {  // (sym implies xsym; both use the same detection layout)
+    rc = launch_mirror_detect(pr.ml, measurement_points, (int)N, workspace, st);
     if (rc) return rc;
-    info = (const int*)((char*)workspace + q.ml.info_off);
+    info = (const int*)((char*)workspace + pr.ml.info_off);
+    skip = info + 2;  // a mirror kernel takes every scan whose points all have x-mirrors
+    const int* pairs = (const int*)((char*)workspace + pr.ml.pairs_off);
+    const int qo = sym ? 1 : 0;
+    switch (transient_dtype) {
+      case TOTRI_F32: rc = run_a6_pair<float>(pr, volume, measurement_points, pairs, info, transient_grad, part, g, qo, st); break;
+      case TOTRI_BF16: rc = run_a6_pair<__hip_bfloat16>(pr, volume, measurement_points, pairs, info, transient_grad, part, g, qo, st); break;
+      default: rc = run_a6_pair<__half>(pr, volume, measurement_points, pairs, info, transient_grad, part, g, qo, st); break;
+    }
+    if (rc) return rc;
+  }
+  if (sym) {
     const int* quads = (const int*)((char*)workspace + q.ml.quads_off);
     switch (transient_dtype) {
       case TOTRI_F32: rc = run_a6_sym<float>(q, volume, measurement_points, quads, info, transient_grad, part, g, st); break;
@@ -3660,17 +3756,17 @@ extern "C" int totri_volume_confocal_grad_measurement_points(
     if (rc) return rc;
   }
   switch (transient_dtype) {
-    case TOTRI_F32: rc = run_a6<float>(p, volume, measurement_points, transient_grad, part, g, st, info); break;
-    case TOTRI_BF16: rc = run_a6<__hip_bfloat16>(p, volume, measurement_points, transient_grad, part, g, st, info); break;
-    default: rc = run_a6<__half>(p, volume, measurement_points, transient_grad, part, g, st, info); break;
+    case TOTRI_F32: rc = run_a6<float>(p, volume, measurement_points, transient_grad, part, g, st, skip); break;
+    case TOTRI_BF16: rc = run_a6<__hip_bfloat16>(p, volume, measurement_points, transient_grad, part, g, st, skip); break;
+    default: rc = run_a6<__half>(p, volume, measurement_points, transient_grad, part, g, st, skip); break;
   }
   if (rc) return rc;
   const size_t n = (size_t)3 * N;
-  if (sym) {
-    // info[0] says which path ran
+  if (xsym) {
+    // info[0] and info[2] say which path ran
     hipLaunchKernelGGL(sum_partials_flag_kernel<float>, dim3(launch_grid(n)), dim3(256), 0, st,
-                       part, info, q.passes * q.n_groups, 0, p.passes * p.r.n_groups, n,
-                       measurement_points_grad);
+                       part, info, sym ? q.passes * q.n_groups : 0, pr.passes * pr.n_groups,
+                       p.passes * p.r.n_groups, n, measurement_points_grad);
     return check_launch("sum_partials_flag_kernel");
   }
   hipLaunchKernelGGL(sum_partials_kernel<float>, dim3(launch_grid(n)), dim3(256), 0, st, part,
