@@ -330,11 +330,12 @@ def test_long_transients_beyond_the_lds_stage():
 
 @pytest.mark.parametrize("case", ["binned", "overflow"])
 def test_truncated_splat_brick_lists(case, report):
-    """Truncated splats list each brick's RBFs first (count, scan, fill, then
-    an index-order sort per brick).  "binned": small supports, every brick
-    takes its list.  "overflow": wide supports put more than 2048 RBFs into
-    most bricks, which then scan all RBFs; a few narrow RBFs keep some bricks
-    on their lists.  Both against the FP32 oracle, zeros included."""
+    """Truncated splats list each brick's RBFs first (one pass into the
+    brick's list slots, then an index-order sort per brick).  "binned": small
+    supports, every brick takes its list.  "overflow": wide supports put more
+    RBFs into most bricks than the bricks have slots (512 here), and these
+    bricks scan all RBFs; a few narrow RBFs keep some bricks on their lists.
+    Both against the FP32 oracle, zeros included."""
     from totrilib_amd.types import VolumeDef
     from totrilib_amd.util import Rbf2VolumeGaussian
     rng = np.random.default_rng(62 if case == "binned" else 63)

@@ -70,16 +70,16 @@ __device__ __forceinline__ bool within(float r2, float sg, float trunc) {
 
 // Binned lists (truncated splats): per (batch, brick) the RBFs whose support
 // may reach the brick, found with the brick-cull test of the full scan, so a
-// brick sums the same RBFs in the same (index) order either way.
+// brick sums the same RBFs in the same (index) order either way.  Each brick
+// has `cap` list slots; a brick reached by more RBFs scans all of them.
 struct SplatBins {
-  int* count;     // [B][bricks]
-  int* offset;    // [B][bricks] exclusive scan of count
-  int* cursor;    // [B][bricks] fill positions
-  int* list;      // [capacity] RBF indices
-  long long capacity;
+  int* count;  // [B][bricks] RBFs that may reach the brick (the fill cursor)
+  int* list;   // [B][bricks][cap] the first cap of their indices, in fill order
+  int cap;
 };
 
-constexpr int kListMax = 2048;  // longest list a brick sorts in LDS; longer lists scan
+constexpr int kSlotMax = 512;  // list slots per brick, at most (LDS sort buffers)
+constexpr int kSlotMin = 64;
 
 // Conservative brick cull (the full scan's test): the squared distance from
 // the RBF centre to the brick's sample box against (trunc sigma)^2.
@@ -93,13 +93,11 @@ __device__ __forceinline__ bool brick_may_hold(float xr, float yr, float zr, flo
   return !((dx * dx + dy * dy + dz * dz) > lim * lim * 1.0001f + 1e-12f);
 }
 
-// The sample box of brick (bx, by, bz): the voxels it really holds.
-__device__ __forceinline__ void brick_box(const SplatGeom& g, int bx, int by, int bz,
-                                          float* lox, float* hix, float* loy, float* hiy,
-                                          float* loz, float* hiz) {
-  const int x0 = bx * kBX, x1 = min(bx * kBX + kBX, g.W) - 1;
-  const int y0 = by * kBY, y1 = min(by * kBY + kBY, g.H) - 1;
-  const int z0 = bz * kBZ, z1 = min(bz * kBZ + kBZ, g.D) - 1;
+// The sample box of voxels [x0, x1] x [y0, y1] x [z0, z1] (inclusive, inside
+// the grid).
+__device__ __forceinline__ void voxel_box(const SplatGeom& g, int x0, int x1, int y0, int y1,
+                                          int z0, int z1, float* lox, float* hix, float* loy,
+                                          float* hiy, float* loz, float* hiz) {
   const float bxa = splat_coord(x0, g.W, g.sx, g.ex), bxb = splat_coord(x1, g.W, g.sx, g.ex);
   const float bya = splat_coord(y0, g.H, g.sy, g.ey), byb = splat_coord(y1, g.H, g.sy, g.ey);
   const float bza = splat_coord(z0, g.D, g.sz, g.ez), bzb = splat_coord(z1, g.D, g.sz, g.ez);
@@ -109,6 +107,16 @@ __device__ __forceinline__ void brick_box(const SplatGeom& g, int bx, int by, in
   *hiy = fmaxf(bya, byb);
   *loz = fminf(bza, bzb);
   *hiz = fmaxf(bza, bzb);
+}
+
+// The sample box of brick (bx, by, bz): the voxels it really holds.
+__device__ __forceinline__ void brick_box(const SplatGeom& g, int bx, int by, int bz,
+                                          float* lox, float* hix, float* loy, float* hiy,
+                                          float* loz, float* hiz) {
+  const int x0 = bx * kBX, x1 = min(bx * kBX + kBX, g.W) - 1;
+  const int y0 = by * kBY, y1 = min(by * kBY + kBY, g.H) - 1;
+  const int z0 = bz * kBZ, z1 = min(bz * kBZ + kBZ, g.D) - 1;
+  voxel_box(g, x0, x1, y0, y1, z0, z1, lox, hix, loy, hiy, loz, hiz);
 }
 
 // Brick index range [lo, hi] along an axis whose bricks hold `bs` samples
@@ -139,12 +147,11 @@ __device__ __forceinline__ void brick_range(float c, float r, int n, int bs, flo
   }
 }
 
-// Pass 1 (FILL = false) counts, per brick, the RBFs that may reach it; pass 2
-// (FILL = true) writes their indices at offset + a fill position.  One wave
+// One pass: per brick, the RBFs that may reach it take the next slot of the
+// brick's list (the count past cap makes the brick scan instead).  One wave
 // per RBF, its lanes over the bricks of the RBF's support box.
 constexpr int kBinWaves = 4;
 
-template <bool FILL>
 __global__ __launch_bounds__(kBinWaves * 64) void splat_bin_kernel(
     const float* __restrict__ rbf, SplatGeom g, int bricks_x, int bricks_y, int bricks_z,
     SplatBins bins) {
@@ -172,37 +179,8 @@ __global__ __launch_bounds__(kBinWaves * 64) void splat_bin_kernel(
     brick_box(g, bx, by, bz, &lox, &hix, &loy, &hiy, &loz, &hiz);
     if (!brick_may_hold(xr, yr, zr, sg, g.trunc, lox, hix, loy, hiy, loz, hiz)) continue;
     const size_t k = b * nb + ((size_t)bz * bricks_y + by) * bricks_x + bx;
-    if (!FILL) {
-      atomicAdd(bins.count + k, 1);
-    } else {
-      const long long at = (long long)bins.offset[k] + atomicAdd(bins.cursor + k, 1);
-      if (at < bins.capacity) bins.list[at] = i;
-    }
-  }
-}
-
-// Exclusive scan of count[0 .. n) into offset (one workgroup, in order), and
-// cursor = 0.  Offsets past the list's capacity make their bricks scan.
-__global__ __launch_bounds__(1024) void splat_scan_kernel(SplatBins bins, int n) {
-  __shared__ long long part[1024];
-  const int tid = threadIdx.x;
-  const int per = (n + 1023) / 1024;
-  const int a = min(tid * per, n), e = min(a + per, n);
-  long long sum = 0;
-  for (int k = a; k < e; ++k) sum += bins.count[k];
-  part[tid] = sum;
-  __syncthreads();
-  for (int d = 1; d < 1024; d <<= 1) {
-    const long long v = tid >= d ? part[tid - d] : 0;
-    __syncthreads();
-    part[tid] += v;
-    __syncthreads();
-  }
-  long long run = tid > 0 ? part[tid - 1] : 0;
-  for (int k = a; k < e; ++k) {
-    bins.offset[k] = (int)(run < 0x7fffffff ? run : 0x7fffffff);
-    bins.cursor[k] = 0;
-    run += bins.count[k];
+    const int at = atomicAdd(bins.count + k, 1);
+    if (at < bins.cap) bins.list[k * bins.cap + at] = i;
   }
 }
 
@@ -211,7 +189,8 @@ __global__ __launch_bounds__(256) void splat_forward_kernel(
     float* __restrict__ vol,        // [B][1+A][D][H][W]
     SplatGeom g, int bricks_x, int bricks_y, SplatBins bins, int binned) {
   __shared__ float prm[kChunk * (4 + kMaxAttr)];
-  __shared__ int keep[kListMax];
+  __shared__ int keep[kSlotMax > kChunk ? kSlotMax : kChunk];
+  __shared__ int fill[kSlotMax];
   __shared__ int wave_cnt[4];
   const int C = 4 + g.A;
   const int b = blockIdx.y;
@@ -222,9 +201,13 @@ __global__ __launch_bounds__(256) void splat_forward_kernel(
   const int bz = brick / bricks_y;
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
-  const int xi = bx * kBX + (tid % kBX);
-  const int yi = by * kBY + ((tid / kBX) % kBY);
-  const int zi = bz * kBZ + tid / (kBX * kBY);
+  // Each wave holds a 4x4x4 quarter of the brick (quarter `wave` of its 8x8
+  // footprint, all 4 z), so that it can skip the RBFs that miss its quarter.
+  static_assert(kBX == 8 && kBY == 8 && kBZ == 4, "four 4x4x4 quarters per brick");
+  const int qx0 = bx * kBX + (wave & 1) * 4, qy0 = by * kBY + (wave >> 1) * 4;
+  const int xi = qx0 + (lane & 3);
+  const int yi = qy0 + ((lane >> 2) & 3);
+  const int zi = bz * kBZ + (lane >> 4);
   const bool ok = xi < g.W && yi < g.H && zi < g.D;
   const float x = splat_coord(xi, g.W, g.sx, g.ex);
   const float y = splat_coord(yi, g.H, g.sy, g.ey);
@@ -240,39 +223,36 @@ __global__ __launch_bounds__(256) void splat_forward_kernel(
   const float* rb = rbf + (size_t)b * g.N * C;
 
   // Binned path: this brick's list, sorted into index order in LDS, then
-  // evaluated in chunks.  Lists past the capacity or kListMax scan instead.
+  // evaluated in chunks.  A brick whose count passed its slots scans instead.
   int n_list = -1;
   if (binned) {
     const size_t k = (size_t)b * gridDim.x + blockIdx.x;
     const int cnt = bins.count[k];
-    const long long off = bins.offset[k];
-    if (cnt <= kListMax && off + cnt <= bins.capacity) n_list = cnt;
+    if (cnt <= bins.cap) n_list = cnt;
     if (n_list >= 0) {
-      int np = 1;
-      while (np < n_list) np <<= 1;
+      const int* src = bins.list + k * bins.cap;
+      for (int e = tid; e < n_list; e += blockDim.x) fill[e] = src[e];
       __syncthreads();
-      for (int e = tid; e < np; e += blockDim.x)
-        keep[e] = e < n_list ? bins.list[off + e] : 0x7fffffff;
-      __syncthreads();
-      // bitonic sort of keep[0 .. np) (ascending)
-      for (int len = 2; len <= np; len <<= 1)
-        for (int st = len >> 1; st > 0; st >>= 1) {
-          for (int e = tid; e < np; e += blockDim.x) {
-            const int o = e ^ st;
-            if (o > e) {
-              const int va = keep[e], vb = keep[o];
-              const bool up = (e & len) == 0;
-              if ((va > vb) == up) {
-                keep[e] = vb;
-                keep[o] = va;
-              }
-            }
-          }
-          __syncthreads();
-        }
+      // Rank sort (the indices are distinct): an entry's place is the number
+      // of smaller entries; every lane reads fill[j] in step (broadcast).
+      for (int e = tid; e < n_list; e += blockDim.x) {
+        const int v = fill[e];
+        int rank = 0;
+        for (int j = 0; j < n_list; ++j) rank += fill[j] < v ? 1 : 0;
+        keep[rank] = v;
+      }
     }
   }
   if (n_list >= 0) {
+    // The wave's quarter box (empty past the grid's edge): an RBF that fails
+    // the brick test on it reaches none of the wave's voxels, so skipping it
+    // leaves every voxel's sum, taken in index order, unchanged.
+    const int qx1 = min(qx0 + 3, g.W - 1), qy1 = min(qy0 + 3, g.H - 1);
+    const int qz0 = bz * kBZ, qz1 = min(qz0 + kBZ, g.D) - 1;
+    const bool qnonempty = qx0 <= qx1 && qy0 <= qy1 && qz0 <= qz1;
+    float qlox = 0.f, qhix = 0.f, qloy = 0.f, qhiy = 0.f, qloz = 0.f, qhiz = 0.f;
+    if (qnonempty)
+      voxel_box(g, qx0, qx1, qy0, qy1, qz0, qz1, &qlox, &qhix, &qloy, &qhiy, &qloz, &qhiz);
     for (int c0 = 0; c0 < n_list; c0 += kChunk) {
       const int nc = min(kChunk, n_list - c0);
       __syncthreads();
@@ -281,15 +261,27 @@ __global__ __launch_bounds__(256) void splat_forward_kernel(
         prm[e] = rb[(size_t)keep[c0 + r] * C + (e - r * C)];
       }
       __syncthreads();
-      for (int k = 0; k < nc; ++k) {
-        const float* p = prm + k * C;
-        const float r2 = rbf_r2(x, y, z, p[0], p[1], p[2]);
-        if (!within(r2, p[3], g.trunc)) continue;
-        const float wgt = gaussian(r2, p[3]);                // :77
+      for (int k0 = 0; k0 < nc; k0 += 64) {
+        const int kk = k0 + lane;
+        bool pass = false;
+        if (qnonempty && kk < nc) {
+          const float* p = prm + kk * C;
+          pass = brick_may_hold(p[0], p[1], p[2], p[3], g.trunc, qlox, qhix, qloy, qhiy, qloz,
+                                qhiz);
+        }
+        unsigned long long m = __ballot(pass);
+        while (m) {  // the wave's RBFs of this group, in index order
+          const int k = k0 + __builtin_ctzll(m);
+          m &= m - 1;
+          const float* p = prm + k * C;
+          const float r2 = rbf_r2(x, y, z, p[0], p[1], p[2]);
+          if (!within(r2, p[3], g.trunc)) continue;
+          const float wgt = gaussian(r2, p[3]);              // :77
 #pragma unroll
-        for (int a = 0; a < kMaxAttr; ++a)
-          if (a < g.A) attr[a] += wgt * p[4 + a];            // :78-80
-        sum += wgt;                                          // :81
+          for (int a = 0; a < kMaxAttr; ++a)
+            if (a < g.A) attr[a] += wgt * p[4 + a];          // :78-80
+          sum += wgt;                                        // :81
+        }
       }
     }
   }
@@ -630,30 +622,31 @@ namespace totri_hip {
 namespace {
 
 struct BinLayout {
-  size_t nb;                    // bricks per batch entry
-  size_t count, offset, cursor; // byte offsets of the [B][nb] int arrays
-  size_t list;                  // byte offset of the list
-  long long capacity;           // list entries
+  size_t nb;     // bricks per batch entry
+  size_t count;  // byte offset of the [B][nb] counts
+  size_t list;   // byte offset of the [B][nb][cap] lists
+  int cap;       // list slots per brick
   size_t end;
 };
 
-// Workspace of the binned splat (truncated splats only): three [B][bricks]
-// int arrays and a list of 64 entries per RBF (about 20 at the usual
-// sigma / voxel ratios); bricks whose lists do not fit scan all RBFs.
-BinLayout bin_layout(int64_t B, int64_t N, int64_t D, int64_t H, int64_t W) {
+// Workspace of the binned splat (truncated splats only): a count per brick
+// and `cap` list slots per brick, kSlotMax at most and fewer for large grids
+// (at most 2^24 slots in all, at least kSlotMin per brick).  About 20-60
+// RBFs reach a brick at the usual sigma / voxel ratios; a brick reached by
+// more than cap scans all RBFs.
+BinLayout bin_layout(int64_t B, int64_t D, int64_t H, int64_t W) {
   BinLayout l;
   const size_t bx = (size_t)((W + kBX - 1) / kBX), by = (size_t)((H + kBY - 1) / kBY),
                bz = (size_t)((D + kBZ - 1) / kBZ);
   l.nb = bx * by * bz;
-  const size_t arr = (((size_t)B * l.nb * sizeof(int)) + 255) & ~(size_t)255;
+  const size_t bricks = (size_t)B * l.nb;
+  size_t cap = ((size_t)1 << 24) / (bricks > 0 ? bricks : 1);
+  if (cap > (size_t)kSlotMax) cap = kSlotMax;
+  if (cap < (size_t)kSlotMin) cap = kSlotMin;
+  l.cap = (int)cap;
   l.count = 0;
-  l.offset = arr;
-  l.cursor = 2 * arr;
-  l.list = 3 * arr;
-  // offsets are ints: at most 2^31 - 1 entries
-  const long long want = 64ll * B * N;
-  l.capacity = want < 0x7fffffffll ? want : 0x7fffffffll;
-  l.end = l.list + (((size_t)l.capacity * sizeof(int) + 255) & ~(size_t)255);
+  l.list = (bricks * sizeof(int) + 255) & ~(size_t)255;
+  l.end = l.list + ((bricks * cap * sizeof(int) + 255) & ~(size_t)255);
   return l;
 }
 
@@ -665,7 +658,7 @@ extern "C" size_t totri_rbf2volume_gaussian_forward_workspace(int64_t B, int64_t
                                                               float truncation) {
   (void)A;
   if (!(truncation > 0.f) || B <= 0 || N <= 0 || D <= 0 || H <= 0 || W <= 0) return 0;
-  return bin_layout(B, N, D, H, W).end;
+  return bin_layout(B, D, H, W).end;
 }
 
 extern "C" int totri_rbf2volume_gaussian_forward(const float* rbf, int64_t B, int64_t N,
@@ -693,13 +686,11 @@ extern "C" int totri_rbf2volume_gaussian_forward(const float* rbf, int64_t B, in
       return fail(TOTRI_EWORKSPACE, "workspace too small: need %zu bytes, got %zu", need,
                   workspace_bytes);
     TOTRI_CHECK_ARG(workspace != nullptr, "null workspace");
-    const BinLayout l = bin_layout(B, N, D, H, W);
+    const BinLayout l = bin_layout(B, D, H, W);
     char* w = (char*)workspace;
     bins.count = (int*)(w + l.count);
-    bins.offset = (int*)(w + l.offset);
-    bins.cursor = (int*)(w + l.cursor);
     bins.list = (int*)(w + l.list);
-    bins.capacity = l.capacity;
+    bins.cap = l.cap;
     const size_t n = (size_t)B * l.nb;
     TOTRI_CHECK_ARG(n < (1u << 31), "too many bricks");
     hipLaunchKernelGGL(fill_zero_i32_kernel, dim3((unsigned)((n + 255) / 256 < 65536 ? (n + 255) / 256 : 65536)),
@@ -707,15 +698,8 @@ extern "C" int totri_rbf2volume_gaussian_forward(const float* rbf, int64_t B, in
     rc = check_launch("fill_zero_i32_kernel");
     if (rc) return rc;
     const dim3 grid((unsigned)((N + kBinWaves - 1) / kBinWaves), (unsigned)B);
-    hipLaunchKernelGGL(splat_bin_kernel<false>, grid, dim3(kBinWaves * 64), 0, st, rbf, g, bx,
-                       by, bz, bins);
-    rc = check_launch("splat_bin_kernel");
-    if (rc) return rc;
-    hipLaunchKernelGGL(splat_scan_kernel, dim3(1), dim3(1024), 0, st, bins, (int)n);
-    rc = check_launch("splat_scan_kernel");
-    if (rc) return rc;
-    hipLaunchKernelGGL(splat_bin_kernel<true>, grid, dim3(kBinWaves * 64), 0, st, rbf, g, bx,
-                       by, bz, bins);
+    hipLaunchKernelGGL(splat_bin_kernel, grid, dim3(kBinWaves * 64), 0, st, rbf, g, bx, by, bz,
+                       bins);
     rc = check_launch("splat_bin_kernel");
     if (rc) return rc;
   }
