@@ -654,7 +654,7 @@ def kernels_main(args):
     sig = rbf[0, :, 3].double()
     in_support = float(torch.sum(4.0 / 3.0 * np.pi * (3 * sig) ** 3) / dv ** 3)
     ms = timeit(lambda: Rbf2VolumeGaussian.apply(rbf, vd, 3.0))
-    row("S1 RBF splat", "splat_bin_kernel, splat_scan_kernel, splat_forward_kernel",
+    row("S1 RBF splat", "splat_bin_kernel, splat_forward_kernel",
         "20,000 RBFs -> 128^3, 3 sigma", ms, in_support,
         "voxel-RBF pairs within 3 sigma", 14, 4 * (2 * RR ** 3 + 5 * M))
     r = rbf.clone().requires_grad_(True)

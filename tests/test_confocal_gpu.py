@@ -531,9 +531,12 @@ def test_a1_mirror_pairs_and_general_path(scan):
       monotonicity condition, the x-mirror window walk ("odd_y_negoff").
       "quads_negoff" takes the window walk too.
     - A broken pair or a random scan takes the general kernel.
+    The grid is 32 x 32 (31 for an odd Y): cell counts over [-1, 1] that are
+    powers of two give centres that are antisymmetric in float, so the
+    mirror walks apply (the walks the planner allows are asserted).
     """
     rng = np.random.default_rng(33)
-    X, Y, Z, T = 30, 26, 44, 600
+    X, Y, Z, T = 32, 32, 44, 600
     mp = wall_grid(8)
     w, off = 0.0045, 0.0
     if scan == "quads_safe":
@@ -546,7 +549,7 @@ def test_a1_mirror_pairs_and_general_path(scan):
         j = int(np.flatnonzero((mp[0] == -mp[0, k]) & (mp[1] == mp[1, k]))[0])
         mp[1, [k, j]] = np.nextafter(mp[1, k], np.float32(2))
     elif scan in ("odd_y", "odd_y_safe", "odd_y_negoff"):
-        Y = 25
+        Y = 31
         off = {"odd_y": 0.0, "odd_y_safe": 0.0005, "odd_y_negoff": -0.05}[scan]
     elif scan == "perturbed":
         mp[1, 9] = np.nextafter(mp[1, 9], np.float32(2))   # breaks one x-pair
@@ -561,10 +564,16 @@ def test_a1_mirror_pairs_and_general_path(scan):
               "odd_y_safe": [1, 16, 1, 32], "odd_y_negoff": [1, 16, 1, 32],
               "xpairs": [0, info[1], 1, 32]}.get(scan, [0, info[1], 0, info[3]])
     assert info == expect
+    from totrilib_amd import _lib
+    walks = _lib.load().totri_debug_a1_walks(X, Y, Z, mp.shape[1], T, _lib.f3(ORIGIN),
+                                             _lib.f3(SIZE), w, off)
+    # bits: 1 x-mirror paths, 2 xy-mirror row walk, 4 general row walk, 8 x-pair row walk
+    assert walks == {"quads_negoff": 1, "odd_y": 13, "odd_y_safe": 13,
+                     "odd_y_negoff": 1}.get(scan, 15), walks
     got = gpu_a1(G, mp, (X, Y, Z), w, off)
     ref = O.a1_transpose(G, mp, X, Y, Z, ORIGIN, SIZE, w, off, mode=O.PINNED64)
     e = rel_l2(got, ref)
-    print(f"{scan}: mirror info {info}, rel-L2 vs FP64 pinned {e:.2e}")
+    print(f"{scan}: mirror info {info}, walks {walks}, rel-L2 vs FP64 pinned {e:.2e}")
     assert e < TOL
     assert np.count_nonzero(got) > 0
 
