@@ -23,13 +23,18 @@ def make_requests(seed=0):
     on the device."""
     from totrilib_amd.serve import BackprojectRequest, RenderRequest
     rng = np.random.default_rng(seed)
+    # Mirror kernels need a grid whose cell centres are antisymmetric in
+    # float: over [-1, 1] that holds for power-of-two sizes (16, 32, 64), not
+    # for 20, 24 or 40, which take the general kernels.
     specs = [  # kind, scan side, (X, Y, Z), T, one-way bin width, offset, where
-        ("r", 8, (24, 20, 40), 300, 0.009, 0.0, "pinned"),
+        ("r", 8, (16, 16, 40), 300, 0.009, 0.0, "pinned"),
         ("r", 16, (32, 32, 32), 1100, 0.0025, 0.0, "device"),
-        ("b", 8, (24, 20, 40), 300, 0.009, 0.0, "pageable"),
+        ("b", 8, (16, 16, 40), 300, 0.009, 0.0, "pageable"),
         ("b", 32, (64, 64, 64), 512, 0.005, 0.0, "pinned"),
         ("r", 32, (64, 64, 64), 512, 0.005, 0.0, "pageable"),
+        ("b", 12, (16, 16, 20), 300, 0.009, 0.0002, "device"),
         ("b", 12, (16, 24, 20), 300, 0.009, 0.0002, "device"),
+        ("r", 6, (16, 16, 24), 64, 0.04, 0.3, "pinned"),
         ("r", 6, (20, 16, 24), 64, 0.04, 0.3, "pinned"),
         ("r", 6, (20, 16, 24), 100, 0.03, 0.1, "device"),
         ("r", 10, (18, 18, 30), 200, 0.02, 0.0, "pinned"),
@@ -41,10 +46,10 @@ def make_requests(seed=0):
         ("r", 12, (24, 20, 32), 400, 0.006, 0.0, "device", "random"),
         # odd Y: x-pairs only (the x-pair row walk; the x-mirror window walk
         # for a negative offset; the x-pair R1)
-        ("b", 10, (20, 15, 24), 300, 0.009, 0.0, "device"),
-        ("b", 10, (20, 15, 24), 300, 0.009, -0.01, "pinned"),
-        ("r", 10, (20, 15, 24), 300, 0.009, 0.0, "device"),
-        ("r", 10, (20, 15, 24), 300, 0.009, -0.01, "pageable"),
+        ("b", 10, (16, 15, 24), 300, 0.009, 0.0, "device"),
+        ("b", 10, (16, 15, 24), 300, 0.009, -0.01, "pinned"),
+        ("r", 10, (16, 15, 24), 300, 0.009, 0.0, "device"),
+        ("r", 10, (16, 15, 24), 300, 0.009, -0.01, "pageable"),
     ]
     reqs = []
     for k, (kind, n, res, T, w, off, where, *scan) in enumerate(specs):
@@ -95,6 +100,30 @@ def collect(tickets):
         out.append(tk.result().clone())
         tk.release()
     return out
+
+
+def paths_allowed(req):
+    """Which kernels the request's geometry allows, from the library's host
+    queries: R1 (1 xy-mirror, 2 x-pair) or A1 walks (1 x-mirror, 2 xy-mirror
+    row walk, 4 general row walk, 8 x-pair row walk)."""
+    from totrilib_amd import _lib
+    from totrilib_amd.serve import RenderRequest
+    lib = _lib.load()
+    X, Y, Z, N, T = req.dims()
+    o, sz = _lib.f3(req.volume_bbox_origin), _lib.f3(req.volume_bbox_size)
+    if isinstance(req, RenderRequest):
+        return "r", lib.totri_debug_r1_paths(X, Y, Z, N, T, o, sz)
+    return "b", lib.totri_debug_a1_walks(X, Y, Z, N, T, o, sz, req.bin_width, req.bin_offset)
+
+
+def test_requests_cover_every_path():
+    """The mix holds renders for the xy-mirror, x-pair and general R1 and
+    backprojections for each A1 walk (the geometry allows; the scans'
+    mirror pairs decide on the device)."""
+    got = {paths_allowed(r) for r in make_requests()}
+    for want in [("r", 3), ("r", 2), ("r", 0), ("b", 15), ("b", 13), ("b", 4), ("b", 1),
+                 ("b", 0)]:
+        assert want in got, (want, sorted(got))
 
 
 @pytest.mark.parametrize("graphs", [True, False])
@@ -168,7 +197,7 @@ def test_shortest_job_first_order():
     share a step); "fifo" puts both into one step in arrival order."""
     from totrilib_amd.serve import ServingEngine, request_cost
     reqs = make_requests(seed=3)
-    small, big = reqs[0], reqs[3]          # 24x20x40 render, 64^3 backprojection
+    small, big = reqs[0], reqs[3]          # 16x16x40 render, 64^3 backprojection
     assert request_cost(big) > 4 * request_cost(small)
     for policy in ("sjf", "fifo"):
         eng = ServingEngine(device_budget=1 << 30, host_budget=256 << 20, lanes=1,
