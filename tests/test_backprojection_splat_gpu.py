@@ -44,6 +44,45 @@ def test_velten_confocal_cfg1_matches_oracle():
     assert np.array_equal(got.astype(np.float64), ref), rel_l2(got, ref)
 
 
+@pytest.mark.parametrize("case", ["exact_formula", "mixed", "batch_origins"])
+def test_velten_confocal_decision_paths(case):
+    """A2's two per-scan-point decision paths, bit for bit against the FP32
+    oracle.  "exact_formula": 0.05 mm bins 2.0 m out put every scan point
+    beyond the estimate's 2^16-bin bound, so each pair takes the reference's
+    formula.  "mixed": per-point origins 0.2 m or 14 m away put half the
+    points on the estimate and half on the formula, all within 70,000 bins.  "batch_origins": two batches with
+    their own scan points and origins (each batch its own threshold rows)."""
+    from totrilib_amd.reconstruct import BackprojectionConfocalVelten
+    from totrilib_amd.types import BinDef, VolumeDef
+    rng = np.random.default_rng(57)
+    res = (32, 28, 36)
+    if case == "exact_formula":
+        B, n, T, w, off = 1, 16, 4096, 0.00005, 2.0
+    elif case == "mixed":
+        B, n, T, w, off = 1, 8, 70000, 0.0005, 1.0
+    else:
+        B, n, T, w, off = 2, 12, 700, 0.004, 0.3
+    S = n * n
+    scan = np.stack([wall_grid(n).T + rng.uniform(-0.01, 0.01, (S, 3)).astype(np.float32)
+                     for _ in range(B)])
+    if case == "exact_formula":
+        org = None
+    elif case == "mixed":
+        far = rng.uniform(0, 1, S) < 0.5
+        org = scan + np.where(far[:, None], [0.0, 0.0, -14.0], [0.0, 0.0, -0.2])
+        org = org.astype(np.float32)
+    else:
+        org = (scan + rng.uniform(-0.5, 0.5, (B, S, 3))).astype(np.float32)
+    tr = rng.random((B, T, S)).astype(np.float32)
+    vd = VolumeDef(VS, VE, res)
+    bd = BinDef(w, off, T)
+    args = (t(tr), t(scan), vd, bd) + ((t(org),) if org is not None else ())
+    got = BackprojectionConfocalVelten.apply(*args).cpu().numpy()
+    ref = O.velten_confocal(tr, scan, org, res, VS, VE, w, off, mode=O.FP32)
+    assert np.count_nonzero(ref) > 1000
+    assert np.array_equal(got.astype(np.float64), ref), rel_l2(got, ref)
+
+
 def test_velten_exhaustive_matches_oracle():
     from totrilib_amd.reconstruct import BackprojectionExhaustiveVelten
     from totrilib_amd.types import BinDef, VolumeDef
@@ -83,6 +122,36 @@ def test_velten_exhaustive_laser_origin_matches_oracle():
                                   mode=O.FP32)
         assert np.count_nonzero(ref) > 1000
         assert np.array_equal(got.astype(np.float64), ref), rel_l2(got, ref)
+
+
+@pytest.mark.parametrize("case", ["exact_formula", "mixed", "long_rows"])
+def test_velten_exhaustive_decision_paths(case):
+    """A3's two per-pair decision paths, bit for bit against the FP32 oracle.
+    "exact_formula": 0.03 mm bins put every pair beyond the estimate's
+    2^16-bin bound, so each (voxel, pair) divides as the reference does.
+    "mixed": laser origins 0.3 m or 28 m away put half the pairs on each
+    path within 72,000 bins.  "long_rows": 2,500 bins, rows too long for the
+    LDS stage (read from global memory)."""
+    from totrilib_amd.reconstruct import BackprojectionExhaustiveVelten
+    from totrilib_amd.types import BinDef, VolumeDef
+    rng = np.random.default_rng(58)
+    B, S, L = 1, 4, 4
+    res = (20, 18, 16)
+    T, w, off = {"exact_formula": (4000, 0.00003, 2.0), "mixed": (72000, 0.0005, 0.0),
+                 "long_rows": (2500, 0.002, 1.0)}[case]
+    scan = np.concatenate([rng.uniform(-1, 1, (B, S, 2)), np.zeros((B, S, 1))], 2).astype(np.float32)
+    laser = np.concatenate([rng.uniform(-1, 1, (B, L, 2)), np.zeros((B, L, 1))], 2).astype(np.float32)
+    lorg = None
+    if case == "mixed":
+        far = np.array([True, False, True, False])[None, :, None]
+        lorg = (laser + np.where(far, [0.0, 0.0, -28.0], [0.0, 0.0, -0.3])).astype(np.float32)
+    tr = rng.random((B, T, S, L)).astype(np.float32)
+    vd, bd = VolumeDef(VS, VE, res), BinDef(w, off, T)
+    got = BackprojectionExhaustiveVelten.apply(
+        t(tr), t(scan), t(laser), vd, bd, None, None if lorg is None else t(lorg)).cpu().numpy()
+    ref = O.velten_exhaustive(tr, scan, laser, None, lorg, res, VS, VE, w, off, mode=O.FP32)
+    assert np.count_nonzero(ref) > 500
+    assert np.array_equal(got.astype(np.float64), ref), rel_l2(got, ref)
 
 
 def test_arellano_with_origins_matches_oracle():

@@ -34,19 +34,27 @@ namespace {
 
 constexpr int kStageFloats = 8192;  // LDS floats of staged transient rows (32 KiB)
 
+// Blocks of a grid-stride loop over n elements (capped).
 int grid1d(size_t n, int block = 256) {
   size_t b = (n + block - 1) / block;
   if (b > 65535 * 4) b = 65535 * 4;
   return (int)(b < 1 ? 1 : b);
 }
 
-// out[b][c][r] = in[b][r][c] for a [B][R][C] array (tiled through LDS).
+// Blocks of a one-lane-per-element launch over n < 2^31 elements (uncapped).
+unsigned blocks_for(size_t n, int block = 256) {
+  const size_t b = (n + block - 1) / block;
+  return (unsigned)(b < 1 ? 1 : b);
+}
+
+// out[b][c][ooff + r] = in[b][r][c] for a [B][R][C] array, out rows of
+// stride ldo (tiled through LDS).
 __global__ void transpose_kernel(const float* __restrict__ in, float* __restrict__ out,
-                                 int R, int C) {
+                                 int R, int C, int ldo, int ooff) {
   __shared__ float tile[32][33];
   const int b = blockIdx.z;
   const float* src = in + (size_t)b * R * C;
-  float* dst = out + (size_t)b * R * C;
+  float* dst = out + (size_t)b * C * ldo + ooff;
   const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
   for (int k = threadIdx.y; k < 32; k += 8) {
     const int r = r0 + k, c = c0 + threadIdx.x;
@@ -55,7 +63,7 @@ __global__ void transpose_kernel(const float* __restrict__ in, float* __restrict
   __syncthreads();
   for (int k = threadIdx.y; k < 32; k += 8) {
     const int c = c0 + k, r = r0 + threadIdx.x;
-    if (r < R && c < C) dst[(size_t)c * R + r] = tile[threadIdx.x][k];
+    if (r < R && c < C) dst[(size_t)c * ldo + r] = tile[threadIdx.x][k];
   }
 }
 
@@ -80,14 +88,178 @@ __device__ __forceinline__ int nearest_bin(float dist, float w, float off) {
 // ---------------------------------------------------------------------------
 // A2: vol[b][z][y][x] = sum_s tr[b][t(s)][s],
 // t = nearest_bin(2 |p_s - o_s| + 2 |p_s - q|)       (backprojection_cuda.cu:152-194)
+//
+// For one scan point, t is a non-decreasing function of the squared distance
+// e = (dx*dx + dy*dy) + dz*dz: every step of the formula (correctly rounded
+// square root, doubling, the sum with 2 |p_s - o_s|, the subtraction of the
+// offset, the division by a positive width, + 0.5, floor, saturating
+// conversion) is monotone.  So t(e) >= k exactly when e >= E_k, with
+// E_k = min{e : t(e) >= k}.  velten_bins_kernel finds E_0 .. E_T per scan
+// point by bisection over the float bit patterns, evaluating the exact
+// formula.  The backprojection then takes a cheap estimate a of t (hardware
+// square root, one FMA with precomputed constants), within one bin of t, and
+// corrects it with two table entries:
+//   t = a - [e < E_a] + [e >= E_(a+1)].
+// That is the exact decision, with no correctly rounded root or division
+// per (voxel, scan point) pair.
 // ---------------------------------------------------------------------------
+
+// The exact bin of squared distance e for a scan point whose origin term is
+// dso2 = 2 |p_s - o_s| (0 without an origin): backprojection_cuda.cu:177-187.
+__device__ __forceinline__ int velten_bin(float e, float dso2, float w, float off) {
+  const float dsv = sqrt_rn(e);
+  const float dist = dso2 + 2.f * dsv;
+  return nearest_bin(dist, w, off);
+}
+
+// Per-scan-point constants of the estimate: the point and
+// c = ((dso2 - off) * (1/w)) + 0.5, so that a = floor(fma(sqrt(e), 2/w, c)).
+// c is NaN where the estimate's error bound does not hold (below); those
+// scan points take the exact formula per pair.
+struct VeltenPoint {
+  float px, py, pz, c;
+};
+
+// One row of floats per scan point, all parts 16-byte aligned:
+//   [0, 4)             the point's constants (VeltenPoint)
+//   [4, 4 + nv)        its transient bins, vals[1 + t] = bin t, zeros at 0
+//                      and T + 1 (nv = T + 2 rounded up to a multiple of 4)
+//   [4 + nv, 4 + nv + nt)  its thresholds { -inf, E_0, ..., E_T, NaN }
+//                      (nt = T + 3 rounded up to a multiple of 4)
+// so that a group of consecutive scan points is one contiguous block.
+struct VeltenRow {
+  int nv, nt, len;
+};
+
+__host__ __device__ __forceinline__ VeltenRow velten_row(int T) {
+  VeltenRow r;
+  r.nv = (T + 5) & ~3;
+  r.nt = (T + 6) & ~3;
+  r.len = 4 + r.nv + r.nt;
+  return r;
+}
+
+// Estimate error: the exact chain rounds five times (root, sum, difference,
+// quotient, + 0.5) and the estimate about eight times (hardware root within
+// 1 ulp, 1/w, 2/w, the constant's three roundings, the FMA), each a relative
+// error of at most 2^-24 (2^-23 for the root) of a term no larger than M bins,
+// M = (2 d_max + |dso2| + |off|) / w + 1 with d_max the largest distance from
+// the scan point to the box's corners.  The two differ by under 16 * 2^-24 * M
+// bins; M < 2^16 keeps that below 1/16 of a bin, so the estimate is within
+// one bin of the exact value.
+constexpr float kVeltenMaxBins = 65536.f;
+
+// Thread (b, s, j), j in [0, T + 3): threshold j of scan point s, that is
+// -inf, E_0, ..., E_T, NaN, where E_k = +inf when no finite e reaches bin k.
+// For an estimate a in [-1, T] the rows' threshold part gives E_a at index
+// a + 1 and E_(a+1) at a + 2 (no e is below -inf, and no e compares >= NaN).
+// Thread j = 0 also writes the point's constants and the zero pads of its
+// bins.
+__global__ __launch_bounds__(256) void velten_bins_kernel(
+    const float* __restrict__ scan, const float* __restrict__ orig, BoxGeom g, int S, int So,
+    float rw, float* __restrict__ rows) {
+  const int T = g.T;
+  const VeltenRow vr = velten_row(T);
+  const int n = T + 3;
+  const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= (size_t)g.B * S * n) return;
+  const int j = (int)(id % n);
+  const size_t bs = id / n;  // b * S + s
+  const int b = (int)(bs / S), s = (int)(bs - (size_t)b * S);
+  float* row = rows + bs * vr.len;
+  const float* p = scan + bs * 3;
+  const float px = p[0], py = p[1], pz = p[2];
+  float dso = 0.f;
+  if (So > 0) {
+    const float* o = orig + ((size_t)b * So + s) * 3;
+    dso = dist3(px, py, pz, o[0], o[1], o[2]);
+  }
+  const float dso2 = 2.f * dso;
+  float e;
+  if (j == 0) {
+    e = -INFINITY;
+  } else if (j == n - 1) {
+    e = __builtin_nanf("");
+  } else {
+    const int k = j - 1;
+    // smallest e >= 0 (as a bit pattern; monotone for non-negative floats)
+    // with velten_bin(e) >= k; +inf (0x7f800000) always qualifies
+    unsigned lo = 0u, hi = 0x7f800000u;
+    if (velten_bin(0.f, dso2, g.w, g.off) >= k) {
+      hi = 0u;
+    } else {
+      while (hi - lo > 1u) {
+        const unsigned mid = lo + (hi - lo) / 2u;
+        if (velten_bin(__uint_as_float(mid), dso2, g.w, g.off) >= k)
+          hi = mid;
+        else
+          lo = mid;
+      }
+    }
+    e = __uint_as_float(hi);
+  }
+  row[4 + vr.nv + j] = e;
+  if (j == 0) {
+    // the largest distance to the box is at a corner
+    float dmax = 0.f;
+    for (int c = 0; c < 8; ++c)
+      dmax = fmaxf(dmax, dist3(px, py, pz, (c & 1) ? g.ex : g.sx, (c & 2) ? g.ey : g.sy,
+                               (c & 4) ? g.ez : g.sz));
+    const float m = (2.f * dmax + fabsf(dso2) + fabsf(g.off)) / g.w + 1.f;
+    const bool est = g.w > 0.f && m < kVeltenMaxBins;  // false for NaN or inf
+    *(float4*)row = make_float4(px, py, pz, est ? (dso2 - g.off) * rw + 0.5f
+                                                : __builtin_nanf(""));
+    row[4] = 0.f;
+    row[4 + T + 1] = 0.f;
+  }
+}
+
+// Bin of squared distance e for the scan point of `row` (stage or global):
+// the corrected estimate, or the exact formula where the point's c is NaN.
+// The result is clamped to [-1, T], whose ends read the zero pads.
+template <bool EXACT>
+__device__ __forceinline__ int velten_take_bin(const float* row, const VeltenRow& vr, float e,
+                                               float c, float k2, int T, float dso2,
+                                               const BoxGeom& g) {
+  if (!EXACT) {
+    int a = (int)floorf(__builtin_fmaf(sqrt_fast(e), k2, c));
+    a = min(max(a, -1), T);
+    const float* E = row + 4 + vr.nv + a + 1;
+    return a + (e < E[0] ? -1 : 0) + (e >= E[1] ? 1 : 0);
+  }
+  const int t = velten_bin(e, dso2, g.w, g.off);                             // :177,185-187
+  return min(max(t, -1), T);
+}
+
+__device__ __forceinline__ float velten_sqdist(const float4& p, float qx, float qy, float qz) {
+  const float dx = p.x - qx, dy = p.y - qy, dz = p.z - qz;
+  float e = dx * dx;
+  e = e + dy * dy;
+  e = e + dz * dz;
+  return e;
+}
+
+// Scan points per unrolled group of the backprojection loop.
+constexpr int kVeltenUnroll = 8;
+
+// LDS floats of one stage of scan-point rows (32 KiB, four workgroups per
+// CU), and the float4s per thread that copy one stage with a 256-lane
+// workgroup.
+constexpr int kVeltenStage = kStageFloats;
+constexpr int kVeltenCopy4 = kVeltenStage / 4 / 256;
+
+// One lane per voxel, flat over [D][H][W]; rows as laid out above.  A
+// workgroup stages the rows of R scan points in LDS (STAGED), loading the
+// next group into registers while it works on the current one, or reads
+// them from global memory (rows too long for the stage).  Sums run over scan
+// points in order, as the reference's loop.
+template <bool STAGED>
 __global__ __launch_bounds__(256) void velten_confocal_kernel(
-    const float* __restrict__ trT,    // [B][S][T]
-    const float* __restrict__ scan,   // [B][S][3]
-    const float* __restrict__ orig,   // [B][So][3], So in {0, S}
-    float* __restrict__ vol,          // [B][D][H][W]
-    BoxGeom g, int S, int So, int rows_per_stage) {
-  extern __shared__ float rows[];
+    const float* __restrict__ rows, const float* __restrict__ orig, float* __restrict__ vol,
+    BoxGeom g, int S, int So, int R, float k2) {
+  extern __shared__ float4 lds4[];
+  const int T = g.T;
+  const VeltenRow vr = velten_row(T);
   const int b = blockIdx.y;
   const int nvox = g.D * g.H * g.W;
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
@@ -99,32 +271,92 @@ __global__ __launch_bounds__(256) void velten_confocal_kernel(
   const float qx = inclusive_coord(x, g.W, g.sx, g.ex);
   const float qy = inclusive_coord(y, g.H, g.sy, g.ey);
   const float qz = inclusive_coord(z, g.D, g.sz, g.ez);
-  const float* sb = scan + (size_t)b * S * 3;
-  const float* ob = orig + (size_t)b * So * 3;
-  const float* tb = trT + (size_t)b * S * g.T;
-  float value = 0.f;
-  // rows_per_stage == 0 (rows longer than the LDS stage): read the rows from
-  // global memory; the choice is uniform over the launch.
-  const bool staged = rows_per_stage > 0;
-  const int step = staged ? rows_per_stage : 1;
-  for (int s0 = 0; s0 < S; s0 += step) {
-    const int ns = min(step, S - s0);
-    const float* src = tb + (size_t)s0 * g.T;
-    if (staged) {
-      __syncthreads();
-      for (int e = threadIdx.x; e < ns * g.T; e += blockDim.x) rows[e] = src[e];
-      __syncthreads();
-      src = rows;
+  const size_t bS = (size_t)b * S;
+  const float* ob = orig + bS * 3;
+  float* stage = (float*)lds4;
+  // the rows of the next group of points, loaded into registers
+  float4 buf[kVeltenCopy4];
+  if (STAGED) {
+    const int n4 = min(R, S) * vr.len / 4;
+    const float4* src = (const float4*)(rows + bS * vr.len);
+#pragma unroll
+    for (int i = 0; i < kVeltenCopy4; ++i) {
+      const int e = i * 256 + (int)threadIdx.x;
+      buf[i] = e < n4 ? src[e] : make_float4(0.f, 0.f, 0.f, 0.f);
     }
-    for (int k = 0; k < ns; ++k) {
-      const int s = s0 + k;
-      const float px = sb[3 * s], py = sb[3 * s + 1], pz = sb[3 * s + 2];
-      const float dsv = dist3(px, py, pz, qx, qy, qz);                     // :177
-      const float dso = So > 0 ? dist3(px, py, pz, ob[3 * s], ob[3 * s + 1], ob[3 * s + 2])
-                               : 0.f;                                        // :178-183
-      const float dist = 2.f * dso + 2.f * dsv;                              // :185
-      const int ti = nearest_bin(dist, g.w, g.off);                          // :187
-      if (ti >= 0 && ti < g.T) value += src[k * g.T + ti];                   // :188-190
+  }
+  float value = 0.f;
+  for (int s0 = 0; s0 < S; s0 += R) {
+    const int ns = min(R, S - s0);
+    const float* base = rows + (bS + s0) * vr.len;
+    if (STAGED) {
+      __syncthreads();  // the previous group is no longer read
+      const int n4 = ns * vr.len / 4;
+#pragma unroll
+      for (int i = 0; i < kVeltenCopy4; ++i) {
+        const int e = i * 256 + (int)threadIdx.x;
+        if (e < n4) lds4[e] = buf[i];
+      }
+      __syncthreads();
+      if (s0 + R < S) {
+        const int n4n = min(R, S - s0 - R) * vr.len / 4;
+        const float4* src = (const float4*)(rows + (bS + s0 + R) * vr.len);
+#pragma unroll
+        for (int i = 0; i < kVeltenCopy4; ++i) {
+          const int e = i * 256 + (int)threadIdx.x;
+          buf[i] = e < n4n ? src[e] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+      }
+      base = stage;
+    }
+    // kVeltenUnroll points at a time (the last group of a stage may have
+    // fewer, nu; its missing slots repeat the last point and add nothing):
+    // their loads and decisions overlap, their values are added in point
+    // order.  A group with a point whose c is NaN takes the per-point path
+    // (the choice is uniform).
+    for (int k = 0; k < ns; k += kVeltenUnroll) {
+      const int nu = min(kVeltenUnroll, ns - k);
+      float4 p[kVeltenUnroll];
+      bool est = true;
+#pragma unroll
+      for (int u = 0; u < kVeltenUnroll; ++u) {
+        p[u] = *(const float4*)(base + (k + min(u, nu - 1)) * vr.len);
+        est = est && p[u].w == p[u].w;
+      }
+      if (est) {
+        float a[kVeltenUnroll];
+#pragma unroll
+        for (int u = 0; u < kVeltenUnroll; ++u) {
+          const float* r = base + (k + min(u, nu - 1)) * vr.len;
+          const float e = velten_sqdist(p[u], qx, qy, qz);
+          a[u] = r[4 + velten_take_bin<false>(r, vr, e, p[u].w, k2, T, 0.f, g) + 1];
+        }
+        if (nu == kVeltenUnroll) {
+#pragma unroll
+          for (int u = 0; u < kVeltenUnroll; ++u) value += a[u];            // :188-190
+        } else {
+#pragma unroll
+          for (int u = 0; u < kVeltenUnroll; ++u)
+            if (u < nu) value += a[u];
+        }
+      } else {
+        for (int u = 0; u < nu; ++u) {
+          const float* r = base + (k + u) * vr.len;
+          const float4 q = *(const float4*)r;
+          const float e = velten_sqdist(q, qx, qy, qz);
+          int t;
+          if (q.w == q.w) {
+            t = velten_take_bin<false>(r, vr, e, q.w, k2, T, 0.f, g);
+          } else {
+            const int s = s0 + k + u;
+            const float dso = So > 0 ? dist3(q.x, q.y, q.z, ob[3 * s], ob[3 * s + 1],
+                                             ob[3 * s + 2])
+                                     : 0.f;                                  // :178-183
+            t = velten_take_bin<true>(r, vr, e, q.w, k2, T, 2.f * dso, g);
+          }
+          value += r[4 + t + 1];  // bins -1 and T read the zero pads (:188-190)
+        }
+      }
     }
   }
   if (ok) vol[(size_t)b * nvox + v] = value;                                 // :192
@@ -134,16 +366,143 @@ __global__ __launch_bounds__(256) void velten_confocal_kernel(
 // A3: vol = sum_{s,l} tr[b][t(s,l)][s][l],
 // t = nearest_bin(|p_s - o_s| + |p_s - q| + |p_l - q| + |p_l - o_l|)
 // (backprojection_cuda.cu:228-287)
+//
+// The path length D is computed as the reference does (correctly rounded
+// roots, sums in its order).  Its bin floor((D - off) / w + 0.5) is a
+// non-decreasing function of D, the same for every pair, so one table of
+// thresholds D_k = min{D : bin(D) >= k} (bisection over the float bit
+// patterns) turns the division into an estimate a = floor(fma(D, 1/w,
+// 0.5 - off/w)), within one bin where D/w stays below 2^16 bins, and its
+// correction a - [D < D_a] + [D >= D_(a+1)].  The pair's constant terms
+// (the points and their origin distances) are computed once per pair.
 // ---------------------------------------------------------------------------
+
+// One row of floats per (scan point, laser point) pair r = s * L + l, 16-byte
+// aligned parts:
+//   [0, 4)    p_s, |p_s - o_s|        [4, 8)  p_l, |p_l - o_l|
+//   [8, 12)   1 where the estimate's bound holds for the pair (else 0), 0, 0, 0
+//   [12, 12 + nv)  its bins, vals[1 + t] = bin t, zeros at 0 and T + 1
+//                  (nv = T + 2 rounded up to a multiple of 4)
+struct ExRow {
+  int nv, len;
+};
+
+__host__ __device__ __forceinline__ ExRow ex_row(int T) {
+  ExRow r;
+  r.nv = (T + 5) & ~3;
+  r.len = 12 + r.nv;
+  return r;
+}
+
+// Thread (b, r): the pair's constants and the zero pads of its bins.  The
+// estimate's error: the reference chain rounds the difference, the quotient
+// and + 0.5, the estimate 1/w, off/w, the constant and the FMA; under
+// 8 * 2^-24 * M bins for M = (D_max + |off|) / w + 1, D_max bounding D over
+// the box (origin terms plus the largest distances to the box's corners).
+// M < 2^16 keeps that far below one bin.
+__global__ __launch_bounds__(256) void velten_ex_rows_kernel(
+    const float* __restrict__ scan, const float* __restrict__ laser,
+    const float* __restrict__ sorig, const float* __restrict__ lorig, BoxGeom g, int S,
+    int L, int So, int Lo, float* __restrict__ rows) {
+  const ExRow xr = ex_row(g.T);
+  const size_t id = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= (size_t)g.B * S * L) return;
+  const int b = (int)(id / ((size_t)S * L));
+  const int r = (int)(id - (size_t)b * S * L);
+  const int s = r / L, l = r - s * L;
+  const float* ps = scan + ((size_t)b * S + s) * 3;
+  const float* pl = laser + ((size_t)b * L + l) * 3;
+  const float px = ps[0], py = ps[1], pz = ps[2];
+  const float lx = pl[0], ly = pl[1], lz = pl[2];
+  float dso = 0.f, dlo = 0.f;
+  if (So > 0) {
+    const float* o = sorig + ((size_t)b * So + s) * 3;
+    dso = dist3(px, py, pz, o[0], o[1], o[2]);                               // :256-261
+  }
+  if (Lo > 0) {
+    const float* o = lorig + ((size_t)b * Lo + l) * 3;
+    dlo = dist3(lx, ly, lz, o[0], o[1], o[2]);                               // :269-274
+  }
+  float ds = 0.f, dl = 0.f;
+  for (int c = 0; c < 8; ++c) {
+    const float cx = (c & 1) ? g.ex : g.sx, cy = (c & 2) ? g.ey : g.sy,
+                cz = (c & 4) ? g.ez : g.sz;
+    ds = fmaxf(ds, dist3(px, py, pz, cx, cy, cz));
+    dl = fmaxf(dl, dist3(lx, ly, lz, cx, cy, cz));
+  }
+  const float m = (dso + dlo + ds + dl + fabsf(g.off)) / g.w + 1.f;
+  const bool est = g.w > 0.f && m < kVeltenMaxBins;  // false for NaN or inf
+  float4* h = (float4*)(rows + id * xr.len);
+  h[0] = make_float4(px, py, pz, dso);
+  h[1] = make_float4(lx, ly, lz, dlo);
+  h[2] = make_float4(est ? 1.f : 0.f, 0.f, 0.f, 0.f);
+  rows[id * xr.len + 12] = 0.f;
+  rows[id * xr.len + 12 + g.T + 1] = 0.f;
+}
+
+// Thread j in [0, T + 3): the path-length thresholds
+// {-inf, D_0, ..., D_T, NaN}, D_k = +inf when no finite D reaches bin k.
+__global__ __launch_bounds__(256) void velten_ex_table_kernel(BoxGeom g,
+                                                              float* __restrict__ tab) {
+  const int n = g.T + 3;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  float d;
+  if (j == 0) {
+    d = -INFINITY;
+  } else if (j == n - 1) {
+    d = __builtin_nanf("");
+  } else {
+    const int k = j - 1;
+    unsigned lo = 0u, hi = 0x7f800000u;
+    if (nearest_bin(0.f, g.w, g.off) >= k) {
+      hi = 0u;
+    } else {
+      while (hi - lo > 1u) {
+        const unsigned mid = lo + (hi - lo) / 2u;
+        if (nearest_bin(__uint_as_float(mid), g.w, g.off) >= k)
+          hi = mid;
+        else
+          lo = mid;
+      }
+    }
+    d = __uint_as_float(hi);
+  }
+  tab[j] = d;
+}
+
+// Bin of one (voxel, pair): D as the reference computes it, then the
+// corrected estimate (EST) or the reference's division; clamped to [-1, T].
+template <bool EST>
+__device__ __forceinline__ int velten_ex_bin(const float* row, const float* tab, float qx,
+                                             float qy, float qz, float rw, float c0,
+                                             const BoxGeom& g) {
+  const float4 hs = *(const float4*)row, hl = *(const float4*)(row + 4);
+  const float dsv = dist3(hs.x, hs.y, hs.z, qx, qy, qz);                     // :255
+  const float dlv = dist3(hl.x, hl.y, hl.z, qx, qy, qz);                     // :268
+  float dist = hs.w + dsv;                                                   // :276-277
+  dist = dist + dlv;
+  dist = dist + hl.w;
+  if (EST) {
+    int a = (int)floorf(__builtin_fmaf(dist, rw, c0));
+    a = min(max(a, -1), g.T);
+    return a + (dist < tab[a + 1] ? -1 : 0) + (dist >= tab[a + 2] ? 1 : 0);
+  }
+  const int t = nearest_bin(dist, g.w, g.off);                               // :279
+  return min(max(t, -1), g.T);
+}
+
+// One lane per voxel, flat over [D][H][W]; rows as laid out above, the
+// threshold table in LDS.  Rows are staged in LDS as in A2 (STAGED), or read
+// from global memory (long rows).  Sums run over pairs in the reference's
+// order (scan point outer, laser point inner).
+template <bool STAGED>
 __global__ __launch_bounds__(256) void velten_exhaustive_kernel(
-    const float* __restrict__ trT,     // [B][S][L][T]
-    const float* __restrict__ scan,    // [B][S][3]
-    const float* __restrict__ laser,   // [B][L][3]
-    const float* __restrict__ sorig,   // [B][So][3]
-    const float* __restrict__ lorig,   // [B][Lo][3]
-    float* __restrict__ vol, BoxGeom g, int S, int L, int So, int Lo,
-    int rows_per_stage) {
-  extern __shared__ float rows[];
+    const float* __restrict__ rows, const float* __restrict__ gtab, float* __restrict__ vol,
+    BoxGeom g, int NR, int R, float rw, float c0) {
+  extern __shared__ float4 lds4[];
+  const int T = g.T;
+  const ExRow xr = ex_row(T);
   const int b = blockIdx.y;
   const int nvox = g.D * g.H * g.W;
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
@@ -155,41 +514,80 @@ __global__ __launch_bounds__(256) void velten_exhaustive_kernel(
   const float qx = inclusive_coord(x, g.W, g.sx, g.ex);
   const float qy = inclusive_coord(y, g.H, g.sy, g.ey);
   const float qz = inclusive_coord(z, g.D, g.sz, g.ez);
-  const float* sb = scan + (size_t)b * S * 3;
-  const float* lb = laser + (size_t)b * L * 3;
-  const float* sob = sorig + (size_t)b * So * 3;
-  const float* lob = lorig + (size_t)b * Lo * 3;
-  const float* tb = trT + (size_t)b * S * L * g.T;
-  const int nrows = S * L;
-  float value = 0.f;
-  // rows_per_stage == 0: rows from global memory (as in A2)
-  const bool staged = rows_per_stage > 0;
-  const int step = staged ? rows_per_stage : 1;
-  for (int r0 = 0; r0 < nrows; r0 += step) {
-    const int nr = min(step, nrows - r0);
-    const float* src = tb + (size_t)r0 * g.T;
-    if (staged) {
-      __syncthreads();
-      for (int e = threadIdx.x; e < nr * g.T; e += blockDim.x) rows[e] = src[e];
-      __syncthreads();
-      src = rows;
+  const size_t bR = (size_t)b * NR;
+  // LDS: the threshold table (T + 3 floats, rounded to float4s), then the stage
+  const int ntab4 = (T + 6) / 4;
+  float* tab = (float*)lds4;
+  float* stage = (float*)(lds4 + ntab4);
+  const float* tb = gtab;
+  if (STAGED) {
+    for (int e = threadIdx.x; e < T + 3; e += blockDim.x) tab[e] = gtab[e];
+    tb = tab;
+  }
+  float4 buf[kVeltenCopy4];
+  if (STAGED) {
+    const int n4 = min(R, NR) * xr.len / 4;
+    const float4* src = (const float4*)(rows + bR * xr.len);
+#pragma unroll
+    for (int i = 0; i < kVeltenCopy4; ++i) {
+      const int e = i * 256 + (int)threadIdx.x;
+      buf[i] = e < n4 ? src[e] : make_float4(0.f, 0.f, 0.f, 0.f);
     }
-    for (int k = 0; k < nr; ++k) {
-      const int r = r0 + k;
-      const int s = r / L, l = r - s * L;
-      const float px = sb[3 * s], py = sb[3 * s + 1], pz = sb[3 * s + 2];
-      const float dsv = dist3(px, py, pz, qx, qy, qz);                     // :255
-      const float dso = So > 0 ? dist3(px, py, pz, sob[3 * s], sob[3 * s + 1], sob[3 * s + 2])
-                               : 0.f;                                        // :256-261
-      const float lx = lb[3 * l], ly = lb[3 * l + 1], lz = lb[3 * l + 2];
-      const float dlv = dist3(lx, ly, lz, qx, qy, qz);                     // :268
-      const float dlo = Lo > 0 ? dist3(lx, ly, lz, lob[3 * l], lob[3 * l + 1], lob[3 * l + 2])
-                               : 0.f;                                        // :269-274
-      float dist = dso + dsv;                                                // :276-277
-      dist = dist + dlv;
-      dist = dist + dlo;
-      const int ti = nearest_bin(dist, g.w, g.off);                          // :279
-      if (ti >= 0 && ti < g.T) value += src[k * g.T + ti];                   // :280-282
+  }
+  float value = 0.f;
+  for (int r0 = 0; r0 < NR; r0 += R) {
+    const int nr = min(R, NR - r0);
+    const float* base = rows + (bR + r0) * xr.len;
+    if (STAGED) {
+      __syncthreads();  // the previous group is no longer read
+      const int n4 = nr * xr.len / 4;
+#pragma unroll
+      for (int i = 0; i < kVeltenCopy4; ++i) {
+        const int e = i * 256 + (int)threadIdx.x;
+        if (e < n4) ((float4*)stage)[e] = buf[i];
+      }
+      __syncthreads();
+      if (r0 + R < NR) {
+        const int n4n = min(R, NR - r0 - R) * xr.len / 4;
+        const float4* src = (const float4*)(rows + (bR + r0 + R) * xr.len);
+#pragma unroll
+        for (int i = 0; i < kVeltenCopy4; ++i) {
+          const int e = i * 256 + (int)threadIdx.x;
+          buf[i] = e < n4n ? src[e] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+      }
+      base = stage;
+    }
+    // groups of kVeltenUnroll pairs, as in A2
+    for (int k = 0; k < nr; k += kVeltenUnroll) {
+      const int nu = min(kVeltenUnroll, nr - k);
+      bool est = true;
+#pragma unroll
+      for (int u = 0; u < kVeltenUnroll; ++u)
+        est = est && base[(k + min(u, nu - 1)) * xr.len + 8] != 0.f;
+      if (est) {
+        float a[kVeltenUnroll];
+#pragma unroll
+        for (int u = 0; u < kVeltenUnroll; ++u) {
+          const float* r = base + (k + min(u, nu - 1)) * xr.len;
+          a[u] = r[12 + velten_ex_bin<true>(r, tb, qx, qy, qz, rw, c0, g) + 1];
+        }
+        if (nu == kVeltenUnroll) {
+#pragma unroll
+          for (int u = 0; u < kVeltenUnroll; ++u) value += a[u];            // :280-282
+        } else {
+#pragma unroll
+          for (int u = 0; u < kVeltenUnroll; ++u)
+            if (u < nu) value += a[u];
+        }
+      } else {
+        for (int u = 0; u < nu; ++u) {
+          const float* r = base + (k + u) * xr.len;
+          const int t = r[8] != 0.f ? velten_ex_bin<true>(r, tb, qx, qy, qz, rw, c0, g)
+                                    : velten_ex_bin<false>(r, tb, qx, qy, qz, rw, c0, g);
+          value += r[12 + t + 1];  // bins -1 and T read the zero pads (:280-282)
+        }
+      }
     }
   }
   if (ok) vol[(size_t)b * nvox + v] = value;                                 // :285
@@ -396,27 +794,44 @@ int check_dims(std::initializer_list<int64_t> dims, int64_t elems_limit_a, int64
 
 }  // namespace
 
-int transpose_batched(const float* in, float* out, int B, int R, int C, hipStream_t st) {
+int transpose_strided(const float* in, float* out, int B, int R, int C, int ldo, int ooff,
+                      hipStream_t st) {
   dim3 grid((C + 31) / 32, (R + 31) / 32, B);
-  hipLaunchKernelGGL(transpose_kernel, grid, dim3(32, 8), 0, st, in, out, R, C);
+  hipLaunchKernelGGL(transpose_kernel, grid, dim3(32, 8), 0, st, in, out, R, C, ldo, ooff);
   return check_launch("transpose_kernel");
+}
+
+int transpose_batched(const float* in, float* out, int B, int R, int C, hipStream_t st) {
+  return transpose_strided(in, out, B, R, C, R, 0, st);
 }
 
 }  // namespace totri_hip
 
 using namespace totri_hip;
 
+namespace totri_hip {
+namespace {
+
+// Workspace of A2: one row per scan point (velten_row), [B][S][len] floats.
+size_t velten_bytes(int64_t B, int64_t T, int64_t S) {
+  return align256((size_t)B * S * velten_row((int)T).len * sizeof(float));
+}
+
+}  // namespace
+}  // namespace totri_hip
+
 extern "C" size_t totri_backprojection_confocal_velten_workspace(int64_t B, int64_t T,
                                                                  int64_t S) {
-  return (size_t)B * T * S * sizeof(float);
+  if (B <= 0 || T < 0 || S <= 0) return 0;
+  return totri_hip::velten_bytes(B, T, S);
 }
 
 namespace totri_hip {
 namespace {
 
-// A2 with its scratch (the transposed transient, B*T*S floats) supplied by
-// the caller: the workspace of the entry point below, or the library's own
-// array for the allocation-free variant.
+// A2 with its scratch (the transient's value rows, the bin thresholds and the
+// per-point constants) supplied by the caller: the workspace of the entry
+// point below, or the library's own array for the allocation-free variant.
 int velten_confocal(const float* transient, int64_t B, int64_t T, int64_t S,
                     const float* scan_points, const float* scan_origin, int64_t So,
                     float* volume, int64_t D, int64_t H, int64_t W,
@@ -426,32 +841,50 @@ int velten_confocal(const float* transient, int64_t B, int64_t T, int64_t S,
   if (rc) return rc;
   TOTRI_CHECK_ARG(So == 0 || So == S, "scan_origin must have 0 or S points");
   TOTRI_CHECK_ARG(volume_start && volume_end, "null volume box");
+  TOTRI_CHECK_ARG(T < (1 << 30), "T=%lld too large", (long long)T);
   hipStream_t st = (hipStream_t)stream;
   const size_t nvox = (size_t)D * H * W;
   if (B == 0 || nvox == 0) return TOTRI_OK;
   TOTRI_CHECK_ARG(volume != nullptr, "null volume");
   const BoxGeom g = box_geom(B, D, H, W, T, volume_start, volume_end, bin_width, bin_offset);
-  if (T == 0 || S == 0) {
-    hipLaunchKernelGGL(velten_confocal_kernel, dim3(grid1d(nvox), (unsigned)B), dim3(256), 0,
-                       st, (const float*)nullptr, scan_points, scan_origin, volume, g, 0,
-                       (int)So, 1);
-    return check_launch("velten_confocal_kernel");
+  if (T == 0 || S == 0) {  // no bins or no scan points: a zero volume
+    hipLaunchKernelGGL(zero_kernel, dim3(grid1d(B * nvox)), dim3(256), 0, st, volume,
+                       (size_t)B * nvox);
+    return check_launch("zero_kernel");
   }
   TOTRI_CHECK_ARG(transient && scan_points, "null input");
   TOTRI_CHECK_ARG(So == 0 || scan_origin, "null scan_origin");
-  const size_t need = totri_backprojection_confocal_velten_workspace(B, T, S);
+  const size_t need = velten_bytes(B, T, S);
   if (workspace_bytes < need)
     return fail(TOTRI_EWORKSPACE, "workspace too small: need %zu bytes, got %zu", need,
                 workspace_bytes);
-  TOTRI_CHECK_ARG(workspace != nullptr, "null workspace");
-  float* trT = (float*)workspace;
-  rc = launch_transpose(transient, trT, (int)B, (int)T, (int)S, st);
+  TOTRI_CHECK_ARG(workspace != nullptr && ((uintptr_t)workspace & 15) == 0,
+                  "workspace must be a non-null, 16-byte aligned device pointer");
+  float* rows = (float*)workspace;
+  const VeltenRow vr = velten_row((int)T);
+  rc = transpose_strided(transient, rows, (int)B, (int)T, (int)S, vr.len, 5, st);
   if (rc) return rc;
-  // rows longer than the LDS stage are read from global memory (rows = 0)
-  const int rows = T <= kStageFloats ? (int)(kStageFloats / T) : 0;
-  hipLaunchKernelGGL(velten_confocal_kernel, dim3(grid1d(nvox), (unsigned)B), dim3(256),
-                     (size_t)rows * T * sizeof(float), st, trT, scan_points, scan_origin,
-                     volume, g, (int)S, (int)So, rows);
+  const float rw = 1.f / bin_width;
+  const size_t nthr = (size_t)B * S * (T + 3);
+  hipLaunchKernelGGL(velten_bins_kernel, dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, st,
+                     scan_points, scan_origin, g, (int)S, (int)So, rw, rows);
+  rc = check_launch("velten_bins_kernel");
+  if (rc) return rc;
+  // Scan points per LDS stage.  Where fewer than four rows fit (T above
+  // about 1020 bins) the stage's barriers would come every few points: the
+  // rows are then read from global memory, through the caches.
+  int R = (int)std::min<int64_t>(S, kVeltenStage / vr.len);
+  if (R < 4 && R < S) R = 0;
+  const float k2 = 2.f * rw;
+  const dim3 grid(blocks_for(nvox), (unsigned)B);
+  if (R > 0) {
+    hipLaunchKernelGGL(velten_confocal_kernel<true>, grid, dim3(256),
+                       (size_t)R * vr.len * sizeof(float), st, rows, scan_origin, volume, g,
+                       (int)S, (int)So, R, k2);
+  } else {
+    hipLaunchKernelGGL(velten_confocal_kernel<false>, grid, dim3(256), 0, st, rows,
+                       scan_origin, volume, g, (int)S, (int)So, (int)S, k2);
+  }
   return check_launch("velten_confocal_kernel");
 }
 
@@ -526,9 +959,28 @@ extern "C" int totri_backprojection_confocal_velten_forward_noalloc(
                          kNoAllocScratchBytes, stream);
 }
 
+namespace totri_hip {
+namespace {
+
+// Workspace of A3: the pair rows [B][S*L][len], then the threshold table.
+struct ExLayout {
+  size_t tab_off, end;
+};
+
+ExLayout ex_layout(int64_t B, int64_t T, int64_t S, int64_t L) {
+  ExLayout l;
+  l.tab_off = align256((size_t)B * S * L * ex_row((int)T).len * sizeof(float));
+  l.end = align256(l.tab_off + (size_t)(T + 3) * sizeof(float));
+  return l;
+}
+
+}  // namespace
+}  // namespace totri_hip
+
 extern "C" size_t totri_backprojection_exhaustive_velten_workspace(int64_t B, int64_t T,
                                                                    int64_t S, int64_t L) {
-  return (size_t)B * T * S * L * sizeof(float);
+  if (B <= 0 || T < 0 || S <= 0 || L <= 0) return 0;
+  return totri_hip::ex_layout(B, T, S, L).end;
 }
 
 extern "C" int totri_backprojection_exhaustive_velten_forward(
@@ -542,32 +994,56 @@ extern "C" int totri_backprojection_exhaustive_velten_forward(
   TOTRI_CHECK_ARG(So == 0 || So == S, "scan_origin must have 0 or S points");
   TOTRI_CHECK_ARG(Lo == 0 || Lo == L, "laser_origin must have 0 or L points");
   TOTRI_CHECK_ARG(volume_start && volume_end, "null volume box");
+  TOTRI_CHECK_ARG(T < (1 << 30), "T=%lld too large", (long long)T);
   hipStream_t st = (hipStream_t)stream;
   const size_t nvox = (size_t)D * H * W;
   if (B == 0 || nvox == 0) return TOTRI_OK;
   TOTRI_CHECK_ARG(volume != nullptr, "null volume");
   const BoxGeom g = box_geom(B, D, H, W, T, volume_start, volume_end, bin_width, bin_offset);
-  if (T == 0 || S == 0 || L == 0) {
-    hipLaunchKernelGGL(velten_exhaustive_kernel, dim3(grid1d(nvox), (unsigned)B), dim3(256),
-                       0, st, (const float*)nullptr, scan_points, laser_points, scan_origin,
-                       laser_origin, volume, g, 0, 1, (int)So, (int)Lo, 1);
-    return check_launch("velten_exhaustive_kernel");
+  if (T == 0 || S == 0 || L == 0) {  // no bins or no pairs: a zero volume
+    hipLaunchKernelGGL(zero_kernel, dim3(grid1d(B * nvox)), dim3(256), 0, st, volume,
+                       (size_t)B * nvox);
+    return check_launch("zero_kernel");
   }
   TOTRI_CHECK_ARG(transient && scan_points && laser_points, "null input");
   TOTRI_CHECK_ARG((So == 0 || scan_origin) && (Lo == 0 || laser_origin), "null origin");
-  const size_t need = totri_backprojection_exhaustive_velten_workspace(B, T, S, L);
-  if (workspace_bytes < need)
-    return fail(TOTRI_EWORKSPACE, "workspace too small: need %zu bytes, got %zu", need,
+  const ExLayout lay = ex_layout(B, T, S, L);
+  if (workspace_bytes < lay.end)
+    return fail(TOTRI_EWORKSPACE, "workspace too small: need %zu bytes, got %zu", lay.end,
                 workspace_bytes);
-  TOTRI_CHECK_ARG(workspace != nullptr, "null workspace");
-  float* trT = (float*)workspace;
-  rc = launch_transpose(transient, trT, (int)B, (int)T, (int)(S * L), st);
+  TOTRI_CHECK_ARG(workspace != nullptr && ((uintptr_t)workspace & 15) == 0,
+                  "workspace must be a non-null, 16-byte aligned device pointer");
+  float* rows = (float*)workspace;
+  float* tab = (float*)((char*)workspace + lay.tab_off);
+  const ExRow xr = ex_row((int)T);
+  const int NR = (int)(S * L);
+  rc = transpose_strided(transient, rows, (int)B, (int)T, NR, xr.len, 13, st);
   if (rc) return rc;
-  const int rows = T <= kStageFloats ? (int)(kStageFloats / T) : 0;
-  hipLaunchKernelGGL(velten_exhaustive_kernel, dim3(grid1d(nvox), (unsigned)B), dim3(256),
-                     (size_t)rows * T * sizeof(float), st, trT, scan_points, laser_points,
-                     scan_origin, laser_origin, volume, g, (int)S, (int)L, (int)So, (int)Lo,
-                     rows);
+  const size_t npairs = (size_t)B * NR;
+  hipLaunchKernelGGL(velten_ex_rows_kernel, dim3((unsigned)((npairs + 255) / 256)), dim3(256),
+                     0, st, scan_points, laser_points, scan_origin, laser_origin, g, (int)S,
+                     (int)L, (int)So, (int)Lo, rows);
+  rc = check_launch("velten_ex_rows_kernel");
+  if (rc) return rc;
+  hipLaunchKernelGGL(velten_ex_table_kernel, dim3((unsigned)((T + 3 + 255) / 256)), dim3(256), 0,
+                     st, g, tab);
+  rc = check_launch("velten_ex_table_kernel");
+  if (rc) return rc;
+  const float rw = 1.f / bin_width;
+  const float c0 = -bin_offset * rw + 0.5f;
+  // pairs per LDS stage (beside the table); with fewer than four per stage
+  // the rows and the table are read from global memory
+  int R = (int)std::min<int64_t>(NR, kVeltenStage / xr.len);
+  if (R < 4 && R < NR) R = 0;
+  const dim3 grid(blocks_for(nvox), (unsigned)B);
+  if (R > 0) {
+    const size_t lds = ((size_t)(T + 6) / 4 * 4 + (size_t)R * xr.len) * sizeof(float);
+    hipLaunchKernelGGL(velten_exhaustive_kernel<true>, grid, dim3(256), lds, st, rows, tab,
+                       volume, g, NR, R, rw, c0);
+  } else {
+    hipLaunchKernelGGL(velten_exhaustive_kernel<false>, grid, dim3(256), 0, st, rows, tab,
+                       volume, g, NR, NR, rw, c0);
+  }
   return check_launch("velten_exhaustive_kernel");
 }
 
