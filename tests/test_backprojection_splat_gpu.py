@@ -332,11 +332,9 @@ def test_splat_render_chain_gradient_matches_fd(report):
 
 
 def test_exhaustive_render_repeats_bit_for_bit(report):
-    """R2 adds each wave's voxels into its private LDS histogram with
-    ds_add_f32; lanes of one instruction that hit the same bin are combined
-    in an order the ISA does not specify.  The result is checked to repeat
-    bit for bit over runs on this hardware (a property observed, not one the
-    ISA guarantees; every other kernel here has a fixed summation order)."""
+    """R2 adds each wave's voxels into private LDS histograms of 64-bit
+    fixed-point sums; integer addition does not depend on the order of the
+    atomics, so every run gives the same bits."""
     from totrilib_amd.render.volume import VolumeRenderExhaustive
     from totrilib_amd.types import BinDef, VolumeDef
     rng = np.random.default_rng(58)
@@ -351,6 +349,29 @@ def test_exhaustive_render_repeats_bit_for_bit(report):
     report(f"R2 exhaustive render, 4 runs at 40^3 (20% occupied), 16 x 4 points, 512 bins: "
            f"{'bit-identical' if same else 'differ in the last bits'}")
     assert same
+
+
+def test_exhaustive_render_with_infinite_occupancy():
+    """A batch whose occupancy holds an infinity takes float histograms, so
+    inf and NaN reach the same bins as in the reference; the other batch
+    keeps the fixed-point sums."""
+    from totrilib_amd.render.volume import VolumeRenderExhaustive
+    from totrilib_amd.types import BinDef, VolumeDef
+    rng = np.random.default_rng(61)
+    B, T, S, L = 2, 300, 4, 2
+    res = (16, 14, 18)
+    vol = np.stack([sparse_albedo(res, 0.1, 62 + b) for b in range(B)])
+    vol[0, 5, 6, 7] = np.inf
+    scan = np.concatenate([rng.uniform(-1, 1, (B, 2, S)), np.zeros((B, 1, S))], 1).astype(np.float32)
+    laser = np.concatenate([rng.uniform(-1, 1, (B, 2, L)), np.zeros((B, 1, L))], 1).astype(np.float32)
+    vd, bd = VolumeDef(VS, VE, res), BinDef(0.01, 0.1, T)
+    got = VolumeRenderExhaustive.apply(t(vol), t(scan), t(laser), vd, bd).cpu().numpy()
+    ref = O.r2_forward(vol, scan, laser, T, VS, VE, 0.01, 0.1, mode=O.FP32)
+    assert np.array_equal(np.isnan(got), np.isnan(ref))
+    assert np.array_equal(np.isinf(got), np.isinf(ref)) and np.isinf(ref[0]).any()
+    fin = np.isfinite(ref)
+    assert np.array_equal(np.sign(got[fin]), np.sign(ref[fin]))
+    assert rel_l2(got[fin], ref[fin]) < 1e-6
 
 
 def test_long_transients_beyond_the_lds_stage():

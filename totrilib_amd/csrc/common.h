@@ -41,6 +41,9 @@ int device_cu_count();
 
 // out[b][c][r] = in[b][r][c] for B arrays of R x C floats, on stream st.
 int transpose_batched(const float* in, float* out, int B, int R, int C, hipStream_t st);
+// out[b][c][ooff + r] = in[b][r][c], out rows of stride ldo.
+int transpose_strided(const float* in, float* out, int B, int R, int C, int ldo, int ooff,
+                      hipStream_t st);
 
 // ---------------------------------------------------------------------------
 // Device math

@@ -39,19 +39,37 @@ __global__ void compact_count_kernel(const float* __restrict__ x, int n,
   if (threadIdx.x == 0) counts[(size_t)b * nblocks + blockIdx.x] = red[0];
 }
 
-// Exclusive scan of the per-block counts of one batch (one workgroup per b).
-__global__ void compact_scan_kernel(int* __restrict__ counts, int nblocks,
-                                    int* __restrict__ totals) {
+// Exclusive scan of the per-block counts of one batch (one 256-lane
+// workgroup per b): each lane scans a contiguous run of counts, the run
+// totals are scanned in LDS, and each lane writes its run's offsets.
+constexpr int kCompactScanLanes = 256;
+
+__global__ __launch_bounds__(kCompactScanLanes) void compact_scan_kernel(
+    int* __restrict__ counts, int nblocks, int* __restrict__ totals) {
+  __shared__ int run_tot[kCompactScanLanes];
   const int b = blockIdx.x;
   int* c = counts + (size_t)b * nblocks;
+  const int per = (nblocks + kCompactScanLanes - 1) / kCompactScanLanes;
+  const int lo = min((int)threadIdx.x * per, nblocks), hi = min(lo + per, nblocks);
+  int sum = 0;
+  for (int k = lo; k < hi; ++k) sum += c[k];
+  run_tot[threadIdx.x] = sum;
+  __syncthreads();
   if (threadIdx.x == 0) {
-    int run = 0;
-    for (int k = 0; k < nblocks; ++k) {
-      const int v = c[k];
-      c[k] = run;
-      run += v;
+    int acc = 0;
+    for (int k = 0; k < kCompactScanLanes; ++k) {
+      const int v = run_tot[k];
+      run_tot[k] = acc;
+      acc += v;
     }
-    totals[b] = run;
+    totals[b] = acc;
+  }
+  __syncthreads();
+  int acc = run_tot[threadIdx.x];
+  for (int k = lo; k < hi; ++k) {
+    const int v = c[k];
+    c[k] = acc;
+    acc += v;
   }
 }
 
@@ -110,8 +128,8 @@ int launch_compact(const float* x, int B, int n, const CompactLayout& c, void* w
                      st, x, n, counts, c.nblocks, pred);
   int rc = check_launch("compact_count_kernel");
   if (rc) return rc;
-  hipLaunchKernelGGL(compact_scan_kernel, dim3((unsigned)B), dim3(64), 0, st, counts,
-                     c.nblocks, totals);
+  hipLaunchKernelGGL(compact_scan_kernel, dim3((unsigned)B), dim3(kCompactScanLanes), 0, st,
+                     counts, c.nblocks, totals);
   rc = check_launch("compact_scan_kernel");
   if (rc) return rc;
   hipLaunchKernelGGL(compact_scatter_kernel<Pred>, dim3(c.nblocks, (unsigned)B), dim3(256), 0,
