@@ -180,7 +180,9 @@ int totri_backprojection_confocal_velten_forward(
  * that cannot allocate device memory (code under HIP graph capture, tight
  * serving loops, harnesses that forbid hipMalloc): the same computation and
  * kernels as totri_backprojection_confocal_velten_forward, with the scratch
- * (the transposed transient, B*T*S floats) in a file-scope device array of
+ * (one row per scan point: the transposed transient, the point's bin
+ * thresholds and constants; totri_backprojection_confocal_velten_workspace()
+ * bytes, about 8*B*S*(T+4)) in a file-scope device array of
  * totri_backprojection_confocal_velten_noalloc_max_bytes() bytes (64 MiB)
  * that each GPU the library runs on holds until the process ends.  Larger
  * transients are rejected with TOTRI_EINVAL.  The call allocates nothing; the
