@@ -484,6 +484,17 @@ int totri_debug_a1_walks(int64_t X, int64_t Y, int64_t Z, int64_t N, int64_t T,
 int totri_debug_r1_paths(int64_t X, int64_t Y, int64_t Z, int64_t N, int64_t T,
                          const float volume_origin[3], const float volume_size[3]);
 
+/*
+ * Testing aid with no counterpart in the reference: counts the floats x
+ * whose bit patterns lie in [lo_bits, hi_bits] for which the library's
+ * short correctly rounded square root (used by the exhaustive backprojection,
+ * render and adjoint where every radicand lies in [2^-64, 2^64]) differs
+ * from the compiler's sqrtf, into *mismatches (device memory, 8 bytes,
+ * overwritten).  Asynchronous on `stream`.
+ */
+int totri_debug_sqrt_mid_mismatches(uint32_t lo_bits, uint32_t hi_bits,
+                                    unsigned long long* mismatches, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -127,6 +127,7 @@ SIGNATURES: dict[str, tuple] = {
     "totri_debug_a1_walks": (
         _c_int, [_c_i64, _c_i64, _c_i64, _c_i64, _c_i64, _F3, _F3, _c_f, _c_f]),
     "totri_debug_r1_paths": (_c_int, [_c_i64, _c_i64, _c_i64, _c_i64, _c_i64, _F3, _F3]),
+    "totri_debug_sqrt_mid_mismatches": (_c_int, [ctypes.c_uint32, ctypes.c_uint32, _c_p, _c_p]),
     "totri_debug_confocal_samples": (
         _c_int,
         [_c_p, _c_i64, _c_p, _c_i64, _c_i64, _c_i64, _c_i64, _c_i64, _F3, _F3,

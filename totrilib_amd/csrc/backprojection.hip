@@ -380,7 +380,9 @@ __global__ __launch_bounds__(256) void velten_confocal_kernel(
 // One row of floats per (scan point, laser point) pair r = s * L + l, 16-byte
 // aligned parts:
 //   [0, 4)    p_s, |p_s - o_s|        [4, 8)  p_l, |p_l - o_l|
-//   [8, 12)   1 where the estimate's bound holds for the pair (else 0), 0, 0, 0
+//   [8, 12)   1 where the estimate's bound holds for the pair (else 0),
+//             1 where both points keep every squared distance to the box in
+//             [2^-64, 2^64] (the short correctly rounded root serves), 0, 0
 //   [12, 12 + nv)  its bins, vals[1 + t] = bin t, zeros at 0 and T + 1
 //                  (nv = T + 2 rounded up to a multiple of 4)
 struct ExRow {
@@ -432,10 +434,14 @@ __global__ __launch_bounds__(256) void velten_ex_rows_kernel(
   }
   const float m = (dso + dlo + ds + dl + fabsf(g.off)) / g.w + 1.f;
   const bool est = g.w > 0.f && m < kVeltenMaxBins;  // false for NaN or inf
+  // at least 2^-20 from the box and at most 2^30 from its far corners
+  const float bs = box_sqdist(px, py, pz, g.sx, g.sy, g.sz, g.ex, g.ey, g.ez);
+  const float bl = box_sqdist(lx, ly, lz, g.sx, g.sy, g.sz, g.ex, g.ey, g.ez);
+  const bool mid = bs >= 0x1p-40f && bl >= 0x1p-40f && ds <= 0x1p30f && dl <= 0x1p30f;
   float4* h = (float4*)(rows + id * xr.len);
   h[0] = make_float4(px, py, pz, dso);
   h[1] = make_float4(lx, ly, lz, dlo);
-  h[2] = make_float4(est ? 1.f : 0.f, 0.f, 0.f, 0.f);
+  h[2] = make_float4(est ? 1.f : 0.f, mid ? 1.f : 0.f, 0.f, 0.f);
   rows[id * xr.len + 12] = 0.f;
   rows[id * xr.len + 12 + g.T + 1] = 0.f;
 }
@@ -471,15 +477,18 @@ __global__ __launch_bounds__(256) void velten_ex_table_kernel(BoxGeom g,
   tab[j] = d;
 }
 
-// Bin of one (voxel, pair): D as the reference computes it, then the
-// corrected estimate (EST) or the reference's division; clamped to [-1, T].
-template <bool EST>
+// Bin of one (voxel, pair): D as the reference computes it (MID: with the
+// short correctly rounded root, the same values), then the corrected estimate
+// (EST) or the reference's division; clamped to [-1, T].
+template <bool EST, bool MID>
 __device__ __forceinline__ int velten_ex_bin(const float* row, const float* tab, float qx,
                                              float qy, float qz, float rw, float c0,
                                              const BoxGeom& g) {
   const float4 hs = *(const float4*)row, hl = *(const float4*)(row + 4);
-  const float dsv = dist3(hs.x, hs.y, hs.z, qx, qy, qz);                     // :255
-  const float dlv = dist3(hl.x, hl.y, hl.z, qx, qy, qz);                     // :268
+  const float dsv = MID ? sqrt_rn_mid(sqdist3(hs.x, hs.y, hs.z, qx, qy, qz))
+                        : dist3(hs.x, hs.y, hs.z, qx, qy, qz);                // :255
+  const float dlv = MID ? sqrt_rn_mid(sqdist3(hl.x, hl.y, hl.z, qx, qy, qz))
+                        : dist3(hl.x, hl.y, hl.z, qx, qy, qz);                // :268
   float dist = hs.w + dsv;                                                   // :276-277
   dist = dist + dlv;
   dist = dist + hl.w;
@@ -563,14 +572,16 @@ __global__ __launch_bounds__(256) void velten_exhaustive_kernel(
       const int nu = min(kVeltenUnroll, nr - k);
       bool est = true;
 #pragma unroll
-      for (int u = 0; u < kVeltenUnroll; ++u)
-        est = est && base[(k + min(u, nu - 1)) * xr.len + 8] != 0.f;
+      for (int u = 0; u < kVeltenUnroll; ++u) {
+        const float* r = base + (k + min(u, nu - 1)) * xr.len;
+        est = est && r[8] != 0.f && r[9] != 0.f;
+      }
       if (est) {
         float a[kVeltenUnroll];
 #pragma unroll
         for (int u = 0; u < kVeltenUnroll; ++u) {
           const float* r = base + (k + min(u, nu - 1)) * xr.len;
-          a[u] = r[12 + velten_ex_bin<true>(r, tb, qx, qy, qz, rw, c0, g) + 1];
+          a[u] = r[12 + velten_ex_bin<true, true>(r, tb, qx, qy, qz, rw, c0, g) + 1];
         }
         if (nu == kVeltenUnroll) {
 #pragma unroll
@@ -583,8 +594,8 @@ __global__ __launch_bounds__(256) void velten_exhaustive_kernel(
       } else {
         for (int u = 0; u < nu; ++u) {
           const float* r = base + (k + u) * xr.len;
-          const int t = r[8] != 0.f ? velten_ex_bin<true>(r, tb, qx, qy, qz, rw, c0, g)
-                                    : velten_ex_bin<false>(r, tb, qx, qy, qz, rw, c0, g);
+          const int t = r[8] != 0.f ? velten_ex_bin<true, false>(r, tb, qx, qy, qz, rw, c0, g)
+                                    : velten_ex_bin<false, false>(r, tb, qx, qy, qz, rw, c0, g);
           value += r[12 + t + 1];  // bins -1 and T read the zero pads (:280-282)
         }
       }

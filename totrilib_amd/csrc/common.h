@@ -54,6 +54,27 @@ int transpose_strided(const float* in, float* out, int B, int R, int C, int ldo,
 // -fhip-fp32-correctly-rounded-divide-sqrt).
 __device__ __forceinline__ float sqrt_rn(float x) { return sqrtf(x); }
 
+// Correctly rounded square root for x in [2^-64, 2^64] (sqrt_mid_range):
+// the hardware root r, within 1 ulp, then the same neighbour tests as the
+// compiler's sqrtf expansion (with r- and r+ the floats 1 ulp below and
+// above r: x - r- * r <= 0 picks r-, then x - r+ * r > 0 picks r+, each
+// product-difference one FMA), without the scaling of tiny inputs and the
+// special-value handling that inputs in this range do not need: 9
+// instructions instead of 16.  Equal to sqrt_rn on every float of the range
+// (totri_debug_sqrt_mid_mismatches checks all 1.07e9 of them on the GPU).
+__device__ __forceinline__ bool sqrt_mid_range(float x) {
+  return x >= 0x1p-64f && x <= 0x1p64f;
+}
+
+__device__ __forceinline__ float sqrt_rn_mid(float x) {
+  const float r = __builtin_amdgcn_sqrtf(x);
+  const float rm = __uint_as_float(__float_as_uint(r) - 1u);
+  const float rp = __uint_as_float(__float_as_uint(r) + 1u);
+  float y = __builtin_fmaf(-rm, r, x) <= 0.f ? rm : r;
+  y = __builtin_fmaf(-rp, r, x) > 0.f ? rp : y;
+  return y;
+}
+
 // Hardware square root, at most 1 ulp from the correctly rounded result.
 // Used only on the fast path, behind a guard that falls back to sqrt_rn
 // wherever the 1-ulp difference could change a discrete decision.
@@ -81,6 +102,25 @@ __device__ __forceinline__ float cell_centre(int i, float o, float size, int n) 
 __device__ __forceinline__ float inclusive_coord(int i, int n, float a, float b) {
   float t = (float)i / (float)(n - 1);
   return a + t * (b - a);
+}
+
+// (dx*dx + dy*dy) + dz*dz for d = a - b, the radicand of dist3.
+__device__ __forceinline__ float sqdist3(float ax, float ay, float az, float bx, float by,
+                                         float bz) {
+  const float dx = ax - bx, dy = ay - by, dz = az - bz;
+  float s = dx * dx;
+  s = s + dy * dy;
+  s = s + dz * dz;
+  return s;
+}
+
+// Squared distance from p to the nearest point of the box [lo, hi] (0 inside).
+__device__ __forceinline__ float box_sqdist(float px, float py, float pz, float lx, float ly,
+                                            float lz, float hx, float hy, float hz) {
+  const float dx = fmaxf(fmaxf(lx - px, px - hx), 0.f);
+  const float dy = fmaxf(fmaxf(ly - py, py - hy), 0.f);
+  const float dz = fmaxf(fmaxf(lz - pz, pz - hz), 0.f);
+  return dx * dx + dy * dy + dz * dz;
 }
 
 // norm3df restated as sqrt((dx*dx + dy*dy) + dz*dz), correctly rounded.

@@ -66,12 +66,21 @@ struct Span {
 // Bin span and weights of one voxel for one (scan, laser) pair
 // (volume_cuda.cu:339-374).  With first == last the whole occupancy goes to
 // that bin (wf = occ).
+// MID: every radicand lies in [2^-64, 2^64] (both points keep a distance
+// from the box, sqrt_mid_range), so the short correctly rounded root serves.
+template <bool MID>
+__device__ __forceinline__ float ex_dist(float ax, float ay, float az, float bx, float by,
+                                         float bz) {
+  return MID ? sqrt_rn_mid(sqdist3(ax, ay, az, bx, by, bz)) : dist3(ax, ay, az, bx, by, bz);
+}
+
+template <bool MID>
 __device__ __forceinline__ Span voxel_span(const ExGeom& g, float occ, float xf, float yf,
                                            float zf0, float zf1, float spx, float spy,
                                            float spz, float lpx, float lpy, float lpz) {
   Span r;
-  float d0 = dist3(spx, spy, spz, xf, yf, zf0) + dist3(lpx, lpy, lpz, xf, yf, zf0);
-  float d1 = dist3(spx, spy, spz, xf, yf, zf1) + dist3(lpx, lpy, lpz, xf, yf, zf1);
+  float d0 = ex_dist<MID>(spx, spy, spz, xf, yf, zf0) + ex_dist<MID>(lpx, lpy, lpz, xf, yf, zf0);
+  float d1 = ex_dist<MID>(spx, spy, spz, xf, yf, zf1) + ex_dist<MID>(lpx, lpy, lpz, xf, yf, zf1);
   if (d1 < d0) {
     const float t = d0;
     d0 = d1;
@@ -105,6 +114,29 @@ __device__ __forceinline__ Span voxel_span(const ExGeom& g, float occ, float xf,
   r.wm = frac;
   r.wl = (bf1 - (float)lb) * frac;
   return r;
+}
+
+// Flag of each point of a [B][3][N] array: 1 where every squared distance
+// from it to a point of the volume box lies in [2^-64, 2^64] (it is at least
+// 2^-20 from the box and at most 2^30 from its far corner), so that the
+// exhaustive kernels may take the short correctly rounded root for it.
+__device__ __forceinline__ bool ex_point_mid(const ExGeom& g, float px, float py, float pz) {
+  const float lo = box_sqdist(px, py, pz, g.sx, g.sy, g.sz, g.ex, g.ey, g.ez);
+  float hi = 0.f;
+  for (int c = 0; c < 8; ++c)
+    hi = fmaxf(hi, sqdist3(px, py, pz, (c & 1) ? g.ex : g.sx, (c & 2) ? g.ey : g.sy,
+                           (c & 4) ? g.ez : g.sz));
+  return lo >= 0x1p-40f && hi <= 0x1p60f;  // false for NaN
+}
+
+__global__ __launch_bounds__(256) void ex_point_flags_kernel(const float* __restrict__ pts,
+                                                             int N, ExGeom g,
+                                                             int* __restrict__ flags) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.B * N) return;
+  const int b = i / N, k = i - b * N;
+  const float* p = pts + (size_t)b * 3 * N;
+  flags[i] = ex_point_mid(g, p[k], p[N + k], p[2 * N + k]) ? 1 : 0;
 }
 
 // ---- R2 -------------------------------------------------------------------
@@ -208,6 +240,7 @@ __global__ __launch_bounds__(kR2Waves * 64) void r2_forward_kernel(
               lpz = laser[(size_t)b * 3 * L + 2 * L + l];
   const float4* gb = geo + (size_t)b * nvox;
   const float* ob = occ + (size_t)b * nvox;
+  const bool mid = ex_point_mid(g, spx, spy, spz) && ex_point_mid(g, lpx, lpy, lpz);
   const int c = (n + kR2Waves * 64 - 1) / (kR2Waves * 64);
   const int k0 = (int)threadIdx.x * c, k1 = min(k0 + c, n);
   float4 f = k0 < k1 ? gb[k0] : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -217,7 +250,9 @@ __global__ __launch_bounds__(kR2Waves * 64) void r2_forward_kernel(
     const float4 fn = k + 1 < k1 ? gb[k + 1] : make_float4(0.f, 0.f, 0.f, 0.f);
     const float on = k + 1 < k1 ? ob[k + 1] : 0.f;
     if (k < k1) {
-      const Span sp = voxel_span(g, o, f.x, f.y, f.z, f.w, spx, spy, spz, lpx, lpy, lpz);
+      const Span sp = mid ? voxel_span<true>(g, o, f.x, f.y, f.z, f.w, spx, spy, spz, lpx, lpy, lpz)
+                          : voxel_span<false>(g, o, f.x, f.y, f.z, f.w, spx, spy, spz, lpx, lpy,
+                                              lpz);
       // The span's bins inside this tile, in increasing order: wf at the
       // first bin, wl at the last (when it differs), wm between.
       const int lo = max(sp.first, t0), hi = min(sp.last, t0 + tn - 1);
@@ -271,6 +306,7 @@ template <bool STAGED>
 __global__ __launch_bounds__(256) void r2_adjoint_kernel(
     const float* __restrict__ gT,      // [B][S*L][a5_stride(T)]
     const float* __restrict__ scan, const float* __restrict__ laser,
+    const int* __restrict__ sflag, const int* __restrict__ lflag,  // [B][S], [B][L]
     float* __restrict__ grad,          // [B][D][H][W]
     ExGeom g, int S, int L, int R) {
   extern __shared__ float4 stage4[];
@@ -285,6 +321,8 @@ __global__ __launch_bounds__(256) void r2_adjoint_kernel(
   voxel_faces(g, xi, yi, zi, &xf, &yf, &zf0, &zf1);
   const float* sb = scan + (size_t)b * 3 * S;
   const float* lb = laser + (size_t)b * 3 * L;
+  const int* sf = sflag + (size_t)b * S;
+  const int* lf = lflag + (size_t)b * L;
   const int nrows = S * L;
   const float* gb = gT + (size_t)b * nrows * P;
   float4 buf[kA5Copy4];
@@ -323,8 +361,12 @@ __global__ __launch_bounds__(256) void r2_adjoint_kernel(
     for (int k = 0; k < nr; ++k) {
       const int r = r0 + k;
       const int s = r / L, l = r - s * L;
-      const Span sp = voxel_span(g, 1.f, xf, yf, zf0, zf1, sb[s], sb[S + s], sb[2 * S + s],
-                                 lb[l], lb[L + l], lb[2 * L + l]);
+      const Span sp =
+          sf[s] & lf[l]
+              ? voxel_span<true>(g, 1.f, xf, yf, zf0, zf1, sb[s], sb[S + s], sb[2 * S + s],
+                                 lb[l], lb[L + l], lb[2 * L + l])
+              : voxel_span<false>(g, 1.f, xf, yf, zf0, zf1, sb[s], sb[S + s], sb[2 * S + s],
+                                  lb[l], lb[L + l], lb[2 * L + l]);
       if (sp.first > sp.last) continue;
       const float* row = src + k * P;
       if (sp.first == sp.last) {
@@ -463,7 +505,9 @@ extern "C" int totri_volume_render_exhaustive_forward(
 extern "C" size_t totri_volume_render_exhaustive_grad_volume_workspace(int64_t B, int64_t T,
                                                                        int64_t S, int64_t L) {
   if (B <= 0 || T < 0 || S <= 0 || L <= 0) return 0;
-  return (size_t)B * S * L * a5_stride((int)T) * sizeof(float);
+  // the transposed gradient rows, then the point flags of the scan and laser points
+  return align256((size_t)B * S * L * a5_stride((int)T) * sizeof(float)) +
+         align256((size_t)B * (S + L) * sizeof(int));
 }
 
 extern "C" int totri_volume_render_exhaustive_grad_volume(
@@ -484,7 +528,8 @@ extern "C" int totri_volume_render_exhaustive_grad_volume(
   const dim3 grid((unsigned)((nvox + 255) / 256), (unsigned)B);
   if (T == 0 || S == 0 || L == 0) {  // no bins or no pairs: a zero gradient
     hipLaunchKernelGGL(r2_adjoint_kernel<false>, grid, dim3(256), 0, st, (const float*)nullptr,
-                       scan_points, laser_points, volume_grad, g, 0, 1, 1);
+                       scan_points, laser_points, (const int*)nullptr, (const int*)nullptr,
+                       volume_grad, g, 0, 1, 1);
     return check_launch("r2_adjoint_kernel");
   }
   TOTRI_CHECK_ARG(transient_grad && scan_points && laser_points, "null input");
@@ -496,17 +541,61 @@ extern "C" int totri_volume_render_exhaustive_grad_volume(
                   "workspace must be a non-null, 16-byte aligned device pointer");
   float* gT = (float*)workspace;
   const int P = a5_stride((int)T);
+  int* sflag = (int*)((char*)workspace + align256((size_t)B * S * L * P * sizeof(float)));
+  int* lflag = sflag + B * S;
   // [B][T][S*L] -> [B][S*L][P]
   rc = transpose_strided(transient_grad, gT, (int)B, (int)T, (int)(S * L), P, 0, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(ex_point_flags_kernel, dim3((unsigned)((B * S + 255) / 256)), dim3(256), 0,
+                     st, scan_points, (int)S, g, sflag);
+  rc = check_launch("ex_point_flags_kernel");
+  if (rc) return rc;
+  hipLaunchKernelGGL(ex_point_flags_kernel, dim3((unsigned)((B * L + 255) / 256)), dim3(256), 0,
+                     st, laser_points, (int)L, g, lflag);
+  rc = check_launch("ex_point_flags_kernel");
   if (rc) return rc;
   // rows per LDS stage; rows longer than the stage are read from global memory
   const int R = (int)std::min<int64_t>(S * L, kStage / P);
   if (R > 0) {
     hipLaunchKernelGGL(r2_adjoint_kernel<true>, grid, dim3(256), (size_t)R * P * sizeof(float),
-                       st, gT, scan_points, laser_points, volume_grad, g, (int)S, (int)L, R);
+                       st, gT, scan_points, laser_points, sflag, lflag, volume_grad, g, (int)S,
+                       (int)L, R);
   } else {
     hipLaunchKernelGGL(r2_adjoint_kernel<false>, grid, dim3(256), 0, st, gT, scan_points,
-                       laser_points, volume_grad, g, (int)S, (int)L, (int)(S * L));
+                       laser_points, sflag, lflag, volume_grad, g, (int)S, (int)L, (int)(S * L));
   }
   return check_launch("r2_adjoint_kernel");
+}
+
+namespace totri_hip {
+namespace {
+
+__global__ void sqrt_mid_check_kernel(uint32_t lo, uint32_t hi,
+                                      unsigned long long* __restrict__ count) {
+  unsigned long long bad = 0;
+  const uint64_t n = (uint64_t)hi - lo + 1;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x) {
+    const float x = __uint_as_float((uint32_t)(lo + i));
+    bad += __float_as_uint(sqrt_rn_mid(x)) != __float_as_uint(sqrt_rn(x)) ? 1 : 0;
+  }
+  for (int m = 32; m >= 1; m >>= 1) bad += __shfl_xor(bad, m, 64);
+  if ((threadIdx.x & 63) == 0 && bad) atomicAdd(count, bad);
+}
+
+__global__ void zero_u64_kernel(unsigned long long* p) { *p = 0ull; }
+
+}  // namespace
+}  // namespace totri_hip
+
+extern "C" int totri_debug_sqrt_mid_mismatches(uint32_t lo_bits, uint32_t hi_bits,
+                                               unsigned long long* mismatches, void* stream) {
+  TOTRI_CHECK_ARG(mismatches != nullptr && lo_bits <= hi_bits, "invalid arguments");
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(1), 0, st, mismatches);
+  int rc = check_launch("zero_u64_kernel");
+  if (rc) return rc;
+  hipLaunchKernelGGL(sqrt_mid_check_kernel, dim3(device_cu_count() * 16), dim3(256), 0, st,
+                     lo_bits, hi_bits, mismatches);
+  return check_launch("sqrt_mid_check_kernel");
 }
