@@ -380,49 +380,68 @@ __global__ __launch_bounds__(256) void splat_grad_kernel(
 #pragma unroll
   for (int c = 0; c < 4 + kMaxAttr; ++c) acc[c] = 0.f;
   const float si = 1.f / sg;                                   // :38
+  // Voxel coordinates of the support box, one table per axis per wave when
+  // each side has at most 64 voxels (written and read by the same wave).
+  __shared__ float ctab[4][3][64];
+  const bool tabbed = nx <= 64 && ny <= 64 && nz <= 64;
+  if (tabbed) {
+    ctab[wave][0][lane] = splat_coord(xa + min(lane, nx - 1), g.W, g.sx, g.ex);
+    ctab[wave][1][lane] = splat_coord(ya + min(lane, ny - 1), g.H, g.sy, g.ey);
+    ctab[wave][2][lane] = splat_coord(za + min(lane, nz - 1), g.D, g.sz, g.ez);
+  }
   if (nx > 0 && ny > 0 && nz > 0) {
     const int total = nx * ny * nz;
+    // lane offsets (xo, yo, zo) of entry e = xo + nx (yo + ny zo), advanced
+    // by 64 entries per step as a mixed-radix addition
+    int xo = lane % nx, yo = (lane / nx) % ny, zo = lane / (nx * ny);
+    const int r1 = 64 % nx, q1 = 64 / nx, r2 = q1 % ny, q2 = q1 / ny;
     for (int e = lane; e < total; e += 64) {
-      const int xi = xa + e % nx;
-      const int yi = ya + (e / nx) % ny;
-      const int zi = za + e / (nx * ny);
-      const float x = splat_coord(xi, g.W, g.sx, g.ex);
-      const float y = splat_coord(yi, g.H, g.sy, g.ey);
-      const float z = splat_coord(zi, g.D, g.sz, g.ez);
-      const float r2 = rbf_r2(x, y, z, xr, yr, zr);
-      if (!within(r2, sg, g.trunc)) continue;
-      const float wgt = gaussian(r2, sg);                     // :150
-      const size_t vi = ((size_t)zi * g.H + yi) * g.W + xi;
-      const float dl0 = g0b[vi];                               // :151
-      const float dxr = (((x - xr) * si) * si) * wgt;          // :39-41
-      const float dyr = (((y - yr) * si) * si) * wgt;
-      const float dzr = (((z - zr) * si) * si) * wgt;
-      const float dsr = (((r2 * si) * si) * si) * wgt;         // :42-46
-      if (g.A == 0) {                                          // :159-163
-        acc[0] += dl0 * dxr;
-        acc[1] += dl0 * dyr;
-        acc[2] += dl0 * dzr;
-        acc[3] += dl0 * dsr;
-      } else {
-        float v0 = v0b[vi];                                    // :165
-        if (v0 < 1.e-8f) v0 = 1.f;                             // :167-169
-        const float inv = 1.f / v0;                            // :170
-        float sum = 0.f;
+      const int xi = xa + xo, yi = ya + yo, zi = za + zo;
+      const float x = tabbed ? ctab[wave][0][xo] : splat_coord(xi, g.W, g.sx, g.ex);
+      const float y = tabbed ? ctab[wave][1][yo] : splat_coord(yi, g.H, g.sy, g.ey);
+      const float z = tabbed ? ctab[wave][2][zo] : splat_coord(zi, g.D, g.sz, g.ez);
+      const float r2v = rbf_r2(x, y, z, xr, yr, zr);
+      if (within(r2v, sg, g.trunc)) {
+        const float wgt = gaussian(r2v, sg);                   // :150
+        const size_t vi = ((size_t)zi * g.H + yi) * g.W + xi;
+        const float dl0 = g0b[vi];                             // :151
+        const float dxr = (((x - xr) * si) * si) * wgt;        // :39-41
+        const float dyr = (((y - yr) * si) * si) * wgt;
+        const float dzr = (((z - zr) * si) * si) * wgt;
+        const float dsr = (((r2v * si) * si) * si) * wgt;      // :42-46
+        if (g.A == 0) {                                        // :159-163
+          acc[0] += dl0 * dxr;
+          acc[1] += dl0 * dyr;
+          acc[2] += dl0 * dzr;
+          acc[3] += dl0 * dsr;
+        } else {
+          float v0 = v0b[vi];                                  // :165
+          if (v0 < 1.e-8f) v0 = 1.f;                           // :167-169
+          const float inv = 1.f / v0;                          // :170
+          float sum = 0.f;
 #pragma unroll
-        for (int a = 0; a < kMaxAttr; ++a) {
-          if (a < g.A) {
-            const float at = v0b[(size_t)(1 + a) * V + vi];
-            const float dla = g0b[(size_t)(1 + a) * V + vi];
-            sum += dla * (av[a] - at);                         // :176
-            acc[4 + a] += (dla * wgt) * inv;                   // :178
+          for (int a = 0; a < kMaxAttr; ++a) {
+            if (a < g.A) {
+              const float at = v0b[(size_t)(1 + a) * V + vi];
+              const float dla = g0b[(size_t)(1 + a) * V + vi];
+              sum += dla * (av[a] - at);                       // :176
+              acc[4 + a] += (dla * wgt) * inv;                 // :178
+            }
           }
+          sum = sum * inv;                                     // :180
+          acc[0] += dxr * (dl0 + sum);                         // :181-184
+          acc[1] += dyr * (dl0 + sum);
+          acc[2] += dzr * (dl0 + sum);
+          acc[3] += dsr * (dl0 + sum);
         }
-        sum = sum * inv;                                       // :180
-        acc[0] += dxr * (dl0 + sum);                           // :181-184
-        acc[1] += dyr * (dl0 + sum);
-        acc[2] += dzr * (dl0 + sum);
-        acc[3] += dsr * (dl0 + sum);
       }
+      xo += r1;
+      const int c1 = xo >= nx ? 1 : 0;
+      xo -= c1 * nx;
+      yo += r2 + c1;
+      const int c2 = yo >= ny ? 1 : 0;
+      yo -= c2 * ny;
+      zo += q2 + c2;
     }
   }
   float* out = rbf_grad + ((size_t)b * g.N + i) * C;
