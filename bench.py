@@ -473,6 +473,10 @@ def serve_main(args):
 
     base_t, base_lat = per_request_baseline(torch, dev, reqs)
     print(f"[serve] per-request baseline {base_t:.2f} s", file=sys.stderr, flush=True)
+    # the smaller and larger halves of the mix by pairs, for per-size latency
+    cost = np.array([request_pairs(r) for r in reqs], dtype=np.float64)
+    large = cost > np.median(cost)
+    service = np.asarray(base_lat, dtype=np.float64) * 1e3
 
     def serve(arrivals, eng=eng):
         steps0 = eng.steps
@@ -483,9 +487,17 @@ def serve_main(args):
         lat = np.array([tk.latency for tk in tks]) * 1e3
         assert all(tk.error is None for tk in tks), [tk.error for tk in tks if tk.error]
         jps = eng.jobs_per_step[jobs0:]
+        slow = lat / service  # latency over the request's own service time
         return dict(span=t_end - t0, p50_ms=float(np.percentile(lat, 50)),
                     p99_ms=float(np.percentile(lat, 99)), steps=eng.steps - steps0,
-                    mean_jobs_per_step=float(np.mean(jps)) if jps else 0.0)
+                    mean_jobs_per_step=float(np.mean(jps)) if jps else 0.0,
+                    small_p50_ms=float(np.percentile(lat[~large], 50)),
+                    small_p99_ms=float(np.percentile(lat[~large], 99)),
+                    large_p50_ms=float(np.percentile(lat[large], 50)),
+                    large_p99_ms=float(np.percentile(lat[large], 99)),
+                    p99_request_large=bool(large[int(np.argmax(lat))]),
+                    slowdown_p50=float(np.percentile(slow, 50)),
+                    slowdown_p99=float(np.percentile(slow, 99)))
 
     sat = serve(np.zeros(len(reqs)))
     print(f"[serve] saturated {sat['span']:.2f} s", file=sys.stderr, flush=True)
@@ -502,7 +514,14 @@ def serve_main(args):
                           pairs_per_s=pairs / r["span"], steps=r["steps"],
                           mean_jobs_per_step=r["mean_jobs_per_step"],
                           fifo_p50_ms=rf["p50_ms"], fifo_p99_ms=rf["p99_ms"],
-                          fifo_pairs_per_s=pairs / rf["span"]))
+                          fifo_pairs_per_s=pairs / rf["span"],
+                          by_size={k: r[k] for k in ("small_p50_ms", "small_p99_ms",
+                                                     "large_p50_ms", "large_p99_ms",
+                                                     "p99_request_large", "slowdown_p50",
+                                                     "slowdown_p99")},
+                          fifo_by_size={k: rf[k] for k in ("small_p50_ms", "small_p99_ms",
+                                                           "large_p50_ms", "large_p99_ms",
+                                                           "slowdown_p50", "slowdown_p99")}))
         print(f"[serve] load {load}: p50 {r['p50_ms']:.1f} ms p99 {r['p99_ms']:.1f} ms "
               f"(fifo p50 {rf['p50_ms']:.1f} p99 {rf['p99_ms']:.1f})",
               file=sys.stderr, flush=True)

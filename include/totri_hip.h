@@ -332,6 +332,15 @@ int totri_rbf2volume_gaussian_grad(const float* rbf, int64_t B, int64_t N, int64
  * each batch_prepare into the same host table.  The host table must not be
  * rewritten until the previous step on it has finished its copy (wait for the
  * stream).  Job buffers must not overlap each other or the table.
+ *
+ * Slices: a job with slices = S > 1 runs over several steps, slices 0 .. S-1
+ * in that order, a step running slice_run consecutive slices from `slice`
+ * (later steps must not start before earlier ones finish, e.g. the same
+ * stream), with the same description and untouched workspace between them.
+ * Each slice is a consecutive 1/S of the job's work items; the mirror
+ * detection and tables run with slice 0 and the output is complete after
+ * slice S-1, bit-identical to the unsliced job.  So a long job can share its
+ * steps with requests that arrive while it runs.
  */
 #define TOTRI_JOB_RENDER 0      /* R1: volume (input) -> transient (output)      */
 #define TOTRI_JOB_BACKPROJECT 1 /* A1: transient (input) -> volume (output)      */
@@ -339,7 +348,7 @@ int totri_rbf2volume_gaussian_grad(const float* rbf, int64_t B, int64_t N, int64
 
 typedef struct totri_confocal_job {
   int kind;                         /* TOTRI_JOB_RENDER or TOTRI_JOB_BACKPROJECT */
-  int reserved;                     /* 0 */
+  int slice;                        /* this step's slice, 0 <= slice < slices */
   float* volume;                    /* [X][Y][Z] FP32 */
   const float* measurement_points;  /* [3][N] FP32 */
   float* transient;                 /* [N][T] FP32 */
@@ -347,6 +356,9 @@ typedef struct totri_confocal_job {
   float volume_origin[3];
   float volume_size[3];
   float bin_width, bin_offset;
+  int slices;                       /* 0 or 1: the whole job in this step */
+  int slice_run;                    /* slices this step runs, from `slice`: 0 or 1
+                                       for one, up to slices - slice */
   void* workspace;                  /* >= totri_confocal_job_workspace(job) */
   size_t workspace_bytes;
 } totri_confocal_job;

@@ -126,16 +126,26 @@ def test_requests_cover_every_path():
         assert want in got, (want, sorted(got))
 
 
-@pytest.mark.parametrize("graphs", [True, False])
-def test_batched_step_matches_single_request_entry_points(graphs):
+@pytest.mark.parametrize("graphs,sliced", [(True, None), (False, None), (True, "greedy"),
+                                           (True, "one")])
+def test_batched_step_matches_single_request_entry_points(graphs, sliced):
+    """Every request of the mix through batched steps equals the single-request
+    entry point bit for bit.  "greedy" and "one" run the larger requests in
+    slices of their work items over several steps (slice cost 20 us of
+    estimated work: up to 9 slices per request here); "greedy" lets a step
+    that would run alone take all remaining slices, "one" takes one slice per
+    step."""
     from totrilib_amd.serve import ServingEngine
     reqs = make_requests()
     eng = ServingEngine(device_budget=1 << 30, host_budget=256 << 20, lanes=2,
-                        use_graphs=graphs)
+                        use_graphs=graphs, slice_cost=2e-5 if sliced else None,
+                        greedy_slices=sliced != "one")
     tickets = [eng.submit(r) for r in reqs]
     eng.drain()
     got = collect(tickets)
     assert eng.steps >= 1
+    if sliced == "one":
+        assert eng.steps > len(eng.lanes) * 2 and max(eng.jobs_per_step) > 1
     for k, (r, g) in enumerate(zip(reqs, got)):
         want = single_request(r)
         assert torch.equal(g, want), (k, float((g - want).abs().max()))
@@ -213,6 +223,28 @@ def test_shortest_job_first_order():
             assert ts.t_done < tb.t_done
         else:
             assert eng.jobs_per_step == [2]
+
+
+def test_sliced_request_shares_its_steps_with_a_later_short_one():
+    """With one lane, a request in slices leaves room for a short request
+    submitted after it: the short one joins one of its steps and finishes
+    first; both results equal the single-request entry points."""
+    from totrilib_amd.serve import ServingEngine, request_cost
+    reqs = make_requests(seed=4)
+    small, big = reqs[0], reqs[3]          # 16x16x40 render, 64^3 backprojection
+    slice_cost = request_cost(big) / 5.5   # six slices
+    assert request_cost(small) < slice_cost
+    eng = ServingEngine(device_budget=1 << 30, host_budget=256 << 20, lanes=1,
+                        slice_cost=slice_cost, group_ratio=None, greedy_slices=False)
+    tb = eng.submit(big)
+    eng.poll()                             # the first slice starts alone
+    ts = eng.submit(small)
+    eng.drain()
+    got = collect([tb, ts])
+    assert torch.equal(got[0], single_request(big))
+    assert torch.equal(got[1], single_request(small))
+    assert eng.jobs_per_step[0] == 1 and eng.steps == 6 and max(eng.jobs_per_step) == 2
+    assert ts.t_done < tb.t_done
 
 
 def test_resident_inputs_wait_for_the_submitting_stream():

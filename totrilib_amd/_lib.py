@@ -143,7 +143,7 @@ class ConfocalJob(ctypes.Structure):
     """totri_confocal_job of include/totri_hip.h (one request of a batched step)."""
     _fields_ = [
         ("kind", ctypes.c_int),
-        ("reserved", ctypes.c_int),
+        ("slice", ctypes.c_int),
         ("volume", _c_p),
         ("measurement_points", _c_p),
         ("transient", _c_p),
@@ -152,6 +152,8 @@ class ConfocalJob(ctypes.Structure):
         ("volume_size", _F3),
         ("bin_width", _c_f),
         ("bin_offset", _c_f),
+        ("slices", ctypes.c_int),
+        ("slice_run", ctypes.c_int),
         ("workspace", _c_p),
         ("workspace_bytes", _c_sz),
     ]

@@ -158,7 +158,10 @@ class Ticket:
 
 
 class _Job:
-    """An admitted request: its arena blocks and the C job description."""
+    """An admitted request: its arena blocks and the C job description.  A
+    job of `slices` > 1 runs over one or more steps, all on one lane (`lane`,
+    set at its first step); `slice` is the next slice to run and `run` the
+    number of slices its current step runs."""
 
     def __init__(self, ticket: Ticket, dev_blocks: List[int], host_block: int):
         self.ticket = ticket
@@ -169,6 +172,17 @@ class _Job:
         self.out_dev: Optional[torch.Tensor] = None
         self.out_host: Optional[torch.Tensor] = None
         self.cjob = _lib.ConfocalJob()
+        self.slices = 1
+        self.slice = 0
+        self.run = 1
+        self.lane: Optional[int] = None
+
+    def step_cost(self) -> float:
+        """Estimated service time of one slice."""
+        return self.ticket.cost / self.slices
+
+    def remaining_cost(self) -> float:
+        return self.ticket.cost * (self.slices - self.slice) / self.slices
 
 
 class _Lane:
@@ -194,16 +208,31 @@ class ServingEngine:
     policy         "sjf" (shortest job first with aging) or "fifo"
     group_ratio    under "sjf", a step takes only jobs at most this many times
                    as costly as its first (None: no limit)
+    slice_cost     under "sjf", a request estimated to take longer than this
+                   (seconds) runs in slices of about this cost over several
+                   steps, each a consecutive share of its work items, so that
+                   requests arriving meanwhile join the next step instead of
+                   waiting for the whole request; results are bit-identical
+                   to the unsliced request (None: no slicing)
+    greedy_slices  a sliced request whose step would run alone (nothing else
+                   admitted or queued, every other lane idle) runs all its
+                   remaining slices in that step: an idle lane serves
+                   arrivals meanwhile
     """
 
     def __init__(self, device_budget: int = 16 << 30, host_budget: int = 4 << 30,
                  lanes: int = 2, max_jobs: int = _lib.TOTRI_BATCH_MAX_JOBS,
                  use_graphs: bool = True, device: Optional[torch.device] = None,
-                 policy: str = "sjf", group_ratio: Optional[float] = 4.0):
+                 policy: str = "sjf", group_ratio: Optional[float] = 4.0,
+                 slice_cost: Optional[float] = 0.03, greedy_slices: bool = True):
         if policy not in ("sjf", "fifo"):
             raise ValueError(f"policy must be 'sjf' or 'fifo', not {policy!r}")
+        if slice_cost is not None and not slice_cost > 0:
+            raise ValueError("slice_cost must be positive or None")
         self.policy = policy
         self.group_ratio = group_ratio
+        self.slice_cost = slice_cost if policy == "sjf" else None
+        self.greedy_slices = greedy_slices
         if not torch.cuda.is_available():
             raise RuntimeError("ServingEngine needs a GPU")
         self.lib = _lib.load()
@@ -284,11 +313,12 @@ class ServingEngine:
         sizes.append(ws)
         return sizes, out, ws
 
-    def _key(self, tk: Ticket, now: float):
-        """Priority of a ticket (smaller first)."""
+    def _key(self, tk: Ticket, now: float, cost: Optional[float] = None):
+        """Priority of a ticket (smaller first): under "sjf" its remaining
+        estimated cost less the time it has waited."""
         if self.policy == "fifo":
             return tk.id
-        return tk.cost - (now - tk.t_submit)
+        return (tk.cost if cost is None else cost) - (now - tk.t_submit)
 
     def _admit(self, now: float) -> None:
         if not self.queue:
@@ -335,9 +365,13 @@ class ServingEngine:
         job.out_dev = self.dev.view(blocks[k], out_shape)
         job.out_host = self.host.view(hb, out_shape)
         ws_ptr = self.dev.ptr(blocks[k + 1]) if ws > 0 else None
+        if self.slice_cost is not None and tk.cost > self.slice_cost:
+            job.slices = min(int(math.ceil(tk.cost / self.slice_cost)), 256)
         cj = job.cjob
         cj.kind = req.kind
-        cj.reserved = 0
+        cj.slice = 0
+        cj.slices = job.slices
+        cj.slice_run = 1
         cj.measurement_points = mp_dev.data_ptr()
         if isinstance(req, RenderRequest):
             cj.volume = data_dev.data_ptr()
@@ -358,16 +392,31 @@ class ServingEngine:
 
     # -- steps --------------------------------------------------------------
     def _launch(self, lane: _Lane, now: float) -> None:
+        li = self.lanes.index(lane)
         jobs = []
-        for j in sorted(self.admitted, key=lambda j: self._key(j.ticket, now)):
+        # a sliced job continues on the lane of its first slice
+        cands = [j for j in self.admitted if j.lane is None or j.lane == li]
+        for j in sorted(cands, key=lambda j: self._key(j.ticket, now, j.remaining_cost())):
             if len(jobs) == self.max_jobs:
                 break
             if (self.policy == "sjf" and self.group_ratio is not None and jobs
-                    and j.ticket.cost > self.group_ratio * jobs[0].ticket.cost):
+                    and j.step_cost() > self.group_ratio * jobs[0].step_cost()):
                 continue
             jobs.append(j)
+        if not jobs:
+            return
         chosen = set(id(j) for j in jobs)
         self.admitted = deque(j for j in self.admitted if id(j) not in chosen)
+        # A sliced job runs one slice when other work waits or another lane is
+        # busy (requests arriving meanwhile then join its next step), and all
+        # its remaining slices otherwise (an idle lane serves new arrivals).
+        contended = (not self.greedy_slices or bool(self.queue) or bool(self.admitted)
+                     or any(l.event is not None for l in self.lanes if l is not lane))
+        for j in jobs:
+            j.lane = li
+            j.run = 1 if contended else j.slices - j.slice
+            j.cjob.slice = j.slice
+            j.cjob.slice_run = j.run
         arr = (_lib.ConfocalJob * len(jobs))(*[j.cjob for j in jobs])
         rc = self.lib.totri_confocal_batch_prepare(arr, len(jobs), lane.host_table.data_ptr(),
                                                    self.table_bytes)
@@ -380,8 +429,9 @@ class ServingEngine:
                 if j.ticket._ready is not None:
                     st.wait_event(j.ticket._ready)
             for j in jobs:
-                for dst, src in j.copies_in:
-                    dst.copy_(src, non_blocking=True)
+                if j.slice == 0:
+                    for dst, src in j.copies_in:
+                        dst.copy_(src, non_blocking=True)
             if self.use_graphs:
                 if lane.graph is None:
                     lane.graph = torch.cuda.CUDAGraph()
@@ -391,7 +441,8 @@ class ServingEngine:
             else:
                 self._run_step(lane)
             for j in jobs:
-                j.out_host.copy_(j.out_dev, non_blocking=True)
+                if j.slice + j.run == j.slices:
+                    j.out_host.copy_(j.out_dev, non_blocking=True)
             ev = torch.cuda.Event()
             ev.record(st)
         lane.jobs = jobs
@@ -408,6 +459,10 @@ class ServingEngine:
 
     def _retire(self, lane: _Lane, now: float) -> None:
         for j in lane.jobs:
+            if j.slice + j.run < j.slices:  # more slices to run: back to the admitted jobs
+                j.slice += j.run
+                self.admitted.append(j)
+                continue
             for b in j.dev_blocks:
                 self.dev.arena.free(b)
             tk = j.ticket
