@@ -181,22 +181,52 @@ __global__ __launch_bounds__(256) void velten_bins_kernel(
   } else if (j == n - 1) {
     e = __builtin_nanf("");
   } else {
+    // E_k: the smallest e >= 0 (as a bit pattern; monotone for non-negative
+    // floats) with velten_bin(e) >= k.  A guess from the bin's edge in
+    // double, ((k - 1/2) w + off - dso2)^2 / 4, is a few ulps from it: walk
+    // from there while the neighbour agrees, in at most 64 steps, else bisect
+    // over all patterns.  +inf (0x7f800000) always qualifies.
     const int k = j - 1;
-    // smallest e >= 0 (as a bit pattern; monotone for non-negative floats)
-    // with velten_bin(e) >= k; +inf (0x7f800000) always qualifies
-    unsigned lo = 0u, hi = 0x7f800000u;
-    if (velten_bin(0.f, dso2, g.w, g.off) >= k) {
-      hi = 0u;
-    } else {
-      while (hi - lo > 1u) {
-        const unsigned mid = lo + (hi - lo) / 2u;
-        if (velten_bin(__uint_as_float(mid), dso2, g.w, g.off) >= k)
-          hi = mid;
-        else
-          lo = mid;
+    const double dk = ((double)k - 0.5) * (double)g.w + (double)g.off - (double)dso2;
+    const double gd = dk > 0.0 ? 0.25 * dk * dk : 0.0;
+    const float gf = gd < 3.0e38 ? (float)gd : INFINITY;
+    unsigned u = __float_as_uint(gf);  // gf >= 0 or NaN (NaN: bisect below)
+    bool found = false;
+    if (gf == gf) {
+      if (velten_bin(__uint_as_float(u), dso2, g.w, g.off) >= k) {
+        for (int it = 0; it < 64; ++it) {  // step down to the first pattern that qualifies
+          if (u == 0u || velten_bin(__uint_as_float(u - 1u), dso2, g.w, g.off) < k) {
+            found = true;
+            break;
+          }
+          --u;
+        }
+      } else {
+        for (int it = 0; it < 64 && u < 0x7f800000u; ++it) {
+          ++u;
+          if (velten_bin(__uint_as_float(u), dso2, g.w, g.off) >= k) {
+            found = true;
+            break;
+          }
+        }
       }
     }
-    e = __uint_as_float(hi);
+    if (!found) {
+      unsigned lo = 0u, hi = 0x7f800000u;
+      if (velten_bin(0.f, dso2, g.w, g.off) >= k) {
+        hi = 0u;
+      } else {
+        while (hi - lo > 1u) {
+          const unsigned mid = lo + (hi - lo) / 2u;
+          if (velten_bin(__uint_as_float(mid), dso2, g.w, g.off) >= k)
+            hi = mid;
+          else
+            lo = mid;
+        }
+      }
+      u = hi;
+    }
+    e = __uint_as_float(u);
   }
   row[4 + vr.nv + j] = e;
   if (j == 0) {
