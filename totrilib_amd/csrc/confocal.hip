@@ -214,7 +214,7 @@ __device__ __forceinline__ void r1_block(
     int chunks_per_group, float2* __restrict__ lds2) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int i = sgrp * kWaves + wave;
+  const int i = uniform_i32(sgrp * kWaves + wave);  // scalar registers
   const bool scan_ok = i < g.N;
   const int n_cols = g.X * g.Y;
   const int chunk_vox = chunk_cols * g.Z;
@@ -667,7 +667,7 @@ __device__ __forceinline__ void r1_sym_block(
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   static_assert(M == 4 || M == 2, "quads or x-pairs");
-  const int q = qgrp * kWaves + wave;
+  const int q = uniform_i32(qgrp * kWaves + wave);  // scalar registers
   const bool quad_ok = q < n_quads;
   const int HX = g.X / 2, HY = M == 4 ? g.Y / 2 : g.Y;
   const int n_reps = HX * HY;
@@ -906,7 +906,7 @@ __device__ __forceinline__ void a6_sym_block(
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   static_assert(M == 4 || M == 2, "quads or x-pairs");
-  const int q = qgrp * kWaves + wave;
+  const int q = uniform_i32(qgrp * kWaves + wave);  // scalar registers
   const bool quad_ok = q < n_quads;
   const int HX = g.X / 2, HY = M == 4 ? g.Y / 2 : g.Y;
   const int n_reps = HX * HY;
@@ -1313,7 +1313,7 @@ __global__ __launch_bounds__(kWaves * 64) void a6_kernel(
   const int wave = threadIdx.x >> 6;
   const int grp = blockIdx.x % n_groups;
   const int sgrp = blockIdx.x / n_groups;
-  const int i = sgrp * kWaves + wave;
+  const int i = uniform_i32(sgrp * kWaves + wave);  // scalar registers
   const bool scan_ok = i < g.N;
   const int n_cols = g.X * g.Y;
   const int chunk_vox = chunk_cols * g.Z;
@@ -2495,7 +2495,9 @@ void a1_row_kernel(
   float* tab = lds1;
   float* acc = lds1 + (TAB ? kRowTabMaxT : 0) + (threadIdx.x >> 6) * (kSlabG + 1) * 64;
   if (TAB) fill_tc2_table(tab, g);
-  const int item = (int)blockIdx.x * kRowWaves + (int)(threadIdx.x >> 6);
+  // wave-uniform (scalar registers): the scan loop, its row pointers and
+  // the wall-point loads derived from it stay scalar
+  const int item = uniform_i32((int)blockIdx.x * kRowWaves + (int)(threadIdx.x >> 6));
   if (item >= n_items) return;  // whole wave; no barrier follows
   a1_row_item<SAFE, TAB>(Kf, itc4t, mp, dst, bnd, g, item, tiles_x, tiles_y, n_slabs,
                          n_ranges, t_stride, acc, tab);
