@@ -524,7 +524,7 @@ __global__ __launch_bounds__(kGradWaves * 64) void splat_grad_all_kernel(
   const int b = blockIdx.y;
   const int A = AT >= 0 ? AT : g.A;
   const int C = 4 + A;
-  const int i0 = (blockIdx.x * kGradWaves + wave) * RPW;
+  const int i0 = uniform_i32((blockIdx.x * kGradWaves + wave) * RPW);  // scalar registers
   // this wave's RBFs (indices past N carry sigma 1 and are not written)
   float xr[RPW], yr[RPW], zr[RPW], si2[RPW], si3[RPW], nh[RPW], av[RPW][AN > 0 ? AN : 1];
 #pragma unroll
