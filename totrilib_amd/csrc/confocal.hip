@@ -150,6 +150,23 @@ __device__ __forceinline__ int decide_fast(const ConfGeom& g, float d2, float cf
   return floor_to_int(zh);
 }
 
+// The fast decision as a row key: z_hat itself, whose floor is the candidate
+// row, with alpha and `near` as in decide_fast.  The row walks test "row <=
+// rr" as key < rr + 1 and so need no float-to-int conversion per step.
+__device__ __forceinline__ float decide_fast_key(const ConfGeom& g, float d2, float cfast,
+                                                 float half_m_delta, float* alpha,
+                                                 bool* near) {
+  const float zh = fmaf(sqrt_fast(d2), g.s, cfast);
+  const float al = __builtin_amdgcn_fractf(zh);
+  *alpha = al;
+  *near = fabsf(al - 0.5f) > half_m_delta;
+  return zh;
+}
+
+// Row key of row r: floor(r + 1/2) == r for |r| < 2^23; larger rows keep a
+// key above every row a walk compares with.
+__device__ __forceinline__ float row_key(int r) { return (float)r + 0.5f; }
+
 // Exact reference sequence (volume_cuda.cu:42-51) for a sample flagged `near`.
 __device__ __forceinline__ int decide_exact(const ConfGeom& g, float d2, float wz,
                                             int zshift, float* alpha) {
@@ -2052,26 +2069,26 @@ __device__ __forceinline__ void a1_quad_item(
           return r;
         };
         // the same for the bin index held as a float (tf == (float)tt)
-        auto decide_bin_f = [&](float tf, float* d2o, float* alo) -> int {
+        auto decide_bin_f = [&](float tf, float* d2o, float* alo) -> float {
           float tc = tf * g.w;
           tc = tc + g.off;
           const float d2 = (tc * tc - dx2) - dy2;
           bool nr;
-          int r = decide_fast(g, d2, cp, delta, half_m_delta, alo, &nr);
+          float zk = decide_fast_key(g, d2, cp, half_m_delta, alo, &nr);
           if (wave_any(nr))
-            if (nr) r = decide_exact(g, d2, wz, zshift, alo);
+            if (nr) zk = row_key(decide_exact(g, d2, wz, zshift, alo));
           *d2o = d2;
-          return r;
+          return zk;
         };
         // the same from d2 = (tc^2 - dx^2) - dy^2 with tc^2 from the LDS
         // table (TAB)
-        auto decide_d2 = [&](float d2, float* d2o, float* alo) -> int {
+        auto decide_d2 = [&](float d2, float* d2o, float* alo) -> float {
           bool nr;
-          int r = decide_fast(g, d2, cp, delta, half_m_delta, alo, &nr);
+          float zk = decide_fast_key(g, d2, cp, half_m_delta, alo, &nr);
           if (wave_any(nr))
-            if (nr) r = decide_exact(g, d2, wz, zshift, alo);
+            if (nr) zk = row_key(decide_exact(g, d2, wz, zshift, alo));
           *d2o = d2;
-          return r;
+          return zk;
         };
         // The walk starts at the lane's first bin with d2 > 0 and row >= 0:
         // estimated from the real-valued inverse (depth of row 0's start), then
@@ -2112,6 +2129,8 @@ __device__ __forceinline__ void a1_quad_item(
         // and on row rr" is the single test r <= rr.
         if (!(t <= thi)) r = 0x7fffffff;
         int rr = uniform_i32(wave_min_i32(r));
+        // from here on the row is held as a key (row_key; key < rr + 1 <=> row <= rr)
+        float zk = row_key(r);
         V carry = zero4;
         // At the start of row rr every live lane's row is >= rr, so a lane
         // that is live with row <= rr is on row rr.
@@ -2140,8 +2159,9 @@ __device__ __forceinline__ void a1_quad_item(
         while (rr <= rspan) {
           if (!wave_any(live())) break;
           V L = carry, H = zero4;
+          const float rr1 = (float)(rr + 1);
           while (true) {
-            const bool act = r <= rr;
+            const bool act = zk < rr1;
             if (!wave_any(act)) break;
             if (act) {
               // this bin's weights, then the next bin's decision in place (past
@@ -2151,7 +2171,7 @@ __device__ __forceinline__ void a1_quad_item(
               // keep both products ahead of the next decision, so that d2 and al
               // are free for it (no register copies); emits no instruction
               asm volatile("" : "+v"(base), "+v"(wh));
-              int rn;
+              float rn;
               if (TAB) {
                 // the next bin's decision from tcn, which then takes tc^2 of
                 // the bin after it (a whole step ahead of its use)
@@ -2167,7 +2187,7 @@ __device__ __forceinline__ void a1_quad_item(
               off16 += kVB;
               kq = *(const V*)(kqp + off16);
               if (SAFE) ic = load_at(itc4t, off16 / (kVB / 4));
-              r = (!STOP || live()) ? rn : 0x7fffffff;
+              zk = (!STOP || live()) ? rn : INFINITY;
             }
           }
           V* row = my + rr * 64;
@@ -2350,24 +2370,24 @@ __device__ __forceinline__ void a1_row_item(
         *d2o = d2;
         return r;
       };
-      auto decide_bin_f = [&](float tf, float* d2o, float* alo) -> int {
+      auto decide_bin_f = [&](float tf, float* d2o, float* alo) -> float {
         float tc = tf * g.w;
         if (!OFF0) tc = tc + g.off;
         const float d2 = (tc * tc - dx2) - dy2;
         bool nr;
-        int r = decide_fast(g, d2, cp, delta, half_m_delta, alo, &nr);
+        float zk = decide_fast_key(g, d2, cp, half_m_delta, alo, &nr);
         if (wave_any(nr))
-          if (nr) r = decide_exact(g, d2, wz, zshift, alo);
+          if (nr) zk = row_key(decide_exact(g, d2, wz, zshift, alo));
         *d2o = d2;
-        return r;
+        return zk;
       };
-      auto decide_d2 = [&](float d2, float* d2o, float* alo) -> int {
+      auto decide_d2 = [&](float d2, float* d2o, float* alo) -> float {
         bool nr;
-        int r = decide_fast(g, d2, cp, delta, half_m_delta, alo, &nr);
+        float zk = decide_fast_key(g, d2, cp, half_m_delta, alo, &nr);
         if (wave_any(nr))
-          if (nr) r = decide_exact(g, d2, wz, zshift, alo);
+          if (nr) zk = row_key(decide_exact(g, d2, wz, zshift, alo));
         *d2o = d2;
-        return r;
+        return zk;
       };
       // Exact walk start (see a1_quad_item).
       const float rho2 = dx2 + dy2;
@@ -2402,6 +2422,7 @@ __device__ __forceinline__ void a1_row_item(
       }
       if (!(t <= thi)) r = 0x7fffffff;
       int rr = uniform_i32(wave_min_i32(r));
+      float zk = row_key(r);  // 0x7fffffff: a key above every row
       float carry = 0.f;
       // t <= thi + 1 <= T < t_stride: every table read is in the row
       uint32_t off4 = (uint32_t)t * 4u;
@@ -2417,14 +2438,15 @@ __device__ __forceinline__ void a1_row_item(
       while (rr <= rspan) {
         if (!wave_any(live())) break;
         float L = carry, H = 0.f;
+        const float rr1 = (float)(rr + 1);
         while (true) {
-          const bool act = r <= rr;
+          const bool act = zk < rr1;
           if (!wave_any(act)) break;
           if (act) {
             float base = SAFE ? sq(d2 * ic) : d2 * d2;
             float wh = base * al;
             asm volatile("" : "+v"(base), "+v"(wh));  // see a1_quad_item
-            int rn;
+            float rn;
             if (TAB) {  // as in a1_quad_item
               const float d2n = (tcn - dx2) - dy2;
               tcn = load_at(tab, off4 + 8u);
@@ -2438,7 +2460,7 @@ __device__ __forceinline__ void a1_row_item(
             off4 += 4u;
             k = load_at(kp, off4);
             if (SAFE) ic = load_at(itc4t, off4);
-            r = (!STOP || live()) ? rn : 0x7fffffff;
+            zk = (!STOP || live()) ? rn : INFINITY;
           }
         }
         float* row = my + rr * 64;
