@@ -2813,7 +2813,7 @@ R1SymPlan plan_r1_sym(int64_t X, int64_t Y, int64_t Z, int64_t N, int64_t T,
   const int64_t total_chunks = (n_reps + rpc - 1) / rpc;
   const int64_t quad_blocks = (N / 4 + kWaves - 1) / kWaves;
   // four workgroups per CU and pass (two run at a time): smaller items
-  // shorten the tail of the last round
+  // shorten the tail of the last wave of workgroups
   const int64_t target = 4 * (int64_t)device_cu_count();
   int64_t G = 1;
   if (quad_blocks < target) G = (target + quad_blocks - 1) / quad_blocks;
