@@ -2075,8 +2075,9 @@ __device__ __forceinline__ void a1_quad_item(
           const float d2 = (tc * tc - dx2) - dy2;
           bool nr;
           float zk = decide_fast_key(g, d2, cp, half_m_delta, alo, &nr);
-          if (wave_any(nr))
-            if (nr) zk = row_key(decide_exact(g, d2, wz, zshift, alo));
+          // a divergent branch, skipped when no lane is near (fewer scalar
+          // instructions than a ballot around it)
+          if (nr) zk = row_key(decide_exact(g, d2, wz, zshift, alo));
           *d2o = d2;
           return zk;
         };
@@ -2085,8 +2086,9 @@ __device__ __forceinline__ void a1_quad_item(
         auto decide_d2 = [&](float d2, float* d2o, float* alo) -> float {
           bool nr;
           float zk = decide_fast_key(g, d2, cp, half_m_delta, alo, &nr);
-          if (wave_any(nr))
-            if (nr) zk = row_key(decide_exact(g, d2, wz, zshift, alo));
+          // a divergent branch, skipped when no lane is near (fewer scalar
+          // instructions than a ballot around it)
+          if (nr) zk = row_key(decide_exact(g, d2, wz, zshift, alo));
           *d2o = d2;
           return zk;
         };
@@ -2160,35 +2162,34 @@ __device__ __forceinline__ void a1_quad_item(
           if (!wave_any(live())) break;
           V L = carry, H = zero4;
           const float rr1 = (float)(rr + 1);
-          while (true) {
-            const bool act = zk < rr1;
-            if (!wave_any(act)) break;
-            if (act) {
-              // this bin's weights, then the next bin's decision in place (past
-              // thi the lane stops and the decision is unused)
-              float base = SAFE ? sq(d2 * ic) : d2 * d2;
-              float wh = base * al;
-              // keep both products ahead of the next decision, so that d2 and al
-              // are free for it (no register copies); emits no instruction
-              asm volatile("" : "+v"(base), "+v"(wh));
-              float rn;
-              if (TAB) {
-                // the next bin's decision from tcn, which then takes tc^2 of
-                // the bin after it (a whole step ahead of its use)
-                const float d2n = (tcn - dx2) - dy2;
-                tcn = load_at(tab, off16 / (kVB / 4) + 8u);
-                rn = decide_d2(d2n, &d2, &al);
-              } else {
-                tf = tf + 1.f;
-                rn = decide_bin_f(tf, &d2, &al);
-              }
-              mv_fma(L, kq, base);
-              mv_fma(H, kq, wh);
-              off16 += kVB;
-              kq = *(const V*)(kqp + off16);
-              if (SAFE) ic = load_at(itc4t, off16 / (kVB / 4));
-              zk = (!STOP || live()) ? rn : INFINITY;
+          // A divergent loop: each lane leaves it at its row's end (the
+          // compiler keeps one exec mask update per step, where a uniform loop
+          // around an `if` costs several scalar instructions and branches)
+          while (zk < rr1) {
+            // this bin's weights, then the next bin's decision in place (past
+            // thi the lane stops and the decision is unused)
+            float base = SAFE ? sq(d2 * ic) : d2 * d2;
+            float wh = base * al;
+            // keep both products ahead of the next decision, so that d2 and al
+            // are free for it (no register copies); emits no instruction
+            asm volatile("" : "+v"(base), "+v"(wh));
+            float rn;
+            if (TAB) {
+              // the next bin's decision from tcn, which then takes tc^2 of
+              // the bin after it (a whole step ahead of its use)
+              const float d2n = (tcn - dx2) - dy2;
+              tcn = load_at(tab, off16 / (kVB / 4) + 8u);
+              rn = decide_d2(d2n, &d2, &al);
+            } else {
+              tf = tf + 1.f;
+              rn = decide_bin_f(tf, &d2, &al);
             }
+            mv_fma(L, kq, base);
+            mv_fma(H, kq, wh);
+            off16 += kVB;
+            kq = *(const V*)(kqp + off16);
+            if (SAFE) ic = load_at(itc4t, off16 / (kVB / 4));
+            zk = (!STOP || live()) ? rn : INFINITY;
           }
           V* row = my + rr * 64;
           V a = *row;
@@ -2376,16 +2377,14 @@ __device__ __forceinline__ void a1_row_item(
         const float d2 = (tc * tc - dx2) - dy2;
         bool nr;
         float zk = decide_fast_key(g, d2, cp, half_m_delta, alo, &nr);
-        if (wave_any(nr))
-          if (nr) zk = row_key(decide_exact(g, d2, wz, zshift, alo));
+        if (nr) zk = row_key(decide_exact(g, d2, wz, zshift, alo));
         *d2o = d2;
         return zk;
       };
       auto decide_d2 = [&](float d2, float* d2o, float* alo) -> float {
         bool nr;
         float zk = decide_fast_key(g, d2, cp, half_m_delta, alo, &nr);
-        if (wave_any(nr))
-          if (nr) zk = row_key(decide_exact(g, d2, wz, zshift, alo));
+        if (nr) zk = row_key(decide_exact(g, d2, wz, zshift, alo));
         *d2o = d2;
         return zk;
       };
@@ -2439,29 +2438,25 @@ __device__ __forceinline__ void a1_row_item(
         if (!wave_any(live())) break;
         float L = carry, H = 0.f;
         const float rr1 = (float)(rr + 1);
-        while (true) {
-          const bool act = zk < rr1;
-          if (!wave_any(act)) break;
-          if (act) {
-            float base = SAFE ? sq(d2 * ic) : d2 * d2;
-            float wh = base * al;
-            asm volatile("" : "+v"(base), "+v"(wh));  // see a1_quad_item
-            float rn;
-            if (TAB) {  // as in a1_quad_item
-              const float d2n = (tcn - dx2) - dy2;
-              tcn = load_at(tab, off4 + 8u);
-              rn = decide_d2(d2n, &d2, &al);
-            } else {
-              tf = tf + 1.f;
-              rn = decide_bin_f(tf, &d2, &al);
-            }
-            L = fmaf(k, base, L);
-            H = fmaf(k, wh, H);
-            off4 += 4u;
-            k = load_at(kp, off4);
-            if (SAFE) ic = load_at(itc4t, off4);
-            zk = (!STOP || live()) ? rn : INFINITY;
+        while (zk < rr1) {  // divergent, as in a1_quad_item
+          float base = SAFE ? sq(d2 * ic) : d2 * d2;
+          float wh = base * al;
+          asm volatile("" : "+v"(base), "+v"(wh));  // see a1_quad_item
+          float rn;
+          if (TAB) {  // as in a1_quad_item
+            const float d2n = (tcn - dx2) - dy2;
+            tcn = load_at(tab, off4 + 8u);
+            rn = decide_d2(d2n, &d2, &al);
+          } else {
+            tf = tf + 1.f;
+            rn = decide_bin_f(tf, &d2, &al);
           }
+          L = fmaf(k, base, L);
+          H = fmaf(k, wh, H);
+          off4 += 4u;
+          k = load_at(kp, off4);
+          if (SAFE) ic = load_at(itc4t, off4);
+          zk = (!STOP || live()) ? rn : INFINITY;
         }
         float* row = my + rr * 64;
         *row += L - H;
