@@ -2063,8 +2063,7 @@ __device__ __forceinline__ void a1_quad_item(
           const float d2 = (tc * tc - dx2) - dy2;
           bool nr;
           int r = decide_fast(g, d2, cp, delta, half_m_delta, alo, &nr);
-          if (wave_any(nr))
-            if (nr) r = decide_exact(g, d2, wz, zshift, alo);
+          if (nr) r = decide_exact(g, d2, wz, zshift, alo);
           *d2o = d2;
           return r;
         };
@@ -2105,30 +2104,27 @@ __device__ __forceinline__ void a1_quad_item(
         float d2, al;
         int r = decide_bin(t, &d2, &al);
         bool back = (t <= thi) & (t > tlo) & (d2 > 0.f) & (r >= 0);
-        while (wave_any(back)) {
+        while (back) {  // divergent: each lane leaves at its own bin
+          float d2p, alp;
+          const int rp = decide_bin(t - 1, &d2p, &alp);
+          back = (d2p > 0.f) & (rp >= 0);
           if (back) {
-            float d2p, alp;
-            const int rp = decide_bin(t - 1, &d2p, &alp);
-            back = (d2p > 0.f) & (rp >= 0);
-            if (back) {
-              --t;
-              r = rp;
-              al = alp;
-              d2 = d2p;
-              back = t > tlo;
-            }
+            --t;
+            r = rp;
+            al = alp;
+            d2 = d2p;
+            back = t > tlo;
           }
         }
         bool fwd = (t <= thi) & !((d2 > 0.f) & (r >= 0));
-        while (wave_any(fwd)) {
-          if (fwd) {
-            ++t;
-            r = decide_bin(t, &d2, &al);
-            fwd = (t <= thi) & !((d2 > 0.f) & (r >= 0));
-          }
+        while (fwd) {  // divergent: each lane leaves at its own bin
+          ++t;
+          r = decide_bin(t, &d2, &al);
+          fwd = (t <= thi) & !((d2 > 0.f) & (r >= 0));
         }
-        // A lane with no bin left carries the row 0x7fffffff, so "still walking
-        // and on row rr" is the single test r <= rr.
+        // A lane with no bin left carries the row 0x7fffffff (a key above
+        // every row), so "still walking and on row rr" is the single test
+        // key < rr + 1.
         if (!(t <= thi)) r = 0x7fffffff;
         int rr = uniform_i32(wave_min_i32(r));
         // from here on the row is held as a key (row_key; key < rr + 1 <=> row <= rr)
@@ -2366,8 +2362,7 @@ __device__ __forceinline__ void a1_row_item(
         const float d2 = (tc * tc - dx2) - dy2;
         bool nr;
         int r = decide_fast(g, d2, cp, delta, half_m_delta, alo, &nr);
-        if (wave_any(nr))
-          if (nr) r = decide_exact(g, d2, wz, zshift, alo);
+        if (nr) r = decide_exact(g, d2, wz, zshift, alo);
         *d2o = d2;
         return r;
       };
@@ -2397,27 +2392,23 @@ __device__ __forceinline__ void a1_row_item(
       float d2, al;
       int r = decide_bin(t, &d2, &al);
       bool back = (t <= thi) & (t > tlo) & (d2 > 0.f) & (r >= 0);
-      while (wave_any(back)) {
+      while (back) {  // divergent: each lane leaves at its own bin
+        float d2p, alp;
+        const int rp = decide_bin(t - 1, &d2p, &alp);
+        back = (d2p > 0.f) & (rp >= 0);
         if (back) {
-          float d2p, alp;
-          const int rp = decide_bin(t - 1, &d2p, &alp);
-          back = (d2p > 0.f) & (rp >= 0);
-          if (back) {
-            --t;
-            r = rp;
-            al = alp;
-            d2 = d2p;
-            back = t > tlo;
-          }
+          --t;
+          r = rp;
+          al = alp;
+          d2 = d2p;
+          back = t > tlo;
         }
       }
       bool fwd = (t <= thi) & !((d2 > 0.f) & (r >= 0));
-      while (wave_any(fwd)) {
-        if (fwd) {
-          ++t;
-          r = decide_bin(t, &d2, &al);
-          fwd = (t <= thi) & !((d2 > 0.f) & (r >= 0));
-        }
+      while (fwd) {  // divergent: each lane leaves at its own bin
+        ++t;
+        r = decide_bin(t, &d2, &al);
+        fwd = (t <= thi) & !((d2 > 0.f) & (r >= 0));
       }
       if (!(t <= thi)) r = 0x7fffffff;
       int rr = uniform_i32(wave_min_i32(r));
