@@ -124,6 +124,27 @@ __device__ __forceinline__ void bin_range(const ConfGeom& g, float rho2, float d
   }
 }
 
+// bin_range for the row walks' geometries (w > 0 and off >= 0, both finite):
+// every bin's radius t w + off is >= 0, so only the positive-radius interval
+// can hold an active bin, and it is the whole result.
+__device__ __forceinline__ void bin_range_pos(const ConfGeom& g, float rho2, float dlo,
+                                              float dhi, int* lo, int* hi) {
+  if (!(dhi > 0.f)) {
+    *lo = 1;
+    *hi = 0;
+    return;
+  }
+  const float dl = fmaxf(dlo, 0.f);
+  const float r_lo = sqrt_fast(rho2 + dl * dl) * (1.f - 1e-5f) - 1e-6f;
+  const float r_hi = sqrt_fast(rho2 + dhi * dhi) * (1.f + 1e-5f) + 1e-6f;
+  const float iw = __builtin_amdgcn_rcpf(g.w);
+  const float top = (float)g.T + 1.f;
+  const float pa = fminf(fmaxf((r_lo - g.off) * iw - 2.f, -1.f), top);
+  const float pb = fminf(fmaxf((r_hi - g.off) * iw + 2.f, -1.f), top);
+  *lo = max((int)floorf(pa), 0);
+  *hi = min((int)ceilf(pb), g.T - 1);
+}
+
 // Depth interval [dlo, dhi) that keeps z_hat in [zlo, zhi) for a wall point
 // at height wz, widened by a relative margin (culling only).
 __device__ __forceinline__ void depth_range(const ConfGeom& g, float wz, float zlo,
@@ -2039,7 +2060,7 @@ __device__ __forceinline__ void a1_quad_item(
       const float dx2 = cvalid ? ((f & 1) ? dxb * dxb : dxa * dxa) : INFINITY;
       const float dy2 = (M == 4 && (f & 2)) ? dyb * dyb : dya * dya;
       int tlo, thi;
-      bin_range(g, dx2 + dy2, dlo, dhi, &tlo, &thi);
+      bin_range_pos(g, dx2 + dy2, dlo, dhi, &tlo, &thi);
       if (!cvalid || rspan < 0 || tlo > thi) {
         tlo = 0;
         thi = -1;
@@ -2343,7 +2364,7 @@ __device__ __forceinline__ void a1_row_item(
     float dlo, dhi;
     depth_range(g, wz, zlo, zhi, &dlo, &dhi);
     int tlo, thi;
-    bin_range(g, dx2 + dy2, dlo, dhi, &tlo, &thi);
+    bin_range_pos(g, dx2 + dy2, dlo, dhi, &tlo, &thi);
     if (!cvalid || rspan < 0 || tlo > thi) {
       tlo = 0;
       thi = -1;
